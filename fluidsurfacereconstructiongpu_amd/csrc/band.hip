@@ -1,0 +1,355 @@
+// Surface band of scalar vertices, its compaction, and the cell-major work
+// list of the field kernels.
+//
+// The reference (estimationOfSurfaceVertices, ReconstructionCUDA.cu:554-595)
+// loops, for every surface particle in scalar cell (i, j, k), over the vertices
+//   x in [clamp(i-3), clamp(i+3))   (upper bound exclusive)
+//   y in [clamp(j-3), clamp(j+3)],  z in [clamp(k-3), clamp(k+3)]
+// (clamp to [0, res-1]) and marks those with FP32 |v - p| < 1.5 h (compared in
+// double) in a dense u32[V] grid, which is then scanned (thrust) and compacted
+// (compactSurfaceVertex, ReconstructionCUDA.cu:293-304).
+//
+// Here the band is a bit mask (V/8 bytes instead of 4V).  When every vertex of
+// such a box lies within 1.5 h of any point of the particle's scalar cell
+// (4*sqrt(3) scalar cells < 1.5 h, checked on the host with a rounding margin),
+// the distance test never rejects and the band of in-grid particles is exactly
+// the dilation of the set S of their scalar cells by the box
+//   x - s_x in [-3, 2] (and x <= resx-2),  |y - s_y| <= 3,  |z - s_z| <= 3,
+// done as three separable passes over the mask.  Particles outside the scalar
+// grid, and every particle when the condition fails, go through the exact
+// per-particle loop of the reference (k_band_exceptions).
+//
+// Compaction: popcount per word -> exclusive scan -> each word writes its set
+// bits in ascending order (the reference's surfaceVerticesIndexArray).
+//
+// Cell-major list: the field kernels process the band grouped by the hash cell
+// each vertex falls in (getIndex3D of the vertex position, exactly as the
+// reference's field kernels compute it), so that all 64 lanes of a wave share
+// one neighbour-cell block.  Cells outside the hash grid are folded into a ring
+// of `radius` virtual cells on each side (their clamped neighbour blocks
+// coincide).  The order of vertices inside a cell's list does not affect any
+// value: every vertex sums its candidates in the cell's fixed order.
+#include <algorithm>
+#include <cmath>
+
+#include "common.hpp"
+
+namespace fsr {
+
+namespace {
+
+__global__ __launch_bounds__(256) void k_dilate_x(const uint32_t* __restrict__ in,
+                                                  uint32_t* __restrict__ out, uint32_t wx,
+                                                  uint64_t nwords, uint32_t rx) {
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwords) return;
+    const uint32_t xw = (uint32_t)(w % wx);
+    const uint32_t cur = in[w];
+    const uint32_t prev = xw > 0 ? in[w - 1] : 0u;
+    const uint32_t next = xw + 1 < wx ? in[w + 1] : 0u;
+    uint32_t o = cur;
+    o |= (cur >> 1) | (next << 31);
+    o |= (cur >> 2) | (next << 30);
+    o |= (cur >> 3) | (next << 29);
+    o |= (cur << 1) | (prev >> 31);
+    o |= (cur << 2) | (prev >> 30);
+    // the reference's exclusive x bound after clamping: x <= rx - 2
+    const uint32_t lim = rx >= 1 ? rx - 1 : 0;
+    const uint32_t x0 = xw * 32u;
+    if (x0 >= lim) o = 0;
+    else if (lim - x0 < 32u) o &= (1u << (lim - x0)) - 1u;
+    out[w] = o;
+}
+
+// out(w, row) = OR of in over rows row + d*stride, |d| <= 3, inside [0, extent)
+__global__ __launch_bounds__(256) void k_dilate_rows(const uint32_t* __restrict__ in,
+                                                     uint32_t* __restrict__ out,
+                                                     uint64_t nwords, uint64_t stride,
+                                                     uint32_t extent, uint64_t period) {
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwords) return;
+    const uint32_t c = (uint32_t)((w / stride) % period);
+    uint32_t o = 0;
+#pragma unroll
+    for (int d = -3; d <= 3; ++d) {
+        const int cc = (int)c + d;
+        if (cc >= 0 && cc < (int)extent) o |= in[(uint64_t)((int64_t)w + (int64_t)d * (int64_t)stride)];
+    }
+    out[w] = o;
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) {
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+__device__ __forceinline__ int trunc_clamped(float q, int res) {
+    if (!(q == q)) return 0;
+    const float lo = -4.0f, hi = (float)res + 4.0f;
+    q = q < lo ? lo : (q > hi ? hi : q);
+    return (int)q;
+}
+
+// The reference's per-particle band loop for the exception list.
+__global__ __launch_bounds__(256) void k_band_exceptions(BandArgs a) {
+    const uint32_t count = a.counters->num_exceptions;
+    const uint32_t wx = row_words(a.sc.rx);
+    const double limit = (double)a.h * 1.5;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < count;
+         e += gridDim.x * blockDim.x) {
+        const float4 p = a.sorted[a.exceptions[e]];
+        const int i = trunc_clamped(cell_coord(p.x, a.sc.minx, a.sc.cell), a.sc.rx);
+        const int j = trunc_clamped(cell_coord(p.y, a.sc.miny, a.sc.cell), a.sc.ry);
+        const int k = trunc_clamped(cell_coord(p.z, a.sc.minz, a.sc.cell), a.sc.rz);
+        const int x0 = clampi(i - 3, 0, a.sc.rx - 1), x1 = clampi(i + 3, 0, a.sc.rx - 1);
+        const int y0 = clampi(j - 3, 0, a.sc.ry - 1), y1 = clampi(j + 3, 0, a.sc.ry - 1);
+        const int z0 = clampi(k - 3, 0, a.sc.rz - 1), z1 = clampi(k + 3, 0, a.sc.rz - 1);
+        for (int z = z0; z <= z1; ++z) {
+            const float vz = vertex_coord((uint32_t)z, a.sc.minz, a.sc.cell);
+            for (int y = y0; y <= y1; ++y) {
+                const float vy = vertex_coord((uint32_t)y, a.sc.miny, a.sc.cell);
+                const uint64_t rowbase = ((uint64_t)z * a.sc.ry + y) * wx;
+                uint32_t cur_w = 0xFFFFFFFFu, m = 0;
+                for (int x = x0; x < x1; ++x) {
+                    const float vx = vertex_coord((uint32_t)x, a.sc.minx, a.sc.cell);
+                    const float dx = vx - p.x, dy = vy - p.y, dz = vz - p.z;
+                    const float dist =
+                        __builtin_sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)));
+                    if ((double)dist < limit) {
+                        const uint32_t ww = (uint32_t)x >> 5;
+                        if (ww != cur_w) {
+                            if (m) atomicOr(&a.bits[rowbase + cur_w], m);
+                            cur_w = ww;
+                            m = 0;
+                        }
+                        m |= 1u << ((uint32_t)x & 31u);
+                    }
+                }
+                if (m) atomicOr(&a.bits[rowbase + cur_w], m);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_band_emit(const uint32_t* __restrict__ bits,
+                                                   const uint32_t* __restrict__ word_off,
+                                                   uint64_t nwords, uint32_t wx, uint32_t rx,
+                                                   uint32_t* __restrict__ band_index) {
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwords) return;
+    uint32_t b = bits[w];
+    if (!b) return;
+    const uint64_t row = w / wx;
+    const uint32_t x0 = (uint32_t)(w % wx) * 32u;
+    const uint32_t base = (uint32_t)(row * rx) + x0;
+    uint32_t o = word_off[w];
+    while (b) {
+        const uint32_t t = __builtin_ctz(b);
+        band_index[o++] = base + t;
+        b &= b - 1u;
+    }
+}
+
+// Per axis: virtual cell of every scalar vertex, and each virtual cell's
+// vertex interval [lo, hi).  Layout of `maps`:
+//   vc_x[rx] vc_y[ry] vc_z[rz] lo_x[vcx] hi_x[vcx] lo_y[vcy] hi_y[vcy] lo_z[vcz] hi_z[vcz]
+struct AxisMaps {
+    uint32_t *vc[3], *lo[3], *hi[3];
+};
+
+__global__ void k_axis_maps(GridDev sc, GridDev sp, int radius, AxisMaps m) {
+    const uint32_t axis = blockIdx.y;
+    const uint32_t res = axis == 0 ? sc.rx : (axis == 1 ? sc.ry : sc.rz);
+    const float smin = axis == 0 ? sc.minx : (axis == 1 ? sc.miny : sc.minz);
+    const float pmin = axis == 0 ? sp.minx : (axis == 1 ? sp.miny : sp.minz);
+    const int pres = (int)(axis == 0 ? sp.rx : (axis == 1 ? sp.ry : sp.rz));
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < res; i += gridDim.x * blockDim.x) {
+        auto vc_of = [&](uint32_t v) -> uint32_t {
+            const float q = cell_coord(vertex_coord(v, smin, sc.cell), pmin, sp.cell);
+            int c = trunc_clamped(q, pres + radius);
+            c = clampi(c, -radius, pres - 1 + radius);
+            return (uint32_t)(c + radius);
+        };
+        const uint32_t c = vc_of(i);
+        m.vc[axis][i] = c;
+        if (i == 0 || vc_of(i - 1) != c) m.lo[axis][c] = i;
+        if (i + 1 == res || vc_of(i + 1) != c) m.hi[axis][c] = i + 1;
+    }
+}
+
+// Thread per scalar row (y, z): counts, per x-virtual-cell, the band bits of
+// that row and adds them to the cell's count.
+__global__ __launch_bounds__(256) void k_cell_count_rows(const uint32_t* __restrict__ bits,
+                                                         GridDev sc, uint32_t wx, AxisMaps m,
+                                                         uint32_t vcx, uint32_t vcy,
+                                                         uint32_t* __restrict__ cnt) {
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= sc.ry * sc.rz) return;
+    const uint32_t y = row % sc.ry, z = row / sc.ry;
+    const uint32_t* rb = bits + (uint64_t)row * wx;
+    const uint32_t vbase = (m.vc[2][z] * vcy + m.vc[1][y]) * vcx;
+    for (uint32_t w = 0; w < wx; ++w) {
+        uint32_t b = rb[w];
+        while (b) {
+            const uint32_t x = w * 32u + __builtin_ctz(b);
+            const uint32_t cx = m.vc[0][x];
+            // all bits of this word inside the same x cell at once
+            const uint32_t hi = m.hi[0][cx];
+            const uint32_t end = hi < (w + 1) * 32u ? hi - w * 32u : 32u;
+            const uint32_t sel = end >= 32u ? b : (b & ((1u << end) - 1u));
+            atomicAdd(&cnt[vbase + cx], (uint32_t)__popc(sel));
+            b &= ~sel;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cell_fill_rows(const uint32_t* __restrict__ bits,
+                                                        GridDev sc, uint32_t wx, AxisMaps m,
+                                                        uint32_t vcx, uint32_t vcy,
+                                                        const uint32_t* __restrict__ cell_off,
+                                                        uint32_t* __restrict__ cursor,
+                                                        uint32_t* __restrict__ cell_band) {
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= sc.ry * sc.rz) return;
+    const uint32_t y = row % sc.ry, z = row / sc.ry;
+    const uint32_t* rb = bits + (uint64_t)row * wx;
+    const uint32_t vbase = (m.vc[2][z] * vcy + m.vc[1][y]) * vcx;
+    const uint32_t lin0 = row * sc.rx;
+    for (uint32_t w = 0; w < wx; ++w) {
+        uint32_t b = rb[w];
+        while (b) {
+            const uint32_t x = w * 32u + __builtin_ctz(b);
+            const uint32_t cx = m.vc[0][x];
+            const uint32_t hi = m.hi[0][cx];
+            const uint32_t end = hi < (w + 1) * 32u ? hi - w * 32u : 32u;
+            uint32_t sel = end >= 32u ? b : (b & ((1u << end) - 1u));
+            b &= ~sel;
+            const uint32_t v = vbase + cx;
+            uint32_t o = cell_off[v] + atomicAdd(&cursor[v], (uint32_t)__popc(sel));
+            while (sel) {
+                cell_band[o++] = lin0 + w * 32u + __builtin_ctz(sel);
+                sel &= sel - 1u;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_chunk_cells(const uint32_t* __restrict__ cnt,
+                                                     const uint32_t* __restrict__ chunk_off,
+                                                     uint32_t nvc, uint32_t* __restrict__ chunk_cell) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvc) return;
+    const uint32_t k = (cnt[v] + 63u) >> 6;
+    const uint32_t o = chunk_off[v];
+    for (uint32_t j = 0; j < k; ++j) chunk_cell[o + j] = v;
+}
+
+}  // namespace
+
+int band_dilation_exact(const fsr_grid_info& sc, float h) {
+    // farthest box vertex from any point of the particle's scalar cell:
+    // |dx| < 4, |dy| < 4, |dz| < 4 cells
+    const double reach = 4.0 * 1.7320508075688772 * (double)sc.cell_size;
+    double maxabs = 0.0;
+    const double lo[3] = {sc.min_pos.x, sc.min_pos.y, sc.min_pos.z};
+    const uint32_t r[3] = {sc.resolution.x, sc.resolution.y, sc.resolution.z};
+    for (int a = 0; a < 3; ++a) {
+        const double hi = lo[a] + (double)sc.cell_size * (double)r[a];
+        maxabs = std::max(maxabs, std::max(std::fabs(lo[a]), std::fabs(hi)));
+    }
+    // FP32 rounding of the vertex position, the difference and the length
+    const double slack = 64.0 * 1.1920928955078125e-7 * std::max(maxabs, (double)h);
+    return reach + slack < 1.5 * (double)h ? 1 : 0;
+}
+
+fsr_status band_build(fsr_context* ctx, const BandArgs& a, hipStream_t s) {
+    const uint32_t wx = row_words(a.sc.rx);
+    const uint64_t nwords = (uint64_t)wx * a.sc.ry * a.sc.rz;
+    const uint32_t blocks = (uint32_t)((nwords + 255) / 256);
+    // S -> tmp (x), tmp -> S (y), S -> bits (z); `sbits` is consumed
+    uint32_t* sb = const_cast<uint32_t*>(a.sbits);
+    hipLaunchKernelGGL(k_dilate_x, dim3(blocks), dim3(256), 0, s, sb, a.tmp, wx, nwords, a.sc.rx);
+    FSR_LAUNCH_CHECK("k_dilate_x");
+    hipLaunchKernelGGL(k_dilate_rows, dim3(blocks), dim3(256), 0, s, a.tmp, sb, nwords,
+                       (uint64_t)wx, a.sc.ry, (uint64_t)a.sc.ry);
+    FSR_LAUNCH_CHECK("k_dilate_rows(y)");
+    hipLaunchKernelGGL(k_dilate_rows, dim3(blocks), dim3(256), 0, s, sb, a.bits, nwords,
+                       (uint64_t)wx * a.sc.ry, a.sc.rz, (uint64_t)a.sc.rz);
+    FSR_LAUNCH_CHECK("k_dilate_rows(z)");
+    const int eb = persistent_blocks(ctx, 256, 4);
+    hipLaunchKernelGGL(k_band_exceptions, dim3(eb), dim3(256), 0, s, a);
+    FSR_LAUNCH_CHECK("k_band_exceptions");
+    return FSR_OK;
+}
+
+fsr_status band_compact(fsr_context* ctx, const uint32_t* bits, const GridDev& sc,
+                        const GridDev& sp, int radius, Counters* counters, fsr_frame_stats* st,
+                        BandCompactOut* out, hipStream_t s) {
+    const uint32_t wx = row_words(sc.rx);
+    const uint64_t nwords = (uint64_t)wx * sc.ry * sc.rz;
+    const uint64_t V = (uint64_t)sc.rx * sc.ry * sc.rz;
+    uint32_t *word_off, *band_index;
+    FSR_TRY(arena_typed(ctx, A_WORD_OFF, nwords, &word_off));
+    FSR_TRY(arena_typed(ctx, A_BAND_INDEX, V, &band_index));
+    FSR_TRY(scan_exclusive(ctx, SCAN_POPC, bits, word_off, nwords, nullptr,
+                           &st->num_band_vertices, s));
+    hipLaunchKernelGGL(k_band_emit, dim3((uint32_t)((nwords + 255) / 256)), dim3(256), 0, s, bits,
+                       word_off, nwords, wx, sc.rx, band_index);
+    FSR_LAUNCH_CHECK("k_band_emit");
+    out->band_index = band_index;
+
+    // cell-major work list
+    const uint32_t vcx = sp.rx + 2 * radius, vcy = sp.ry + 2 * radius, vcz = sp.rz + 2 * radius;
+    const uint64_t nvc = (uint64_t)vcx * vcy * vcz;
+    uint32_t *maps, *cnt, *cell_off, *cursor, *chunk_off, *cell_band, *chunk_cell;
+    const uint64_t map_len = (uint64_t)sc.rx + sc.ry + sc.rz + 2ull * (vcx + vcy + vcz);
+    FSR_TRY(arena_typed(ctx, A_AXIS_MAPS, map_len, &maps));
+    FSR_TRY(arena_typed(ctx, A_CELL_CNT, nvc, &cnt));
+    FSR_TRY(arena_typed(ctx, A_CELL_OFF, nvc, &cell_off));
+    FSR_TRY(arena_typed(ctx, A_CELL_CHUNKS, nvc, &cursor));
+    FSR_TRY(arena_typed(ctx, A_CHUNK_OFF, nvc, &chunk_off));
+    FSR_TRY(arena_typed(ctx, A_CELL_BAND, V, &cell_band));
+    const uint64_t max_chunks = V / 64 + nvc + 1;
+    FSR_TRY(arena_typed(ctx, A_CHUNK_CELL, max_chunks, &chunk_cell));
+    AxisMaps m;
+    m.vc[0] = maps;
+    m.vc[1] = m.vc[0] + sc.rx;
+    m.vc[2] = m.vc[1] + sc.ry;
+    m.lo[0] = m.vc[2] + sc.rz;
+    m.hi[0] = m.lo[0] + vcx;
+    m.lo[1] = m.hi[0] + vcx;
+    m.hi[1] = m.lo[1] + vcy;
+    m.lo[2] = m.hi[1] + vcy;
+    m.hi[2] = m.lo[2] + vcz;
+    FSR_TRY(fill_u32(m.lo[0], 0u, 2ull * (vcx + vcy + vcz), s));
+    FSR_TRY(fill_u32(cnt, 0u, nvc, s));
+    FSR_TRY(fill_u32(cursor, 0u, nvc, s));
+    const uint32_t maxres = std::max(sc.rx, std::max(sc.ry, sc.rz));
+    hipLaunchKernelGGL(k_axis_maps, dim3((maxres + 255) / 256, 3), dim3(256), 0, s, sc, sp, radius, m);
+    FSR_LAUNCH_CHECK("k_axis_maps");
+    const uint32_t rows = sc.ry * sc.rz;
+    hipLaunchKernelGGL(k_cell_count_rows, dim3((rows + 255) / 256), dim3(256), 0, s, bits, sc, wx,
+                       m, vcx, vcy, cnt);
+    FSR_LAUNCH_CHECK("k_cell_count_rows");
+    FSR_TRY(scan_exclusive(ctx, SCAN_U32, cnt, cell_off, nvc, nullptr, nullptr, s));
+    FSR_TRY(scan_exclusive(ctx, SCAN_CHUNKS64, cnt, chunk_off, nvc, nullptr,
+                           &counters->num_chunks, s));
+    hipLaunchKernelGGL(k_cell_fill_rows, dim3((rows + 255) / 256), dim3(256), 0, s, bits, sc, wx,
+                       m, vcx, vcy, cell_off, cursor, cell_band);
+    FSR_LAUNCH_CHECK("k_cell_fill_rows");
+    hipLaunchKernelGGL(k_chunk_cells, dim3((uint32_t)((nvc + 255) / 256)), dim3(256), 0, s, cnt,
+                       chunk_off, (uint32_t)nvc, chunk_cell);
+    FSR_LAUNCH_CHECK("k_chunk_cells");
+    out->cell_off = cell_off;
+    out->cell_cnt = cnt;
+    out->chunk_off = chunk_off;
+    out->cell_band = cell_band;
+    out->chunk_cell = chunk_cell;
+    out->vcx = vcx;
+    out->vcy = vcy;
+    out->vcz = vcz;
+    out->radius = radius;
+    out->max_chunks = max_chunks;
+    return FSR_OK;
+}
+
+}  // namespace fsr

@@ -1,0 +1,306 @@
+// Internal declarations shared by the HIP translation units of libfsr.
+//
+// Data layout in HBM (one frame, N particles, C spatial cells, V scalar
+// vertices, B band vertices):
+//   particles      float3[N] caller buffer, sorted in place by cell id
+//   packed         float4[N] (x, y, z, density) copy taken by the hash pass
+//   sorted         float4[N] (x, y, z, density) in cell order, used by every
+//                  neighbour gather (16-byte aligned loads)
+//   ranges         fsr_index_range[C]
+//   band bits      u32[Wx * resy * resz], Wx = ceil(resx / 32): one bit per
+//                  scalar vertex instead of the reference's dense u32[V] flags
+//   band index     u32[B] ascending linear vertex ids (the reference's
+//                  surfaceVerticesIndexArray)
+//   cell-major     u32[B] the same ids grouped by the hash cell each vertex
+//                  falls in, cut into 64-vertex chunks for the field kernels
+//   phi            float[V] caller buffer
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/fsr.h"
+
+namespace fsr {
+
+// ---------------------------------------------------------------------------
+// Error handling
+// ---------------------------------------------------------------------------
+void set_error(const std::string& msg);
+fsr_status hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define FSR_HIP(expr)                                                      \
+    do {                                                                   \
+        hipError_t _e = (expr);                                            \
+        if (_e != hipSuccess) return ::fsr::hip_fail(_e, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+#define FSR_LAUNCH_CHECK(name)                                             \
+    do {                                                                   \
+        hipError_t _e = hipGetLastError();                                 \
+        if (_e != hipSuccess) return ::fsr::hip_fail(_e, name, __FILE__, __LINE__); \
+    } while (0)
+
+#define FSR_TRY(expr)                                                      \
+    do {                                                                   \
+        fsr_status _s = (expr);                                            \
+        if (_s != FSR_OK) return _s;                                       \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Grid description used inside kernels (passed by value)
+// ---------------------------------------------------------------------------
+struct GridDev {
+    float minx, miny, minz;
+    float cell;
+    uint32_t rx, ry, rz;
+};
+
+inline GridDev to_dev(const fsr_grid_info& g) {
+    GridDev d;
+    d.minx = g.min_pos.x; d.miny = g.min_pos.y; d.minz = g.min_pos.z;
+    d.cell = g.cell_size;
+    d.rx = g.resolution.x; d.ry = g.resolution.y; d.rz = g.resolution.z;
+    return d;
+}
+
+inline uint64_t grid_count(const fsr_grid_info& g) {
+    return (uint64_t)g.resolution.x * g.resolution.y * g.resolution.z;
+}
+
+// Words per x-row of a band bit mask.
+__host__ __device__ inline uint32_t row_words(uint32_t rx) { return (rx + 31u) / 32u; }
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+#ifdef __HIPCC__
+// The reference's getIndex3D (ReconstructionCUDA.cuh:354-358): IEEE FP32
+// subtract and divide, then truncation toward zero (int3(float3)).  Returns the
+// FP32 quotient so callers can range-check it before truncating.
+__device__ __forceinline__ float cell_coord(float p, float mn, float cell) {
+    return (p - mn) / cell;
+}
+
+// The reference's getVertexPos (ReconstructionCUDA.cuh:366-370), evaluated as
+// one fused multiply-add per axis (the contraction nvcc applies to it).
+__device__ __forceinline__ float vertex_coord(uint32_t i, float mn, float cell) {
+    return __builtin_fmaf((float)i, cell, mn);
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Number of active lanes below this one whose bit is set in `mask`.
+__device__ __forceinline__ uint32_t mask_rank(uint64_t mask) {
+    uint32_t lo = __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u);
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), lo);
+}
+
+// Wave-wide inclusive sum of a u32 (64 lanes).
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t o = __shfl_up(v, d, 64);
+        if (lane >= (uint32_t)d) v += o;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// Adds a per-lane count into a global counter with one atomic per wave.
+__device__ __forceinline__ void wave_count_add(uint32_t* counter, uint32_t v) {
+    uint32_t s = wave_sum(v);
+    if (__lane_id() == 0 && s) atomicAdd(counter, s);
+}
+#endif
+
+// ---------------------------------------------------------------------------
+// Scratch arena: named device buffers that only grow.
+// ---------------------------------------------------------------------------
+enum ArenaSlot {
+    A_PACKED = 0, A_SORTED, A_KEYS0, A_KEYS1, A_VALS0, A_VALS1, A_RADIX_HIST,
+    A_RANGES, A_OCCUPANCY, A_SURF_CELL, A_SURF_FLAGS, A_SBITS, A_BAND_TMP,
+    A_BAND_BITS, A_EXCEPTIONS, A_WORD_OFF, A_BAND_INDEX, A_AXIS_MAPS,
+    A_CELL_CNT, A_CELL_OFF, A_CELL_CHUNKS, A_CHUNK_OFF, A_CELL_BAND,
+    A_CHUNK_CELL, A_NVERTS, A_VALID, A_VALID_OFF, A_VERT_OFF, A_VALID_IDX,
+    A_COUNTERS, A_SCAN_PARTIALS, A_STATS, A_INVOLVE, A_INVOLVE_OFF,
+    A_INVOLVED_IDX, A_MEAN, A_SMOOTH, A_GMAT, A_SIGMA, A_AUX0, A_AUX1,
+    A_NUM_SLOTS
+};
+
+struct Arena {
+    void* ptr[A_NUM_SLOTS] = {};
+    size_t bytes[A_NUM_SLOTS] = {};
+    uint64_t count[A_NUM_SLOTS] = {};
+};
+
+// Internal device counters (reset by a fill kernel at frame start, so a frame
+// can be captured in a graph without memset nodes).
+struct Counters {
+    uint32_t field_next_chunk;
+    uint32_t num_chunks;
+    uint32_t num_exceptions;
+    uint32_t mc_next;
+    uint32_t aux[12];
+};
+
+}  // namespace fsr
+
+struct fsr_context {
+    int device = 0;
+    int num_cus = 256;
+    int last_sort_passes = 0;
+    fsr::Arena arena;
+    // stage profiling: ring of (FSR_NUM_STAGES + 1) events per frame
+    std::vector<hipEvent_t> prof_events;
+    uint32_t prof_ring = 0;
+    uint32_t prof_frames = 0;  // frames recorded since the last read
+};
+
+namespace fsr {
+// Records stage boundary `k` (0..FSR_NUM_STAGES) of the current frame.
+void prof_mark(fsr_context* ctx, int k, hipStream_t s);
+}  // namespace fsr
+
+namespace fsr {
+
+fsr_status arena_get(fsr_context* ctx, ArenaSlot s, size_t bytes, void** out,
+                     uint64_t count_for_report);
+
+template <typename T>
+fsr_status arena_typed(fsr_context* ctx, ArenaSlot s, uint64_t n, T** out) {
+    void* p = nullptr;
+    size_t b = (size_t)(n ? n : 1) * sizeof(T);
+    FSR_TRY(arena_get(ctx, s, b, &p, n));
+    *out = (T*)p;
+    return FSR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers (defined in the .hip files)
+// ---------------------------------------------------------------------------
+
+// scan.hip ------------------------------------------------------------------
+enum ScanInput { SCAN_U32 = 0, SCAN_POPC = 1, SCAN_CHUNKS64 = 2, SCAN_NONZERO = 3 };
+// Exclusive scan of f(in[i]) for i < n, where n = *d_count when d_count is not
+// null (bounded by n_max) and n_max otherwise.  Writes the total to *d_total
+// when d_total is not null.
+fsr_status scan_exclusive(fsr_context* ctx, ScanInput f, const uint32_t* in,
+                          uint32_t* out, uint64_t n_max, const uint32_t* d_count,
+                          uint32_t* d_total, hipStream_t s);
+fsr_status fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
+
+// sort.hip ------------------------------------------------------------------
+struct HashOut {
+    float4* sorted;        // A_SORTED
+    uint32_t* keys;        // sorted cell ids
+    uint32_t* perm;        // sorted -> original index
+    fsr_index_range* ranges;
+    float* occupancy;
+};
+fsr_status spatial_hash(fsr_context* ctx, float* pos, float* dens, uint32_t n,
+                        const fsr_grid_info& sp, fsr_frame_stats* st,
+                        HashOut* out, hipStream_t s);
+
+// surface.hip ---------------------------------------------------------------
+struct SurfaceArgs {
+    const float4* sorted;
+    const uint32_t* keys;
+    const fsr_index_range* ranges;
+    const float* occupancy;
+    uint32_t n;
+    GridDev sp;
+    GridDev sc;
+    float h;
+    uint32_t* flags;        // u32[N] surface flags (sorted order)
+    uint32_t* sbits;        // band-shaped mask of surface scalar cells
+    uint32_t* exceptions;   // particles routed to the exact band path
+    Counters* counters;
+    fsr_frame_stats* st;
+    int all_exceptions;     // 1: every surface particle takes the exact path
+};
+fsr_status surface_occupancy(fsr_context* ctx, const SurfaceArgs& a, uint8_t* surf_cell,
+                             uint32_t* involve, hipStream_t s);
+fsr_status surface_colorfield(fsr_context* ctx, const SurfaceArgs& a,
+                              float mass, hipStream_t s);
+
+// band.hip ------------------------------------------------------------------
+struct BandArgs {
+    const float4* sorted;
+    const uint32_t* exceptions;
+    uint32_t max_exceptions;
+    const uint32_t* sbits;
+    uint32_t* tmp;
+    uint32_t* bits;
+    GridDev sc;
+    float h;
+    Counters* counters;
+    fsr_frame_stats* st;
+};
+fsr_status band_build(fsr_context* ctx, const BandArgs& a, hipStream_t s);
+// Returns 1 when every vertex of the reference's band box lies within 1.5 h of
+// the particle, so the band equals a box dilation of the surface cells.
+int band_dilation_exact(const fsr_grid_info& sc, float h);
+
+struct BandCompactOut {
+    uint32_t* band_index;   // u32[B]
+    uint32_t* cell_off;     // per virtual cell offset into cell_band
+    uint32_t* cell_cnt;
+    uint32_t* chunk_off;
+    uint32_t* cell_band;    // u32[B] cell-major
+    uint32_t* chunk_cell;   // chunk -> virtual cell
+    uint32_t vcx, vcy, vcz; // virtual cell grid (hash grid padded by `radius`)
+    int radius;             // 1 (3^3 blocks) or 2 (5^3 blocks)
+    uint64_t max_chunks;
+};
+fsr_status band_compact(fsr_context* ctx, const uint32_t* bits, const GridDev& sc,
+                        const GridDev& sp, int radius, Counters* counters,
+                        fsr_frame_stats* st, BandCompactOut* out, hipStream_t s);
+
+// field_zb05.hip ------------------------------------------------------------
+struct FieldArgs {
+    const float4* sorted;
+    const fsr_index_range* ranges;
+    const BandCompactOut* bc;
+    GridDev sp;
+    GridDev sc;
+    float radius_sq;       // R^2
+    float particle_radius;
+    float* phi;
+    Counters* counters;
+};
+fsr_status field_zb05(fsr_context* ctx, const FieldArgs& a, hipStream_t s);
+fsr_status fill_f32(float* p, float value, uint64_t n, hipStream_t s);
+
+// mc.hip --------------------------------------------------------------------
+struct McArgs {
+    const uint32_t* band_index;
+    const uint32_t* bits;
+    const float* phi;
+    GridDev sc;
+    float iso;
+    int cpu_flavor;        // 1: the reference's host MC semantics (Akinci12)
+    uint32_t* nverts;
+    uint32_t* valid;
+    uint32_t* valid_off;
+    uint32_t* vert_off;
+    uint32_t* valid_idx;
+    float* tri_pos;
+    float* tri_nrm;
+    uint32_t capacity;
+    uint64_t max_band;
+    fsr_frame_stats* st;
+};
+fsr_status mc_run(fsr_context* ctx, const McArgs& a, hipStream_t s);
+
+int persistent_blocks(fsr_context* ctx, int threads, int per_cu);
+
+}  // namespace fsr

@@ -1,0 +1,215 @@
+// Marching cubes over the band: classify, compact, emit a triangle soup.
+//
+//   k_mc_classify  detectValidSurfaceCubes (ReconstructionCUDA.cu:328-364) and
+//                  detectionOfValidSurfaceCubesForAkinci (:974-1008): the cube
+//                  whose corner 0 is band vertex b is skipped on the last grid
+//                  layer; cube index bit i is set iff phi(corner i) <= iso
+//                  (ReconstructionCUDA.cuh:534-544); the cube is valid iff all 8
+//                  corners are band vertices (isAllSfVertex, .cuh:602-609) and its
+//                  case emits vertices.
+//   scans          valid flags and vertex counts (the reference's two thrust scans,
+//                  GPU/ReconstructorGPUOurs.cpp:195-235), totals kept on the device.
+//   k_mc_compact   compactValidSurfaceCubes (ReconstructionCUDA.cu:367-380).
+//   k_mc_emit      generateTriangles (ReconstructionCUDA.cu:383-463): per crossed
+//                  edge t = (iso - phi0)/(phi1 - phi0) (0.5 if |delta| < 1e-7),
+//                  position lerp of the corner positions, normal lerp of the
+//                  normalised central-difference corner normals (out-of-grid
+//                  samples read 0), negated when iso < 0; triangles written at the
+//                  cube's scanned offset, in table order.
+// The `cpu` flavour reproduces the reference's host marching cubes used by the
+// Akinci12 method (GPU/MarchingCubesCPU.cpp:3-89, MarchingCubesCPU.h:142-181):
+// positions min + i*cell without fusion, normals normalised only when their
+// length exceeds 1e-7, no negation.  (Its out-of-range neighbour read is
+// undefined in the reference; here it reads 0 like the GPU path.)
+#include "common.hpp"
+#include "mc_tables.h"
+
+namespace fsr {
+
+namespace {
+
+__constant__ uint64_t cMcTri[256] = {FSR_MC_TRI_PACKED_VALUES};
+
+__device__ __forceinline__ uint32_t case_vertices(uint32_t flag) {
+    const uint64_t x = cMcTri[flag];
+    // nibbles equal to 0xF mark unused entries
+    const uint64_t f = x & (x >> 1) & (x >> 2) & (x >> 3) & 0x1111111111111111ull;
+    return 16u - (uint32_t)__popcll(f);
+}
+
+__device__ __forceinline__ bool band_bit(const uint32_t* __restrict__ bits, uint32_t wx,
+                                         uint32_t ry, uint32_t x, uint32_t y, uint32_t z) {
+    return (bits[((uint64_t)z * ry + y) * wx + (x >> 5)] >> (x & 31u)) & 1u;
+}
+
+__device__ __forceinline__ uint32_t cube_flag(const float* __restrict__ phi, uint64_t v,
+                                              uint32_t rx, uint64_t rxy, float iso) {
+    uint32_t f = 0;
+    f |= (phi[v] <= iso) ? 1u : 0u;
+    f |= (phi[v + rxy] <= iso) ? 2u : 0u;
+    f |= (phi[v + rxy + rx] <= iso) ? 4u : 0u;
+    f |= (phi[v + rx] <= iso) ? 8u : 0u;
+    f |= (phi[v + 1] <= iso) ? 16u : 0u;
+    f |= (phi[v + 1 + rxy] <= iso) ? 32u : 0u;
+    f |= (phi[v + 1 + rxy + rx] <= iso) ? 64u : 0u;
+    f |= (phi[v + 1 + rx] <= iso) ? 128u : 0u;
+    return f;
+}
+
+__global__ __launch_bounds__(256) void k_mc_classify(McArgs a) {
+    const uint32_t B = a.st->num_band_vertices;
+    const uint32_t rx = a.sc.rx, ry = a.sc.ry, rz = a.sc.rz;
+    const uint64_t rxy = (uint64_t)rx * ry;
+    const uint32_t wx = row_words(rx);
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < B; b += gridDim.x * blockDim.x) {
+        const uint32_t vid = a.band_index[b];
+        const uint32_t x = vid % rx, y = (vid / rx) % ry, z = (uint32_t)(vid / rxy);
+        uint32_t nv = 0;
+        if (x < rx - 1 && y < ry - 1 && z < rz - 1) {
+            const uint32_t f = cube_flag(a.phi, vid, rx, rxy, a.iso);
+            bool all = true;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const uint32_t dx = (c >= 4) ? 1u : 0u;
+                const uint32_t dy = (c == 2 || c == 3 || c == 6 || c == 7) ? 1u : 0u;
+                const uint32_t dz = (c == 1 || c == 2 || c == 5 || c == 6) ? 1u : 0u;
+                all = all && band_bit(a.bits, wx, ry, x + dx, y + dy, z + dz);
+            }
+            nv = all ? case_vertices(f) : 0u;
+        }
+        a.nverts[b] = nv;
+        a.valid[b] = nv ? 1u : 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mc_compact(McArgs a) {
+    const uint32_t B = a.st->num_band_vertices;
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < B; b += gridDim.x * blockDim.x)
+        if (a.valid[b]) a.valid_idx[a.valid_off[b]] = b;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        a.st->overflow = a.st->num_mesh_vertices > a.capacity ? 1u : 0u;
+}
+
+__device__ __forceinline__ float sample(const float* __restrict__ phi, int x, int y, int z,
+                                        uint32_t rx, uint32_t ry, uint32_t rz) {
+    if (x < 0 || y < 0 || z < 0 || x >= (int)rx || y >= (int)ry || z >= (int)rz) return 0.0f;
+    return phi[((uint64_t)z * ry + y) * rx + x];
+}
+
+template <bool kCpu>
+__global__ __launch_bounds__(256) void k_mc_emit(McArgs a) {
+    const uint32_t Q = a.st->num_valid_cubes;
+    const uint32_t rx = a.sc.rx, ry = a.sc.ry, rz = a.sc.rz;
+    const uint64_t rxy = (uint64_t)rx * ry;
+    const float sign = (!kCpu && a.iso < 0.0f) ? -1.0f : 1.0f;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < Q; q += gridDim.x * blockDim.x) {
+        const uint32_t b = a.valid_idx[q];
+        const uint32_t vid = a.band_index[b];
+        const uint32_t nv = a.nverts[b];
+        const uint32_t off = a.vert_off[b];
+        if ((uint64_t)off + nv > a.capacity) continue;
+        const int x = (int)(vid % rx), y = (int)((vid / rx) % ry), z = (int)(vid / rxy);
+        float val[8], cpx[8], cpy[8], cpz[8], cnx[8], cny[8], cnz[8];
+        uint32_t f = 0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int dx = (c >= 4) ? 1 : 0;
+            const int dy = (c == 2 || c == 3 || c == 6 || c == 7) ? 1 : 0;
+            const int dz = (c == 1 || c == 2 || c == 5 || c == 6) ? 1 : 0;
+            const int X = x + dx, Y = y + dy, Z = z + dz;
+            val[c] = a.phi[((uint64_t)Z * ry + Y) * rx + X];
+            f |= (val[c] <= a.iso) ? (1u << c) : 0u;
+            if (kCpu) {
+                cpx[c] = a.sc.minx + (float)X * a.sc.cell;
+                cpy[c] = a.sc.miny + (float)Y * a.sc.cell;
+                cpz[c] = a.sc.minz + (float)Z * a.sc.cell;
+            } else {
+                cpx[c] = vertex_coord((uint32_t)X, a.sc.minx, a.sc.cell);
+                cpy[c] = vertex_coord((uint32_t)Y, a.sc.miny, a.sc.cell);
+                cpz[c] = vertex_coord((uint32_t)Z, a.sc.minz, a.sc.cell);
+            }
+            float nx = sample(a.phi, X - 1, Y, Z, rx, ry, rz) - sample(a.phi, X + 1, Y, Z, rx, ry, rz);
+            float ny = sample(a.phi, X, Y - 1, Z, rx, ry, rz) - sample(a.phi, X, Y + 1, Z, rx, ry, rz);
+            float nz = sample(a.phi, X, Y, Z - 1, rx, ry, rz) - sample(a.phi, X, Y, Z + 1, rx, ry, rz);
+            if (kCpu) {
+                const float len = __builtin_sqrtf(nx * nx + ny * ny + nz * nz);
+                if (len > 1.0e-7f) {
+                    const float fac = 1.0f / len;
+                    nx *= fac; ny *= fac; nz *= fac;
+                }
+            } else {
+                const float rs = __builtin_amdgcn_rsqf(
+                    __builtin_fmaf(nz, nz, __builtin_fmaf(ny, ny, nx * nx)));
+                nx *= rs; ny *= rs; nz *= rs;
+            }
+            cnx[c] = nx; cny[c] = ny; cnz[c] = nz;
+        }
+        const uint64_t tri = cMcTri[f];
+        for (uint32_t k = 0; k < nv; ++k) {
+            const uint32_t e = (uint32_t)(tri >> (4 * k)) & 0xFu;
+            // corners of edge e
+            const uint32_t s0 = (e < 8) ? (e & 3u) + (e & 4u) : e - 8u;
+            const uint32_t s1 = (e < 8) ? (((e & 3u) + 1u) & 3u) + (e & 4u) : e - 4u;
+            const float v0 = val[s0], v1 = val[s1];
+            const float delta = v1 - v0;
+            float t;
+            if (kCpu) t = ((double)delta > -1.0e-7 && (double)delta < 1.0e-7) ? 0.5f : (a.iso - v0) / delta;
+            else t = (delta > -1.0e-7f && delta < 1.0e-7f) ? 0.5f : (a.iso - v0) / delta;
+            float px, py, pz, nx, ny, nz;
+            if (kCpu) {
+                px = cpx[s0] + (cpx[s1] - cpx[s0]) * t;
+                py = cpy[s0] + (cpy[s1] - cpy[s0]) * t;
+                pz = cpz[s0] + (cpz[s1] - cpz[s0]) * t;
+                nx = cnx[s0] + (cnx[s1] - cnx[s0]) * t;
+                ny = cny[s0] + (cny[s1] - cny[s0]) * t;
+                nz = cnz[s0] + (cnz[s1] - cnz[s0]) * t;
+                const float len = __builtin_sqrtf(nx * nx + ny * ny + nz * nz);
+                if (len > 1.0e-7f) {
+                    const float fac = 1.0f / len;
+                    nx *= fac; ny *= fac; nz *= fac;
+                }
+            } else {
+                px = __builtin_fmaf(t, cpx[s1] - cpx[s0], cpx[s0]);
+                py = __builtin_fmaf(t, cpy[s1] - cpy[s0], cpy[s0]);
+                pz = __builtin_fmaf(t, cpz[s1] - cpz[s0], cpz[s0]);
+                nx = __builtin_fmaf(t, cnx[s1] - cnx[s0], cnx[s0]);
+                ny = __builtin_fmaf(t, cny[s1] - cny[s0], cny[s0]);
+                nz = __builtin_fmaf(t, cnz[s1] - cnz[s0], cnz[s0]);
+                const float rs = __builtin_amdgcn_rsqf(
+                    __builtin_fmaf(nz, nz, __builtin_fmaf(ny, ny, nx * nx)));
+                nx = sign * (nx * rs); ny = sign * (ny * rs); nz = sign * (nz * rs);
+            }
+            const uint64_t o = 3ull * (off + k);
+            a.tri_pos[o + 0] = px;
+            a.tri_pos[o + 1] = py;
+            a.tri_pos[o + 2] = pz;
+            if (a.tri_nrm) {
+                a.tri_nrm[o + 0] = nx;
+                a.tri_nrm[o + 1] = ny;
+                a.tri_nrm[o + 2] = nz;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+fsr_status mc_run(fsr_context* ctx, const McArgs& a, hipStream_t s) {
+    const int blocks = persistent_blocks(ctx, 256, 8);
+    hipLaunchKernelGGL(k_mc_classify, dim3(blocks), dim3(256), 0, s, a);
+    FSR_LAUNCH_CHECK("k_mc_classify");
+    FSR_TRY(scan_exclusive(ctx, SCAN_U32, a.valid, a.valid_off, a.max_band,
+                           &a.st->num_band_vertices, &a.st->num_valid_cubes, s));
+    FSR_TRY(scan_exclusive(ctx, SCAN_U32, a.nverts, a.vert_off, a.max_band,
+                           &a.st->num_band_vertices, &a.st->num_mesh_vertices, s));
+    hipLaunchKernelGGL(k_mc_compact, dim3(blocks), dim3(256), 0, s, a);
+    FSR_LAUNCH_CHECK("k_mc_compact");
+    if (a.cpu_flavor)
+        hipLaunchKernelGGL(k_mc_emit<true>, dim3(blocks), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_mc_emit<false>, dim3(blocks), dim3(256), 0, s, a);
+    FSR_LAUNCH_CHECK("k_mc_emit");
+    return FSR_OK;
+}
+
+}  // namespace fsr

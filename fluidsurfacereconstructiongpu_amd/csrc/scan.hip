@@ -1,0 +1,213 @@
+// Exclusive prefix sums over u32 arrays whose live length may sit in device
+// memory.  Replaces launchThrustExclusivePrefixSumScan
+// (ReconstructionCUDA.cu:1154-1172), which ran thrust::exclusive_scan and then
+// copied the total back to the host with two blocking copies; here the total
+// stays on the device, so later stages size themselves from it without a host
+// round trip.
+//
+// Three kernels on a fixed grid of G blocks (G <= 1024): each block reduces a
+// contiguous segment, one block scans the G partial sums, each block rescans
+// its segment in 1024-element tiles (uint4 loads, wave scans, one LDS exchange
+// per tile).  HBM traffic: read n, read n, write n words.
+#include "common.hpp"
+
+namespace fsr {
+
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanTile = kScanThreads * 4;
+constexpr uint32_t kMaxScanBlocks = 1024;
+
+template <int F>
+__device__ __forceinline__ uint32_t xform(uint32_t v) {
+    if (F == SCAN_POPC) return (uint32_t)__popc(v);
+    if (F == SCAN_CHUNKS64) return (v + 63u) >> 6;
+    if (F == SCAN_NONZERO) return v ? 1u : 0u;
+    return v;
+}
+
+__device__ __forceinline__ uint64_t live_length(uint64_t n_max, const uint32_t* d_count) {
+    if (!d_count) return n_max;
+    uint64_t c = *d_count;
+    return c < n_max ? c : n_max;
+}
+
+__device__ __forceinline__ uint64_t segment_len(uint64_t n, uint32_t g) {
+    uint64_t seg = (n + g - 1) / g;
+    return (seg + kScanTile - 1) / kScanTile * kScanTile;
+}
+
+// Block-wide exclusive scan of one value per thread; returns the block total.
+__device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t* lds_wave,
+                                                    uint32_t* out_excl) {
+    const uint32_t lane = __lane_id();
+    const uint32_t wave = threadIdx.x >> 6;
+    uint32_t inc = wave_inclusive_sum(v);
+    if (lane == 63) lds_wave[wave] = inc;
+    __syncthreads();
+    uint32_t wave_off = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / 64; ++w) {
+        uint32_t t = lds_wave[w];
+        if ((uint32_t)w < wave) wave_off += t;
+        total += t;
+    }
+    __syncthreads();
+    *out_excl = wave_off + inc - v;
+    return total;
+}
+
+template <int F>
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(
+    const uint32_t* __restrict__ in, uint64_t n_max, const uint32_t* d_count,
+    uint32_t* __restrict__ partials) {
+    __shared__ uint32_t lds[kScanThreads / 64];
+    const uint64_t n = live_length(n_max, d_count);
+    const uint64_t seg = segment_len(n, gridDim.x);
+    const uint64_t lo = (uint64_t)blockIdx.x * seg;
+    const uint64_t hi = lo + seg < n ? lo + seg : n;
+    uint32_t sum = 0;
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += kScanThreads) sum += xform<F>(in[i]);
+    uint32_t excl;
+    uint32_t total = block_exclusive(sum, lds, &excl);
+    if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_partials(uint32_t* partials, uint32_t g,
+                                                        uint32_t* d_total) {
+    __shared__ uint32_t lds[16];
+    const uint32_t t = threadIdx.x;
+    const uint32_t lane = __lane_id();
+    const uint32_t wave = t >> 6;
+    uint32_t v = t < g ? partials[t] : 0u;
+    uint32_t inc = wave_inclusive_sum(v);
+    if (lane == 63) lds[wave] = inc;
+    __syncthreads();
+    uint32_t off = 0, total = 0;
+    for (uint32_t w = 0; w < 16; ++w) {
+        uint32_t s = lds[w];
+        if (w < wave) off += s;
+        total += s;
+    }
+    if (t < g) partials[t] = off + inc - v;
+    if (t == 0 && d_total) *d_total = total;
+}
+
+template <int F>
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(
+    const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint64_t n_max,
+    const uint32_t* d_count, const uint32_t* __restrict__ partials) {
+    __shared__ uint32_t lds[kScanThreads / 64];
+    const uint64_t n = live_length(n_max, d_count);
+    const uint64_t seg = segment_len(n, gridDim.x);
+    const uint64_t lo = (uint64_t)blockIdx.x * seg;
+    const uint64_t hi = lo + seg < n ? lo + seg : n;
+    uint32_t running = partials[blockIdx.x];
+    for (uint64_t base = lo; base < hi; base += kScanTile) {
+        const uint64_t i0 = base + 4ull * threadIdx.x;
+        uint32_t v[4];
+        if (base + kScanTile <= hi) {
+            uint4 q = *reinterpret_cast<const uint4*>(in + i0);
+            v[0] = xform<F>(q.x); v[1] = xform<F>(q.y); v[2] = xform<F>(q.z); v[3] = xform<F>(q.w);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[k] = (i0 + k < hi) ? xform<F>(in[i0 + k]) : 0u;
+        }
+        uint32_t tsum = v[0] + v[1] + v[2] + v[3];
+        uint32_t excl;
+        uint32_t total = block_exclusive(tsum, lds, &excl);
+        uint32_t o0 = running + excl;
+        uint32_t r[4];
+        r[0] = o0; r[1] = o0 + v[0]; r[2] = r[1] + v[1]; r[3] = r[2] + v[2];
+        if (base + kScanTile <= hi) {
+            *reinterpret_cast<uint4*>(out + i0) = make_uint4(r[0], r[1], r[2], r[3]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (i0 + k < hi) out[i0 + k] = r[k];
+        }
+        running += total;
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_f32(float* p, float v, uint64_t n) {
+    // float4 stores for the bulk; the tail element by element
+    const uint64_t n4 = n / 4;
+    float4* p4 = reinterpret_cast<float4*>(p);
+    const float4 q = make_float4(v, v, v, v);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride)
+        p4[i] = q;
+    for (uint64_t i = 4 * n4 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        p[i] = v;
+}
+
+template <int F>
+fsr_status scan_impl(fsr_context* ctx, const uint32_t* in, uint32_t* out, uint64_t n_max,
+                     const uint32_t* d_count, uint32_t* d_total, hipStream_t s) {
+    uint64_t want = (n_max + 2047) / 2048;
+    uint32_t g = (uint32_t)(want < 1 ? 1 : (want > kMaxScanBlocks ? kMaxScanBlocks : want));
+    uint32_t* partials = nullptr;
+    FSR_TRY(arena_typed(ctx, A_SCAN_PARTIALS, kMaxScanBlocks, &partials));
+    hipLaunchKernelGGL(k_scan_reduce<F>, dim3(g), dim3(kScanThreads), 0, s, in, n_max, d_count,
+                       partials);
+    FSR_LAUNCH_CHECK("k_scan_reduce");
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, partials, g, d_total);
+    FSR_LAUNCH_CHECK("k_scan_partials");
+    if (out) {
+        hipLaunchKernelGGL(k_scan_down<F>, dim3(g), dim3(kScanThreads), 0, s, in, out, n_max,
+                           d_count, partials);
+        FSR_LAUNCH_CHECK("k_scan_down");
+    }
+    return FSR_OK;
+}
+
+}  // namespace
+
+fsr_status scan_exclusive(fsr_context* ctx, ScanInput f, const uint32_t* in, uint32_t* out,
+                          uint64_t n_max, const uint32_t* d_count, uint32_t* d_total,
+                          hipStream_t s) {
+    switch (f) {
+        case SCAN_U32: return scan_impl<SCAN_U32>(ctx, in, out, n_max, d_count, d_total, s);
+        case SCAN_POPC: return scan_impl<SCAN_POPC>(ctx, in, out, n_max, d_count, d_total, s);
+        case SCAN_CHUNKS64:
+            return scan_impl<SCAN_CHUNKS64>(ctx, in, out, n_max, d_count, d_total, s);
+        case SCAN_NONZERO:
+            return scan_impl<SCAN_NONZERO>(ctx, in, out, n_max, d_count, d_total, s);
+    }
+    set_error("scan_exclusive: unknown input transform");
+    return FSR_ERR_INVALID_ARG;
+}
+
+fsr_status fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s) {
+    if (n == 0) return FSR_OK;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)blocks), dim3(256), 0, s, p, value, n);
+    FSR_LAUNCH_CHECK("k_fill_u32");
+    return FSR_OK;
+}
+
+fsr_status fill_f32(float* p, float value, uint64_t n, hipStream_t s) {
+    if (n == 0) return FSR_OK;
+    if (((uintptr_t)p & 15u) != 0) {
+        uint32_t bits;
+        __builtin_memcpy(&bits, &value, 4);
+        return fill_u32(reinterpret_cast<uint32_t*>(p), bits, n, s);
+    }
+    uint64_t blocks = (n / 4 + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_fill_f32, dim3((uint32_t)blocks), dim3(256), 0, s, p, value, n);
+    FSR_LAUNCH_CHECK("k_fill_f32");
+    return FSR_OK;
+}
+
+}  // namespace fsr

@@ -1,0 +1,234 @@
+/*
+ * fsr.h -- C-ABI of the MI355X-native particle -> scalar field -> triangle mesh
+ * library (libfsr.so).
+ *
+ * This header is the drop-in boundary.  It replaces the `extern "C"` launcher
+ * layer of the reference (FluidSurfaceReconstruction/GPU/ReconstructionCUDA.cuh:17-248)
+ * and the per-frame stage order of the reference's reconstructor classes
+ * (GPU/ReconstructorGPUOursZB05.cpp:22-78, GPU/ReconstructorGPUAkinci12.cpp:34-84,
+ * GPU/ReconstructorGPUOursYu13.cpp:24-64).
+ *
+ * Conventions (differences from the reference are deliberate and listed here):
+ *   - every entry point returns fsr_status (0 = FSR_OK); nothing calls exit();
+ *   - every entry point that launches GPU work takes a `stream` (a hipStream_t
+ *     passed as void*, NULL = the null stream) and is asynchronous on it unless
+ *     its comment says "blocking";
+ *   - all buffers are device pointers owned by the caller unless noted; the
+ *     context owns only its scratch arena;
+ *   - POD layouts are binary compatible with the reference's SimParam, GridInfo,
+ *     IndexRange and MatrixValue (GPU/Defines.h:18-68), so reference-side structs
+ *     can be passed as they are;
+ *   - particles are AoS float3 (12 bytes), densities float, as in the reference.
+ *
+ * Compiled for gfx950 only.
+ */
+#ifndef FSR_H_
+#define FSR_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* POD types                                                                  */
+/* ------------------------------------------------------------------------- */
+
+typedef struct { float x, y, z; } fsr_float3;
+typedef struct { uint32_t x, y, z; } fsr_uint3;
+
+/* IndexRange (GPU/Defines.h:45): [start, end) of a cell's particles in sorted
+ * order; an empty cell holds start = end = 0xFFFFFFFF. */
+typedef struct { uint32_t start, end; } fsr_index_range;
+
+/* MatrixValue (GPU/Defines.h:54-59): row-major 3x3, 36 bytes. */
+typedef struct {
+    float a11, a12, a13;
+    float a21, a22, a23;
+    float a31, a32, a33;
+} fsr_matrix;
+
+/* SimParam (GPU/Defines.h:18-32), same field order and size (48 bytes). */
+typedef struct {
+    float particle_radius;        /* r_p */
+    float particle_mass;          /* m */
+    float smoothing_radius;       /* h, the kernel radius */
+    float smoothing_radius_sq;    /* h^2 */
+    float smoothing_radius_inv;   /* 1/h */
+    float lambda_for_smoothed;    /* Laplacian smoothing blend (Yu13) */
+    float anisotropic_radius;     /* covariance radius, 2h in the reference */
+    float iso_value;              /* iso-contour */
+    uint32_t sc_sp_grid_res_ratio;/* spatial cell / scalar cell (reference grid rule) */
+    uint32_t expand_extent;       /* unused on every path (the band extent is 3 cells) */
+    uint32_t min_num_neighbors;   /* Ne (Yu13) */
+    float spatial_cell_size_scale;/* hash cell = h * scale */
+} fsr_sim_param;
+
+/* GridInfo (GPU/Defines.h:62-68), 40 bytes. */
+typedef struct {
+    fsr_float3 min_pos;
+    fsr_float3 max_pos;
+    float cell_size;
+    fsr_uint3 resolution;
+} fsr_grid_info;
+
+/* Anisotropy constants of the Yu & Turk kernel that the reference hard-codes.
+ * Reference values: kr = 4 (singular values clamped to sigma_max / kr,
+ * ReconstructionCUDA.cu:796), kn = 1 (isolated particles get G = I / (kn h),
+ * ReconstructionCUDA.cu:772).  ks scales the covariance and cancels under the
+ * volume normalisation (ReconstructionCUDA.cu:816-822); it is accepted and
+ * ignored. */
+typedef struct {
+    float kr;
+    float ks;
+    float kn;
+    uint32_t reserved;
+} fsr_aniso_param;
+
+typedef enum {
+    FSR_OK = 0,
+    FSR_ERR_INVALID_ARG = 1,
+    FSR_ERR_HIP = 2,
+    FSR_ERR_CAPACITY = 3,
+    FSR_ERR_OUT_OF_MEMORY = 4,
+    FSR_ERR_NO_DEVICE = 5,
+    FSR_ERR_UNSUPPORTED = 6
+} fsr_status;
+
+typedef enum {
+    FSR_METHOD_ZB05 = 0,      /* occupancy surface cells + ZB05 field (ReconstructorGPUOursZB05) */
+    FSR_METHOD_AKINCI12 = 1,  /* colour-field surface particles + ZB05 field (ReconstructorGPUAkinci12) */
+    FSR_METHOD_YU13 = 2       /* anisotropic kernel (ReconstructorGPUOursYu13), "TC01" */
+} fsr_method;
+
+/* Counters of one frame, written by the GPU into a caller-provided device
+ * buffer (read it back after the stream is synchronised). */
+typedef struct {
+    uint32_t num_particles;
+    uint32_t num_out_of_grid;        /* particles whose hash cell is outside the spatial grid */
+    uint32_t num_surface_particles;
+    uint32_t num_involved_particles; /* Yu13: particles that carry a G matrix */
+    uint32_t num_band_vertices;      /* B: scalar vertices in the surface band */
+    uint32_t num_valid_cubes;        /* Q: cubes that emit triangles */
+    uint32_t num_mesh_vertices;      /* 3T: soup vertices required */
+    uint32_t overflow;               /* nonzero: num_mesh_vertices exceeded the capacity */
+    uint32_t band_exceptions;        /* surface particles handled by the exact band path */
+    uint32_t reserved[7];
+} fsr_frame_stats;
+
+/* ------------------------------------------------------------------------- */
+/* Context                                                                    */
+/* ------------------------------------------------------------------------- */
+
+typedef struct fsr_context fsr_context;
+
+/* Creates a context on HIP device `device`: uploads the marching-cubes case
+ * tables (replaces bindTextures, ReconstructionCUDA.cuh:17-22 /
+ * ReconstructionCUDA.cu:50-65) and creates an empty scratch arena. */
+fsr_status fsr_create(fsr_context** out_ctx, int device);
+fsr_status fsr_destroy(fsr_context* ctx);
+
+/* Thread-local text of the last error raised on this thread. */
+const char* fsr_last_error(void);
+const char* fsr_status_string(fsr_status s);
+/* Library version string. */
+const char* fsr_version(void);
+
+/* ------------------------------------------------------------------------- */
+/* Grid set-up                                                                */
+/* ------------------------------------------------------------------------- */
+
+/* Host helper: the reference's grid rule.  From the particle AABB, kernel
+ * radius h, cell scale and scalar:spatial ratio computes the padded spatial
+ * hashing grid and the scalar grid exactly as
+ * GPU/ReconstructorGPU.cpp:237-267 and :99-105 do (FP32 arithmetic, truncating
+ * conversion to uint). */
+fsr_status fsr_grid_from_aabb(fsr_float3 aabb_min, fsr_float3 aabb_max,
+                              float smoothing_radius, float cell_scale,
+                              uint32_t sc_sp_ratio,
+                              fsr_grid_info* out_spatial, fsr_grid_info* out_scalar);
+
+/* ------------------------------------------------------------------------- */
+/* Whole-frame entry point                                                    */
+/* ------------------------------------------------------------------------- */
+
+/* Runs one frame of `method` on the stream, with no host synchronisation:
+ *   spatial hashing (positions and densities are sorted in place, as
+ *   launchSpatialGridBuilding does, ReconstructionCUDA.cu:1461-1466) ->
+ *   surface particles -> band -> field -> marching cubes.
+ *
+ *   phi        [out] float[V], V = product of sc_grid->resolution; non-band
+ *                    vertices hold 0 (ZB05, Akinci12) or the byte pattern
+ *                    0x64646464 (Yu13), as the reference leaves them.
+ *   tri_pos    [out] float3[tri_capacity] triangle-soup positions;
+ *   tri_nrm    [out] float3[tri_capacity] soup normals (may be NULL);
+ *   stats      [out] device pointer to one fsr_frame_stats.
+ * When 3T > tri_capacity nothing beyond the capacity is written, stats->overflow
+ * is set and stats->num_mesh_vertices holds the size needed.
+ * aniso may be NULL (reference constants). */
+fsr_status fsr_reconstruct_frame(fsr_context* ctx, fsr_method method,
+                                 const fsr_sim_param* sim, const fsr_aniso_param* aniso,
+                                 const fsr_grid_info* sp_grid, const fsr_grid_info* sc_grid,
+                                 float* pos, float* dens, uint32_t num_particles,
+                                 float* phi, float* tri_pos, float* tri_nrm,
+                                 uint32_t tri_capacity, fsr_frame_stats* stats,
+                                 void* stream);
+
+/* Intermediate buffers of the last frame run on `ctx`, for inspection and
+ * parity checks.  The pointer stays valid until the next call that grows the
+ * arena.  `count` is the element capacity of the buffer (the live length is
+ * in fsr_frame_stats). */
+typedef enum {
+    FSR_BUF_CELL_KEYS = 0,      /* u32[N]   sorted hash cell id of every particle */
+    FSR_BUF_PERMUTATION = 1,    /* u32[N]   original index of every sorted particle */
+    FSR_BUF_CELL_RANGES = 2,    /* fsr_index_range[C] */
+    FSR_BUF_OCCUPANCY = 3,      /* float[C] 1.0 for occupied cells (the reference's flag grid) */
+    FSR_BUF_SURFACE_FLAGS = 4,  /* u32[N]   1 for surface particles (sorted order) */
+    FSR_BUF_BAND_BITS = 5,      /* u32[ceil(resx/32) * resy * resz] band bit mask */
+    FSR_BUF_BAND_INDEX = 6,     /* u32[B]   band vertex linear ids, ascending */
+    FSR_BUF_CUBE_NVERTS = 7,    /* u32[B]   soup vertices emitted by the cube at band entry b */
+    FSR_BUF_VALID_CUBES = 8,    /* u32[Q]   band entries of valid cubes, ascending */
+    FSR_BUF_INVOLVE_COUNTS = 9, /* u32[C]   Yu13 involved-particle count per cell */
+    FSR_BUF_MEAN_POS = 10,      /* float3[N] Yu13 weighted mean positions */
+    FSR_BUF_SMOOTH_POS = 11,    /* float3[N] Yu13 smoothed positions */
+    FSR_BUF_G_MATRICES = 12,    /* fsr_matrix[I] Yu13 G of the involved particles */
+    FSR_BUF_SINGULAR = 13,      /* float3[I] Yu13 covariance singular values (before clamping) */
+    FSR_BUF_INVOLVED_INDEX = 14 /* u32[I]   Yu13 involved particle ids (sorted order) */
+} fsr_buffer;
+
+fsr_status fsr_frame_buffer(fsr_context* ctx, fsr_buffer which,
+                            void** dev_ptr, uint64_t* count);
+
+/* ------------------------------------------------------------------------- */
+/* Stage timing (the reference's optional per-stage timers,                   */
+/* GPU/ReconstructorGPUOursZB05.cpp:26-76, as device events)                  */
+/* ------------------------------------------------------------------------- */
+
+/* Stages of one frame, in order. */
+typedef enum {
+    FSR_STAGE_HASH = 0,     /* cell ids, radix sort, gather, cell ranges */
+    FSR_STAGE_SURFACE = 1,  /* surface particles and band seeds */
+    FSR_STAGE_BAND = 2,     /* band dilation, compaction, cell-major work list */
+    FSR_STAGE_FILL = 3,     /* fill of the dense field with the non-band value */
+    FSR_STAGE_FIELD = 4,    /* field kernel(s) on the band */
+    FSR_STAGE_MC = 5,       /* marching cubes classify, scans, compaction, emit */
+    FSR_NUM_STAGES = 6
+} fsr_stage;
+
+/* ring_frames > 0: every following frame records HIP events at its stage
+ * boundaries on its stream, into a ring of ring_frames slots; 0 disables. */
+fsr_status fsr_set_profiling(fsr_context* ctx, uint32_t ring_frames);
+
+/* Blocking.  Writes the stage durations (ms) of the frames recorded since the
+ * last call, oldest first, at most max_frames: ms[f * FSR_NUM_STAGES + s].
+ * *num_frames receives the number written; the record is then cleared. */
+fsr_status fsr_stage_times(fsr_context* ctx, float* ms, uint32_t max_frames,
+                           uint32_t* num_frames);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FSR_H_ */

@@ -1,0 +1,208 @@
+"""Python binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the timed CPU baseline,
+never as part of the product path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(_HERE, "liboracle.so")
+REF_MC = os.path.join(_HERE, "_ref", "libref_mctables.so")
+
+u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+class OGrid(ctypes.Structure):
+    _fields_ = [("minx", ctypes.c_float), ("miny", ctypes.c_float), ("minz", ctypes.c_float),
+                ("cell", ctypes.c_float), ("rx", ctypes.c_uint32), ("ry", ctypes.c_uint32),
+                ("rz", ctypes.c_uint32)]
+
+
+def ogrid(g) -> OGrid:
+    return OGrid(float(g.min[0]), float(g.min[1]), float(g.min[2]), float(g.cell),
+                 int(g.res[0]), int(g.res[1]), int(g.res[2]))
+
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-B", "-C", _HERE, "all"], check=True, stdout=subprocess.DEVNULL)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        build()
+    lib = ctypes.CDLL(LIB)
+    lib.orc_num_threads.restype = ctypes.c_int
+    lib.orc_set_threads.argtypes = [ctypes.c_int]
+    lib.orc_hash_sort.argtypes = [f32p, f32p, ctypes.c_uint32, OGrid, u32p, u32p, f32p, f32p,
+                                  u32p, f32p]
+    lib.orc_surface_occupancy.argtypes = [f32p, u32p, OGrid, u32p, ctypes.c_uint32, u8p, u32p,
+                                          ctypes.c_void_p]
+    lib.orc_surface_colorfield.argtypes = [f32p, f32p, ctypes.c_uint32, u32p, OGrid,
+                                           ctypes.c_float, ctypes.c_float, u32p, f64p, u32p]
+    lib.orc_band.argtypes = [f32p, u32p, ctypes.c_uint32, OGrid, ctypes.c_float, u8p]
+    lib.orc_compact.argtypes = [u8p, ctypes.c_uint64, u32p]
+    lib.orc_compact.restype = ctypes.c_uint32
+    lib.orc_field_zb05.argtypes = [u32p, ctypes.c_uint32, f32p, u32p, OGrid, OGrid,
+                                   ctypes.c_float, ctypes.c_float, f64p, f64p]
+    lib.orc_mc.argtypes = [u32p, ctypes.c_uint32, u8p, f64p, OGrid, ctypes.c_float, ctypes.c_int,
+                           u32p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
+    lib.orc_mc.restype = ctypes.c_uint64
+    lib.orc_frame.argtypes = [ctypes.c_int, f32p, f32p, ctypes.c_uint32, OGrid, OGrid,
+                              ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                              ctypes.c_float, ctypes.c_void_p, u64p]
+    lib.orc_mc_tri_packed.argtypes = [ctypes.c_uint32]
+    lib.orc_mc_tri_packed.restype = ctypes.c_uint64
+    _lib = lib
+    return lib
+
+
+def threads() -> int:
+    return int(load().orc_num_threads())
+
+
+def set_threads(n: int) -> None:
+    load().orc_set_threads(int(n))
+
+
+def hash_sort(pos, dens, sp):
+    lib = load()
+    n = pos.shape[0]
+    C = sp.count
+    keys = np.empty(n, np.uint32)
+    perm = np.empty(n, np.uint32)
+    pos_o = np.empty((n, 3), np.float32)
+    dens_o = np.empty(n, np.float32)
+    ranges = np.empty(2 * C, np.uint32)
+    occ = np.empty(C, np.float32)
+    lib.orc_hash_sort(np.ascontiguousarray(pos, np.float32).ravel(),
+                      np.ascontiguousarray(dens, np.float32), n, ogrid(sp), keys, perm,
+                      pos_o.ravel(), dens_o, ranges, occ)
+    return dict(keys=keys, perm=perm, pos=pos_o, dens=dens_o, ranges=ranges.reshape(C, 2),
+                occ=occ)
+
+
+def surface_occupancy(occ, ranges, sp, keys, involve=False):
+    lib = load()
+    C = sp.count
+    n = keys.shape[0]
+    surf = np.empty(C, np.uint8)
+    flags = np.empty(n, np.uint32)
+    inv = np.empty(C, np.uint32) if involve else None
+    lib.orc_surface_occupancy(occ, np.ascontiguousarray(ranges).ravel(), ogrid(sp), keys, n, surf,
+                              flags, inv.ctypes.data if inv is not None else None)
+    return surf, flags, inv
+
+
+def surface_colorfield(pos, dens, ranges, sp, h, mass):
+    lib = load()
+    n = pos.shape[0]
+    flags = np.empty(n, np.uint32)
+    g = np.empty(n, np.float64)
+    cnt = np.empty(n, np.uint32)
+    lib.orc_surface_colorfield(np.ascontiguousarray(pos, np.float32).ravel(),
+                               np.ascontiguousarray(dens, np.float32), n,
+                               np.ascontiguousarray(ranges).ravel(), ogrid(sp), h, mass, flags,
+                               g, cnt)
+    return flags, g, cnt
+
+
+def band(pos, flags, sc, h):
+    lib = load()
+    out = np.empty(sc.count, np.uint8)
+    lib.orc_band(np.ascontiguousarray(pos, np.float32).ravel(),
+                 np.ascontiguousarray(flags, np.uint32), pos.shape[0], ogrid(sc), h, out)
+    return out
+
+
+def compact(band_u8):
+    lib = load()
+    idx = np.empty(band_u8.shape[0], np.uint32)
+    b = lib.orc_compact(band_u8, band_u8.shape[0], idx)
+    return idx[:b].copy()
+
+
+def field_zb05(index, pos, ranges, sp, sc, radius_sq, particle_radius, fill=0.0):
+    lib = load()
+    phi = np.full(sc.count, fill, np.float64)
+    wsum = np.empty(index.shape[0], np.float64)
+    lib.orc_field_zb05(np.ascontiguousarray(index, np.uint32), index.shape[0],
+                       np.ascontiguousarray(pos, np.float32).ravel(),
+                       np.ascontiguousarray(ranges).ravel(), ogrid(sp), ogrid(sc), radius_sq,
+                       particle_radius, phi, wsum)
+    return phi, wsum
+
+
+def mc(index, band_u8, phi, sc, iso, cpu=False, with_mesh=True):
+    lib = load()
+    index = np.ascontiguousarray(index, np.uint32)
+    B = index.shape[0]
+    nverts = np.empty(B, np.uint32)
+    phi = np.ascontiguousarray(phi, np.float64)
+    total = lib.orc_mc(index, B, band_u8, phi, ogrid(sc), iso, int(cpu), nverts, None, None, 0)
+    pos = nrm = None
+    if with_mesh:
+        pos = np.empty((total, 3), np.float64)
+        nrm = np.empty((total, 3), np.float64)
+        lib.orc_mc(index, B, band_u8, phi, ogrid(sc), iso, int(cpu), nverts, pos.ctypes.data,
+                   nrm.ctypes.data, total)
+    return nverts, pos, nrm
+
+
+def frame(method, fr, want_phi=False):
+    """Whole frame on copies of the frame's arrays; returns (counts, phi|None)."""
+    lib = load()
+    pos = np.ascontiguousarray(fr.pos, np.float32).copy()
+    dens = np.ascontiguousarray(fr.dens, np.float32).copy()
+    phi = np.empty(fr.sc.count, np.float32) if want_phi else None
+    counts = np.zeros(6, np.uint64)
+    h = np.float32(fr.h)
+    lib.orc_frame({"zb05": 0, "akinci12": 1}[method], pos.ravel(), dens, pos.shape[0],
+                  ogrid(fr.sp), ogrid(fr.sc), float(h), float(h * h),
+                  float(np.float32(fr.particle_radius)), float(np.float32(fr.mass)),
+                  float(np.float32(fr.iso)), phi.ctypes.data if want_phi else None, counts)
+    keys = ["surface_particles", "band_vertices", "valid_cubes", "mesh_vertices", "out_of_grid"]
+    return {k: int(counts[i]) for i, k in enumerate(keys)}, phi
+
+
+def mc_tri_packed():
+    lib = load()
+    return np.array([lib.orc_mc_tri_packed(c) for c in range(256)], dtype=np.uint64)
+
+
+def reference_mc_tables():
+    """The reference's own MC tables (oracle/_ref, built from the reference's
+    GPU/MarchingCubesHelper.cpp), or None when that build is absent."""
+    if not os.path.exists(REF_MC):
+        return None
+    lib = ctypes.CDLL(REF_MC)
+    for f in ("ref_mc_edge_flags", "ref_mc_tri_table", "ref_mc_num_vertices",
+              "ref_mc_edge_vertices", "ref_mc_corner_offsets"):
+        getattr(lib, f).restype = ctypes.c_void_p
+
+    def arr(addr, ctype, n):
+        return np.ctypeslib.as_array((ctype * n).from_address(addr)).copy()
+
+    return dict(
+        edge_flags=arr(lib.ref_mc_edge_flags(), ctypes.c_uint32, 256),
+        tri=arr(lib.ref_mc_tri_table(), ctypes.c_int32, 256 * 16).reshape(256, 16),
+        num_vertices=arr(lib.ref_mc_num_vertices(), ctypes.c_uint32, 256),
+        edge_vertices=arr(lib.ref_mc_edge_vertices(), ctypes.c_uint32, 24).reshape(12, 2),
+        corner_offsets=arr(lib.ref_mc_corner_offsets(), ctypes.c_int32, 24).reshape(8, 3),
+    )

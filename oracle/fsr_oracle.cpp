@@ -1,0 +1,436 @@
+// CPU oracle of the particle -> field -> mesh path.  TEST INFRASTRUCTURE ONLY
+// (the checker of the parity tests and the CPU baseline of bench.py).
+//
+// Every function restates one reference kernel; the citation is given at the
+// function.  Reference files are under FluidSurfaceReconstruction/GPU/.
+// Build: g++ -O2 -fopenmp -ffp-contract=off (oracle/Makefile), so FP32
+// expressions are evaluated exactly as written, one IEEE operation at a time,
+// and fused multiply-adds appear only where fmaf() is written.
+#include "fsr_oracle.h"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../fluidsurfacereconstructiongpu_amd/csrc/mc_tables.h"
+
+namespace {
+
+const uint64_t kTri[256] = {FSR_MC_TRI_PACKED_VALUES};
+
+inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// int3((p - min) / cell) of ReconstructionCUDA.cuh:354-358 in FP32, with the
+// quotient clamped first so that the truncation stays defined for far-away or
+// NaN positions (NaN -> 0).
+inline float cell_q(float p, float mn, float cell) { return (p - mn) / cell; }
+inline int trunc_q(float q, int res) {
+    if (!(q == q)) return 0;
+    const float lo = -4.0f, hi = (float)res + 4.0f;
+    q = q < lo ? lo : (q > hi ? hi : q);
+    return (int)q;
+}
+
+// getVertexPos (ReconstructionCUDA.cuh:366-370) as one FP32 fused multiply-add.
+inline float vpos(uint32_t i, float mn, float cell) { return fmaf((float)i, cell, mn); }
+
+inline uint32_t case_nverts(uint32_t flag) {
+    uint32_t n = 0;
+    for (int k = 0; k < 16; ++k) {
+        if (((kTri[flag] >> (4 * k)) & 0xF) == 0xF) break;
+        ++n;
+    }
+    return n;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_num_threads(void) { return omp_get_max_threads(); }
+void orc_set_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+}
+
+uint64_t orc_mc_tri_packed(uint32_t c) { return c < 256 ? kTri[c] : ~0ull; }
+
+// calcParticlesHashKernel (ReconstructionCUDA.cu:1379-1396), thrust::sort_by_key
+// (:1461-1466, a stable sort by key) and findCellRangeKernel (:1399-1435).
+int orc_hash_sort(const float* pos, const float* dens, uint32_t n, orc_grid sp, uint32_t* keys,
+                  uint32_t* perm, float* pos_out, float* dens_out, uint32_t* ranges, float* occ) {
+    const uint64_t C = (uint64_t)sp.rx * sp.ry * sp.rz;
+    std::vector<uint32_t> key(n);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        const float qx = cell_q(pos[3 * i + 0], sp.minx, sp.cell);
+        const float qy = cell_q(pos[3 * i + 1], sp.miny, sp.cell);
+        const float qz = cell_q(pos[3 * i + 2], sp.minz, sp.cell);
+        const bool ok = qx > -1.0f && qx < (float)sp.rx && qy > -1.0f && qy < (float)sp.ry &&
+                        qz > -1.0f && qz < (float)sp.rz;
+        key[i] = ok ? (uint32_t)(int)qz * sp.rx * sp.ry + (uint32_t)(int)qy * sp.rx +
+                          (uint32_t)(int)qx
+                    : (uint32_t)C;
+    }
+    // stable counting sort by key
+    std::vector<uint32_t> start(C + 2, 0);
+    for (uint32_t i = 0; i < n; ++i) start[key[i] + 1]++;
+    for (uint64_t c = 0; c <= C; ++c) start[c + 1] += start[c];
+    std::vector<uint32_t> cursor(start.begin(), start.end() - 1);
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; ++i) order[cursor[key[i]]++] = i;
+    for (uint64_t c = 0; c < 2 * C; ++c) ranges[c] = 0xFFFFFFFFu;
+    for (uint64_t c = 0; c < C; ++c) occ[c] = 0.0f;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        const uint32_t j = order[i];
+        keys[i] = key[j];
+        if (perm) perm[i] = j;
+        pos_out[3 * i + 0] = pos[3 * j + 0];
+        pos_out[3 * i + 1] = pos[3 * j + 1];
+        pos_out[3 * i + 2] = pos[3 * j + 2];
+        dens_out[i] = dens[j];
+    }
+    for (uint64_t c = 0; c < C; ++c) {
+        if (start[c + 1] > start[c]) {
+            ranges[2 * c] = start[c];
+            ranges[2 * c + 1] = start[c + 1];
+            occ[c] = 1.0f;
+        }
+    }
+    return 0;
+}
+
+// extractionOfSurfaceParticles (ReconstructionCUDA.cu:70-120); with involve,
+// extractionOfSurfaceAndInvolveParticles (:466-551).
+int orc_surface_occupancy(const float* occ, const uint32_t* ranges, orc_grid sp,
+                          const uint32_t* keys, uint32_t n, uint8_t* surf_cell, uint32_t* flags,
+                          uint32_t* involve) {
+    const uint64_t C = (uint64_t)sp.rx * sp.ry * sp.rz;
+    const int rx = sp.rx, ry = sp.ry, rz = sp.rz;
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < (int64_t)C; ++c) {
+        const int x = (int)(c % rx), y = (int)((c / rx) % ry), z = (int)(c / ((int64_t)rx * ry));
+        uint8_t s = 0;
+        if (x < rx - 1 && y < ry - 1 && z < rz - 1 && !(occ[c] < 1.0f)) {
+            for (int zz = clampi(z - 1, 0, rz - 1); zz <= clampi(z + 1, 0, rz - 1); ++zz)
+                for (int yy = clampi(y - 1, 0, ry - 1); yy <= clampi(y + 1, 0, ry - 1); ++yy)
+                    for (int xx = clampi(x - 1, 0, rx - 1); xx <= clampi(x + 1, 0, rx - 1); ++xx)
+                        if (occ[((int64_t)zz * ry + yy) * rx + xx] < 1.0f) s = 1;
+        }
+        surf_cell[c] = s;
+    }
+    if (involve) {
+        for (uint64_t c = 0; c < C; ++c) involve[c] = 0;
+        for (uint64_t c = 0; c < C; ++c) {
+            if (!surf_cell[c]) continue;
+            const int x = (int)(c % rx), y = (int)((c / rx) % ry), z = (int)(c / ((int64_t)rx * ry));
+            for (int zz = clampi(z - 2, 0, rz - 1); zz <= clampi(z + 2, 0, rz - 1); ++zz)
+                for (int yy = clampi(y - 2, 0, ry - 1); yy <= clampi(y + 2, 0, ry - 1); ++yy)
+                    for (int xx = clampi(x - 2, 0, rx - 1); xx <= clampi(x + 2, 0, rx - 1); ++xx) {
+                        const uint64_t cc = ((uint64_t)zz * ry + yy) * rx + xx;
+                        if (ranges[2 * cc] != 0xFFFFFFFFu)
+                            involve[cc] = ranges[2 * cc + 1] - ranges[2 * cc];
+                    }
+        }
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) flags[i] = (keys[i] < C && surf_cell[keys[i]]) ? 1u : 0u;
+    return 0;
+}
+
+// extractionOfSurfaceParticlesUsingColorField (ReconstructionCUDA.cu:909-971)
+// with wSpikyGrad (ReconstructionCUDA.cuh:325-334); the gradient sum in FP64.
+int orc_surface_colorfield(const float* pos, const float* dens, uint32_t n,
+                           const uint32_t* ranges, orc_grid sp, float h, float mass,
+                           uint32_t* flags, double* gnorm, uint32_t* count) {
+    const int rx = sp.rx, ry = sp.ry, rz = sp.rz;
+    const double H = h;
+    const double coef = -45.0 / (M_PI * H * H * H * H);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+        const int cx = trunc_q(cell_q(px, sp.minx, sp.cell), rx);
+        const int cy = trunc_q(cell_q(py, sp.miny, sp.cell), ry);
+        const int cz = trunc_q(cell_q(pz, sp.minz, sp.cell), rz);
+        uint32_t cnt = 0;
+        double gx = 0, gy = 0, gz = 0;
+        for (int zz = clampi(cz - 1, 0, rz - 1); zz <= clampi(cz + 1, 0, rz - 1); ++zz)
+            for (int yy = clampi(cy - 1, 0, ry - 1); yy <= clampi(cy + 1, 0, ry - 1); ++yy)
+                for (int xx = clampi(cx - 1, 0, rx - 1); xx <= clampi(cx + 1, 0, rx - 1); ++xx) {
+                    const uint64_t c = ((uint64_t)zz * ry + yy) * rx + xx;
+                    if (ranges[2 * c] == 0xFFFFFFFFu) continue;
+                    for (uint32_t j = ranges[2 * c]; j < ranges[2 * c + 1]; ++j) {
+                        // |r|^2 > 0 decided in FP32 as the kernel computes it
+                        const float fx = px - pos[3 * j], fy = py - pos[3 * j + 1],
+                                    fz = pz - pos[3 * j + 2];
+                        const float d2f = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+                        if (!(d2f > 0.0f)) continue;
+                        ++cnt;
+                        const double dx = (double)px - pos[3 * j], dy = (double)py - pos[3 * j + 1],
+                                     dz = (double)pz - pos[3 * j + 2];
+                        const double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+                        if (dist >= H || dist == 0.0) continue;
+                        const double x = 1.0 - dist / H;
+                        const double g = (double)mass / (double)dens[j] * coef * x * x / dist;
+                        gx += g * dx;
+                        gy += g * dy;
+                        gz += g * dz;
+                    }
+                }
+        const double gn = std::sqrt(gx * gx + gy * gy + gz * gz);
+        flags[i] = (cnt < 25u || gn > 3.0) ? 1u : 0u;
+        if (gnorm) gnorm[i] = gn;
+        if (count) count[i] = cnt;
+    }
+    return 0;
+}
+
+// estimationOfSurfaceVertices (ReconstructionCUDA.cu:554-595): box [i-3, i+3)
+// x [j-3, j+3] x [k-3, k+3] clamped to the grid, FP32 |v - p| compared in double
+// with 1.5 h.
+int orc_band(const float* pos, const uint32_t* flags, uint32_t n, orc_grid sc, float h,
+             uint8_t* band) {
+    const int rx = sc.rx, ry = sc.ry, rz = sc.rz;
+    const uint64_t V = (uint64_t)rx * ry * rz;
+    std::memset(band, 0, V);
+    const double limit = (double)h * 1.5;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        if (!flags[i]) continue;
+        const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+        const int ix = trunc_q(cell_q(px, sc.minx, sc.cell), rx);
+        const int iy = trunc_q(cell_q(py, sc.miny, sc.cell), ry);
+        const int iz = trunc_q(cell_q(pz, sc.minz, sc.cell), rz);
+        const int x0 = clampi(ix - 3, 0, rx - 1), x1 = clampi(ix + 3, 0, rx - 1);
+        const int y0 = clampi(iy - 3, 0, ry - 1), y1 = clampi(iy + 3, 0, ry - 1);
+        const int z0 = clampi(iz - 3, 0, rz - 1), z1 = clampi(iz + 3, 0, rz - 1);
+        for (int z = z0; z <= z1; ++z)
+            for (int y = y0; y <= y1; ++y)
+                for (int x = x0; x < x1; ++x) {
+                    const float dx = vpos(x, sc.minx, sc.cell) - px;
+                    const float dy = vpos(y, sc.miny, sc.cell) - py;
+                    const float dz = vpos(z, sc.minz, sc.cell) - pz;
+                    const float dist = sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+                    if ((double)dist < limit) {
+#pragma omp atomic write
+                        band[((uint64_t)z * ry + y) * rx + x] = 1;
+                    }
+                }
+    }
+    return 0;
+}
+
+uint32_t orc_compact(const uint8_t* band, uint64_t V, uint32_t* index) {
+    uint32_t b = 0;
+    for (uint64_t v = 0; v < V; ++v)
+        if (band[v]) index[b++] = (uint32_t)v;
+    return b;
+}
+
+// updateScalarFieldValueZB05 (ReconstructionCUDA.cu:230-289) with kernelZB05
+// (ReconstructionCUDA.cuh:262-268), in FP64.  The vertex position and its hash
+// cell are the FP32 values the kernel computes.
+int orc_field_zb05(const uint32_t* band_index, uint32_t B, const float* pos,
+                   const uint32_t* ranges, orc_grid sp, orc_grid sc, float radius_sq,
+                   float particle_radius, double* phi, double* wsum) {
+    const int rx = sp.rx, ry = sp.ry, rz = sp.rz;
+    const double R2 = radius_sq;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t b = 0; b < (int64_t)B; ++b) {
+        const uint32_t vid = band_index[b];
+        const uint32_t ix = vid % sc.rx, iy = (vid / sc.rx) % sc.ry,
+                       iz = vid / (sc.rx * sc.ry);
+        const float vxf = vpos(ix, sc.minx, sc.cell), vyf = vpos(iy, sc.miny, sc.cell),
+                    vzf = vpos(iz, sc.minz, sc.cell);
+        const int cx = trunc_q(cell_q(vxf, sp.minx, sp.cell), rx);
+        const int cy = trunc_q(cell_q(vyf, sp.miny, sp.cell), ry);
+        const int cz = trunc_q(cell_q(vzf, sp.minz, sp.cell), rz);
+        const double vx = vxf, vy = vyf, vz = vzf;
+        double sw = 0, sx = 0, sy = 0, sz = 0;
+        for (int zz = clampi(cz - 1, 0, rz - 1); zz <= clampi(cz + 1, 0, rz - 1); ++zz)
+            for (int yy = clampi(cy - 1, 0, ry - 1); yy <= clampi(cy + 1, 0, ry - 1); ++yy)
+                for (int xx = clampi(cx - 1, 0, rx - 1); xx <= clampi(cx + 1, 0, rx - 1); ++xx) {
+                    const uint64_t c = ((uint64_t)zz * ry + yy) * rx + xx;
+                    if (ranges[2 * c] == 0xFFFFFFFFu) continue;
+                    for (uint32_t j = ranges[2 * c]; j < ranges[2 * c + 1]; ++j) {
+                        const double px = pos[3 * j], py = pos[3 * j + 1], pz = pos[3 * j + 2];
+                        const double dx = vx - px, dy = vy - py, dz = vz - pz;
+                        const double d2 = dx * dx + dy * dy + dz * dz;
+                        if (d2 >= R2) continue;
+                        const double t = 1.0 - d2 / R2;
+                        const double w = t * t * t;
+                        sw += w;
+                        sx += w * px;
+                        sy += w * py;
+                        sz += w * pz;
+                    }
+                }
+        if (wsum) wsum[b] = sw;
+        if (sw > 0.0) {
+            const double ax = vx - sx / sw, ay = vy - sy / sw, az = vz - sz / sw;
+            phi[vid] = std::sqrt(ax * ax + ay * ay + az * az) - (double)particle_radius;
+        }
+    }
+    return 0;
+}
+
+// detectValidSurfaceCubes + compactValidSurfaceCubes + generateTriangles
+// (ReconstructionCUDA.cu:328-463, .cuh:366-430, 534-609); cpu = 1 follows the
+// host marching cubes of GPU/MarchingCubesCPU.cpp:3-89 instead (no normal
+// negation, normals normalised only above length 1e-7, positions min + i*cell).
+uint64_t orc_mc(const uint32_t* band_index, uint32_t B, const uint8_t* band, const double* phi,
+                orc_grid sc, float iso, int cpu, uint32_t* nverts, double* tri_pos,
+                double* tri_nrm, uint64_t capacity) {
+    const int rx = sc.rx, ry = sc.ry, rz = sc.rz;
+    const uint64_t rxy = (uint64_t)rx * ry;
+    static const int off[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {0, 1, 0},
+                                  {1, 0, 0}, {1, 0, 1}, {1, 1, 1}, {1, 1, 0}};
+    static const int edge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                    {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+    const double isod = iso;
+#pragma omp parallel for schedule(static)
+    for (int64_t b = 0; b < (int64_t)B; ++b) {
+        const uint32_t vid = band_index[b];
+        const int x = (int)(vid % rx), y = (int)((vid / rx) % ry), z = (int)(vid / rxy);
+        uint32_t nv = 0;
+        if (x < rx - 1 && y < ry - 1 && z < rz - 1) {
+            uint32_t f = 0;
+            bool all = true;
+            for (int c = 0; c < 8; ++c) {
+                const uint64_t v = ((uint64_t)(z + off[c][2]) * ry + (y + off[c][1])) * rx +
+                                   (x + off[c][0]);
+                if (phi[v] <= isod) f |= 1u << c;
+                if (!band[v]) all = false;
+            }
+            nv = all ? case_nverts(f) : 0;
+        }
+        nverts[b] = nv;
+    }
+    std::vector<uint64_t> offs(B + 1, 0);
+    for (uint32_t b = 0; b < B; ++b) offs[b + 1] = offs[b] + nverts[b];
+    const uint64_t total = offs[B];
+    if (!tri_pos || total > capacity) return total;
+    auto sample = [&](int X, int Y, int Z) -> double {
+        if (X < 0 || Y < 0 || Z < 0 || X >= rx || Y >= ry || Z >= rz) return 0.0;
+        return phi[((uint64_t)Z * ry + Y) * rx + X];
+    };
+    const double sign = (!cpu && iso < 0.0f) ? -1.0 : 1.0;
+#pragma omp parallel for schedule(static)
+    for (int64_t b = 0; b < (int64_t)B; ++b) {
+        if (!nverts[b]) continue;
+        const uint32_t vid = band_index[b];
+        const int x = (int)(vid % rx), y = (int)((vid / rx) % ry), z = (int)(vid / rxy);
+        double val[8], cp[8][3], cn[8][3];
+        uint32_t f = 0;
+        for (int c = 0; c < 8; ++c) {
+            const int X = x + off[c][0], Y = y + off[c][1], Z = z + off[c][2];
+            val[c] = phi[((uint64_t)Z * ry + Y) * rx + X];
+            if (val[c] <= isod) f |= 1u << c;
+            if (cpu) {
+                cp[c][0] = sc.minx + (float)X * sc.cell;
+                cp[c][1] = sc.miny + (float)Y * sc.cell;
+                cp[c][2] = sc.minz + (float)Z * sc.cell;
+            } else {
+                cp[c][0] = vpos(X, sc.minx, sc.cell);
+                cp[c][1] = vpos(Y, sc.miny, sc.cell);
+                cp[c][2] = vpos(Z, sc.minz, sc.cell);
+            }
+            double n0 = sample(X - 1, Y, Z) - sample(X + 1, Y, Z);
+            double n1 = sample(X, Y - 1, Z) - sample(X, Y + 1, Z);
+            double n2 = sample(X, Y, Z - 1) - sample(X, Y, Z + 1);
+            const double len = std::sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+            if (cpu) {
+                if (len > 1e-7) { n0 /= len; n1 /= len; n2 /= len; }
+            } else {
+                // v * rsqrt(dot(v, v)): NaN for a zero vector, as on the GPU
+                const double r = 1.0 / len;
+                n0 *= r; n1 *= r; n2 *= r;
+            }
+            cn[c][0] = n0; cn[c][1] = n1; cn[c][2] = n2;
+        }
+        const uint32_t nv = nverts[b];
+        for (uint32_t k = 0; k < nv; ++k) {
+            const int e = (int)((kTri[f] >> (4 * k)) & 0xF);
+            const int s0 = edge[e][0], s1 = edge[e][1];
+            const double delta = val[s1] - val[s0];
+            const double t = (delta > -1e-7 && delta < 1e-7) ? 0.5 : (isod - val[s0]) / delta;
+            double p[3], nn[3];
+            for (int a = 0; a < 3; ++a) {
+                p[a] = cp[s0][a] + t * (cp[s1][a] - cp[s0][a]);
+                nn[a] = cn[s0][a] + t * (cn[s1][a] - cn[s0][a]);
+            }
+            const double len = std::sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+            if (cpu) {
+                if (len > 1e-7) for (int a = 0; a < 3; ++a) nn[a] /= len;
+            } else {
+                const double r = 1.0 / len;
+                for (int a = 0; a < 3; ++a) nn[a] = sign * (nn[a] * r);
+            }
+            const uint64_t o = 3 * (offs[b] + k);
+            for (int a = 0; a < 3; ++a) {
+                tri_pos[o + a] = p[a];
+                if (tri_nrm) tri_nrm[o + a] = nn[a];
+            }
+        }
+    }
+    return total;
+}
+
+// One frame of the ZB05 (method 0) or Akinci12 (method 1) reconstructor
+// (GPU/ReconstructorGPUOursZB05.cpp:22-78, GPU/ReconstructorGPUAkinci12.cpp:34-84).
+int orc_frame(int method, float* pos, float* dens, uint32_t n, orc_grid sp, orc_grid sc, float h,
+              float radius_sq, float particle_radius, float mass, float iso, float* phi_out,
+              uint64_t* counts) {
+    const uint64_t C = (uint64_t)sp.rx * sp.ry * sp.rz;
+    const uint64_t V = (uint64_t)sc.rx * sc.ry * sc.rz;
+    std::vector<uint32_t> keys(n), ranges(2 * C), flags(n);
+    std::vector<float> occ(C), pos_s(3ull * n), dens_s(n);
+    orc_hash_sort(pos, dens, n, sp, keys.data(), nullptr, pos_s.data(), dens_s.data(),
+                  ranges.data(), occ.data());
+    std::memcpy(pos, pos_s.data(), sizeof(float) * 3ull * n);
+    std::memcpy(dens, dens_s.data(), sizeof(float) * n);
+    if (method == 1) {
+        orc_surface_colorfield(pos, dens, n, ranges.data(), sp, h, mass, flags.data(), nullptr,
+                               nullptr);
+    } else {
+        std::vector<uint8_t> surf(C);
+        orc_surface_occupancy(occ.data(), ranges.data(), sp, keys.data(), n, surf.data(),
+                              flags.data(), nullptr);
+    }
+    uint64_t nsurf = 0, nout = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        nsurf += flags[i];
+        nout += keys[i] == C;
+    }
+    std::vector<uint8_t> band(V);
+    orc_band(pos, flags.data(), n, sc, h, band.data());
+    std::vector<uint32_t> index(V);
+    const uint32_t B = orc_compact(band.data(), V, index.data());
+    std::vector<double> phi(V, 0.0);
+    orc_field_zb05(index.data(), B, pos, ranges.data(), sp, sc, radius_sq, particle_radius,
+                   phi.data(), nullptr);
+    std::vector<uint32_t> nverts(B);
+    const uint64_t total = orc_mc(index.data(), B, band.data(), phi.data(), sc, iso, method == 1,
+                                  nverts.data(), nullptr, nullptr, 0);
+    std::vector<double> tp(3 * total), tn(3 * total);
+    orc_mc(index.data(), B, band.data(), phi.data(), sc, iso, method == 1, nverts.data(),
+           tp.data(), tn.data(), total);
+    uint64_t q = 0;
+    for (uint32_t b = 0; b < B; ++b) q += nverts[b] > 0;
+    if (phi_out)
+        for (uint64_t v = 0; v < V; ++v) phi_out[v] = (float)phi[v];
+    if (counts) {
+        counts[0] = nsurf;
+        counts[1] = B;
+        counts[2] = q;
+        counts[3] = total;
+        counts[4] = nout;
+        counts[5] = 0;
+    }
+    return 0;
+}
+
+}  // extern "C"

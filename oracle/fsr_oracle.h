@@ -1,0 +1,84 @@
+/*
+ * CPU oracle of the particle -> field -> mesh path.  TEST INFRASTRUCTURE ONLY:
+ * it is the checker the parity tests compare libfsr against, and the CPU
+ * baseline timed by bench.py; no product code links or calls it.
+ *
+ * FP64 restatement of the reference's kernels (FluidSurfaceReconstruction/GPU/
+ * ReconstructionCUDA.cu and .cuh), OpenMP-parallel.  Discrete decisions (cell
+ * ids, band membership, cube indices of a given field) follow the reference's
+ * FP32 operation order exactly; continuous quantities (fields, gradients,
+ * covariances, eigenvalues, mesh vertices) are computed in FP64.
+ */
+#ifndef FSR_ORACLE_H_
+#define FSR_ORACLE_H_
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    float minx, miny, minz;
+    float cell;
+    uint32_t rx, ry, rz;
+} orc_grid;
+
+int orc_num_threads(void);
+void orc_set_threads(int n);
+
+/* Spatial hashing (ReconstructionCUDA.cu:1379-1481).  Outputs in sorted order.
+ * ranges: u32[2C] (start, end) pairs, 0xFFFFFFFF for empty cells; occ: f32[C].
+ * Particles outside the grid get key C and sort last. */
+int orc_hash_sort(const float* pos, const float* dens, uint32_t n, orc_grid sp,
+                  uint32_t* keys, uint32_t* perm, float* pos_out, float* dens_out,
+                  uint32_t* ranges, float* occ);
+
+/* extractionOfSurfaceParticles (ReconstructionCUDA.cu:70-120) and, with
+ * involve != NULL, extractionOfSurfaceAndInvolveParticles (:466-551). */
+int orc_surface_occupancy(const float* occ, const uint32_t* ranges, orc_grid sp,
+                          const uint32_t* keys, uint32_t n, uint8_t* surf_cell,
+                          uint32_t* flags, uint32_t* involve);
+
+/* extractionOfSurfaceParticlesUsingColorField (ReconstructionCUDA.cu:909-971),
+ * gradient in FP64.  gnorm (may be NULL): |sum|; count (may be NULL). */
+int orc_surface_colorfield(const float* pos, const float* dens, uint32_t n,
+                           const uint32_t* ranges, orc_grid sp, float h, float mass,
+                           uint32_t* flags, double* gnorm, uint32_t* count);
+
+/* estimationOfSurfaceVertices (ReconstructionCUDA.cu:554-595): band[V] = 1. */
+int orc_band(const float* pos, const uint32_t* flags, uint32_t n, orc_grid sc, float h,
+             uint8_t* band);
+
+/* Ascending compaction of band[V]; returns B. */
+uint32_t orc_compact(const uint8_t* band, uint64_t V, uint32_t* index);
+
+/* ZB05 field (ReconstructionCUDA.cu:230-289, .cuh:262-268) on the band, FP64.
+ * phi[V] is written at band vertices with sum w > 0 only; wsum[B] (may be NULL). */
+int orc_field_zb05(const uint32_t* band_index, uint32_t B, const float* pos,
+                   const uint32_t* ranges, orc_grid sp, orc_grid sc, float radius_sq,
+                   float particle_radius, double* phi, double* wsum);
+
+/* Marching cubes over the band (ReconstructionCUDA.cu:328-463; with cpu = 1 the
+ * host MC of GPU/MarchingCubesCPU.cpp).  Cube index from phi (FP64 compare
+ * against iso), vertices in FP64.  nverts[B] out; tri_pos/tri_nrm (3 doubles
+ * per soup vertex) written when not NULL and the total fits `capacity`.
+ * Returns the total soup vertex count (3T). */
+uint64_t orc_mc(const uint32_t* band_index, uint32_t B, const uint8_t* band,
+                const double* phi, orc_grid sc, float iso, int cpu, uint32_t* nverts,
+                double* tri_pos, double* tri_nrm, uint64_t capacity);
+
+/* Whole frame (method 0 = ZB05 occupancy band, 1 = Akinci12 colour-field band),
+ * as the reconstructors run it.  Positions and densities are sorted in place.
+ * phi_out (float[V], may be NULL).  counts[6] = {surface particles, B, Q, 3T,
+ * out of grid, 0}.  Returns 0 on success. */
+int orc_frame(int method, float* pos, float* dens, uint32_t n, orc_grid sp, orc_grid sc,
+              float h, float radius_sq, float particle_radius, float mass, float iso,
+              float* phi_out, uint64_t* counts);
+
+/* Case tables (for the table parity test). */
+uint64_t orc_mc_tri_packed(uint32_t c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
