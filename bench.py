@@ -94,9 +94,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--method", default="zb05", choices=["zb05", "akinci12"])
+    ap.add_argument("--method", default="zb05", choices=["zb05", "akinci12", "yu13"])
     ap.add_argument("--particles", type=int, default=1_000_000)
-    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--grid", type=int, default=None,
+                    help="scalar grid resolution (default 256, 384 for yu13)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -110,12 +111,22 @@ def main():
     from fluidsurfacereconstructiongpu_amd import datasets as ds
     from fluidsurfacereconstructiongpu_amd.reconstructor import Reconstructor, stats_dict
 
+    yu13 = args.method == "yu13"
+    if args.grid is None:
+        args.grid = 384 if yu13 else 256
     t0 = time.perf_counter()
-    fr = ds.dam_break(n_target=args.particles, grid_res=args.grid, method=args.method)
+
+    def make_frame():
+        if yu13:
+            return ds.dam_break(n_target=args.particles, grid_res=args.grid, method="yu13",
+                                scale=1.3, iso=0.0)
+        return ds.dam_break(n_target=args.particles, grid_res=args.grid, method=args.method)
+
+    fr = make_frame()
     print(f"[bench] workload generated in {time.perf_counter() - t0:.1f} s: N={fr.n} "
           f"sp={fr.sp.res} sc={fr.sc.res}", file=sys.stderr, flush=True)
     dev = torch.device("cuda", 0)
-    rec = Reconstructor(args.method)
+    rec = Reconstructor(args.method, aniso=ds.yu13_params(fr, True) if yu13 else None)
     K, W = args.steps, args.warmup
     src_pos = torch.from_numpy(fr.pos).to(dev)
     src_dens = torch.from_numpy(fr.dens).to(dev)
@@ -146,16 +157,27 @@ def main():
     frame_ms = stages.sum(axis=1)
     field_ms = float(stages[:, 4].mean())
     fill_ms = float(stages[:, 3].mean())
-    # algorithmic counts of the field stage
     B = st["num_band_vertices"]
-    band_index = rec.buffer("band_index", np.uint32, B)
-    ranges = rec.buffer("cell_ranges", np.uint32, 2 * fr.sp.count).reshape(-1, 2)
-    p27, n_touched = field_counts(fr, band_index, ranges)
     C, V = fr.sp.count, fr.sc.count
-    flops = 20.0 * p27
-    b_comp = 8.0 * B + 8.0 * C + 12.0 * n_touched + 4.0 * V
-    achieved_tflops = flops / (field_ms * 1e-3) / 1e12
-    t_roof = max(b_comp / HBM_PEAK, flops / FP32_PEAK)
+    if yu13:
+        roofline = None
+    else:
+        # algorithmic counts of the field stage
+        band_index = rec.buffer("band_index", np.uint32, B)
+        ranges = rec.buffer("cell_ranges", np.uint32, 2 * fr.sp.count).reshape(-1, 2)
+        p27, n_touched = field_counts(fr, band_index, ranges)
+        flops = 20.0 * p27
+        b_comp = 8.0 * B + 8.0 * C + 12.0 * n_touched + 4.0 * V
+        achieved_tflops = flops / (field_ms * 1e-3) / 1e12
+        t_roof = max(b_comp / HBM_PEAK, flops / FP32_PEAK)
+        roofline = {"bound": "valu", "kernel": "k_field_zb05", "achieved": achieved_tflops,
+                    "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
+                    "frac": achieved_tflops / (FP32_PEAK / 1e12), "traffic": None,
+                    "flops_per_launch": flops, "pairs_p27": p27,
+                    "field_stage_roofline_frac": t_roof / ((field_ms + fill_ms) * 1e-3),
+                    "field_stage_hbm_gbps": b_comp / ((field_ms + fill_ms) * 1e-3) / 1e9,
+                    "compulsory_bytes": b_comp, "field_kernel_ms": field_ms,
+                    "fill_kernel_ms": fill_ms}
     out = {
         "metric": METRIC,
         "value": K / (total_ms * 1e-3),
@@ -174,14 +196,7 @@ def main():
                                                                          args.method.upper()),
                    "particles": fr.n, "grid": list(fr.sc.res), "hash_grid": list(fr.sp.res),
                    "method": args.method, "parallelism": "single GPU"},
-        "roofline": {"bound": "valu", "kernel": "k_field_zb05", "achieved": achieved_tflops,
-                     "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved_tflops / (FP32_PEAK / 1e12), "traffic": None,
-                     "flops_per_launch": flops, "pairs_p27": p27,
-                     "field_stage_roofline_frac": t_roof / ((field_ms + fill_ms) * 1e-3),
-                     "field_stage_hbm_gbps": b_comp / ((field_ms + fill_ms) * 1e-3) / 1e9,
-                     "compulsory_bytes": b_comp, "field_kernel_ms": field_ms,
-                     "fill_kernel_ms": fill_ms},
+        "roofline": roofline,
         "frame_ms_p50": float(np.percentile(frame_ms, 50)),
         "frame_ms_p99": float(np.percentile(frame_ms, 99)),
         "stage_ms": {k: float(stages[:, i].mean()) for i, k in enumerate(
@@ -189,9 +204,8 @@ def main():
         "counts": {k: st[k] for k in ("num_surface_particles", "num_band_vertices",
                                       "num_valid_cubes", "num_mesh_vertices")},
     }
-    if not args.no_cpu_baseline:
-        fr2 = ds.dam_break(n_target=args.particles, grid_res=args.grid, method=args.method)
-        out["cpu_baseline"] = cpu_baseline(fr2, args.method)
+    if not args.no_cpu_baseline and not yu13:
+        out["cpu_baseline"] = cpu_baseline(make_frame(), args.method)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

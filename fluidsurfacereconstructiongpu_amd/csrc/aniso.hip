@@ -1,0 +1,577 @@
+// Anisotropic kernel of Yu & Turk 2013 (the reference's "Yu13" method, TC01).
+//
+//   k_smooth        calculationOfMeanAndSmoothParticles (ReconstructionCUDA.cu:598-658):
+//                   over the clamped 5^3 cell block, w = 1 - (d/r)^3 for d < r
+//                   (wij, ReconstructionCUDA.cuh:282-292), r = anisotropic radius;
+//                   mean = sum w p / sum w; smoothed = (1 - lambda) x + lambda mean.
+//   k_involved      compactationOfSurfaceVerticesAndParticles, particle half
+//                   (ReconstructionCUDA.cu:661-686): particles of cells with a
+//                   nonzero involve count, in cell order.
+//   k_covariance    calculationOfTransformMatricesForParticles (ReconstructionCUDA.cu:689-827):
+//                   C = (h^2 I + sum w v v^T) / sum w with v = p_j - mean_i over the
+//                   clamped 5^3 block; #{|v| <= r} < Ne gives G = I / (kn h); otherwise
+//                   the singular values of C (|sigma|) are clamped to sigma_max / kr and
+//                   G = cbrt(s1 s2 s3) / h * R diag(1/s) R^T.
+//   k_field_aniso   computationOfScalarFieldGrid (ReconstructionCUDA.cu:830-906):
+//                   phi = sum m/rho_j W - 0.5 over the clamped 5^3 block, with
+//                   W = sigma det(G) (1 - |G r|^2)^3 for particles of involved cells
+//                   (anisotropicW, .cuh:295-310) and W = sigma (1 - |r|^2)^3 with r
+//                   in world units for the others (anisotropicWOpt, .cuh:313-322).
+//
+// SVD: C is symmetric positive definite (it starts at h^2 I), so its singular
+// values are its eigenvalues and U = V = R; G depends only on that
+// eigen-decomposition.  It is computed with the cyclic Jacobi method for
+// symmetric matrices (Golub & Van Loan, Matrix Computations, sec. 8.5: the
+// symmetric Schur 2x2 rotation, 6 sweeps) in registers.
+//
+// MI355X mapping: as for the ZB05 field, a wave owns 64 particles (or band
+// vertices) of one hash cell, all lanes share the candidate block, and the 25
+// x-runs of the 5^3 block are staged through LDS 64 candidates at a time and
+// read as broadcasts.  The field kernel evaluates |G r|^2 as the quadratic form
+// r^T M r with M = G^T G stored per particle (6 values): with v the vertex and
+// p the particle relative to the cell corner, |G(p - v)|^2 =
+// v^T M v - 2 v.(M p) + p^T M p, where M p and p^T M p are formed once per staged
+// candidate; the per-pair cost is 9 fused multiply-adds.  Particles of
+// non-involved cells carry M = I and the unscaled kernel, as the reference has it.
+#include "common.hpp"
+
+namespace fsr {
+
+namespace {
+
+constexpr int kAnisoThreads = 256;
+constexpr int kMaxRows = 25;
+constexpr float kSigma = 1.5666814710608448f;  // 315 / (64 pi)
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) {
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Rows (x-runs) of the clamped (2R+1)^3 block around cell (cx, cy, cz); lane k
+// computes row k.  Returns the total candidate count; row starts and
+// exclusive prefix sums are left in LDS.
+template <int R>
+__device__ __forceinline__ uint32_t block_rows(const fsr_index_range* __restrict__ ranges,
+                                               const GridDev& sp, int cx, int cy, int cz,
+                                               uint32_t* row_start, uint32_t* row_pref,
+                                               int* nrows_out) {
+    const uint32_t lane = __lane_id();
+    const int x0 = clampi(cx - R, 0, sp.rx - 1), x1 = clampi(cx + R, 0, sp.rx - 1);
+    const int y0 = clampi(cy - R, 0, sp.ry - 1), y1 = clampi(cy + R, 0, sp.ry - 1);
+    const int z0 = clampi(cz - R, 0, sp.rz - 1), z1 = clampi(cz + R, 0, sp.rz - 1);
+    const int ny = y1 - y0 + 1;
+    const int nrows = ny * (z1 - z0 + 1);
+    uint32_t len = 0, rs = 0;
+    if ((int)lane < nrows) {
+        const int yy = y0 + (int)lane % ny, zz = z0 + (int)lane / ny;
+        const uint32_t rowc = ((uint32_t)zz * sp.ry + yy) * sp.rx;
+        uint32_t s = 0xFFFFFFFFu, e = 0;
+        for (int xx = x0; xx <= x1; ++xx) {
+            const fsr_index_range rg = ranges[rowc + xx];
+            if (rg.start != 0xFFFFFFFFu) {
+                if (s == 0xFFFFFFFFu) s = rg.start;
+                e = rg.end;
+            }
+        }
+        if (s != 0xFFFFFFFFu) { rs = s; len = e - s; }
+    }
+    const uint32_t incl = wave_inclusive_sum(len);
+    const uint32_t total = __shfl(incl, 63, 64);
+    if ((int)lane < nrows) {
+        row_start[lane] = rs;
+        row_pref[lane] = incl - len;
+    }
+    *nrows_out = nrows;
+    wave_lds_sync();
+    return total;
+}
+
+__device__ __forceinline__ uint32_t candidate_index(uint32_t g, int nrows,
+                                                    const uint32_t* row_start,
+                                                    const uint32_t* row_pref) {
+    int j = 0;
+    while (j + 1 < nrows && row_pref[j + 1] <= g) ++j;
+    return row_start[j] + (g - row_pref[j]);
+}
+
+// Next chunk of the dynamic work list (one atomic per wave).
+__device__ __forceinline__ uint32_t next_chunk(uint32_t* counter) {
+    uint32_t ch = 0;
+    if (__lane_id() == 0) ch = atomicAdd(counter, 1u);
+    return __shfl(ch, 0, 64);
+}
+
+struct ChunkList {
+    const uint32_t* chunk_cell;  // chunk -> cell
+    const uint32_t* chunk_off;   // cell -> first chunk
+    const uint32_t* num_chunks;  // device count
+    uint32_t* next;              // fetch counter
+};
+
+// ---------------------------------------------------------------- K10 -------
+__global__ __launch_bounds__(kAnisoThreads) void k_smooth(
+    const float4* __restrict__ sorted, const fsr_index_range* __restrict__ ranges, GridDev sp,
+    float radius, float lambda, ChunkList cl, float4* __restrict__ mean_out,
+    float4* __restrict__ smooth_out) {
+    __shared__ float4 stage[kAnisoThreads / 64][64];
+    __shared__ uint32_t rstart[kAnisoThreads / 64][kMaxRows];
+    __shared__ uint32_t rpref[kAnisoThreads / 64][kMaxRows];
+    const uint32_t lane = __lane_id();
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t nch = *cl.num_chunks;
+    const float inv_r2 = 1.0f / (radius * radius);
+    for (;;) {
+        const uint32_t ch = next_chunk(cl.next);
+        if (ch >= nch) break;
+        const uint32_t c = cl.chunk_cell[ch];
+        const uint32_t k = ch - cl.chunk_off[c];
+        const fsr_index_range own = ranges[c];
+        const uint32_t i = own.start + 64u * k + lane;
+        const bool live = i < own.end;
+        const int cx = (int)(c % sp.rx), cy = (int)((c / sp.rx) % sp.ry), cz = (int)(c / (sp.rx * sp.ry));
+        int nrows;
+        const uint32_t total = block_rows<2>(ranges, sp, cx, cy, cz, rstart[wave], rpref[wave], &nrows);
+        const float ox = __builtin_fmaf((float)cx, sp.cell, sp.minx);
+        const float oy = __builtin_fmaf((float)cy, sp.cell, sp.miny);
+        const float oz = __builtin_fmaf((float)cz, sp.cell, sp.minz);
+        float4 self = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (live) self = sorted[i];
+        const float px = self.x - ox, py = self.y - oy, pz = self.z - oz;
+        const float ax = 2.0f * px * inv_r2, ay = 2.0f * py * inv_r2, az = 2.0f * pz * inv_r2;
+        const float c0 = __builtin_fmaf(pz, pz, __builtin_fmaf(py, py, px * px)) * inv_r2;
+        float sw = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+        for (uint32_t base = 0; base < total; base += 64u) {
+            const uint32_t g = base + lane;
+            float4 st = make_float4(0.f, 0.f, 0.f, 1.0e30f);
+            if (g < total) {
+                const float4 q = sorted[candidate_index(g, nrows, rstart[wave], rpref[wave])];
+                const float qx = q.x - ox, qy = q.y - oy, qz = q.z - oz;
+                st = make_float4(qx, qy, qz,
+                                 __builtin_fmaf(qz, qz, __builtin_fmaf(qy, qy, qx * qx)) * inv_r2);
+            }
+            stage[wave][lane] = st;
+            wave_lds_sync();
+#pragma unroll 8
+            for (int t = 0; t < 64; ++t) {
+                const float4 q = stage[wave][t];
+                // s2 = |x - q|^2 / r^2
+                float s2 = __builtin_fmaf(-az, q.z, c0 + q.w);
+                s2 = __builtin_fmaf(-ay, q.y, s2);
+                s2 = __builtin_fmaf(-ax, q.x, s2);
+                s2 = fmaxf(s2, 0.0f);
+                const float w = fmaxf(1.0f - s2 * __builtin_sqrtf(s2), 0.0f);
+                sw += w;
+                sx = __builtin_fmaf(w, q.x, sx);
+                sy = __builtin_fmaf(w, q.y, sy);
+                sz = __builtin_fmaf(w, q.z, sz);
+            }
+            wave_lds_sync();
+        }
+        if (live && sw > 0.0f) {
+            const float inv = 1.0f / sw;
+            const float mx = ox + sx * inv, my = oy + sy * inv, mz = oz + sz * inv;
+            mean_out[i] = make_float4(mx, my, mz, 0.0f);
+            const float a = 1.0f - lambda;
+            smooth_out[i] = make_float4(a * self.x + lambda * mx, a * self.y + lambda * my,
+                                        a * self.z + lambda * mz, 0.0f);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- K8 --------
+// Involved list and the per-particle field records of non-involved particles.
+__global__ __launch_bounds__(256) void k_involved(
+    const uint32_t* __restrict__ keys, const fsr_index_range* __restrict__ ranges,
+    const uint32_t* __restrict__ involve, const uint32_t* __restrict__ involve_off,
+    const float4* __restrict__ sorted, uint32_t n, uint32_t C, float mass,
+    uint32_t* __restrict__ involved_idx, float4* __restrict__ rec) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = keys[i];
+    if (c < C && involve[c] > 0) {
+        involved_idx[involve_off[c] + (i - ranges[c].start)] = i;
+    } else {
+        // M = I, unscaled kernel (anisotropicWOpt)
+        rec[2ull * i + 0] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        rec[2ull * i + 1] = make_float4(0.0f, 0.0f, kSigma * (mass / sorted[i].w), 0.0f);
+    }
+}
+
+// ------------------------------------------------------ 3x3 Jacobi eigen -----
+// Symmetric A (a00 a01 a02 / a11 a12 / a22) -> eigenvalues d and rotation V
+// (columns = eigenvectors), A = V diag(d) V^T.
+__device__ __forceinline__ void jacobi_rotate(float& app, float& aqq, float& apq, float& arp,
+                                              float& arq, float* vp, float* vq) {
+    if (fabsf(apq) <= 1.0e-30f) return;
+    const float theta = (aqq - app) / (2.0f * apq);
+    const float t = copysignf(1.0f, theta) / (fabsf(theta) + __builtin_sqrtf(__builtin_fmaf(theta, theta, 1.0f)));
+    const float c = 1.0f / __builtin_sqrtf(__builtin_fmaf(t, t, 1.0f));
+    const float s = t * c;
+    app = __builtin_fmaf(-t, apq, app);
+    aqq = __builtin_fmaf(t, apq, aqq);
+    apq = 0.0f;
+    const float rp = arp, rq = arq;
+    arp = c * rp - s * rq;
+    arq = s * rp + c * rq;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float a = vp[k], b = vq[k];
+        vp[k] = c * a - s * b;
+        vq[k] = s * a + c * b;
+    }
+}
+
+__device__ __forceinline__ void sym_eigen3(float a00, float a01, float a02, float a11, float a12,
+                                           float a22, float d[3], float V[3][3]) {
+    // V stored by columns: V[k] is eigenvector k
+    float v0[3] = {1.f, 0.f, 0.f}, v1[3] = {0.f, 1.f, 0.f}, v2[3] = {0.f, 0.f, 1.f};
+#pragma unroll 1
+    for (int sweep = 0; sweep < 6; ++sweep) {
+        jacobi_rotate(a00, a11, a01, a02, a12, v0, v1);   // (p,q) = (0,1), r = 2
+        jacobi_rotate(a00, a22, a02, a01, a12, v0, v2);   // (0,2), r = 1
+        jacobi_rotate(a11, a22, a12, a01, a02, v1, v2);   // (1,2), r = 0
+    }
+    d[0] = a00; d[1] = a11; d[2] = a22;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { V[0][k] = v0[k]; V[1][k] = v1[k]; V[2][k] = v2[k]; }
+}
+
+// ---------------------------------------------------------------- K11 -------
+struct CovArgs {
+    const float4* sorted;
+    const float4* mean;
+    const fsr_index_range* ranges;
+    const uint32_t* involve_off;
+    GridDev sp;
+    float radius;           // anisotropic radius r
+    float h2;               // smoothing radius squared
+    float inv_h;            // 1 / h
+    float kr, kn;
+    uint32_t min_neighbors;
+    float mass;
+    fsr_matrix* gmat;       // [I]
+    float4* sigma;          // [I] eigenvalues, descending
+    float4* rec;            // [2N] field records
+};
+
+__global__ __launch_bounds__(kAnisoThreads) void k_covariance(CovArgs a, ChunkList cl) {
+    __shared__ float4 stage[kAnisoThreads / 64][64];
+    __shared__ uint32_t rstart[kAnisoThreads / 64][kMaxRows];
+    __shared__ uint32_t rpref[kAnisoThreads / 64][kMaxRows];
+    const uint32_t lane = __lane_id();
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t nch = *cl.num_chunks;
+    const float r = a.radius;
+    const float inv_r = 1.0f / r;
+    for (;;) {
+        const uint32_t ch = next_chunk(cl.next);
+        if (ch >= nch) break;
+        const uint32_t c = cl.chunk_cell[ch];
+        const uint32_t k = ch - cl.chunk_off[c];
+        const fsr_index_range own = a.ranges[c];
+        const uint32_t i = own.start + 64u * k + lane;
+        const bool live = i < own.end;
+        const int cx = (int)(c % a.sp.rx), cy = (int)((c / a.sp.rx) % a.sp.ry),
+                  cz = (int)(c / (a.sp.rx * a.sp.ry));
+        int nrows;
+        const uint32_t total =
+            block_rows<2>(a.ranges, a.sp, cx, cy, cz, rstart[wave], rpref[wave], &nrows);
+        const float ox = __builtin_fmaf((float)cx, a.sp.cell, a.sp.minx);
+        const float oy = __builtin_fmaf((float)cy, a.sp.cell, a.sp.miny);
+        const float oz = __builtin_fmaf((float)cz, a.sp.cell, a.sp.minz);
+        float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (live) m = a.mean[i];
+        const float mx = m.x - ox, my = m.y - oy, mz = m.z - oz;
+        float sw = 0.f, c00 = 0.f, c01 = 0.f, c02 = 0.f, c11 = 0.f, c12 = 0.f, c22 = 0.f;
+        uint32_t cnt = 0;
+        for (uint32_t base = 0; base < total; base += 64u) {
+            const uint32_t g = base + lane;
+            float4 st = make_float4(1.0e18f, 1.0e18f, 1.0e18f, 0.f);
+            if (g < total) {
+                const float4 q = a.sorted[candidate_index(g, nrows, rstart[wave], rpref[wave])];
+                st = make_float4(q.x - ox, q.y - oy, q.z - oz, 0.f);
+            }
+            stage[wave][lane] = st;
+            wave_lds_sync();
+#pragma unroll 4
+            for (int t = 0; t < 64; ++t) {
+                const float4 q = stage[wave][t];
+                const float vx = q.x - mx, vy = q.y - my, vz = q.z - mz;
+                const float d2 = __builtin_fmaf(vz, vz, __builtin_fmaf(vy, vy, vx * vx));
+                const float dist = __builtin_sqrtf(d2);
+                cnt += dist <= r ? 1u : 0u;
+                const float s = dist * inv_r;
+                const float w = dist < r ? 1.0f - s * s * s : 0.0f;
+                sw += w;
+                const float wx = w * vx, wy = w * vy;
+                c00 = __builtin_fmaf(wx, vx, c00);
+                c01 = __builtin_fmaf(wx, vy, c01);
+                c02 = __builtin_fmaf(wx, vz, c02);
+                c11 = __builtin_fmaf(wy, vy, c11);
+                c12 = __builtin_fmaf(wy, vz, c12);
+                c22 = __builtin_fmaf(w * vz, vz, c22);
+            }
+            wave_lds_sync();
+        }
+        if (live) {
+        const uint32_t gi = a.involve_off[c] + (i - own.start);
+        float G[3][3];
+        float4 sig;
+        if (cnt < a.min_neighbors) {
+            const float gv = a.inv_h / a.kn;
+            for (int p = 0; p < 3; ++p)
+                for (int q = 0; q < 3; ++q) G[p][q] = p == q ? gv : 0.0f;
+            sig = make_float4(0.f, 0.f, 0.f, 0.f);
+        } else {
+            const float iw = 1.0f / sw;
+            const float a00 = (a.h2 + c00) * iw, a11 = (a.h2 + c11) * iw, a22 = (a.h2 + c22) * iw;
+            const float a01 = c01 * iw, a02 = c02 * iw, a12 = c12 * iw;
+            float d[3], V[3][3];
+            sym_eigen3(a00, a01, a02, a11, a12, a22, d, V);
+            float s0 = fabsf(d[0]), s1 = fabsf(d[1]), s2 = fabsf(d[2]);
+            // report sorted descending
+            float e0 = fmaxf(s0, fmaxf(s1, s2)), e2 = fminf(s0, fminf(s1, s2));
+            sig = make_float4(e0, s0 + s1 + s2 - e0 - e2, e2, 0.f);
+            const float floor_v = e0 / a.kr;
+            s0 = fmaxf(s0, floor_v);
+            s1 = fmaxf(s1, floor_v);
+            s2 = fmaxf(s2, floor_v);
+            const float scale = cbrtf(s0 * s1 * s2) * a.inv_h;
+            const float g0 = scale / s0, g1 = scale / s1, g2 = scale / s2;
+            for (int p = 0; p < 3; ++p)
+                for (int q = 0; q < 3; ++q)
+                    G[p][q] = V[0][p] * g0 * V[0][q] + V[1][p] * g1 * V[1][q] + V[2][p] * g2 * V[2][q];
+        }
+        fsr_matrix out;
+        out.a11 = G[0][0]; out.a12 = G[0][1]; out.a13 = G[0][2];
+        out.a21 = G[1][0]; out.a22 = G[1][1]; out.a23 = G[1][2];
+        out.a31 = G[2][0]; out.a32 = G[2][1]; out.a33 = G[2][2];
+        a.gmat[gi] = out;
+        a.sigma[gi] = sig;
+        // determinant (ReconstructionCUDA.cuh:690-700) and M = G^T G
+        const float det = G[0][0] * G[1][1] * G[2][2] + G[0][1] * G[1][2] * G[2][0] +
+                          G[0][2] * G[1][0] * G[2][1] - G[0][0] * G[1][2] * G[2][1] -
+                          G[0][1] * G[1][0] * G[2][2] - G[0][2] * G[1][1] * G[2][0];
+        float M[3][3];
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q)
+                M[p][q] = G[0][p] * G[0][q] + G[1][p] * G[1][q] + G[2][p] * G[2][q];
+        const float coef = kSigma * det * (a.mass / a.sorted[i].w);
+        a.rec[2ull * i + 0] = make_float4(M[0][0], M[1][1], M[2][2], M[0][1]);
+        a.rec[2ull * i + 1] = make_float4(M[0][2], M[1][2], coef, 0.0f);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- K12 -------
+struct AnisoFieldArgs {
+    const float4* smooth;
+    const float4* rec;
+    const fsr_index_range* ranges;
+    BandCompactOut bc;
+    GridDev sp, sc;
+    float* phi;
+    uint32_t* next;
+    const uint32_t* num_chunks;
+};
+
+__global__ __launch_bounds__(kAnisoThreads) void k_field_aniso(AnisoFieldArgs a) {
+    __shared__ float4 stage[kAnisoThreads / 64][3][64];
+    __shared__ uint32_t rstart[kAnisoThreads / 64][kMaxRows];
+    __shared__ uint32_t rpref[kAnisoThreads / 64][kMaxRows];
+    const uint32_t lane = __lane_id();
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t nch = *a.num_chunks;
+    const int R = a.bc.radius;
+    for (;;) {
+        const uint32_t ch = next_chunk(a.next);
+        if (ch >= nch) break;
+        const uint32_t v = a.bc.chunk_cell[ch];
+        const int cx = (int)(v % a.bc.vcx) - R;
+        const int cy = (int)((v / a.bc.vcx) % a.bc.vcy) - R;
+        const int cz = (int)(v / (a.bc.vcx * a.bc.vcy)) - R;
+        const uint32_t k = ch - a.bc.chunk_off[v];
+        const uint32_t first = a.bc.cell_off[v] + 64u * k;
+        const uint32_t cnt = min(64u, a.bc.cell_cnt[v] - 64u * k);
+        int nrows;
+        const uint32_t total =
+            block_rows<2>(a.ranges, a.sp, cx, cy, cz, rstart[wave], rpref[wave], &nrows);
+        const float ox = __builtin_fmaf((float)cx, a.sp.cell, a.sp.minx);
+        const float oy = __builtin_fmaf((float)cy, a.sp.cell, a.sp.miny);
+        const float oz = __builtin_fmaf((float)cz, a.sp.cell, a.sp.minz);
+        uint32_t vid = 0;
+        float vx = 0.f, vy = 0.f, vz = 0.f;
+        if (lane < cnt) {
+            vid = a.bc.cell_band[first + lane];
+            const uint32_t ix = vid % a.sc.rx;
+            const uint32_t iy = (vid / a.sc.rx) % a.sc.ry;
+            const uint32_t iz = vid / (a.sc.rx * a.sc.ry);
+            vx = vertex_coord(ix, a.sc.minx, a.sc.cell) - ox;
+            vy = vertex_coord(iy, a.sc.miny, a.sc.cell) - oy;
+            vz = vertex_coord(iz, a.sc.minz, a.sc.cell) - oz;
+        }
+        const float xx = vx * vx, yy = vy * vy, zz = vz * vz;
+        const float xy2 = 2.0f * vx * vy, xz2 = 2.0f * vx * vz, yz2 = 2.0f * vy * vz;
+        const float mx2 = -2.0f * vx, my2 = -2.0f * vy, mz2 = -2.0f * vz;
+        float sum = 0.f;
+        for (uint32_t base = 0; base < total; base += 64u) {
+            const uint32_t g = base + lane;
+            // padding candidate: |G r|^2 = 2 (outside the support) and zero weight
+            float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0, s2 = make_float4(0.f, 2.0f, 0.f, 0.f);
+            if (g < total) {
+                const uint32_t j = candidate_index(g, nrows, rstart[wave], rpref[wave]);
+                const float4 p4 = a.smooth[j];
+                const float4 r0 = a.rec[2ull * j + 0];
+                const float4 r1 = a.rec[2ull * j + 1];
+                const float px = p4.x - ox, py = p4.y - oy, pz = p4.z - oz;
+                const float M00 = r0.x, M11 = r0.y, M22 = r0.z, M01 = r0.w, M02 = r1.x, M12 = r1.y;
+                const float bx = M00 * px + M01 * py + M02 * pz;
+                const float by = M01 * px + M11 * py + M12 * pz;
+                const float bz = M02 * px + M12 * py + M22 * pz;
+                const float cc = px * bx + py * by + pz * bz;
+                s0 = make_float4(M00, M11, M22, M01);
+                s1 = make_float4(M02, M12, bx, by);
+                s2 = make_float4(bz, cc, r1.z, 0.f);
+            }
+            stage[wave][0][lane] = s0;
+            stage[wave][1][lane] = s1;
+            stage[wave][2][lane] = s2;
+            wave_lds_sync();
+#pragma unroll 4
+            for (int t = 0; t < 64; ++t) {
+                const float4 q0 = stage[wave][0][t];
+                const float4 q1 = stage[wave][1][t];
+                const float4 q2 = stage[wave][2][t];
+                // |G(p - v)|^2 = v^T M v - 2 v.(M p) + p^T M p
+                float d2 = __builtin_fmaf(q0.x, xx, q2.y);
+                d2 = __builtin_fmaf(q0.y, yy, d2);
+                d2 = __builtin_fmaf(q0.z, zz, d2);
+                d2 = __builtin_fmaf(q0.w, xy2, d2);
+                d2 = __builtin_fmaf(q1.x, xz2, d2);
+                d2 = __builtin_fmaf(q1.y, yz2, d2);
+                d2 = __builtin_fmaf(q1.z, mx2, d2);
+                d2 = __builtin_fmaf(q1.w, my2, d2);
+                d2 = __builtin_fmaf(q2.x, mz2, d2);
+                const float x = fmaxf(1.0f - d2, 0.0f);
+                sum = __builtin_fmaf(q2.z, x * x * x, sum);
+            }
+            wave_lds_sync();
+        }
+        if (lane < cnt) a.phi[vid] = sum - 0.5f;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cell_counts(const fsr_index_range* __restrict__ ranges,
+                                                     const uint32_t* __restrict__ filter,
+                                                     uint32_t C, uint32_t* __restrict__ cnt) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const fsr_index_range r = ranges[c];
+    uint32_t n = r.start == 0xFFFFFFFFu ? 0u : r.end - r.start;
+    if (filter && filter[c] == 0) n = 0;
+    cnt[c] = n;
+}
+
+__global__ __launch_bounds__(256) void k_chunks_of_cells(const uint32_t* __restrict__ cnt,
+                                                         const uint32_t* __restrict__ chunk_off,
+                                                         uint32_t C, uint32_t* __restrict__ chunk_cell) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const uint32_t k = (cnt[c] + 63u) >> 6;
+    const uint32_t o = chunk_off[c];
+    for (uint32_t j = 0; j < k; ++j) chunk_cell[o + j] = c;
+}
+
+}  // namespace
+
+fsr_status aniso_run(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
+    const uint32_t C = a.sp.rx * a.sp.ry * a.sp.rz;
+    const uint32_t n = a.n;
+    const int blocks = persistent_blocks(ctx, kAnisoThreads, 8);
+    // per-cell particle counts and chunk offsets of the two particle passes
+    uint32_t *cnt, *chunk_off, *chunk_cell, *cnt2, *chunk_off2, *chunk_cell2;
+    FSR_TRY(arena_typed(ctx, A_AUX0, 4ull * C, &cnt));
+    chunk_off = cnt + C;
+    cnt2 = chunk_off + C;
+    chunk_off2 = cnt2 + C;
+    const uint64_t max_chunks = (uint64_t)n / 64 + C + 1;
+    FSR_TRY(arena_typed(ctx, A_AUX1, 2ull * max_chunks, &chunk_cell));
+    chunk_cell2 = chunk_cell + max_chunks;
+    // K10 over all particles
+    hipLaunchKernelGGL(k_cell_counts, dim3((C + 255) / 256), dim3(256), 0, s, a.ranges,
+                       (const uint32_t*)nullptr, C, cnt);
+    FSR_LAUNCH_CHECK("k_cell_counts");
+    FSR_TRY(scan_exclusive(ctx, SCAN_CHUNKS64, cnt, chunk_off, C, nullptr,
+                           &a.counters->aux[0], s));
+    hipLaunchKernelGGL(k_chunks_of_cells, dim3((C + 255) / 256), dim3(256), 0, s, cnt, chunk_off,
+                       C, chunk_cell);
+    FSR_LAUNCH_CHECK("k_chunks_of_cells");
+    ChunkList cl{chunk_cell, chunk_off, &a.counters->aux[0], &a.counters->aux[1]};
+    hipLaunchKernelGGL(k_smooth, dim3(blocks), dim3(kAnisoThreads), 0, s, a.sorted, a.ranges,
+                       a.sp, a.aniso_radius, a.lambda, cl, a.mean, a.smooth);
+    FSR_LAUNCH_CHECK("k_smooth");
+    // K8: involved list (offsets = exclusive scan of the involve counts)
+    FSR_TRY(scan_exclusive(ctx, SCAN_U32, a.involve, a.involve_off, C, nullptr,
+                           &a.st->num_involved_particles, s));
+    if (n) {
+        hipLaunchKernelGGL(k_involved, dim3((n + 255) / 256), dim3(256), 0, s, a.keys, a.ranges,
+                           a.involve, a.involve_off, a.sorted, n, C, a.mass, a.involved_idx,
+                           a.rec);
+        FSR_LAUNCH_CHECK("k_involved");
+    }
+    // K11 over the particles of involved cells
+    hipLaunchKernelGGL(k_cell_counts, dim3((C + 255) / 256), dim3(256), 0, s, a.ranges,
+                       a.involve, C, cnt2);
+    FSR_LAUNCH_CHECK("k_cell_counts");
+    FSR_TRY(scan_exclusive(ctx, SCAN_CHUNKS64, cnt2, chunk_off2, C, nullptr,
+                           &a.counters->aux[2], s));
+    hipLaunchKernelGGL(k_chunks_of_cells, dim3((C + 255) / 256), dim3(256), 0, s, cnt2,
+                       chunk_off2, C, chunk_cell2);
+    FSR_LAUNCH_CHECK("k_chunks_of_cells");
+    ChunkList cl2{chunk_cell2, chunk_off2, &a.counters->aux[2], &a.counters->aux[3]};
+    CovArgs ca;
+    ca.sorted = a.sorted;
+    ca.mean = a.mean;
+    ca.ranges = a.ranges;
+    ca.involve_off = a.involve_off;
+    ca.sp = a.sp;
+    ca.radius = a.aniso_radius;
+    ca.h2 = a.h2;
+    ca.inv_h = a.inv_h;
+    ca.kr = a.kr;
+    ca.kn = a.kn;
+    ca.min_neighbors = a.min_neighbors;
+    ca.mass = a.mass;
+    ca.gmat = a.gmat;
+    ca.sigma = a.sigma;
+    ca.rec = a.rec;
+    hipLaunchKernelGGL(k_covariance, dim3(blocks), dim3(kAnisoThreads), 0, s, ca, cl2);
+    FSR_LAUNCH_CHECK("k_covariance");
+    return FSR_OK;
+}
+
+fsr_status aniso_field(fsr_context* ctx, const AnisoArgs& a, const BandCompactOut& bc,
+                       float* phi, hipStream_t s) {
+    const int blocks = persistent_blocks(ctx, kAnisoThreads, 8);
+    AnisoFieldArgs fa;
+    fa.smooth = a.smooth;
+    fa.rec = a.rec;
+    fa.ranges = a.ranges;
+    fa.bc = bc;
+    fa.sp = a.sp;
+    fa.sc = a.sc;
+    fa.phi = phi;
+    fa.next = &a.counters->field_next_chunk;
+    fa.num_chunks = &a.counters->num_chunks;
+    hipLaunchKernelGGL(k_field_aniso, dim3(blocks), dim3(kAnisoThreads), 0, s, fa);
+    FSR_LAUNCH_CHECK("k_field_aniso");
+    return FSR_OK;
+}
+
+}  // namespace fsr

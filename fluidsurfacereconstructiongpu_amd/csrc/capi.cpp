@@ -197,15 +197,21 @@ fsr_status fsr_reconstruct_frame(fsr_context* ctx, fsr_method method, const fsr_
         set_error("fsr_reconstruct_frame: tri_pos is null but tri_capacity > 0");
         return FSR_ERR_INVALID_ARG;
     }
-    if (method != FSR_METHOD_ZB05 && method != FSR_METHOD_AKINCI12) {
-        if (method == FSR_METHOD_YU13) {
-            set_error("fsr_reconstruct_frame: the anisotropic method is not built in this library");
-            return FSR_ERR_UNSUPPORTED;
-        }
+    if (method != FSR_METHOD_ZB05 && method != FSR_METHOD_AKINCI12 && method != FSR_METHOD_YU13) {
         set_error("fsr_reconstruct_frame: unknown method");
         return FSR_ERR_INVALID_ARG;
     }
-    (void)aniso;
+    const bool yu13 = method == FSR_METHOD_YU13;
+    fsr_aniso_param ap;
+    ap.kr = 4.0f;
+    ap.ks = 1.0f;
+    ap.kn = 1.0f;
+    ap.reserved = 0;
+    if (aniso) ap = *aniso;
+    if (yu13 && !(ap.kr >= 1.0f && ap.kn > 0.0f && sim->anisotropic_radius > 0.0f)) {
+        set_error("fsr_reconstruct_frame: anisotropic parameters need kr >= 1, kn > 0, r > 0");
+        return FSR_ERR_INVALID_ARG;
+    }
     FSR_TRY(check_grid(sp_grid, "sp_grid"));
     FSR_TRY(check_grid(sc_grid, "sc_grid"));
     FSR_HIP(hipSetDevice(ctx->device));
@@ -253,12 +259,14 @@ fsr_status fsr_reconstruct_frame(fsr_context* ctx, fsr_method method, const fsr_
     sa.counters = counters;
     sa.st = stats;
     sa.all_exceptions = band_dilation_exact(scg, sim->smoothing_radius) ? 0 : 1;
+    uint32_t* involve = nullptr;
+    if (yu13) FSR_TRY(arena_typed(ctx, A_INVOLVE, C, &involve));
     if (method == FSR_METHOD_AKINCI12) {
         FSR_TRY(surface_colorfield(ctx, sa, sim->particle_mass, s));
     } else {
         uint8_t* surf_cell;
         FSR_TRY(arena_typed(ctx, A_SURF_CELL, C, &surf_cell));
-        FSR_TRY(surface_occupancy(ctx, sa, surf_cell, nullptr, s));
+        FSR_TRY(surface_occupancy(ctx, sa, surf_cell, involve, s));
     }
 
     // 3. band
@@ -276,23 +284,59 @@ fsr_status fsr_reconstruct_frame(fsr_context* ctx, fsr_method method, const fsr_
     ba.st = stats;
     FSR_TRY(band_build(ctx, ba, s));
     BandCompactOut bc;
-    FSR_TRY(band_compact(ctx, bits, sc, sp, 1, counters, stats, &bc, s));
+    FSR_TRY(band_compact(ctx, bits, sc, sp, yu13 ? 2 : 1, counters, stats, &bc, s));
 
     // 4. field
     prof_mark(ctx, FSR_STAGE_FILL, s);
-    FSR_TRY(fill_f32(phi, 0.0f, V, s));
+    if (yu13) {
+        // the reference fills the Yu13 grid with the byte 100 (0x64646464)
+        FSR_TRY(fill_u32(reinterpret_cast<uint32_t*>(phi), 0x64646464u, V, s));
+    } else {
+        FSR_TRY(fill_f32(phi, 0.0f, V, s));
+    }
     prof_mark(ctx, FSR_STAGE_FIELD, s);
-    FieldArgs fa;
-    fa.sorted = ho.sorted;
-    fa.ranges = ho.ranges;
-    fa.bc = &bc;
-    fa.sp = sp;
-    fa.sc = sc;
-    fa.radius_sq = sim->smoothing_radius_sq;
-    fa.particle_radius = sim->particle_radius;
-    fa.phi = phi;
-    fa.counters = counters;
-    FSR_TRY(field_zb05(ctx, fa, s));
+    if (yu13) {
+        AnisoArgs aa;
+        const uint64_t nn = n ? n : 1;
+        aa.sorted = ho.sorted;
+        aa.keys = ho.keys;
+        aa.ranges = ho.ranges;
+        aa.involve = involve;
+        FSR_TRY(arena_typed(ctx, A_INVOLVE_OFF, C, &aa.involve_off));
+        aa.n = n;
+        aa.sp = sp;
+        aa.sc = sc;
+        aa.aniso_radius = sim->anisotropic_radius;
+        aa.lambda = sim->lambda_for_smoothed;
+        aa.h2 = sim->smoothing_radius_sq;
+        aa.inv_h = sim->smoothing_radius_inv;
+        aa.kr = ap.kr;
+        aa.kn = ap.kn;
+        aa.mass = sim->particle_mass;
+        aa.min_neighbors = sim->min_num_neighbors;
+        FSR_TRY(arena_typed(ctx, A_MEAN, nn, &aa.mean));
+        FSR_TRY(arena_typed(ctx, A_SMOOTH, nn, &aa.smooth));
+        FSR_TRY(arena_typed(ctx, A_INVOLVED_IDX, nn, &aa.involved_idx));
+        FSR_TRY(arena_typed(ctx, A_GMAT, nn, &aa.gmat));
+        FSR_TRY(arena_typed(ctx, A_SIGMA, nn, &aa.sigma));
+        FSR_TRY(arena_typed(ctx, A_REC, 2 * nn, &aa.rec));
+        aa.counters = counters;
+        aa.st = stats;
+        FSR_TRY(aniso_run(ctx, aa, s));
+        FSR_TRY(aniso_field(ctx, aa, bc, phi, s));
+    } else {
+        FieldArgs fa;
+        fa.sorted = ho.sorted;
+        fa.ranges = ho.ranges;
+        fa.bc = &bc;
+        fa.sp = sp;
+        fa.sc = sc;
+        fa.radius_sq = sim->smoothing_radius_sq;
+        fa.particle_radius = sim->particle_radius;
+        fa.phi = phi;
+        fa.counters = counters;
+        FSR_TRY(field_zb05(ctx, fa, s));
+    }
 
     // 5. marching cubes
     prof_mark(ctx, FSR_STAGE_MC, s);

@@ -132,7 +132,7 @@ enum ArenaSlot {
     A_CELL_CNT, A_CELL_OFF, A_CELL_CHUNKS, A_CHUNK_OFF, A_CELL_BAND,
     A_CHUNK_CELL, A_NVERTS, A_VALID, A_VALID_OFF, A_VERT_OFF, A_VALID_IDX,
     A_COUNTERS, A_SCAN_PARTIALS, A_STATS, A_INVOLVE, A_INVOLVE_OFF,
-    A_INVOLVED_IDX, A_MEAN, A_SMOOTH, A_GMAT, A_SIGMA, A_AUX0, A_AUX1,
+    A_INVOLVED_IDX, A_MEAN, A_SMOOTH, A_GMAT, A_SIGMA, A_AUX0, A_AUX1, A_REC,
     A_NUM_SLOTS
 };
 
@@ -300,6 +300,34 @@ struct McArgs {
     fsr_frame_stats* st;
 };
 fsr_status mc_run(fsr_context* ctx, const McArgs& a, hipStream_t s);
+
+// aniso.hip -----------------------------------------------------------------
+struct AnisoArgs {
+    const float4* sorted;
+    const uint32_t* keys;
+    const fsr_index_range* ranges;
+    const uint32_t* involve;      // u32[C] involve counts (0: not involved)
+    uint32_t* involve_off;        // u32[C] exclusive scan of involve
+    uint32_t n;
+    GridDev sp, sc;
+    float aniso_radius;           // covariance / smoothing radius r
+    float lambda;                 // smoothing blend
+    float h2, inv_h;              // smoothing radius squared, inverse
+    float kr, kn;
+    float mass;
+    uint32_t min_neighbors;
+    float4* mean;                 // [N] xyz
+    float4* smooth;               // [N] xyz
+    uint32_t* involved_idx;       // [I]
+    fsr_matrix* gmat;             // [I]
+    float4* sigma;                // [I] covariance eigenvalues, descending
+    float4* rec;                  // [2N] field records (M = G^T G, coefficient)
+    Counters* counters;
+    fsr_frame_stats* st;
+};
+fsr_status aniso_run(fsr_context* ctx, const AnisoArgs& a, hipStream_t s);
+fsr_status aniso_field(fsr_context* ctx, const AnisoArgs& a, const BandCompactOut& bc,
+                       float* phi, hipStream_t s);
 
 int persistent_blocks(fsr_context* ctx, int threads, int per_cu);
 

@@ -172,16 +172,26 @@ def lattice(nx: int, ny: int, nz: int, spacing: float, origin=(0.0, 0.0, 0.0)) -
 
 def jittered_lattice(n_side: int = 40, spacing: float = 0.01, jitter: float = 0.1,
                      seed: int = BASE_SEED + 0, grid_res: int = 128, iso: float = -1e-4,
-                     method: str = "zb05") -> Frame:
+                     method: str = "zb05", scale: float = 1.0, cell=None) -> Frame:
     """Plain method, small: n_side^3 lattice at `spacing`, each coordinate
-    jittered by U(-jitter, +jitter) * spacing; ZB05 with h = 4 s, hash cell = h;
-    scalar grid grid_res^3 with cell 0.5 s at the padded spatial min."""
+    jittered by U(-jitter, +jitter) * spacing; h = 4 s, hash cell = h * scale;
+    scalar grid grid_res^3 with cell 0.5 s (or `cell`) at the padded spatial min."""
     rng = SplitMix64(seed)
     p = lattice(n_side, n_side, n_side, spacing)
     u = rng.uniform(3 * p.shape[0]).reshape(-1, 3)
     p = p + (2.0 * u - 1.0) * (jitter * spacing)
-    return _finish("lattice%d" % n_side, p, spacing, ("cubic_cell", (grid_res, 0.5 * spacing)),
-                   iso, method, seed)
+    c = 0.5 * spacing if cell is None else cell
+    return _finish("lattice%d" % n_side, p, spacing, ("cubic_cell", (grid_res, c)), iso, method,
+                   seed, scale)
+
+
+def yu13_params(fr: Frame, config_mode: bool = True) -> dict:
+    """Anisotropic constants: the configuration values (lambda 0.9, kn 0.5,
+    kr 4, Ne 25) or the reference's (lambda 0.5, kn 1, kr 4, Ne 25,
+    GPU/ReconstructorGPUOursYu13.cpp:78-99 and ReconstructionCUDA.cu:772,796)."""
+    if config_mode:
+        return dict(lam=0.9, kn=0.5, kr=4.0, ks=1400.0, ne=25)
+    return dict(lam=0.5, kn=1.0, kr=4.0, ks=1.0, ne=25)
 
 
 def sphere_rejection(rng: SplitMix64, center, radius: float, count: int) -> np.ndarray:

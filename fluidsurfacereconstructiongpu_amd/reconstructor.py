@@ -98,21 +98,37 @@ def stats_dict(t) -> dict:
 class Reconstructor:
     """One method (zb05 / akinci12 / yu13) over frames that live on the GPU."""
 
-    def __init__(self, method: str = "zb05", device: int = 0, ctx: Context | None = None):
+    def __init__(self, method: str = "zb05", device: int = 0, ctx: Context | None = None,
+                 aniso: dict | None = None):
+        """aniso (Yu13 only): dict(lam, kn, kr, ks, ne); default the reference's
+        constants (lambda 0.5, kn 1, kr 4, Ne 25)."""
         import torch
 
         self.torch = torch
         self.method = method
         self.ctx = ctx or Context(device)
         self.device = torch.device("cuda", device)
+        self.aniso = dict(lam=0.5, kn=1.0, kr=4.0, ks=1.0, ne=25)
+        if aniso:
+            self.aniso.update(aniso)
         self._phi = None
         self._tri = None
         self._nrm = None
         self._stats = torch.zeros(_lib.STATS_WORDS, dtype=torch.int32, device=self.device)
 
     def params(self, fr: Frame, **kw) -> _lib.SimParam:
+        if self.method == "yu13":
+            kw.setdefault("ratio", 3)
+            kw.setdefault("scale", 1.3)
+            kw.setdefault("lam", self.aniso["lam"])
+            kw.setdefault("min_neighbors", self.aniso["ne"])
         ratio = kw.pop("ratio", 2)
         return sim_param(sim_param_fields(fr, ratio=ratio, **kw))
+
+    def aniso_param(self) -> _lib.AnisoParam:
+        a = _lib.AnisoParam()
+        a.kr, a.ks, a.kn = self.aniso["kr"], self.aniso["ks"], self.aniso["kn"]
+        return a
 
     def _ensure(self, V: int, cap: int):
         torch = self.torch
@@ -129,10 +145,12 @@ class Reconstructor:
         V = fr.sc.count
         self._ensure(V, max(1, tri_capacity))
         sim = sim or self.params(fr)
+        ap = self.aniso_param()
         spg, scg = grid_info(fr.sp), grid_info(fr.sc)
         s = stream if stream is not None else torch.cuda.current_stream(self.device)
         st = self.ctx.lib.fsr_reconstruct_frame(
-            self.ctx.handle, METHODS[self.method], ctypes.byref(sim), None, ctypes.byref(spg),
+            self.ctx.handle, METHODS[self.method], ctypes.byref(sim), ctypes.byref(ap),
+            ctypes.byref(spg),
             ctypes.byref(scg), ctypes.c_void_p(pos.data_ptr()), ctypes.c_void_p(dens.data_ptr()),
             pos.shape[0], ctypes.c_void_p(self._phi.data_ptr()),
             ctypes.c_void_p(self._tri.data_ptr()), ctypes.c_void_p(self._nrm.data_ptr()),

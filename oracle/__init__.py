@@ -69,8 +69,83 @@ def load():
                               ctypes.c_float, ctypes.c_void_p, u64p]
     lib.orc_mc_tri_packed.argtypes = [ctypes.c_uint32]
     lib.orc_mc_tri_packed.restype = ctypes.c_uint64
+    lib.orc_smooth.argtypes = [f32p, ctypes.c_uint32, u32p, OGrid, ctypes.c_float, ctypes.c_float,
+                               f64p, f64p]
+    lib.orc_eigen3.argtypes = [f64p, f64p, f64p]
+    lib.orc_aniso_matrices.argtypes = [f32p, f64p, f32p, ctypes.c_uint32, u32p, u32p, OGrid,
+                                       ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                       ctypes.c_float, ctypes.c_uint32, f64p, f64p, u32p, u32p]
+    lib.orc_field_aniso.argtypes = [u32p, ctypes.c_uint32, f64p, f32p, u32p, u32p, u32p, f64p,
+                                    OGrid, OGrid, ctypes.c_float, ctypes.c_void_p, f64p,
+                                    ctypes.c_void_p]
     _lib = lib
     return lib
+
+
+def smooth(pos, ranges, sp, radius, lam):
+    lib = load()
+    n = pos.shape[0]
+    mean = np.empty((n, 3), np.float64)
+    sm = np.empty((n, 3), np.float64)
+    lib.orc_smooth(np.ascontiguousarray(pos, np.float32).ravel(), n,
+                   np.ascontiguousarray(ranges).ravel(), ogrid(sp), radius, lam, mean.ravel(),
+                   sm.ravel())
+    return mean, sm
+
+
+def eigen3(a):
+    lib = load()
+    d = np.empty(3, np.float64)
+    v = np.empty(9, np.float64)
+    lib.orc_eigen3(np.ascontiguousarray(a, np.float64).ravel(), d, v)
+    return d, v.reshape(3, 3)
+
+
+def involved_list(keys, ranges, involve):
+    """Sorted ids of the particles of involved cells, in cell order, and the
+    per-cell offsets (the reference's involve scan)."""
+    off = np.zeros(involve.shape[0], np.uint32)
+    off[1:] = np.cumsum(involve[:-1], dtype=np.uint64).astype(np.uint32)
+    C = involve.shape[0]
+    valid = keys < C
+    sel = np.zeros(keys.shape[0], bool)
+    sel[valid] = involve[keys[valid]] > 0
+    return np.nonzero(sel)[0].astype(np.uint32), off
+
+
+def aniso_matrices(pos, mean, dens, index, ranges, sp, h, radius, kr=4.0, kn=1.0, ne=25):
+    lib = load()
+    I = index.shape[0]
+    G = np.empty((I, 3, 3), np.float64)
+    sig = np.empty((I, 3), np.float64)
+    lo = np.empty(I, np.uint32)
+    hi = np.empty(I, np.uint32)
+    lib.orc_aniso_matrices(np.ascontiguousarray(pos, np.float32).ravel(),
+                           np.ascontiguousarray(mean, np.float64).ravel(),
+                           np.ascontiguousarray(dens, np.float32), I,
+                           np.ascontiguousarray(index, np.uint32),
+                           np.ascontiguousarray(ranges).ravel(), ogrid(sp), h, radius, kr, kn,
+                           ne, G.ravel(), sig.ravel(), lo, hi)
+    return G, sig, lo, hi
+
+
+def field_aniso(index, smooth_pos, dens, ranges, involve, involve_off, G, sp, sc, mass,
+                ambiguous=None):
+    lib = load()
+    B = index.shape[0]
+    phi = np.full(sc.count, np.frombuffer(np.uint32(0x64646464).tobytes(), np.float32)[0],
+                  np.float64)
+    flag = np.zeros(B, np.uint8)
+    amb = None if ambiguous is None else np.ascontiguousarray(ambiguous, np.uint8)
+    lib.orc_field_aniso(np.ascontiguousarray(index, np.uint32), B,
+                        np.ascontiguousarray(smooth_pos, np.float64).ravel(),
+                        np.ascontiguousarray(dens, np.float32), np.ascontiguousarray(ranges).ravel(),
+                        np.ascontiguousarray(involve, np.uint32),
+                        np.ascontiguousarray(involve_off, np.uint32),
+                        np.ascontiguousarray(G, np.float64).ravel(), ogrid(sp), ogrid(sc), mass,
+                        amb.ctypes.data if amb is not None else None, phi,
+                        flag.ctypes.data)
+    return phi, flag
 
 
 def threads() -> int:

@@ -379,6 +379,234 @@ uint64_t orc_mc(const uint32_t* band_index, uint32_t B, const uint8_t* band, con
     return total;
 }
 
+// ------------------------------------------------------------- Yu13 ---------
+
+// calculationOfMeanAndSmoothParticles (ReconstructionCUDA.cu:598-658) with wij
+// (ReconstructionCUDA.cuh:282-292), FP64: clamped 5^3 block of the particle's
+// cell, w = 1 - (d/r)^3 for d < r; mean = sum w p / sum w; smoothed =
+// (1 - lambda) x + lambda mean.
+int orc_smooth(const float* pos, uint32_t n, const uint32_t* ranges, orc_grid sp, float radius,
+               float lambda, double* mean, double* smooth) {
+    const int rx = sp.rx, ry = sp.ry, rz = sp.rz;
+    const double r = radius, lam = lambda;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+        const int cx = trunc_q(cell_q(px, sp.minx, sp.cell), rx);
+        const int cy = trunc_q(cell_q(py, sp.miny, sp.cell), ry);
+        const int cz = trunc_q(cell_q(pz, sp.minz, sp.cell), rz);
+        double sw = 0, sx = 0, sy = 0, sz = 0;
+        for (int zz = clampi(cz - 2, 0, rz - 1); zz <= clampi(cz + 2, 0, rz - 1); ++zz)
+            for (int yy = clampi(cy - 2, 0, ry - 1); yy <= clampi(cy + 2, 0, ry - 1); ++yy)
+                for (int xx = clampi(cx - 2, 0, rx - 1); xx <= clampi(cx + 2, 0, rx - 1); ++xx) {
+                    const uint64_t c = ((uint64_t)zz * ry + yy) * rx + xx;
+                    if (ranges[2 * c] == 0xFFFFFFFFu) continue;
+                    for (uint32_t j = ranges[2 * c]; j < ranges[2 * c + 1]; ++j) {
+                        const double qx = pos[3 * j], qy = pos[3 * j + 1], qz = pos[3 * j + 2];
+                        const double dx = px - qx, dy = py - qy, dz = pz - qz;
+                        const double d = std::sqrt(dx * dx + dy * dy + dz * dz);
+                        if (d >= r) continue;
+                        const double s = d / r;
+                        const double w = 1.0 - s * s * s;
+                        sw += w;
+                        sx += w * qx;
+                        sy += w * qy;
+                        sz += w * qz;
+                    }
+                }
+        const double m[3] = {sx / sw, sy / sw, sz / sw};
+        const double x[3] = {px, py, pz};
+        for (int a = 0; a < 3; ++a) {
+            mean[3 * i + a] = m[a];
+            smooth[3 * i + a] = (1.0 - lam) * x[a] + lam * m[a];
+        }
+    }
+    return 0;
+}
+
+// Cyclic Jacobi eigen-decomposition of a symmetric 3x3 matrix in FP64
+// (Golub & Van Loan, Matrix Computations, sec. 8.5), iterated to convergence.
+// A is overwritten; d = eigenvalues, V columns = eigenvectors.
+static void jacobi_eigen3(double A[3][3], double d[3], double V[3][3]) {
+    for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q) V[p][q] = p == q ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 50; ++sweep) {
+        const double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+        const double dia = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
+        if (off <= 1e-34 * dia) break;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                if (A[p][q] == 0.0) continue;
+                const double theta = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < 3; ++k) {  // A <- A J (columns p, q)
+                    const double akp = A[k][p], akq = A[k][q];
+                    A[k][p] = c * akp - s * akq;
+                    A[k][q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 3; ++k) {  // A <- J^T A (rows p, q)
+                    const double apk = A[p][k], aqk = A[q][k];
+                    A[p][k] = c * apk - s * aqk;
+                    A[q][k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    const double vkp = V[k][p], vkq = V[k][q];
+                    V[k][p] = c * vkp - s * vkq;
+                    V[k][q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    for (int a = 0; a < 3; ++a) d[a] = A[a][a];
+}
+
+int orc_eigen3(const double* a, double* d, double* v) {
+    double A[3][3], V[3][3];
+    for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q) A[p][q] = a[3 * p + q];
+    jacobi_eigen3(A, d, V);
+    for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q) v[3 * p + q] = V[p][q];
+    return 0;
+}
+
+// calculationOfTransformMatricesForParticles (ReconstructionCUDA.cu:689-827) in
+// FP64 for the involved particles `index[I]` (sorted ids): G (9 per particle,
+// row-major), singular values descending (3), and the neighbour count bounds
+// count_lo/hi = #{|v| <= r (1 -/+ 1e-5)}, which flag particles whose FP32 count
+// could fall on the other side of Ne.
+int orc_aniso_matrices(const float* pos, const double* mean, const float* dens, uint32_t I,
+                       const uint32_t* index, const uint32_t* ranges, orc_grid sp, float h,
+                       float radius, float kr, float kn, uint32_t min_neighbors, double* G,
+                       double* sigma, uint32_t* count_lo, uint32_t* count_hi) {
+    (void)dens;
+    const int rx = sp.rx, ry = sp.ry, rz = sp.rz;
+    const double r = radius, H = h;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t k = 0; k < (int64_t)I; ++k) {
+        const uint32_t i = index[k];
+        const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+        const int cx = trunc_q(cell_q(px, sp.minx, sp.cell), rx);
+        const int cy = trunc_q(cell_q(py, sp.miny, sp.cell), ry);
+        const int cz = trunc_q(cell_q(pz, sp.minz, sp.cell), rz);
+        const double m[3] = {mean[3 * i], mean[3 * i + 1], mean[3 * i + 2]};
+        double C[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        double sw = 0;
+        uint32_t cnt = 0, lo = 0, hi = 0;
+        for (int zz = clampi(cz - 2, 0, rz - 1); zz <= clampi(cz + 2, 0, rz - 1); ++zz)
+            for (int yy = clampi(cy - 2, 0, ry - 1); yy <= clampi(cy + 2, 0, ry - 1); ++yy)
+                for (int xx = clampi(cx - 2, 0, rx - 1); xx <= clampi(cx + 2, 0, rx - 1); ++xx) {
+                    const uint64_t c = ((uint64_t)zz * ry + yy) * rx + xx;
+                    if (ranges[2 * c] == 0xFFFFFFFFu) continue;
+                    for (uint32_t j = ranges[2 * c]; j < ranges[2 * c + 1]; ++j) {
+                        const double v[3] = {pos[3 * j] - m[0], pos[3 * j + 1] - m[1],
+                                             pos[3 * j + 2] - m[2]};
+                        const double d = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                        cnt += d <= r;
+                        lo += d <= r * (1.0 - 1e-5);
+                        hi += d <= r * (1.0 + 1e-5);
+                        if (d >= r) continue;
+                        const double s = d / r;
+                        const double w = 1.0 - s * s * s;
+                        sw += w;
+                        for (int p = 0; p < 3; ++p)
+                            for (int q = 0; q < 3; ++q) C[p][q] += w * v[p] * v[q];
+                    }
+                }
+        count_lo[k] = lo;
+        count_hi[k] = hi;
+        double g[3][3];
+        if (cnt < min_neighbors) {
+            for (int p = 0; p < 3; ++p)
+                for (int q = 0; q < 3; ++q) g[p][q] = p == q ? 1.0 / ((double)kn * H) : 0.0;
+            sigma[3 * k] = sigma[3 * k + 1] = sigma[3 * k + 2] = 0.0;
+        } else {
+            for (int p = 0; p < 3; ++p) {
+                C[p][p] += (double)(float)(h * h);
+                for (int q = 0; q < 3; ++q) C[p][q] /= sw;
+            }
+            double d[3], V[3][3];
+            jacobi_eigen3(C, d, V);
+            double s[3] = {std::fabs(d[0]), std::fabs(d[1]), std::fabs(d[2])};
+            double srt[3] = {s[0], s[1], s[2]};
+            std::sort(srt, srt + 3);
+            sigma[3 * k] = srt[2];
+            sigma[3 * k + 1] = srt[1];
+            sigma[3 * k + 2] = srt[0];
+            const double fl = srt[2] / kr;
+            for (int a = 0; a < 3; ++a) s[a] = std::max(s[a], fl);
+            const double scale = std::cbrt(s[0] * s[1] * s[2]) / H;
+            for (int p = 0; p < 3; ++p)
+                for (int q = 0; q < 3; ++q) {
+                    double acc = 0;
+                    for (int a = 0; a < 3; ++a) acc += V[p][a] * (scale / s[a]) * V[q][a];
+                    g[p][q] = acc;
+                }
+        }
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q) G[9 * k + 3 * p + q] = g[p][q];
+    }
+    return 0;
+}
+
+// computationOfScalarFieldGrid (ReconstructionCUDA.cu:830-906) with
+// anisotropicW / anisotropicWOpt (ReconstructionCUDA.cuh:295-322), FP64.
+// G is indexed by involve_off[c] + (j - start_c) for involved cells.
+// ambiguous[N] (may be NULL): vertices that receive weight from a flagged
+// particle get flag_out[b] = 1.
+int orc_field_aniso(const uint32_t* band_index, uint32_t B, const double* smooth,
+                    const float* dens, const uint32_t* ranges, const uint32_t* involve,
+                    const uint32_t* involve_off, const double* G, orc_grid sp, orc_grid sc,
+                    float mass, const uint8_t* ambiguous, double* phi, uint8_t* flag_out) {
+    const int rx = sp.rx, ry = sp.ry, rz = sp.rz;
+    const double sigma = 315.0 / (64.0 * M_PI);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t b = 0; b < (int64_t)B; ++b) {
+        const uint32_t vid = band_index[b];
+        const uint32_t ix = vid % sc.rx, iy = (vid / sc.rx) % sc.ry, iz = vid / (sc.rx * sc.ry);
+        const float vxf = vpos(ix, sc.minx, sc.cell), vyf = vpos(iy, sc.miny, sc.cell),
+                    vzf = vpos(iz, sc.minz, sc.cell);
+        const int cx = trunc_q(cell_q(vxf, sp.minx, sp.cell), rx);
+        const int cy = trunc_q(cell_q(vyf, sp.miny, sp.cell), ry);
+        const int cz = trunc_q(cell_q(vzf, sp.minz, sp.cell), rz);
+        const double v[3] = {vxf, vyf, vzf};
+        double sum = 0;
+        uint8_t flag = 0;
+        for (int zz = clampi(cz - 2, 0, rz - 1); zz <= clampi(cz + 2, 0, rz - 1); ++zz)
+            for (int yy = clampi(cy - 2, 0, ry - 1); yy <= clampi(cy + 2, 0, ry - 1); ++yy)
+                for (int xx = clampi(cx - 2, 0, rx - 1); xx <= clampi(cx + 2, 0, rx - 1); ++xx) {
+                    const uint64_t c = ((uint64_t)zz * ry + yy) * rx + xx;
+                    if (ranges[2 * c] == 0xFFFFFFFFu) continue;
+                    const uint32_t s0 = ranges[2 * c], e0 = ranges[2 * c + 1];
+                    for (uint32_t j = s0; j < e0; ++j) {
+                        const double r[3] = {smooth[3 * j] - v[0], smooth[3 * j + 1] - v[1],
+                                             smooth[3 * j + 2] - v[2]};
+                        const double mr = (double)mass / (double)dens[j];
+                        double d2, det = 1.0;
+                        if (involve[c] > 0) {
+                            const double* g = G + 9ull * (involve_off[c] + (j - s0));
+                            double t[3];
+                            for (int p = 0; p < 3; ++p)
+                                t[p] = g[3 * p] * r[0] + g[3 * p + 1] * r[1] + g[3 * p + 2] * r[2];
+                            d2 = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+                            det = g[0] * (g[4] * g[8] - g[5] * g[7]) -
+                                  g[1] * (g[3] * g[8] - g[5] * g[6]) +
+                                  g[2] * (g[3] * g[7] - g[4] * g[6]);
+                        } else {
+                            d2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+                        }
+                        if (d2 >= 1.0) continue;
+                        const double x = 1.0 - d2;
+                        sum += mr * sigma * det * x * x * x;
+                        if (ambiguous && ambiguous[j]) flag = 1;
+                    }
+                }
+        phi[vid] = sum - 0.5;
+        if (flag_out) flag_out[b] = flag;
+    }
+    return 0;
+}
+
 // One frame of the ZB05 (method 0) or Akinci12 (method 1) reconstructor
 // (GPU/ReconstructorGPUOursZB05.cpp:22-78, GPU/ReconstructorGPUAkinci12.cpp:34-84).
 int orc_frame(int method, float* pos, float* dens, uint32_t n, orc_grid sp, orc_grid sc, float h,

@@ -67,6 +67,23 @@ uint64_t orc_mc(const uint32_t* band_index, uint32_t B, const uint8_t* band,
                 const double* phi, orc_grid sc, float iso, int cpu, uint32_t* nverts,
                 double* tri_pos, double* tri_nrm, uint64_t capacity);
 
+/* Yu13: smoothing (ReconstructionCUDA.cu:598-658); mean/smooth: 3 doubles each. */
+int orc_smooth(const float* pos, uint32_t n, const uint32_t* ranges, orc_grid sp, float radius,
+               float lambda, double* mean, double* smooth);
+/* FP64 cyclic Jacobi eigen-decomposition of a symmetric 3x3 (row-major a[9]);
+ * d[3] eigenvalues, v[9] eigenvectors as columns. */
+int orc_eigen3(const double* a, double* d, double* v);
+/* Yu13: covariance -> G (ReconstructionCUDA.cu:689-827) for involved particles. */
+int orc_aniso_matrices(const float* pos, const double* mean, const float* dens, uint32_t I,
+                       const uint32_t* index, const uint32_t* ranges, orc_grid sp, float h,
+                       float radius, float kr, float kn, uint32_t min_neighbors, double* G,
+                       double* sigma, uint32_t* count_lo, uint32_t* count_hi);
+/* Yu13: anisotropic field (ReconstructionCUDA.cu:830-906). */
+int orc_field_aniso(const uint32_t* band_index, uint32_t B, const double* smooth,
+                    const float* dens, const uint32_t* ranges, const uint32_t* involve,
+                    const uint32_t* involve_off, const double* G, orc_grid sp, orc_grid sc,
+                    float mass, const uint8_t* ambiguous, double* phi, uint8_t* flag_out);
+
 /* Whole frame (method 0 = ZB05 occupancy band, 1 = Akinci12 colour-field band),
  * as the reconstructors run it.  Positions and densities are sorted in place.
  * phi_out (float[V], may be NULL).  counts[6] = {surface particles, B, Q, 3T,

@@ -58,6 +58,34 @@ class Stage:
                                                 np.float32(fr.particle_radius))
 
 
+class StageYu13:
+    """Oracle results of every stage of one anisotropic (Yu13) frame."""
+
+    def __init__(self, fr, aniso):
+        self.fr = fr
+        f = np.float32
+        h = f(fr.h)
+        r = f(h * f(2.0))
+        self.hs = hs = oracle.hash_sort(fr.pos, fr.dens, fr.sp)
+        self.surf_cell, self.flags, self.involve = oracle.surface_occupancy(
+            hs["occ"], hs["ranges"], fr.sp, hs["keys"], involve=True)
+        self.band = oracle.band(hs["pos"], self.flags, fr.sc, h)
+        self.index = oracle.compact(self.band)
+        self.mean, self.smooth = oracle.smooth(hs["pos"], hs["ranges"], fr.sp, r,
+                                               f(aniso["lam"]))
+        self.inv_idx, self.inv_off = oracle.involved_list(hs["keys"], hs["ranges"], self.involve)
+        self.G, self.sigma, lo, hi = oracle.aniso_matrices(
+            hs["pos"], self.mean, hs["dens"], self.inv_idx, hs["ranges"], fr.sp, h, r,
+            aniso["kr"], aniso["kn"], aniso["ne"])
+        ne = aniso["ne"]
+        self.ambiguous_inv = (lo < ne) & (hi >= ne)
+        amb = np.zeros(fr.n, np.uint8)
+        amb[self.inv_idx[self.ambiguous_inv]] = 1
+        self.phi, self.flag = oracle.field_aniso(self.index, self.smooth, hs["dens"],
+                                                 hs["ranges"], self.involve, self.inv_off,
+                                                 self.G, fr.sp, fr.sc, f(fr.mass), amb)
+
+
 def gpu_buffers(rec, fr, st):
     n, C = fr.n, fr.sp.count
     rx, ry, rz = fr.sc.res

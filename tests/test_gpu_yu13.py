@@ -1,0 +1,117 @@
+"""GPU parity of the anisotropic (Yu13 / TC01) frame against the FP64 oracle.
+
+Tolerances: smoothed and mean positions |d| <= 1e-5 h; covariance singular
+values |d sigma_i| <= 1e-5 sigma_max; G matrices |dG|_F <= 1e-4 |G|_F; field
+|d phi| <= 1e-5 max(|phi|, 1).  Particles whose neighbour count lies within
+1e-5 r of the Ne threshold (the FP32 count may fall either side) are exempt,
+with the band vertices they weigh on."""
+import numpy as np
+import pytest
+
+import oracle
+from fluidsurfacereconstructiongpu_amd import datasets as ds
+from fluidsurfacereconstructiongpu_amd.reconstructor import Reconstructor
+from tests import parity
+
+pytestmark = pytest.mark.gpu
+
+
+def _frames():
+    return {
+        "lattice16": lambda: ds.jittered_lattice(n_side=16, grid_res=72, scale=1.3, iso=0.0,
+                                                 method="yu13", cell=0.004),
+        # large enough for a hash cell more than 2 cells from every surface cell
+        # (a non-involved cell, whose particles use the unscaled kernel)
+        "lattice36": lambda: ds.jittered_lattice(n_side=36, grid_res=96, scale=1.3, iso=0.0,
+                                                 method="yu13", cell=0.0055),
+        "dambreak_mini": lambda: ds.dam_break(n_target=30_000, block=(50, 25, 25),
+                                              sphere_center=(0.125, 0.25, 0.0625),
+                                              sphere_radius=0.035, grid_res=112, scale=1.3,
+                                              iso=0.0, method="yu13"),
+    }
+
+
+CASES = [("lattice16", True), ("lattice16", False), ("lattice36", True), ("dambreak_mini", True)]
+
+
+@pytest.fixture(scope="module", params=CASES,
+                ids=["%s-%s" % (c[0], "config" if c[1] else "reference") for c in CASES])
+def case(request, torch_cuda):
+    name, config_mode = request.param
+    print(f"\n[gpu case] yu13 {name} config_mode={config_mode}", flush=True)
+    fr = _frames()[name]()
+    aniso = ds.yu13_params(fr, config_mode)
+    rec = Reconstructor("yu13", aniso=aniso)
+    res = rec.run(fr)
+    st = res.stats
+    n, C = fr.n, fr.sp.count
+    I = st["num_involved_particles"]
+    buf = parity.gpu_buffers(rec, fr, st)
+    buf["involve"] = rec.buffer("involve_counts", np.uint32, C)
+    buf["inv_idx"] = rec.buffer("involved_index", np.uint32, I)
+    buf["mean"] = rec.buffer("mean_pos", np.float32, 4 * n).reshape(n, 4)[:, :3]
+    buf["smooth"] = rec.buffer("smooth_pos", np.float32, 4 * n).reshape(n, 4)[:, :3]
+    buf["G"] = rec.buffer("g_matrices", np.float32, 9 * I).reshape(I, 3, 3)
+    buf["sigma"] = rec.buffer("singular", np.float32, 4 * I).reshape(I, 4)[:, :3]
+    T = st["num_mesh_vertices"]
+    ref = parity.StageYu13(fr, aniso)
+    return dict(fr=fr, st=st, buf=buf, phi=res.phi.cpu().numpy(),
+                pos=res.tri_pos[:T].cpu().numpy(), nrm=res.tri_nrm[:T].cpu().numpy(), ref=ref,
+                aniso=aniso)
+
+
+def test_discrete_stages_bit_exact(case):
+    buf, ref = case["buf"], case["ref"]
+    assert np.array_equal(buf["keys"], ref.hs["keys"])
+    assert np.array_equal(buf["ranges"], ref.hs["ranges"])
+    assert np.array_equal(buf["flags"], ref.flags)
+    assert np.array_equal(buf["involve"], ref.involve)
+    assert np.array_equal(buf["inv_idx"], ref.inv_idx)
+    assert np.array_equal(buf["band"], ref.band)
+    assert np.array_equal(buf["index"], ref.index)
+    assert case["st"]["num_involved_particles"] == ref.inv_idx.shape[0]
+
+
+def test_smoothing(case):
+    buf, ref, fr = case["buf"], case["ref"], case["fr"]
+    tol = 1e-5 * fr.h
+    assert np.max(np.abs(buf["mean"] - ref.mean)) <= tol
+    assert np.max(np.abs(buf["smooth"] - ref.smooth)) <= tol
+
+
+def test_singular_values_and_G(case):
+    buf, ref = case["buf"], case["ref"]
+    ok = ~ref.ambiguous_inv
+    s_gpu, s_ref = buf["sigma"][ok].astype(np.float64), ref.sigma[ok]
+    smax = np.maximum(s_ref[:, :1], 1e-30)
+    err = np.abs(s_gpu - s_ref) / smax
+    print(f"sigma: {ok.sum()} particles, max rel err {err.max():.2e}, "
+          f"{int(ref.ambiguous_inv.sum())} exempt at the Ne threshold")
+    assert err.max() <= 1e-5
+    G_gpu, G_ref = buf["G"][ok].astype(np.float64), ref.G[ok]
+    gerr = np.linalg.norm(G_gpu - G_ref, axis=(1, 2)) / np.linalg.norm(G_ref, axis=(1, 2))
+    assert gerr.max() <= 1e-4
+
+
+def test_field(case):
+    ref, fr = case["ref"], case["fr"]
+    phi = case["phi"].astype(np.float64)
+    idx = ref.index
+    keep = ref.flag == 0
+    tol = 1e-5 * np.maximum(np.abs(ref.phi[idx]), 1.0)
+    bad = (np.abs(phi[idx] - ref.phi[idx]) > tol) & keep
+    print(f"field: {idx.shape[0]} band vertices, {int((~keep).sum())} exempt")
+    assert not bad.any(), np.max(np.abs(phi[idx] - ref.phi[idx])[keep])
+    nonband = np.ones(fr.sc.count, bool)
+    nonband[idx] = False
+    assert np.all(case["phi"].view(np.uint32)[nonband] == 0x64646464)
+
+
+def test_mc_given_gpu_field(case):
+    fr, buf = case["fr"], case["buf"]
+    nv, pos, nrm = oracle.mc(buf["index"], buf["band"], case["phi"].astype(np.float64), fr.sc,
+                             np.float32(fr.iso))
+    assert np.array_equal(nv, buf["nverts"])
+    assert case["st"]["num_mesh_vertices"] == pos.shape[0] > 0
+    vbad, nbad, worst = parity.compare_mesh(fr, case["pos"], case["nrm"], pos, nrm)
+    assert vbad == 0 and nbad == 0
