@@ -141,6 +141,33 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.fsr_reconstruct_frame.restype = st
     lib.fsr_frame_buffer.argtypes = [c_void_p, c_int, POINTER(c_void_p), POINTER(c_uint64)]
     lib.fsr_frame_buffer.restype = st
+    P = c_void_p
+    stage_sigs = {
+        "fsr_bind_textures": [P, P, P, P, P],
+        "fsr_spatial_grid_building": [P, P, P, c_uint32, P, P, P, P],
+        "fsr_extraction_of_surface_particles": [P, P, P, P, c_uint32, P, P, P],
+        "fsr_extraction_of_surface_and_involve_particles": [P, P, P, P, P, c_uint32, P, P, P],
+        "fsr_extraction_of_surface_particles_for_akinci": [P, P, P, P, c_uint32, P, P, P, P],
+        "fsr_estimation_of_surface_vertices": [P, P, P, P, P, c_uint32, P, P, P],
+        "fsr_thrust_exclusive_prefix_sum_scan": [P, P, P, c_uint32, POINTER(c_uint32), P],
+        "fsr_compact_surface_vertex": [P, P, P, P, c_uint64, P],
+        "fsr_update_scalar_grid_values_compacted": [P, P, c_uint32, P, P, c_uint32, P, P, P, P, P],
+        "fsr_detect_valid_surface_cubes": [P, P, c_uint32, P, P, P, P, P, P, P],
+        "fsr_detection_of_valid_surface_cubes_for_akinci": [P, P, c_uint32, P, P, P, P, P, P],
+        "fsr_compact_valid_surface_cubes": [P, P, P, P, c_uint32, P],
+        "fsr_generate_triangles": [P, P, P, c_uint32, P, P, P, P, P, c_uint32, P, P],
+        "fsr_calculation_of_mean_and_smooth_particles": [P, P, P, c_uint32, P, P, P, P, P],
+        "fsr_compactation_of_surface_vertices_and_particles": [P, P, P, c_uint64, P, P, P,
+                                                               c_uint64, P, P, P],
+        "fsr_calculation_of_transform_matrices_for_particles": [P, P, P, P, P, P, c_uint32, P, P,
+                                                                c_uint32, P, P],
+        "fsr_computation_of_scalar_field_grid": [P, P, c_uint32, P, P, P, P, P, c_uint32, P, P, P,
+                                                 P, P, P],
+    }
+    for name, sig in stage_sigs.items():
+        f = getattr(lib, name)
+        f.argtypes = sig
+        f.restype = st
     lib.fsr_set_profiling.argtypes = [c_void_p, c_uint32]
     lib.fsr_set_profiling.restype = st
     lib.fsr_stage_times.argtypes = [c_void_p, POINTER(c_float), c_uint32, POINTER(c_uint32)]

@@ -490,51 +490,70 @@ __global__ __launch_bounds__(256) void k_chunks_of_cells(const uint32_t* __restr
 
 }  // namespace
 
-fsr_status aniso_run(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
-    const uint32_t C = a.sp.rx * a.sp.ry * a.sp.rz;
-    const uint32_t n = a.n;
-    const int blocks = persistent_blocks(ctx, kAnisoThreads, 8);
-    // per-cell particle counts and chunk offsets of the two particle passes
-    uint32_t *cnt, *chunk_off, *chunk_cell, *cnt2, *chunk_off2, *chunk_cell2;
-    FSR_TRY(arena_typed(ctx, A_AUX0, 4ull * C, &cnt));
-    chunk_off = cnt + C;
-    cnt2 = chunk_off + C;
-    chunk_off2 = cnt2 + C;
+// Scratch of the two particle passes: per-cell counts and chunk offsets, and
+// the chunk -> cell lists.
+struct PassScratch {
+    uint32_t *cnt, *chunk_off, *cnt2, *chunk_off2, *chunk_cell, *chunk_cell2;
+};
+
+static fsr_status pass_scratch(fsr_context* ctx, uint32_t C, uint32_t n, PassScratch* ps) {
+    FSR_TRY(arena_typed(ctx, A_AUX0, 4ull * C, &ps->cnt));
+    ps->chunk_off = ps->cnt + C;
+    ps->cnt2 = ps->chunk_off + C;
+    ps->chunk_off2 = ps->cnt2 + C;
     const uint64_t max_chunks = (uint64_t)n / 64 + C + 1;
-    FSR_TRY(arena_typed(ctx, A_AUX1, 2ull * max_chunks, &chunk_cell));
-    chunk_cell2 = chunk_cell + max_chunks;
-    // K10 over all particles
+    FSR_TRY(arena_typed(ctx, A_AUX1, 2ull * max_chunks, &ps->chunk_cell));
+    ps->chunk_cell2 = ps->chunk_cell + max_chunks;
+    return FSR_OK;
+}
+
+fsr_status aniso_smooth(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
+    const uint32_t C = a.sp.rx * a.sp.ry * a.sp.rz;
+    const int blocks = persistent_blocks(ctx, kAnisoThreads, 8);
+    PassScratch ps;
+    FSR_TRY(pass_scratch(ctx, C, a.n, &ps));
     hipLaunchKernelGGL(k_cell_counts, dim3((C + 255) / 256), dim3(256), 0, s, a.ranges,
-                       (const uint32_t*)nullptr, C, cnt);
+                       (const uint32_t*)nullptr, C, ps.cnt);
     FSR_LAUNCH_CHECK("k_cell_counts");
-    FSR_TRY(scan_exclusive(ctx, SCAN_CHUNKS64, cnt, chunk_off, C, nullptr,
+    FSR_TRY(scan_exclusive(ctx, SCAN_CHUNKS64, ps.cnt, ps.chunk_off, C, nullptr,
                            &a.counters->aux[0], s));
-    hipLaunchKernelGGL(k_chunks_of_cells, dim3((C + 255) / 256), dim3(256), 0, s, cnt, chunk_off,
-                       C, chunk_cell);
+    hipLaunchKernelGGL(k_chunks_of_cells, dim3((C + 255) / 256), dim3(256), 0, s, ps.cnt,
+                       ps.chunk_off, C, ps.chunk_cell);
     FSR_LAUNCH_CHECK("k_chunks_of_cells");
-    ChunkList cl{chunk_cell, chunk_off, &a.counters->aux[0], &a.counters->aux[1]};
+    ChunkList cl{ps.chunk_cell, ps.chunk_off, &a.counters->aux[0], &a.counters->aux[1]};
     hipLaunchKernelGGL(k_smooth, dim3(blocks), dim3(kAnisoThreads), 0, s, a.sorted, a.ranges,
                        a.sp, a.aniso_radius, a.lambda, cl, a.mean, a.smooth);
     FSR_LAUNCH_CHECK("k_smooth");
-    // K8: involved list (offsets = exclusive scan of the involve counts)
+    return FSR_OK;
+}
+
+fsr_status aniso_involved(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
+    const uint32_t C = a.sp.rx * a.sp.ry * a.sp.rz;
     FSR_TRY(scan_exclusive(ctx, SCAN_U32, a.involve, a.involve_off, C, nullptr,
                            &a.st->num_involved_particles, s));
-    if (n) {
-        hipLaunchKernelGGL(k_involved, dim3((n + 255) / 256), dim3(256), 0, s, a.keys, a.ranges,
-                           a.involve, a.involve_off, a.sorted, n, C, a.mass, a.involved_idx,
+    if (a.n) {
+        hipLaunchKernelGGL(k_involved, dim3((a.n + 255) / 256), dim3(256), 0, s, a.keys, a.ranges,
+                           a.involve, a.involve_off, a.sorted, a.n, C, a.mass, a.involved_idx,
                            a.rec);
         FSR_LAUNCH_CHECK("k_involved");
     }
-    // K11 over the particles of involved cells
+    return FSR_OK;
+}
+
+fsr_status aniso_cov(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
+    const uint32_t C = a.sp.rx * a.sp.ry * a.sp.rz;
+    const int blocks = persistent_blocks(ctx, kAnisoThreads, 8);
+    PassScratch ps;
+    FSR_TRY(pass_scratch(ctx, C, a.n, &ps));
     hipLaunchKernelGGL(k_cell_counts, dim3((C + 255) / 256), dim3(256), 0, s, a.ranges,
-                       a.involve, C, cnt2);
+                       a.involve, C, ps.cnt2);
     FSR_LAUNCH_CHECK("k_cell_counts");
-    FSR_TRY(scan_exclusive(ctx, SCAN_CHUNKS64, cnt2, chunk_off2, C, nullptr,
+    FSR_TRY(scan_exclusive(ctx, SCAN_CHUNKS64, ps.cnt2, ps.chunk_off2, C, nullptr,
                            &a.counters->aux[2], s));
-    hipLaunchKernelGGL(k_chunks_of_cells, dim3((C + 255) / 256), dim3(256), 0, s, cnt2,
-                       chunk_off2, C, chunk_cell2);
+    hipLaunchKernelGGL(k_chunks_of_cells, dim3((C + 255) / 256), dim3(256), 0, s, ps.cnt2,
+                       ps.chunk_off2, C, ps.chunk_cell2);
     FSR_LAUNCH_CHECK("k_chunks_of_cells");
-    ChunkList cl2{chunk_cell2, chunk_off2, &a.counters->aux[2], &a.counters->aux[3]};
+    ChunkList cl2{ps.chunk_cell2, ps.chunk_off2, &a.counters->aux[2], &a.counters->aux[3]};
     CovArgs ca;
     ca.sorted = a.sorted;
     ca.mean = a.mean;
@@ -554,6 +573,12 @@ fsr_status aniso_run(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_covariance, dim3(blocks), dim3(kAnisoThreads), 0, s, ca, cl2);
     FSR_LAUNCH_CHECK("k_covariance");
     return FSR_OK;
+}
+
+fsr_status aniso_run(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
+    FSR_TRY(aniso_smooth(ctx, a, s));
+    FSR_TRY(aniso_involved(ctx, a, s));
+    return aniso_cov(ctx, a, s);
 }
 
 fsr_status aniso_field(fsr_context* ctx, const AnisoArgs& a, const BandCompactOut& bc,

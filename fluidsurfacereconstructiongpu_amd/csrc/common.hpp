@@ -300,6 +300,11 @@ struct McArgs {
     fsr_frame_stats* st;
 };
 fsr_status mc_run(fsr_context* ctx, const McArgs& a, hipStream_t s);
+// single steps of mc_run (classify reads a.st->num_band_vertices, emit reads
+// a.st->num_valid_cubes)
+fsr_status mc_classify(fsr_context* ctx, const McArgs& a, hipStream_t s);
+fsr_status mc_compact(fsr_context* ctx, const McArgs& a, hipStream_t s);
+fsr_status mc_emit(fsr_context* ctx, const McArgs& a, hipStream_t s);
 
 // aniso.hip -----------------------------------------------------------------
 struct AnisoArgs {
@@ -326,6 +331,11 @@ struct AnisoArgs {
     fsr_frame_stats* st;
 };
 fsr_status aniso_run(fsr_context* ctx, const AnisoArgs& a, hipStream_t s);
+// steps of aniso_run: smoothing (K10), involved list + non-involved field
+// records (K8), covariance -> G (K11)
+fsr_status aniso_smooth(fsr_context* ctx, const AnisoArgs& a, hipStream_t s);
+fsr_status aniso_involved(fsr_context* ctx, const AnisoArgs& a, hipStream_t s);
+fsr_status aniso_cov(fsr_context* ctx, const AnisoArgs& a, hipStream_t s);
 fsr_status aniso_field(fsr_context* ctx, const AnisoArgs& a, const BandCompactOut& bc,
                        float* phi, hipStream_t s);
 

@@ -105,9 +105,7 @@ __global__ __launch_bounds__(256) void k_mc_emit(McArgs a) {
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < Q; q += gridDim.x * blockDim.x) {
         const uint32_t b = a.valid_idx[q];
         const uint32_t vid = a.band_index[b];
-        const uint32_t nv = a.nverts[b];
         const uint32_t off = a.vert_off[b];
-        if ((uint64_t)off + nv > a.capacity) continue;
         const int x = (int)(vid % rx), y = (int)((vid / rx) % ry), z = (int)(vid / rxy);
         float val[8], cpx[8], cpy[8], cpz[8], cnx[8], cny[8], cnz[8];
         uint32_t f = 0;
@@ -144,6 +142,8 @@ __global__ __launch_bounds__(256) void k_mc_emit(McArgs a) {
             }
             cnx[c] = nx; cny[c] = ny; cnz[c] = nz;
         }
+        const uint32_t nv = case_vertices(f);
+        if ((uint64_t)off + nv > a.capacity) continue;
         const uint64_t tri = cMcTri[f];
         for (uint32_t k = 0; k < nv; ++k) {
             const uint32_t e = (uint32_t)(tri >> (4 * k)) & 0xFu;
@@ -194,22 +194,38 @@ __global__ __launch_bounds__(256) void k_mc_emit(McArgs a) {
 
 }  // namespace
 
-fsr_status mc_run(fsr_context* ctx, const McArgs& a, hipStream_t s) {
+fsr_status mc_classify(fsr_context* ctx, const McArgs& a, hipStream_t s) {
     const int blocks = persistent_blocks(ctx, 256, 8);
     hipLaunchKernelGGL(k_mc_classify, dim3(blocks), dim3(256), 0, s, a);
     FSR_LAUNCH_CHECK("k_mc_classify");
-    FSR_TRY(scan_exclusive(ctx, SCAN_U32, a.valid, a.valid_off, a.max_band,
-                           &a.st->num_band_vertices, &a.st->num_valid_cubes, s));
-    FSR_TRY(scan_exclusive(ctx, SCAN_U32, a.nverts, a.vert_off, a.max_band,
-                           &a.st->num_band_vertices, &a.st->num_mesh_vertices, s));
+    return FSR_OK;
+}
+
+fsr_status mc_compact(fsr_context* ctx, const McArgs& a, hipStream_t s) {
+    const int blocks = persistent_blocks(ctx, 256, 8);
     hipLaunchKernelGGL(k_mc_compact, dim3(blocks), dim3(256), 0, s, a);
     FSR_LAUNCH_CHECK("k_mc_compact");
+    return FSR_OK;
+}
+
+fsr_status mc_emit(fsr_context* ctx, const McArgs& a, hipStream_t s) {
+    const int blocks = persistent_blocks(ctx, 256, 8);
     if (a.cpu_flavor)
         hipLaunchKernelGGL(k_mc_emit<true>, dim3(blocks), dim3(256), 0, s, a);
     else
         hipLaunchKernelGGL(k_mc_emit<false>, dim3(blocks), dim3(256), 0, s, a);
     FSR_LAUNCH_CHECK("k_mc_emit");
     return FSR_OK;
+}
+
+fsr_status mc_run(fsr_context* ctx, const McArgs& a, hipStream_t s) {
+    FSR_TRY(mc_classify(ctx, a, s));
+    FSR_TRY(scan_exclusive(ctx, SCAN_U32, a.valid, a.valid_off, a.max_band,
+                           &a.st->num_band_vertices, &a.st->num_valid_cubes, s));
+    FSR_TRY(scan_exclusive(ctx, SCAN_U32, a.nverts, a.vert_off, a.max_band,
+                           &a.st->num_band_vertices, &a.st->num_mesh_vertices, s));
+    FSR_TRY(mc_compact(ctx, a, s));
+    return mc_emit(ctx, a, s);
 }
 
 }  // namespace fsr

@@ -76,6 +76,10 @@ __global__ __launch_bounds__(256) void k_cell_surface(const float* __restrict__ 
 // Band seeding for one particle (called by every lane; `surf` = its flag).
 __device__ __forceinline__ void seed_band(const SurfaceArgs& a, uint32_t i, float px, float py,
                                           float pz, bool surf) {
+    if (!a.sbits) {  // detection only (uniform across the launch)
+        if (a.st) wave_count_add(&a.st->num_surface_particles, surf ? 1u : 0u);
+        return;
+    }
     const float qx = cell_coord(px, a.sc.minx, a.sc.cell);
     const float qy = cell_coord(py, a.sc.miny, a.sc.cell);
     const float qz = cell_coord(pz, a.sc.minz, a.sc.cell);
@@ -170,7 +174,29 @@ __global__ __launch_bounds__(256) void k_colorfield(SurfaceArgs a, float mass) {
     seed_band(a, i, p.x, p.y, p.z, surf);
 }
 
+// band seeding from given flags (stage entry point of the band)
+__global__ __launch_bounds__(256) void k_seed_flags(SurfaceArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < a.n;
+    bool surf = false;
+    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (live) {
+        surf = a.flags[i] != 0;
+        p = a.sorted[i];
+    }
+    seed_band(a, i, p.x, p.y, p.z, surf);
+}
+
 }  // namespace
+
+fsr_status surface_seed_from_flags(fsr_context* ctx, const SurfaceArgs& a, hipStream_t s) {
+    (void)ctx;
+    if (a.n) {
+        hipLaunchKernelGGL(k_seed_flags, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+        FSR_LAUNCH_CHECK("k_seed_flags");
+    }
+    return FSR_OK;
+}
 
 fsr_status surface_occupancy(fsr_context* ctx, const SurfaceArgs& a, uint8_t* surf_cell,
                              uint32_t* involve, hipStream_t s) {

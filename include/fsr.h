@@ -202,6 +202,148 @@ fsr_status fsr_frame_buffer(fsr_context* ctx, fsr_buffer which,
                             void** dev_ptr, uint64_t* count);
 
 /* ------------------------------------------------------------------------- */
+/* Stage entry points: one per reference launcher                            */
+/* ------------------------------------------------------------------------- */
+/* Same buffers and meaning as the reference launcher named in each comment
+ * (ReconstructionCUDA.cuh), so a caller that drives the stages itself (the
+ * reconstructor classes) replaces each launchX(...) call by fsr_X(ctx, ...).
+ * The reference's dim3 launch geometry arguments are dropped; buffer sizes the
+ * reference read from Grid<T> are explicit.  Particles are float3[n] in the
+ * cell order produced by fsr_spatial_grid_building.  Stream-ordered and
+ * asynchronous unless marked blocking. */
+
+/* bindTextures (ReconstructionCUDA.cuh:17-22): the case tables are built in;
+ * when tri_table (device int[256*16]) is given it is read back (blocking) and
+ * must equal the built-in table. */
+fsr_status fsr_bind_textures(fsr_context* ctx, const uint32_t* edge_table,
+                             const int32_t* tri_table, const uint32_t* num_vertices_table,
+                             const uint32_t* vertex_indices_of_edge_table);
+
+/* launchSpatialGridBuilding (ReconstructionCUDA.cuh:241-248): cell id, stable
+ * sort of particles and densities in place, ranges[C] (empty = 0xFFFFFFFF) and
+ * flag_grid[C] (1.0 occupied, 0.0 empty) of the spatial grid sp. */
+fsr_status fsr_spatial_grid_building(fsr_context* ctx, float* particles, float* densities,
+                                     uint32_t num_particles, fsr_index_range* ranges,
+                                     float* flag_grid, const fsr_grid_info* sp, void* stream);
+
+/* launchExtractionOfSurfaceParticles (ReconstructionCUDA.cuh:25-32):
+ * surface_flags[n] = 1 for particles of surface cells, 0 otherwise. */
+fsr_status fsr_extraction_of_surface_particles(fsr_context* ctx, const fsr_sim_param* sim,
+                                               const float* flag_grid, uint32_t* surface_flags,
+                                               uint32_t num_particles,
+                                               const fsr_index_range* ranges,
+                                               const fsr_grid_info* sp, void* stream);
+
+/* launchExtractionOfSurfaceAndInvolveParticles (ReconstructionCUDA.cuh:134-142):
+ * as above, plus num_involve[C] = particle count of every cell in the 5^3 block
+ * of a surface cell (0 elsewhere). */
+fsr_status fsr_extraction_of_surface_and_involve_particles(
+    fsr_context* ctx, const fsr_sim_param* sim, const float* flag_grid, uint32_t* num_involve,
+    uint32_t* surface_flags, uint32_t num_particles, const fsr_index_range* ranges,
+    const fsr_grid_info* sp, void* stream);
+
+/* launchExtractionOfSurfaceParticlesForAkinci (ReconstructionCUDA.cuh:216-225):
+ * colour-field surface test (fewer than 25 neighbours in the 3^3 block or
+ * gradient norm above 3). */
+fsr_status fsr_extraction_of_surface_particles_for_akinci(
+    fsr_context* ctx, const fsr_sim_param* sim, const float* particles, const float* densities,
+    uint32_t num_particles, const fsr_index_range* ranges, const fsr_grid_info* sp,
+    uint32_t* surface_flags, void* stream);
+
+/* launchEstimationOfSurfaceVertices (ReconstructionCUDA.cuh:145-154):
+ * is_surface_grid[V] = 1 for band vertices, 0 otherwise (ranges unused). */
+fsr_status fsr_estimation_of_surface_vertices(fsr_context* ctx, const fsr_sim_param* sim,
+                                              const fsr_grid_info* sc, const float* particles,
+                                              const uint32_t* surface_flags,
+                                              uint32_t num_particles,
+                                              const fsr_index_range* ranges,
+                                              uint32_t* is_surface_grid, void* stream);
+
+/* launchThrustExclusivePrefixSumScan (ReconstructionCUDA.cuh:118-119):
+ * exclusive scan; *total (host, may be NULL) receives the sum: blocking when
+ * total is given, as the reference is. */
+fsr_status fsr_thrust_exclusive_prefix_sum_scan(fsr_context* ctx, uint32_t* output,
+                                                const uint32_t* input, uint32_t num_elements,
+                                                uint32_t* total, void* stream);
+
+/* launchCompactSurfaceVertex (ReconstructionCUDA.cuh:58-65):
+ * sv_index[scan[v]] = v for is_surface[v] == 1, v < num_vertices. */
+fsr_status fsr_compact_surface_vertex(fsr_context* ctx, uint32_t* sv_index,
+                                      const uint32_t* is_surface_scan, const uint32_t* is_surface,
+                                      uint64_t num_vertices, void* stream);
+
+/* launchUpdateScalarGridValuesCompacted (ReconstructionCUDA.cuh:68-79): ZB05
+ * field at the num_sv vertices of sv_index; vertices with zero weight keep the
+ * caller's value (the reference's caller zeroes the grid first). */
+fsr_status fsr_update_scalar_grid_values_compacted(
+    fsr_context* ctx, const uint32_t* sv_index, uint32_t num_sv, const fsr_index_range* ranges,
+    const float* particles, uint32_t num_particles, float* scalar_field, const fsr_grid_info* sp,
+    const fsr_grid_info* sc, const fsr_sim_param* sim, void* stream);
+
+/* launchDetectValidSurfaceCubes (ReconstructionCUDA.cuh:82-92): is_valid[b],
+ * num_vertices[b] of the cube at band entry b. */
+fsr_status fsr_detect_valid_surface_cubes(fsr_context* ctx, const uint32_t* sv_index,
+                                          uint32_t num_sv, const float* scalar_field,
+                                          const fsr_grid_info* sc, uint32_t* is_valid,
+                                          uint32_t* num_vertices, const uint32_t* is_surface_grid,
+                                          const fsr_sim_param* sim, void* stream);
+
+/* launchDetectionOfValidSurfaceCubesForAkinci (ReconstructionCUDA.cuh:228-237). */
+fsr_status fsr_detection_of_valid_surface_cubes_for_akinci(
+    fsr_context* ctx, const uint32_t* sv_index, uint32_t num_sv, const float* scalar_field,
+    const fsr_grid_info* sc, uint32_t* is_valid, const uint32_t* is_surface_grid,
+    const fsr_sim_param* sim, void* stream);
+
+/* launchCompactValidSurfaceCubes (ReconstructionCUDA.cuh:95-102). */
+fsr_status fsr_compact_valid_surface_cubes(fsr_context* ctx, uint32_t* valid_index,
+                                           const uint32_t* valid_scan, const uint32_t* valid,
+                                           uint32_t num_sv, void* stream);
+
+/* launchGenerateTriangles (ReconstructionCUDA.cuh:105-116): soup positions and
+ * normals (float3) at num_vertices_scan[b]; capacity = soup vertices allocated. */
+fsr_status fsr_generate_triangles(fsr_context* ctx, const uint32_t* sv_index,
+                                  const uint32_t* valid_index, uint32_t num_valid,
+                                  const fsr_grid_info* sc, const uint32_t* num_vertices_scan,
+                                  const float* scalar_field, float* positions, float* normals,
+                                  uint32_t capacity, const fsr_sim_param* sim, void* stream);
+
+/* launchCalculationOfMeanAndSmoothParticles (ReconstructionCUDA.cuh:157-166):
+ * mean and smoothed positions (float3[n]). */
+fsr_status fsr_calculation_of_mean_and_smooth_particles(fsr_context* ctx,
+                                                        const fsr_sim_param* sim,
+                                                        const float* particles,
+                                                        uint32_t num_particles, float* mean,
+                                                        float* smoothed,
+                                                        const fsr_index_range* ranges,
+                                                        const fsr_grid_info* sp, void* stream);
+
+/* launchCompactationOfSurfaceVerticesAndParticles (ReconstructionCUDA.cuh:169-180):
+ * band vertex list and involved particle list (particles of cells with
+ * num_involve > 0, in cell order). */
+fsr_status fsr_compactation_of_surface_vertices_and_particles(
+    fsr_context* ctx, const uint32_t* is_surface, const uint32_t* is_surface_scan,
+    uint64_t num_vertices, const uint32_t* num_involve, const uint32_t* num_involve_scan,
+    const fsr_index_range* ranges, uint64_t num_cells, uint32_t* sv_index,
+    uint32_t* involved_index, void* stream);
+
+/* launchCalculationOfTransformMatricesForParticles (ReconstructionCUDA.cuh:183-193):
+ * G of every listed involved particle (aniso may be NULL: reference constants). */
+fsr_status fsr_calculation_of_transform_matrices_for_particles(
+    fsr_context* ctx, const fsr_sim_param* sim, const fsr_aniso_param* aniso,
+    const fsr_grid_info* sp, const float* mean, const float* particles, uint32_t num_particles,
+    const fsr_index_range* ranges, const uint32_t* involved_index, uint32_t num_involved,
+    fsr_matrix* g, void* stream);
+
+/* launchComputationOfScalarFieldGrid (ReconstructionCUDA.cuh:196-211):
+ * anisotropic field at the band vertices (sum m/rho W - 0.5). */
+fsr_status fsr_computation_of_scalar_field_grid(
+    fsr_context* ctx, const fsr_sim_param* sim, uint32_t num_sv, const fsr_grid_info* sc,
+    const fsr_grid_info* sp, const fsr_matrix* g, const float* smoothed, const float* densities,
+    uint32_t num_particles, const fsr_index_range* ranges, const uint32_t* sv_index,
+    const uint32_t* num_involve, const uint32_t* num_involve_scan, float* scalar_field,
+    void* stream);
+
+/* ------------------------------------------------------------------------- */
 /* Stage timing (the reference's optional per-stage timers,                   */
 /* GPU/ReconstructorGPUOursZB05.cpp:26-76, as device events)                  */
 /* ------------------------------------------------------------------------- */
