@@ -89,6 +89,106 @@ def cpu_baseline(fr, method: str, budget_s: float = 25.0):
             "nproc": os.cpu_count()}
 
 
+def batched_vs_per_frame(method: str, frames: int, lanes: int, reps: int, autotune: bool):
+    """The batched workload: `frames` random-shape frames (particle counts
+    log-uniform in [2e4, 2e6], 96^3-320^3 grids), each starting from its own
+    resident unsorted copy.  Per-frame path: the frames one after the other on
+    one stream (copy + frame each).  Batched path: the same work through
+    fsr_batch_run (lanes concurrent, one graph launch per pass)."""
+    import torch
+
+    from fluidsurfacereconstructiongpu_amd import datasets as ds
+    from fluidsurfacereconstructiongpu_amd.reconstructor import Batch
+
+    kw = dict(method="yu13", iso=0.0, scale=1.3) if method == "yu13" else {}
+    t0 = time.perf_counter()
+    frs = [ds.random_shapes_frame(i, **kw) for i in range(frames)]
+    print(f"[bench] batch workload: {frames} frames, {sum(f.n for f in frs)} particles, "
+          f"generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    aniso = ds.yu13_params(frs[0], True) if method == "yu13" else None
+    batch = Batch(method, frs, lanes=lanes, aniso=aniso)
+    dev = batch.dev
+    stream = torch.cuda.current_stream(dev)
+    tuned = None
+    if autotune:
+        import ctypes
+        from fluidsurfacereconstructiongpu_amd import _lib
+
+        big = int(np.argmax([f.n for f in frs]))
+        choice = (ctypes.c_uint32 * 4)()
+        ap = batch.rec.aniso_param()
+        _lib.check(batch.ctx.lib.fsr_autotune(batch.ctx.handle, {"zb05": 0, "akinci12": 1, "yu13": 2}[method],
+                                              ctypes.byref(ap), ctypes.byref(batch.descs[big]),
+                                              5, choice, ctypes.c_void_p(stream.cuda_stream)))
+        tuned = list(choice)
+    # per-frame path on the batch's parent context: copy + frame, one stream
+    lib, ctx = batch.ctx.lib, batch.ctx.handle
+    from fluidsurfacereconstructiongpu_amd.reconstructor import METHODS
+    ap = batch.rec.aniso_param()
+    import ctypes
+
+    def per_frame_pass(events=None):
+        for i, d in enumerate(batch.descs):
+            b = batch.bufs[i]
+            if events is not None:
+                events[i][0].record(stream)
+            b["pos"].copy_(b["src_pos"])
+            b["dens"].copy_(b["src_dens"])
+            st = lib.fsr_reconstruct_frame(ctx, METHODS[method], ctypes.byref(d.sim),
+                                           ctypes.byref(ap), ctypes.byref(d.sp_grid),
+                                           ctypes.byref(d.sc_grid), d.pos, d.dens,
+                                           d.num_particles, d.phi, d.tri_pos, d.tri_nrm,
+                                           d.tri_capacity, d.stats,
+                                           ctypes.c_void_p(stream.cuda_stream))
+            if st != 0:
+                raise RuntimeError("frame %d failed: %d" % (i, st))
+            if events is not None:
+                events[i][1].record(stream)
+
+    per_frame_pass()                       # sizes the arena
+    torch.cuda.synchronize(dev)
+    if batch.grow_overflowed():
+        per_frame_pass()
+        torch.cuda.synchronize(dev)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in frs]
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for r in range(reps):
+        per_frame_pass(evs if r == reps - 1 else None)
+    e1.record(stream)
+    torch.cuda.synchronize(dev)
+    per_ms = e0.elapsed_time(e1) / reps
+    per_frame_ms = np.array([a.elapsed_time(b) for a, b in evs])
+    # batched path
+    batch.run(stream=stream)                # eager + capture
+    torch.cuda.synchronize(dev)
+    batch.run(stream=stream)                # first replay
+    torch.cuda.synchronize(dev)
+    e0.record(stream)
+    for _ in range(reps):
+        batch.run(stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize(dev)
+    bat_ms = e0.elapsed_time(e1) / reps
+    bat_frame_ms = batch.frame_ms()
+    batch.close()
+    n = len(frs)
+    return {
+        "workload": "%d random-shape frames (%s), %d particles in total, grids %d^3-%d^3"
+                    % (n, method, sum(f.n for f in frs), min(f.sc.res[0] for f in frs),
+                       max(f.sc.res[0] for f in frs)),
+        "lanes": lanes, "passes": reps, "autotuned_workgroups_per_cu": tuned,
+        "batched_frames_per_s": n / (bat_ms * 1e-3),
+        "per_frame_frames_per_s": n / (per_ms * 1e-3),
+        "speedup": per_ms / bat_ms,
+        "batched_frame_ms_p50": float(np.percentile(bat_frame_ms, 50)),
+        "batched_frame_ms_p99": float(np.percentile(bat_frame_ms, 99)),
+        "per_frame_ms_p50": float(np.percentile(per_frame_ms, 50)),
+        "per_frame_ms_p99": float(np.percentile(per_frame_ms, 99)),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +199,11 @@ def main():
     ap.add_argument("--grid", type=int, default=None,
                     help="scalar grid resolution (default 256, 384 for yu13)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched comparison")
+    ap.add_argument("--batch-frames", type=int, default=64)
+    ap.add_argument("--lanes", type=int, default=8)
+    ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus %d is not supported "
@@ -204,6 +309,12 @@ def main():
         "counts": {k: st[k] for k in ("num_surface_particles", "num_band_vertices",
                                       "num_valid_cubes", "num_mesh_vertices")},
     }
+    del pos_k, dens_k, rec
+    torch.cuda.empty_cache()
+    if not args.no_batch:
+        out["batched"] = batched_vs_per_frame(args.method, args.batch_frames, args.lanes,
+                                              args.batch_reps, args.autotune)
+        print(f"[bench] batched: {out['batched']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline and not yu13:
         out["cpu_baseline"] = cpu_baseline(make_frame(), args.method)
     else:

@@ -79,6 +79,17 @@ class FrameStats(ctypes.Structure):
 
 STATS_WORDS = ctypes.sizeof(FrameStats) // 4
 
+
+class FrameDesc(ctypes.Structure):
+    """fsr_frame_desc."""
+
+    _fields_ = [
+        ("src_pos", c_void_p), ("src_dens", c_void_p), ("pos", c_void_p), ("dens", c_void_p),
+        ("num_particles", c_uint32), ("tri_capacity", c_uint32), ("sp_grid", GridInfo),
+        ("sc_grid", GridInfo), ("sim", SimParam), ("phi", c_void_p), ("tri_pos", c_void_p),
+        ("tri_nrm", c_void_p), ("stats", c_void_p),
+    ]
+
 STAGES = ["hash", "surface", "band", "fill", "field", "mc"]
 
 METHOD_ZB05 = 0
@@ -168,6 +179,18 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         f = getattr(lib, name)
         f.argtypes = sig
         f.restype = st
+    lib.fsr_batch_create.argtypes = [c_void_p, c_uint32, POINTER(c_void_p)]
+    lib.fsr_batch_create.restype = st
+    lib.fsr_batch_destroy.argtypes = [c_void_p]
+    lib.fsr_batch_destroy.restype = st
+    lib.fsr_batch_run.argtypes = [c_void_p, c_int, POINTER(AnisoParam), POINTER(FrameDesc),
+                                  c_uint32, c_void_p, c_void_p]
+    lib.fsr_batch_run.restype = st
+    lib.fsr_autotune.argtypes = [c_void_p, c_int, POINTER(AnisoParam), POINTER(FrameDesc),
+                                 c_uint32, POINTER(c_uint32), c_void_p]
+    lib.fsr_autotune.restype = st
+    lib.fsr_set_launch_shape.argtypes = [c_void_p, POINTER(c_uint32)]
+    lib.fsr_set_launch_shape.restype = st
     lib.fsr_set_profiling.argtypes = [c_void_p, c_uint32]
     lib.fsr_set_profiling.restype = st
     lib.fsr_stage_times.argtypes = [c_void_p, POINTER(c_float), c_uint32, POINTER(c_uint32)]

@@ -275,7 +275,7 @@ fsr_status band_build(fsr_context* ctx, const BandArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_dilate_rows, dim3(blocks), dim3(256), 0, s, sb, a.bits, nwords,
                        (uint64_t)wx * a.sc.ry, a.sc.rz, (uint64_t)a.sc.rz);
     FSR_LAUNCH_CHECK("k_dilate_rows(z)");
-    const int eb = persistent_blocks(ctx, 256, 4);
+    const int eb = persistent_blocks(ctx, 256, 4, TUNE_BAND);
     hipLaunchKernelGGL(k_band_exceptions, dim3(eb), dim3(256), 0, s, a);
     FSR_LAUNCH_CHECK("k_band_exceptions");
     return FSR_OK;

@@ -23,6 +23,10 @@ fsr_status hip_fail(hipError_t e, const char* what, const char* file, int line) 
 fsr_status arena_get(fsr_context* ctx, ArenaSlot s, size_t bytes, void** out,
                      uint64_t count_for_report) {
     Arena& a = ctx->arena;
+    if (bytes > a.bytes[s] && ctx->capturing) {
+        set_error("scratch arena too small inside a graph capture (run the frames eagerly first)");
+        return FSR_ERR_CAPACITY;
+    }
     if (bytes > a.bytes[s]) {
         // grow (never shrinks); other slots may be in use by queued work
         FSR_HIP(hipDeviceSynchronize());
@@ -57,9 +61,10 @@ static void prof_release(fsr_context* ctx) {
     ctx->prof_frames = 0;
 }
 
-int persistent_blocks(fsr_context* ctx, int threads, int per_cu) {
+int persistent_blocks(fsr_context* ctx, int threads, int per_cu, TuneClass cls) {
     (void)threads;
-    return std::max(1, ctx->num_cus * per_cu);
+    const int t = ctx->tune[cls];
+    return std::max(1, ctx->num_cus * (t > 0 ? t : per_cu));
 }
 
 static fsr_status check_grid(const fsr_grid_info* g, const char* name) {

@@ -154,10 +154,18 @@ struct Counters {
 
 }  // namespace fsr
 
+namespace fsr {
+// Launch-shape classes of the persistent kernels (workgroups per CU), set by
+// the autotuner.
+enum TuneClass { TUNE_FIELD = 0, TUNE_MC = 1, TUNE_ANISO = 2, TUNE_BAND = 3, TUNE_NUM = 4 };
+}  // namespace fsr
+
 struct fsr_context {
     int device = 0;
     int num_cus = 256;
     int last_sort_passes = 0;
+    int tune[fsr::TUNE_NUM] = {0, 0, 0, 0};  // 0: kernel default
+    bool capturing = false;                  // inside a graph capture: no arena growth
     fsr::Arena arena;
     // stage profiling: ring of (FSR_NUM_STAGES + 1) events per frame
     std::vector<hipEvent_t> prof_events;
@@ -339,6 +347,11 @@ fsr_status aniso_cov(fsr_context* ctx, const AnisoArgs& a, hipStream_t s);
 fsr_status aniso_field(fsr_context* ctx, const AnisoArgs& a, const BandCompactOut& bc,
                        float* phi, hipStream_t s);
 
-int persistent_blocks(fsr_context* ctx, int threads, int per_cu);
+// Grid of a persistent kernel: CUs x workgroups per CU (the tuned value of
+// `cls` when set, else `per_cu`).
+int persistent_blocks(fsr_context* ctx, int threads, int per_cu, TuneClass cls);
+
+// util.hip: writes the 100 MHz device clock (s_memrealtime) to *out
+fsr_status device_timestamp(uint64_t* out, hipStream_t s);
 
 }  // namespace fsr

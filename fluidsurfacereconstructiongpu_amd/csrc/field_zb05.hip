@@ -156,7 +156,7 @@ __global__ __launch_bounds__(kFieldThreads) void k_field_zb05(FieldArgs a, BandC
 }  // namespace
 
 fsr_status field_zb05(fsr_context* ctx, const FieldArgs& a, hipStream_t s) {
-    const int blocks = persistent_blocks(ctx, kFieldThreads, 8);
+    const int blocks = persistent_blocks(ctx, kFieldThreads, 8, TUNE_FIELD);
     hipLaunchKernelGGL(k_field_zb05, dim3(blocks), dim3(kFieldThreads), 0, s, a, *a.bc);
     FSR_LAUNCH_CHECK("k_field_zb05");
     return FSR_OK;

@@ -195,21 +195,21 @@ __global__ __launch_bounds__(256) void k_mc_emit(McArgs a) {
 }  // namespace
 
 fsr_status mc_classify(fsr_context* ctx, const McArgs& a, hipStream_t s) {
-    const int blocks = persistent_blocks(ctx, 256, 8);
+    const int blocks = persistent_blocks(ctx, 256, 8, TUNE_MC);
     hipLaunchKernelGGL(k_mc_classify, dim3(blocks), dim3(256), 0, s, a);
     FSR_LAUNCH_CHECK("k_mc_classify");
     return FSR_OK;
 }
 
 fsr_status mc_compact(fsr_context* ctx, const McArgs& a, hipStream_t s) {
-    const int blocks = persistent_blocks(ctx, 256, 8);
+    const int blocks = persistent_blocks(ctx, 256, 8, TUNE_MC);
     hipLaunchKernelGGL(k_mc_compact, dim3(blocks), dim3(256), 0, s, a);
     FSR_LAUNCH_CHECK("k_mc_compact");
     return FSR_OK;
 }
 
 fsr_status mc_emit(fsr_context* ctx, const McArgs& a, hipStream_t s) {
-    const int blocks = persistent_blocks(ctx, 256, 8);
+    const int blocks = persistent_blocks(ctx, 256, 8, TUNE_MC);
     if (a.cpu_flavor)
         hipLaunchKernelGGL(k_mc_emit<true>, dim3(blocks), dim3(256), 0, s, a);
     else

@@ -254,6 +254,57 @@ def lattice_ball(n_target: int = 256_000, radius: float = 0.25, seed: int = BASE
     return _finish("ball%d" % n_target, p, s, ("cubic", grid_res), iso, "zb05", seed)
 
 
+def random_shapes_frame(index: int, seed: int = BASE_SEED + 3, n_min: float = 2e4,
+                        n_max: float = 2e6, res_min: int = 96, res_max: int = 320,
+                        method: str = "zb05", iso: float = -1e-4, scale: float = 1.0,
+                        n_override: int | None = None) -> Frame:
+    """One frame of the batched workload: stream seed + index; a particle count
+    N log-uniform in [n_min, n_max]; 1-4 shapes (block or ball, 50:50) at random
+    sizes and positions in the unit cube; spacing s = (sum of shape volumes /
+    N)^(1/3); every shape filled with the lattice of spacing s anchored at its
+    minimum corner (balls keep the lattice points inside), jittered by
+    U(+-0.1 s); a cubic scalar grid at 0.5 s over the padded AABB, its
+    resolution clamped to [res_min, res_max]."""
+    rng = SplitMix64(seed + index)
+    u = rng.uniform(1)[0]
+    n_target = n_override or int(round(math.exp(math.log(n_min) + u * (math.log(n_max) - math.log(n_min)))))
+    nshape = 1 + int(rng.uniform(1)[0] * 4)
+    shapes = []
+    vol = 0.0
+    for _ in range(nshape):
+        kind, a, b, c, d, e, f = rng.uniform(7)
+        if kind < 0.5:
+            size = 0.15 + 0.45 * np.array([a, b, c])
+            lo = np.array([d, e, f]) * (1.0 - size)
+            shapes.append(("block", lo, size))
+            vol += float(np.prod(size))
+        else:
+            r = 0.08 + 0.22 * a
+            cen = r + np.array([b, c, d]) * (1.0 - 2 * r)
+            shapes.append(("ball", cen, r))
+            vol += 4.0 / 3.0 * math.pi * r ** 3
+    s = (vol / n_target) ** (1.0 / 3.0)
+    pts = []
+    for kind, p, q in shapes:
+        if kind == "block":
+            n = np.maximum(1, np.floor(q / s).astype(int))
+            pts.append(lattice(n[0], n[1], n[2], s, origin=p))
+        else:
+            m = int(math.ceil(2 * q / s)) + 1
+            g = lattice(m, m, m, s, origin=p - q)
+            pts.append(g[np.sum((g - p) ** 2, axis=1) <= q * q])
+    pos = np.concatenate(pts, axis=0)
+    pos = pos + (2.0 * rng.uniform(3 * pos.shape[0]).reshape(-1, 3) - 1.0) * (0.1 * s)
+    h = 4.0 * s
+    sp = spatial_grid(pos.min(0).astype(np.float32), pos.max(0).astype(np.float32),
+                      float(np.float32(h)), scale)
+    ext = float(np.max(sp.max - sp.min))
+    res = int(min(max(math.ceil(ext / (0.5 * s)) + 1, res_min), res_max))
+    fr = _finish("shapes%02d" % index, pos, s, ("cubic", res), iso, method, seed + index, scale,
+                 meta={"target": n_target, "shapes": [k for k, _, _ in shapes]})
+    return fr
+
+
 def small_block(nx=12, ny=10, nz=8, spacing=0.01, seed=BASE_SEED + 100, ratio=2, jitter=0.1,
                 iso=-1e-4, method="zb05") -> Frame:
     """Small jittered block on the reference's own grid rule (scalar cell =

@@ -192,6 +192,90 @@ class Reconstructor:
         return out
 
 
+class Batch:
+    """Many frames through fsr_batch_run: per-frame device buffers (pristine
+    input, working copy, field, soup, stats) and one descriptor each."""
+
+    def __init__(self, method: str, frames, lanes: int = 8, device: int = 0,
+                 aniso: dict | None = None, ctx: Context | None = None, cap_factor: float = 4.0):
+        import torch
+
+        self.torch = torch
+        self.rec = Reconstructor(method, device=device, ctx=ctx, aniso=aniso)
+        self.ctx = self.rec.ctx
+        self.method = method
+        self.frames = frames
+        self.dev = self.rec.device
+        h = ctypes.c_void_p()
+        _lib.check(self.ctx.lib.fsr_batch_create(self.ctx.handle, lanes, ctypes.byref(h)))
+        self.handle = h
+        self.bufs = []
+        for fr in frames:
+            cap = int(max(3 * 1024, cap_factor * fr.n))
+            b = dict(src_pos=torch.from_numpy(fr.pos).to(self.dev),
+                     src_dens=torch.from_numpy(fr.dens).to(self.dev))
+            b["pos"] = torch.empty_like(b["src_pos"])
+            b["dens"] = torch.empty_like(b["src_dens"])
+            b["phi"] = torch.empty(fr.sc.count, dtype=torch.float32, device=self.dev)
+            b["tri"] = torch.empty((cap, 3), dtype=torch.float32, device=self.dev)
+            b["nrm"] = torch.empty((cap, 3), dtype=torch.float32, device=self.dev)
+            b["stats"] = torch.zeros(_lib.STATS_WORDS, dtype=torch.int32, device=self.dev)
+            b["cap"] = cap
+            self.bufs.append(b)
+        self.ts = torch.zeros(2 * len(frames), dtype=torch.int64, device=self.dev)
+        self._build_descs()
+
+    def _build_descs(self):
+        n = len(self.frames)
+        self.descs = (_lib.FrameDesc * n)()
+        for i, (fr, b) in enumerate(zip(self.frames, self.bufs)):
+            d = self.descs[i]
+            d.src_pos, d.src_dens = b["src_pos"].data_ptr(), b["src_dens"].data_ptr()
+            d.pos, d.dens = b["pos"].data_ptr(), b["dens"].data_ptr()
+            d.num_particles = fr.n
+            d.tri_capacity = b["cap"]
+            d.sp_grid, d.sc_grid = grid_info(fr.sp), grid_info(fr.sc)
+            d.sim = self.rec.params(fr)
+            d.phi = b["phi"].data_ptr()
+            d.tri_pos, d.tri_nrm = b["tri"].data_ptr(), b["nrm"].data_ptr()
+            d.stats = b["stats"].data_ptr()
+
+    def run(self, stream=None, timestamps: bool = True):
+        torch = self.torch
+        s = stream if stream is not None else torch.cuda.current_stream(self.dev)
+        ap = self.rec.aniso_param()
+        _lib.check(self.ctx.lib.fsr_batch_run(
+            self.handle, METHODS[self.method], ctypes.byref(ap), self.descs, len(self.frames),
+            ctypes.c_void_p(self.ts.data_ptr() if timestamps else 0),
+            ctypes.c_void_p(s.cuda_stream)))
+
+    def grow_overflowed(self) -> bool:
+        """After a synchronised run: grows the soup of frames that overflowed;
+        returns True when descriptors changed (the next run recaptures)."""
+        changed = False
+        for b in self.bufs:
+            st = stats_dict(b["stats"])
+            if st["overflow"]:
+                cap = st["num_mesh_vertices"] + 1024
+                b["tri"] = self.torch.empty((cap, 3), dtype=self.torch.float32, device=self.dev)
+                b["nrm"] = self.torch.empty((cap, 3), dtype=self.torch.float32, device=self.dev)
+                b["cap"] = cap
+                changed = True
+        if changed:
+            self._build_descs()
+        return changed
+
+    def frame_ms(self) -> np.ndarray:
+        """Per-frame device time (start/end timestamps, 100 MHz clock)."""
+        t = self.ts.cpu().numpy().reshape(-1, 2)
+        return (t[:, 1] - t[:, 0]) * 1e-5
+
+    def close(self):
+        if self.handle:
+            self.ctx.lib.fsr_batch_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+
 _hip = None
 
 

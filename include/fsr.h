@@ -202,6 +202,59 @@ fsr_status fsr_frame_buffer(fsr_context* ctx, fsr_buffer which,
                             void** dev_ptr, uint64_t* count);
 
 /* ------------------------------------------------------------------------- */
+/* Batched frames (no counterpart in the reference: it runs one frame at a    */
+/* time, GPU/ReconstructorGPU.cpp:37-69)                                      */
+/* ------------------------------------------------------------------------- */
+
+/* One frame of a batch: the arguments of fsr_reconstruct_frame.  When src_pos
+ * and src_dens are set, the frame first copies them into pos/dens (so a
+ * replayed batch starts again from unsorted particles). */
+typedef struct {
+    const float* src_pos;
+    const float* src_dens;
+    float* pos;
+    float* dens;
+    uint32_t num_particles;
+    uint32_t tri_capacity;
+    fsr_grid_info sp_grid;
+    fsr_grid_info sc_grid;
+    fsr_sim_param sim;
+    float* phi;
+    float* tri_pos;
+    float* tri_nrm;
+    fsr_frame_stats* stats;
+} fsr_frame_desc;
+
+typedef struct fsr_batch fsr_batch;
+
+/* A batch executor with `lanes` concurrent lanes, each with its own scratch
+ * arena and stream on the device of ctx. */
+fsr_status fsr_batch_create(fsr_context* ctx, uint32_t lanes, fsr_batch** out);
+fsr_status fsr_batch_destroy(fsr_batch* batch);
+
+/* Runs the frames (frame f on lane f % lanes, lanes concurrently).  The first
+ * call for a frame list runs it eagerly (sizing the lane arenas) and captures
+ * the whole batch, forked over the lanes and joined on `stream`, into one HIP
+ * graph; a later call with the same list (same descriptors) replays that graph
+ * with a single launch.  timestamps (device u64[2 * num_frames], may be NULL):
+ * start and end of every frame on the 100 MHz device clock. */
+fsr_status fsr_batch_run(fsr_batch* batch, fsr_method method, const fsr_aniso_param* aniso,
+                         const fsr_frame_desc* frames, uint32_t num_frames,
+                         uint64_t* timestamps, void* stream);
+
+/* Autotuner: runs `frame` (blocking) with each candidate launch shape of the
+ * persistent kernels (workgroups per CU in {2, 4, 8, 16} for the field, MC,
+ * anisotropic and band kernels), times each stage with HIP events (median of
+ * `repeats`), and keeps the fastest per kernel class in ctx.  choice (may be
+ * NULL) receives the chosen workgroups per CU, one per class
+ * (field, MC, anisotropic, band). */
+fsr_status fsr_autotune(fsr_context* ctx, fsr_method method, const fsr_aniso_param* aniso,
+                        const fsr_frame_desc* frame, uint32_t repeats, uint32_t* choice,
+                        void* stream);
+/* Sets the launch shape directly (0 = default), one value per class. */
+fsr_status fsr_set_launch_shape(fsr_context* ctx, const uint32_t* workgroups_per_cu);
+
+/* ------------------------------------------------------------------------- */
 /* Stage entry points: one per reference launcher                            */
 /* ------------------------------------------------------------------------- */
 /* Same buffers and meaning as the reference launcher named in each comment
