@@ -127,7 +127,7 @@ def batched_vs_per_frame(method: str, frames: int, lanes: int, reps: int, autotu
     ap = batch.rec.aniso_param()
     import ctypes
 
-    def per_frame_pass(events=None):
+    def per_frame_pass(events=None, sync=False):
         for i, d in enumerate(batch.descs):
             b = batch.bufs[i]
             if events is not None:
@@ -144,6 +144,9 @@ def batched_vs_per_frame(method: str, frames: int, lanes: int, reps: int, autotu
                 raise RuntimeError("frame %d failed: %d" % (i, st))
             if events is not None:
                 events[i][1].record(stream)
+            if sync:
+                # a per-frame consumer reads the frame's counts before using its mesh
+                int(b["stats"][6].item())
 
     per_frame_pass()                       # sizes the arena
     torch.cuda.synchronize(dev)
@@ -160,6 +163,12 @@ def batched_vs_per_frame(method: str, frames: int, lanes: int, reps: int, autotu
     torch.cuda.synchronize(dev)
     per_ms = e0.elapsed_time(e1) / reps
     per_frame_ms = np.array([a.elapsed_time(b) for a, b in evs])
+    # per-frame path with the count read back after every frame (host sync)
+    t_sync = time.perf_counter()
+    for r in range(reps):
+        per_frame_pass(sync=True)
+    torch.cuda.synchronize(dev)
+    sync_ms = (time.perf_counter() - t_sync) * 1e3 / reps
     # batched path
     batch.run(stream=stream)                # eager + capture
     torch.cuda.synchronize(dev)
@@ -180,8 +189,10 @@ def batched_vs_per_frame(method: str, frames: int, lanes: int, reps: int, autotu
                        max(f.sc.res[0] for f in frs)),
         "lanes": lanes, "passes": reps, "autotuned_workgroups_per_cu": tuned,
         "batched_frames_per_s": n / (bat_ms * 1e-3),
-        "per_frame_frames_per_s": n / (per_ms * 1e-3),
-        "speedup": per_ms / bat_ms,
+        "per_frame_sync_frames_per_s": n / (sync_ms * 1e-3),
+        "per_frame_async_frames_per_s": n / (per_ms * 1e-3),
+        "speedup_vs_per_frame_sync": sync_ms / bat_ms,
+        "speedup_vs_per_frame_async": per_ms / bat_ms,
         "batched_frame_ms_p50": float(np.percentile(bat_frame_ms, 50)),
         "batched_frame_ms_p99": float(np.percentile(bat_frame_ms, 99)),
         "per_frame_ms_p50": float(np.percentile(per_frame_ms, 50)),

@@ -48,9 +48,12 @@ def test_batch_equals_per_frame(torch_cuda, method):
     frames = small_frames(method)
     aniso = ds.yu13_params(frames[0], True) if method == "yu13" else None
     ref = per_frame(method, frames, aniso)
-    batch = Batch(method, frames, lanes=3, aniso=aniso)
+    batch = Batch(method, frames, lanes=3, aniso=aniso, cap_factor=2.0)
     batch.run()                      # eager + capture
     torch.cuda.synchronize()
+    if batch.grow_overflowed():      # a frame's soup did not fit: new descriptors, recapture
+        batch.run()
+        torch.cuda.synchronize()
     assert not batch.grow_overflowed()
     check_same(batch, ref)
     for b in batch.bufs:             # poison the outputs, replay the graph twice
