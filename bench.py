@@ -200,6 +200,79 @@ def batched_vs_per_frame(method: str, frames: int, lanes: int, reps: int, autotu
     }
 
 
+def adjoint_cost(reps: int = 20):
+    """Adjoint workload: 256,000-particle jittered lattice ball (radius 0.25),
+    192^3 grid, ZB05, sphere loss (r = 0.3) over the grid and the soup.  Device
+    time of the forward frame against the backward (vertex term + particle
+    gather), both on the same stream, inputs resident."""
+    import ctypes
+
+    import torch
+
+    from fluidsurfacereconstructiongpu_amd import _lib
+    from fluidsurfacereconstructiongpu_amd import datasets as ds
+    from fluidsurfacereconstructiongpu_amd.autograd import SurfaceField, sphere_loss
+
+    fr = ds.lattice_ball()
+    sf = SurfaceField(fr, "zb05")
+    dev = sf.device
+    src = torch.from_numpy(fr.pos).to(dev)
+    phi, soup = sf.forward_frame(src)
+    ph = phi.detach().clone().requires_grad_(True)
+    so = soup.detach().clone().requires_grad_(True)
+    sphere_loss(ph, so, fr).backward()
+    g_phi0 = ph.grad.to(torch.float32).contiguous()
+    g_soup = so.grad.to(torch.float32).contiguous()
+    stream = torch.cuda.current_stream(dev)
+    s = ctypes.c_void_p(stream.cuda_stream)
+    lib, h = sf.ctx.lib, sf.ctx.handle
+    pos_k = [src.clone() for _ in range(reps + 1)]
+    dens_k = [sf.dens.clone() for _ in range(reps + 1)]
+    g_k = [g_phi0.clone() for _ in range(2 * reps + 1)]
+    out = torch.empty((fr.n, 3), dtype=torch.float32, device=dev)
+
+    def bwd(g, mc=True):
+        if mc:
+            _lib.check(lib.fsr_mc_vertex_backward(h, ctypes.c_void_p(g_soup.data_ptr()),
+                                                  ctypes.c_void_p(g.data_ptr()), s))
+        _lib.check(lib.fsr_field_zb05_backward(h, ctypes.c_void_p(g.data_ptr()),
+                                               ctypes.c_void_p(out.data_ptr()), s))
+
+    sf.rec.launch(fr, pos_k[reps], dens_k[reps], sf.cap, sim=sf.sim, stream=stream)
+    bwd(g_k[2 * reps])
+    torch.cuda.synchronize(dev)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record(stream)
+    for i in range(reps):
+        sf.rec.launch(fr, pos_k[i], dens_k[i], sf.cap, sim=sf.sim, stream=stream)
+    ev[1].record(stream)
+    for i in range(reps):
+        bwd(g_k[i])
+    ev[2].record(stream)
+    for i in range(reps):
+        bwd(g_k[reps + i], mc=False)
+    ev[3].record(stream)
+    torch.cuda.synchronize(dev)
+    t_fwd = ev[0].elapsed_time(ev[1]) / reps
+    t_bwd = ev[1].elapsed_time(ev[2]) / reps
+    t_field = ev[2].elapsed_time(ev[3]) / reps
+    st = stats_dict_of(sf)
+    res = {"workload": "jittered lattice ball, %d particles, 192^3 grid, ZB05, sphere loss "
+                       "over grid and soup" % fr.n,
+           "t_fwd_ms": t_fwd, "t_bwd_ms": t_bwd, "t_bwd_field_ms": t_field,
+           "t_bwd_vertex_term_ms": t_bwd - t_field, "ratio_bwd_fwd": t_bwd / t_fwd,
+           "band_vertices": st["num_band_vertices"], "soup_vertices": st["num_mesh_vertices"],
+           "reps": reps}
+    sf.close()
+    return res
+
+
+def stats_dict_of(sf):
+    from fluidsurfacereconstructiongpu_amd.reconstructor import stats_dict
+
+    return stats_dict(sf.rec._stats)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +288,7 @@ def main():
     ap.add_argument("--lanes", type=int, default=8)
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus %d is not supported "
@@ -326,6 +400,9 @@ def main():
         out["batched"] = batched_vs_per_frame(args.method, args.batch_frames, args.lanes,
                                               args.batch_reps, args.autotune)
         print(f"[bench] batched: {out['batched']}", file=sys.stderr, flush=True)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_cost()
+        print(f"[bench] adjoint: {out['adjoint']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline and not yu13:
         out["cpu_baseline"] = cpu_baseline(make_frame(), args.method)
     else:

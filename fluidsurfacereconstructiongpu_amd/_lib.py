@@ -195,6 +195,12 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.fsr_set_profiling.restype = st
     lib.fsr_stage_times.argtypes = [c_void_p, POINTER(c_float), c_uint32, POINTER(c_uint32)]
     lib.fsr_stage_times.restype = st
+    lib.fsr_set_adjoint.argtypes = [c_void_p, c_int]
+    lib.fsr_set_adjoint.restype = st
+    lib.fsr_mc_vertex_backward.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]
+    lib.fsr_mc_vertex_backward.restype = st
+    lib.fsr_field_zb05_backward.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]
+    lib.fsr_field_zb05_backward.restype = st
     _lib = lib
     return lib
 

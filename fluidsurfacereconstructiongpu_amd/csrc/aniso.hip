@@ -490,6 +490,21 @@ __global__ __launch_bounds__(256) void k_chunks_of_cells(const uint32_t* __restr
 
 }  // namespace
 
+fsr_status cell_chunk_list(fsr_context* ctx, const fsr_index_range* ranges,
+                           const uint32_t* filter, uint32_t C, uint32_t* cnt,
+                           uint32_t* chunk_off, uint32_t* chunk_cell, uint32_t* d_total,
+                           hipStream_t s) {
+    (void)ctx;
+    hipLaunchKernelGGL(k_cell_counts, dim3((C + 255) / 256), dim3(256), 0, s, ranges, filter, C,
+                       cnt);
+    FSR_LAUNCH_CHECK("k_cell_counts");
+    FSR_TRY(scan_exclusive(ctx, SCAN_CHUNKS64, cnt, chunk_off, C, nullptr, d_total, s));
+    hipLaunchKernelGGL(k_chunks_of_cells, dim3((C + 255) / 256), dim3(256), 0, s, cnt, chunk_off,
+                       C, chunk_cell);
+    FSR_LAUNCH_CHECK("k_chunks_of_cells");
+    return FSR_OK;
+}
+
 // Scratch of the two particle passes: per-cell counts and chunk offsets, and
 // the chunk -> cell lists.
 struct PassScratch {

@@ -340,6 +340,8 @@ fsr_status fsr_reconstruct_frame(fsr_context* ctx, fsr_method method, const fsr_
         fa.particle_radius = sim->particle_radius;
         fa.phi = phi;
         fa.counters = counters;
+        fa.saved = nullptr;
+        if (ctx->save_adjoint) FSR_TRY(arena_typed(ctx, A_ADJ_VERT, V, &fa.saved));
         FSR_TRY(field_zb05(ctx, fa, s));
     }
 
@@ -370,6 +372,30 @@ fsr_status fsr_reconstruct_frame(fsr_context* ctx, fsr_method method, const fsr_
     ma.st = stats;
     FSR_TRY(mc_run(ctx, ma, s));
     prof_mark(ctx, FSR_NUM_STAGES, s);
+    AdjointState& st = ctx->adj;
+    st = AdjointState();
+    if (ctx->save_adjoint && !yu13) {
+        st.valid = true;
+        st.cpu_flavor = ma.cpu_flavor;
+        st.n = n;
+        st.sp = sp;
+        st.sc = sc;
+        st.radius_sq = sim->smoothing_radius_sq;
+        st.iso = sim->iso_value;
+        st.sorted = ho.sorted;
+        st.ranges = ho.ranges;
+        st.perm = ho.perm;
+        st.bc = bc;
+        st.saved = static_cast<const float4*>(ctx->arena.ptr[A_ADJ_VERT]);
+        st.phi = phi;
+        st.bits = bits;
+        st.word_off = static_cast<const uint32_t*>(ctx->arena.ptr[A_WORD_OFF]);
+        st.nverts = nverts;
+        st.vert_off = vert_off;
+        st.tri_capacity = tri_capacity;
+        st.stats = stats;
+        st.counters = counters;
+    }
     return FSR_OK;
 }
 
@@ -423,6 +449,56 @@ fsr_status fsr_stage_times(fsr_context* ctx, float* ms, uint32_t max_frames, uin
     *num_frames = n;
     ctx->prof_frames = 0;
     return FSR_OK;
+}
+
+fsr_status fsr_set_adjoint(fsr_context* ctx, int enable) {
+    if (!ctx) {
+        set_error("fsr_set_adjoint: null context");
+        return FSR_ERR_INVALID_ARG;
+    }
+    ctx->save_adjoint = enable != 0;
+    if (!ctx->save_adjoint) ctx->adj = AdjointState();
+    return FSR_OK;
+}
+
+static fsr_status adjoint_ready(fsr_context* ctx, const char* who) {
+    if (!ctx) {
+        set_error(std::string(who) + ": null context");
+        return FSR_ERR_INVALID_ARG;
+    }
+    if (!ctx->adj.valid) {
+        set_error(std::string(who) +
+                  ": no adjoint state; enable it with fsr_set_adjoint before a ZB05 or "
+                  "Akinci12 frame (Yu13 frames have no adjoint)");
+        return FSR_ERR_UNSUPPORTED;
+    }
+    return FSR_OK;
+}
+
+fsr_status fsr_mc_vertex_backward(fsr_context* ctx, const float* g_tri_pos, float* g_phi,
+                                  void* stream) {
+    FSR_TRY(adjoint_ready(ctx, "fsr_mc_vertex_backward"));
+    if (!g_phi || (!g_tri_pos && ctx->adj.tri_capacity)) {
+        set_error("fsr_mc_vertex_backward: g_tri_pos and g_phi are required");
+        return FSR_ERR_INVALID_ARG;
+    }
+    FSR_HIP(hipSetDevice(ctx->device));
+    return mc_vertex_backward(ctx, g_tri_pos, g_phi, (hipStream_t)stream);
+}
+
+fsr_status fsr_field_zb05_backward(fsr_context* ctx, const float* g_phi, float* g_pos,
+                                   void* stream) {
+    FSR_TRY(adjoint_ready(ctx, "fsr_field_zb05_backward"));
+    if (!g_phi || (!g_pos && ctx->adj.n)) {
+        set_error("fsr_field_zb05_backward: g_phi and g_pos are required");
+        return FSR_ERR_INVALID_ARG;
+    }
+    if (ctx->adj.bc.radius != 1) {
+        set_error("fsr_field_zb05_backward: the frame's band work list is not a ZB05 one");
+        return FSR_ERR_UNSUPPORTED;
+    }
+    FSR_HIP(hipSetDevice(ctx->device));
+    return field_zb05_backward(ctx, g_phi, g_pos, (hipStream_t)stream);
 }
 
 fsr_status fsr_frame_buffer(fsr_context* ctx, fsr_buffer which, void** dev_ptr, uint64_t* count) {

@@ -133,7 +133,21 @@ enum ArenaSlot {
     A_CHUNK_CELL, A_NVERTS, A_VALID, A_VALID_OFF, A_VERT_OFF, A_VALID_IDX,
     A_COUNTERS, A_SCAN_PARTIALS, A_STATS, A_INVOLVE, A_INVOLVE_OFF,
     A_INVOLVED_IDX, A_MEAN, A_SMOOTH, A_GMAT, A_SIGMA, A_AUX0, A_AUX1, A_REC,
+    A_ADJ_VERT, A_ADJ_COEF, A_ADJ_AUX,
     A_NUM_SLOTS
+};
+
+// Band vertices grouped by hash cell (band.hip).
+struct BandCompactOut {
+    uint32_t* band_index;   // u32[B]
+    uint32_t* cell_off;     // per virtual cell offset into cell_band
+    uint32_t* cell_cnt;
+    uint32_t* chunk_off;
+    uint32_t* cell_band;    // u32[B] cell-major
+    uint32_t* chunk_cell;   // chunk -> virtual cell
+    uint32_t vcx, vcy, vcz; // virtual cell grid (hash grid padded by `radius`)
+    int radius;             // 1 (3^3 blocks) or 2 (5^3 blocks)
+    uint64_t max_chunks;
 };
 
 struct Arena {
@@ -158,6 +172,29 @@ namespace fsr {
 // Launch-shape classes of the persistent kernels (workgroups per CU), set by
 // the autotuner.
 enum TuneClass { TUNE_FIELD = 0, TUNE_MC = 1, TUNE_ANISO = 2, TUNE_BAND = 3, TUNE_NUM = 4 };
+
+// What the last frame left for its adjoint (valid while no other frame runs on
+// the context).
+struct AdjointState {
+    bool valid = false;
+    int cpu_flavor = 0;                      // marching cubes flavour of the frame
+    uint32_t n = 0;
+    GridDev sp{}, sc{};
+    float radius_sq = 0.f, iso = 0.f;
+    const float4* sorted = nullptr;          // hash-sorted particles
+    const fsr_index_range* ranges = nullptr;
+    const uint32_t* perm = nullptr;          // sorted -> caller index
+    BandCompactOut bc{};
+    const float4* saved = nullptr;           // cell-major (x - xbar, sum w)
+    const float* phi = nullptr;
+    const uint32_t* bits = nullptr;          // band mask
+    const uint32_t* word_off = nullptr;      // band rank of each mask word
+    const uint32_t* nverts = nullptr;        // soup vertices of the cube at band slot b
+    const uint32_t* vert_off = nullptr;      // their first soup index
+    uint32_t tri_capacity = 0;
+    const fsr_frame_stats* stats = nullptr;
+    Counters* counters = nullptr;
+};
 }  // namespace fsr
 
 struct fsr_context {
@@ -166,6 +203,8 @@ struct fsr_context {
     int last_sort_passes = 0;
     int tune[fsr::TUNE_NUM] = {0, 0, 0, 0};  // 0: kernel default
     bool capturing = false;                  // inside a graph capture: no arena growth
+    bool save_adjoint = false;               // frames keep what the adjoint needs
+    fsr::AdjointState adj;
     fsr::Arena arena;
     // stage profiling: ring of (FSR_NUM_STAGES + 1) events per frame
     std::vector<hipEvent_t> prof_events;
@@ -258,17 +297,6 @@ fsr_status band_build(fsr_context* ctx, const BandArgs& a, hipStream_t s);
 // the particle, so the band equals a box dilation of the surface cells.
 int band_dilation_exact(const fsr_grid_info& sc, float h);
 
-struct BandCompactOut {
-    uint32_t* band_index;   // u32[B]
-    uint32_t* cell_off;     // per virtual cell offset into cell_band
-    uint32_t* cell_cnt;
-    uint32_t* chunk_off;
-    uint32_t* cell_band;    // u32[B] cell-major
-    uint32_t* chunk_cell;   // chunk -> virtual cell
-    uint32_t vcx, vcy, vcz; // virtual cell grid (hash grid padded by `radius`)
-    int radius;             // 1 (3^3 blocks) or 2 (5^3 blocks)
-    uint64_t max_chunks;
-};
 fsr_status band_compact(fsr_context* ctx, const uint32_t* bits, const GridDev& sc,
                         const GridDev& sp, int radius, Counters* counters,
                         fsr_frame_stats* st, BandCompactOut* out, hipStream_t s);
@@ -284,6 +312,7 @@ struct FieldArgs {
     float particle_radius;
     float* phi;
     Counters* counters;
+    float4* saved;         // optional, cell-major [B]: (x - xbar, sum w) for the adjoint
 };
 fsr_status field_zb05(fsr_context* ctx, const FieldArgs& a, hipStream_t s);
 fsr_status fill_f32(float* p, float value, uint64_t n, hipStream_t s);
@@ -346,6 +375,19 @@ fsr_status aniso_involved(fsr_context* ctx, const AnisoArgs& a, hipStream_t s);
 fsr_status aniso_cov(fsr_context* ctx, const AnisoArgs& a, hipStream_t s);
 fsr_status aniso_field(fsr_context* ctx, const AnisoArgs& a, const BandCompactOut& bc,
                        float* phi, hipStream_t s);
+// Chunks of up to 64 particles per hash cell (cells with filter[c] == 0 are
+// skipped when filter is not null): counts, exclusive chunk offsets, chunk ->
+// cell map, total chunk count into *d_total.
+fsr_status cell_chunk_list(fsr_context* ctx, const fsr_index_range* ranges,
+                           const uint32_t* filter, uint32_t C, uint32_t* cnt,
+                           uint32_t* chunk_off, uint32_t* chunk_cell, uint32_t* d_total,
+                           hipStream_t s);
+
+// adjoint.hip: backward passes over the state a frame left in ctx->adj
+fsr_status mc_vertex_backward(fsr_context* ctx, const float* g_tri_pos, float* g_phi,
+                              hipStream_t s);
+fsr_status field_zb05_backward(fsr_context* ctx, const float* g_phi, float* g_pos,
+                               hipStream_t s);
 
 // Grid of a persistent kernel: CUs x workgroups per CU (the tuned value of
 // `cls` when set, else `per_cu`).

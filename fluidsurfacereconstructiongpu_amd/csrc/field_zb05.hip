@@ -143,12 +143,18 @@ __global__ __launch_bounds__(kFieldThreads) void k_field_zb05(FieldArgs a, BandC
             }
             wave_lds_sync();
         }
-        if (lane < cnt && sw > 0.0f) {
-            const float inv = 1.0f / sw;
-            const float dx = vx - sx * inv, dy = vy - sy * inv, dz = vz - sz * inv;
-            a.phi[vid] =
-                __builtin_sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx))) -
-                a.particle_radius;
+        if (lane < cnt) {
+            float4 keep = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (sw > 0.0f) {
+                const float inv = 1.0f / sw;
+                const float dx = vx - sx * inv, dy = vy - sy * inv, dz = vz - sz * inv;
+                a.phi[vid] =
+                    __builtin_sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx))) -
+                    a.particle_radius;
+                keep = make_float4(dx, dy, dz, sw);
+            }
+            // for the adjoint: x - xbar and sum w, at the vertex's cell-major slot
+            if (a.saved) a.saved[first + lane] = keep;
         }
     }
 }

@@ -468,6 +468,7 @@ fsr_status fsr_update_scalar_grid_values_compacted(
     fa.particle_radius = sim->particle_radius;
     fa.phi = scalar_field;
     fa.counters = ct;
+    fa.saved = nullptr;
     return field_zb05(ctx, fa, s);
 }
 

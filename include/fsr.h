@@ -422,6 +422,36 @@ fsr_status fsr_set_profiling(fsr_context* ctx, uint32_t ring_frames);
 fsr_status fsr_stage_times(fsr_context* ctx, float* ms, uint32_t max_frames,
                            uint32_t* num_frames);
 
+/* ------------------------------------------------------------------------- */
+/* Adjoint (no counterpart in the reference, which has no backward pass):     */
+/* dL/d(particle positions) through the ZB05 field                            */
+/* (updateScalarFieldValueZB05, ReconstructionCUDA.cu:230-289) and the        */
+/* marching-cubes vertex interpolation (generateTriangles,                    */
+/* ReconstructionCUDA.cu:383-463), topology held fixed.  ZB05 and Akinci12    */
+/* frames only (both use the ZB05 field).                                     */
+/* ------------------------------------------------------------------------- */
+
+/* enable != 0: every following fsr_reconstruct_frame on ctx keeps what the
+ * adjoint needs (16 bytes per band vertex beyond the forward).  The state of
+ * the last frame stays valid until the next frame on ctx; the particle,
+ * phi and arena buffers of that frame must not change in between. */
+fsr_status fsr_set_adjoint(fsr_context* ctx, int enable);
+
+/* Adds the vertex term of the last frame to g_phi (float[V], dL/dphi on the
+ * scalar grid, updated in place at band vertices only):
+ *   g_phi[a] += sum over emitted soup vertices k on an edge (a, b) of
+ *               g_tri_pos[k] . (c_b - c_a) * dt/dphi_a.
+ * g_tri_pos: float[3 * num_mesh_vertices] of that frame (soup order).
+ * Deterministic (gather, no atomics).  Enqueued on `stream`; no host sync. */
+fsr_status fsr_mc_vertex_backward(fsr_context* ctx, const float* g_tri_pos, float* g_phi,
+                                  void* stream);
+
+/* g_pos (float[3 * num_particles], the caller's particle order of the last
+ * frame) = sum over band vertices v of g_phi[v] * dphi_v/dx_p.  Particles
+ * outside the hash grid get 0.  Deterministic.  Enqueued on `stream`. */
+fsr_status fsr_field_zb05_backward(fsr_context* ctx, const float* g_phi, float* g_pos,
+                                   void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,6 +78,26 @@ def load():
     lib.orc_field_aniso.argtypes = [u32p, ctypes.c_uint32, f64p, f32p, u32p, u32p, u32p, f64p,
                                     OGrid, OGrid, ctypes.c_float, ctypes.c_void_p, f64p,
                                     ctypes.c_void_p]
+    lib.orc_mc_vertex_backward.argtypes = [u32p, ctypes.c_uint32, u8p, f64p, OGrid,
+                                           ctypes.c_float, ctypes.c_int, f64p, ctypes.c_uint64,
+                                           f64p]
+    lib.orc_field_zb05_backward.argtypes = [u32p, ctypes.c_uint32, f32p, ctypes.c_uint32, u32p,
+                                            OGrid, OGrid, ctypes.c_float, f64p, f64p]
+    lib.orc_adj_create.restype = ctypes.c_void_p
+    lib.orc_adj_create.argtypes = [ctypes.c_int, f32p, f32p, ctypes.c_uint32, OGrid, OGrid,
+                                   ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                   ctypes.c_float, ctypes.c_float, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_double]
+    lib.orc_adj_destroy.argtypes = [ctypes.c_void_p]
+    lib.orc_adj_info.argtypes = [ctypes.c_void_p, u64p]
+    lib.orc_adj_state.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_void_p]
+    lib.orc_adj_loss.argtypes = [ctypes.c_void_p]
+    lib.orc_adj_loss.restype = ctypes.c_double
+    lib.orc_adj_upstream.argtypes = [ctypes.c_void_p, f64p, f64p]
+    lib.orc_adj_grad.argtypes = [ctypes.c_void_p, f64p]
+    lib.orc_adj_fd.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_double]
+    lib.orc_adj_fd.restype = ctypes.c_double
     _lib = lib
     return lib
 
@@ -281,3 +301,83 @@ def reference_mc_tables():
         edge_vertices=arr(lib.ref_mc_edge_vertices(), ctypes.c_uint32, 24).reshape(12, 2),
         corner_offsets=arr(lib.ref_mc_corner_offsets(), ctypes.c_int32, 24).reshape(8, 3),
     )
+
+
+def mc_vertex_backward(index, band_u8, phi, sc, iso, g_tri, cpu=False, g_phi=None):
+    """g_phi (float64[V], accumulated) += the marching-cubes vertex term for the
+    soup over `phi` (soup order of the device emitter)."""
+    lib = load()
+    index = np.ascontiguousarray(index, np.uint32)
+    out = np.zeros(sc.count, np.float64) if g_phi is None else np.array(g_phi, np.float64)
+    g = np.ascontiguousarray(g_tri, np.float64).ravel()
+    lib.orc_mc_vertex_backward(index, index.shape[0], np.ascontiguousarray(band_u8, np.uint8),
+                               np.ascontiguousarray(phi, np.float64), ogrid(sc), iso, int(cpu),
+                               g, g.shape[0] // 3, out)
+    return out
+
+
+def field_zb05_backward(index, pos_sorted, ranges, sp, sc, radius_sq, g_phi):
+    """dL/dp (float64[n, 3], hash-sorted order) from g_phi (float64[V])."""
+    lib = load()
+    index = np.ascontiguousarray(index, np.uint32)
+    n = pos_sorted.shape[0]
+    out = np.empty((n, 3), np.float64)
+    lib.orc_field_zb05_backward(index, index.shape[0],
+                                np.ascontiguousarray(pos_sorted, np.float32).ravel(), n,
+                                np.ascontiguousarray(ranges).ravel(), ogrid(sp), ogrid(sc),
+                                radius_sq, np.ascontiguousarray(g_phi, np.float64), out.ravel())
+    return out
+
+
+class AdjointWorkload:
+    """FP64 adjoint workload on a frame: field + soup with frozen topology, the
+    sphere loss, its analytic gradient and incremental central differences."""
+
+    def __init__(self, method, fr, sphere_radius=0.3, w_grid=1.0, w_soup=1.0):
+        self.lib = load()
+        h = np.float32(fr.h)
+        pos = np.ascontiguousarray(fr.pos, np.float32)
+        dens = np.ascontiguousarray(fr.dens, np.float32)
+        self.n = pos.shape[0]
+        self.ptr = self.lib.orc_adj_create(
+            {"zb05": 0, "akinci12": 1}[method], pos.ravel(), dens, self.n, ogrid(fr.sp),
+            ogrid(fr.sc), float(h), float(h * h), float(np.float32(fr.particle_radius)),
+            float(np.float32(fr.mass)), float(np.float32(fr.iso)), float(sphere_radius),
+            float(w_grid), float(w_soup))
+        info = np.zeros(6, np.uint64)
+        self.lib.orc_adj_info(self.ptr, info)
+        self.B, self.K, self.V = int(info[1]), int(info[2]), int(info[3])
+
+    def close(self):
+        if self.ptr:
+            self.lib.orc_adj_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.close()
+
+    def state(self):
+        phi = np.empty(self.V, np.float64)
+        soup = np.empty((self.K, 3), np.float64)
+        index = np.empty(self.B, np.uint32)
+        perm = np.empty(self.n, np.uint32)
+        self.lib.orc_adj_state(self.ptr, phi.ctypes.data, soup.ctypes.data, index.ctypes.data,
+                               perm.ctypes.data)
+        return dict(phi=phi, soup=soup, band_index=index, perm=perm)
+
+    def loss(self) -> float:
+        return float(self.lib.orc_adj_loss(self.ptr))
+
+    def upstream(self):
+        g_phi = np.empty(self.V, np.float64)
+        g_soup = np.empty((self.K, 3), np.float64)
+        self.lib.orc_adj_upstream(self.ptr, g_phi, g_soup.ravel())
+        return g_phi, g_soup
+
+    def grad(self):
+        g = np.empty((self.n, 3), np.float64)
+        self.lib.orc_adj_grad(self.ptr, g.ravel())
+        return g
+
+    def fd(self, j: int, axis: int, eps: float = 1e-6) -> float:
+        return float(self.lib.orc_adj_fd(self.ptr, int(j), int(axis), float(eps)))

@@ -230,12 +230,17 @@ uint32_t orc_compact(const uint8_t* band, uint64_t V, uint32_t* index) {
     return b;
 }
 
+}  // extern "C"
+
+namespace {
 // updateScalarFieldValueZB05 (ReconstructionCUDA.cu:230-289) with kernelZB05
 // (ReconstructionCUDA.cuh:262-268), in FP64.  The vertex position and its hash
-// cell are the FP32 values the kernel computes.
-int orc_field_zb05(const uint32_t* band_index, uint32_t B, const float* pos,
-                   const uint32_t* ranges, orc_grid sp, orc_grid sc, float radius_sq,
-                   float particle_radius, double* phi, double* wsum) {
+// cell are the FP32 values the kernel computes.  Positions are FP32 (the
+// frame's) or FP64 (the adjoint's perturbed ones).
+template <typename P>
+void field_zb05_impl(const uint32_t* band_index, uint32_t B, const P* pos,
+                     const uint32_t* ranges, orc_grid sp, orc_grid sc, float radius_sq,
+                     float particle_radius, double* phi, double* wsum) {
     const int rx = sp.rx, ry = sp.ry, rz = sp.rz;
     const double R2 = radius_sq;
 #pragma omp parallel for schedule(dynamic, 64)
@@ -274,6 +279,15 @@ int orc_field_zb05(const uint32_t* band_index, uint32_t B, const float* pos,
             phi[vid] = std::sqrt(ax * ax + ay * ay + az * az) - (double)particle_radius;
         }
     }
+}
+}  // namespace
+
+extern "C" {
+
+int orc_field_zb05(const uint32_t* band_index, uint32_t B, const float* pos,
+                   const uint32_t* ranges, orc_grid sp, orc_grid sc, float radius_sq,
+                   float particle_radius, double* phi, double* wsum) {
+    field_zb05_impl(band_index, B, pos, ranges, sp, sc, radius_sq, particle_radius, phi, wsum);
     return 0;
 }
 
@@ -659,6 +673,448 @@ int orc_frame(int method, float* pos, float* dens, uint32_t n, orc_grid sp, orc_
         counts[5] = 0;
     }
     return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Adjoint of the ZB05 field and of the marching-cubes vertex lerp, topology
+// held fixed.  The reference has no backward pass: these are the derivatives of
+// the FP64 restatements above, written out by hand, plus the loss of the
+// adjoint workload with incremental central differences that pin them.
+// ---------------------------------------------------------------------------
+namespace {
+
+const int kCornerOff[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {0, 1, 0},
+                              {1, 0, 0}, {1, 0, 1}, {1, 1, 1}, {1, 1, 0}};
+const int kEdgeCorner[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+
+// Corner position as the frame's marching cubes evaluates it (orc_mc).
+inline double corner_pos(int cpu, uint32_t X, float mn, float cell) {
+    return cpu ? (double)(mn + (float)X * cell) : (double)vpos(X, mn, cell);
+}
+
+// One band vertex of the ZB05 field: sum w, sum w p (FP64), same candidate
+// cells as field_zb05_impl.
+template <typename P>
+void zb05_sums(const P* pos, const uint32_t* ranges, orc_grid sp, orc_grid sc, double R2,
+               uint32_t vid, double x[3], double* sw, double s[3]) {
+    const int rx = sp.rx, ry = sp.ry, rz = sp.rz;
+    const uint32_t ix = vid % sc.rx, iy = (vid / sc.rx) % sc.ry, iz = vid / (sc.rx * sc.ry);
+    const float vxf = vpos(ix, sc.minx, sc.cell), vyf = vpos(iy, sc.miny, sc.cell),
+                vzf = vpos(iz, sc.minz, sc.cell);
+    const int cx = trunc_q(cell_q(vxf, sp.minx, sp.cell), rx);
+    const int cy = trunc_q(cell_q(vyf, sp.miny, sp.cell), ry);
+    const int cz = trunc_q(cell_q(vzf, sp.minz, sp.cell), rz);
+    x[0] = vxf; x[1] = vyf; x[2] = vzf;
+    *sw = 0; s[0] = s[1] = s[2] = 0;
+    for (int zz = clampi(cz - 1, 0, rz - 1); zz <= clampi(cz + 1, 0, rz - 1); ++zz)
+        for (int yy = clampi(cy - 1, 0, ry - 1); yy <= clampi(cy + 1, 0, ry - 1); ++yy)
+            for (int xx = clampi(cx - 1, 0, rx - 1); xx <= clampi(cx + 1, 0, rx - 1); ++xx) {
+                const uint64_t c = ((uint64_t)zz * ry + yy) * rx + xx;
+                if (ranges[2 * c] == 0xFFFFFFFFu) continue;
+                for (uint32_t j = ranges[2 * c]; j < ranges[2 * c + 1]; ++j) {
+                    const double px = pos[3 * j], py = pos[3 * j + 1], pz = pos[3 * j + 2];
+                    const double dx = x[0] - px, dy = x[1] - py, dz = x[2] - pz;
+                    const double d2 = dx * dx + dy * dy + dz * dz;
+                    if (d2 >= R2) continue;
+                    const double t = 1.0 - d2 / R2;
+                    const double w = t * t * t;
+                    *sw += w;
+                    s[0] += w * px;
+                    s[1] += w * py;
+                    s[2] += w * pz;
+                }
+            }
+}
+
+// g_pos (sorted order, += ) of sum_v g_phi[v] phi_v over the band.
+template <typename P>
+void field_zb05_backward_impl(const uint32_t* band_index, uint32_t B, const P* pos,
+                              const uint32_t* ranges, orc_grid sp, orc_grid sc, float radius_sq,
+                              const double* g_phi, double* g_pos) {
+    const int rx = sp.rx, ry = sp.ry, rz = sp.rz;
+    const double R2 = radius_sq;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t b = 0; b < (int64_t)B; ++b) {
+        const uint32_t vid = band_index[b];
+        const double g = g_phi[vid];
+        if (g == 0.0) continue;
+        double x[3], sw, s[3];
+        zb05_sums(pos, ranges, sp, sc, R2, vid, x, &sw, s);
+        if (!(sw > 0.0)) continue;
+        const double xb[3] = {s[0] / sw, s[1] / sw, s[2] / sw};
+        const double d[3] = {x[0] - xb[0], x[1] - xb[1], x[2] - xb[2]};
+        const double len = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        if (!(len > 0.0)) continue;
+        const double nrm[3] = {d[0] / len, d[1] / len, d[2] / len};
+        const double gw = g / sw;
+        const int cx = trunc_q(cell_q((float)x[0], sp.minx, sp.cell), rx);
+        const int cy = trunc_q(cell_q((float)x[1], sp.miny, sp.cell), ry);
+        const int cz = trunc_q(cell_q((float)x[2], sp.minz, sp.cell), rz);
+        for (int zz = clampi(cz - 1, 0, rz - 1); zz <= clampi(cz + 1, 0, rz - 1); ++zz)
+            for (int yy = clampi(cy - 1, 0, ry - 1); yy <= clampi(cy + 1, 0, ry - 1); ++yy)
+                for (int xx = clampi(cx - 1, 0, rx - 1); xx <= clampi(cx + 1, 0, rx - 1); ++xx) {
+                    const uint64_t c = ((uint64_t)zz * ry + yy) * rx + xx;
+                    if (ranges[2 * c] == 0xFFFFFFFFu) continue;
+                    for (uint32_t j = ranges[2 * c]; j < ranges[2 * c + 1]; ++j) {
+                        const double p[3] = {(double)pos[3 * j], (double)pos[3 * j + 1],
+                                             (double)pos[3 * j + 2]};
+                        const double e[3] = {x[0] - p[0], x[1] - p[1], x[2] - p[2]};
+                        const double d2 = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+                        if (d2 >= R2) continue;
+                        const double t = 1.0 - d2 / R2;
+                        const double w = t * t * t;
+                        // d phi / d p_j = -(1/W)[w n + (n.(p - xbar)) (6/R^2) t^2 (x - p)]
+                        const double m = nrm[0] * (p[0] - xb[0]) + nrm[1] * (p[1] - xb[1]) +
+                                         nrm[2] * (p[2] - xb[2]);
+                        const double k = m * 6.0 / R2 * t * t;
+                        for (int a = 0; a < 3; ++a) {
+                            const double v = -gw * (w * nrm[a] + k * e[a]);
+#pragma omp atomic
+                            g_pos[3 * j + a] += v;
+                        }
+                    }
+                }
+    }
+}
+
+// Soup vertices of the marching cubes over `phi` in soup order (cube band
+// position, then table order), with the per-cube capacity rule of the device
+// emitter: a = grid id of corner s0, b = of corner s1 of each vertex's edge.
+struct Soup {
+    std::vector<uint32_t> a, b;
+    std::vector<double> ca, dc;  // corner s0 position, corner s1 - corner s0
+};
+
+uint64_t soup_edges(const uint32_t* band_index, uint32_t B, const uint8_t* band,
+                    const double* phi, orc_grid sc, float iso, int cpu, uint64_t capacity,
+                    Soup* out) {
+    const uint64_t rx = sc.rx, ry = sc.ry, rz = sc.rz, rxy = rx * ry;
+    std::vector<uint32_t> nverts(B);
+    orc_mc(band_index, B, band, phi, sc, iso, cpu, nverts.data(), nullptr, nullptr, 0);
+    uint64_t off = 0;
+    for (uint32_t bb = 0; bb < B; ++bb) {
+        const uint32_t nv = nverts[bb];
+        if (!nv) continue;
+        const uint64_t first = off;
+        off += nv;
+        if (first + nv > capacity) continue;
+        const uint32_t vid = band_index[bb];
+        const uint32_t x = vid % rx, y = (vid / rx) % ry, z = (uint32_t)(vid / rxy);
+        (void)rz;
+        uint32_t f = 0;
+        for (int c = 0; c < 8; ++c) {
+            const uint64_t v = ((uint64_t)(z + kCornerOff[c][2]) * ry + (y + kCornerOff[c][1])) * rx +
+                               (x + kCornerOff[c][0]);
+            if (phi[v] <= (double)iso) f |= 1u << c;
+        }
+        for (uint32_t k = 0; k < nv; ++k) {
+            const int e = (int)((kTri[f] >> (4 * k)) & 0xF);
+            uint32_t id[2];
+            double cp[2][3];
+            for (int q = 0; q < 2; ++q) {
+                const int c = kEdgeCorner[e][q];
+                const uint32_t X = x + kCornerOff[c][0], Y = y + kCornerOff[c][1],
+                               Z = z + kCornerOff[c][2];
+                id[q] = (uint32_t)(((uint64_t)Z * ry + Y) * rx + X);
+                cp[q][0] = corner_pos(cpu, X, sc.minx, sc.cell);
+                cp[q][1] = corner_pos(cpu, Y, sc.miny, sc.cell);
+                cp[q][2] = corner_pos(cpu, Z, sc.minz, sc.cell);
+            }
+            out->a.push_back(id[0]);
+            out->b.push_back(id[1]);
+            for (int t = 0; t < 3; ++t) {
+                out->ca.push_back(cp[0][t]);
+                out->dc.push_back(cp[1][t] - cp[0][t]);
+            }
+        }
+    }
+    return off;
+}
+
+inline double lerp_t(double pa, double pb, double iso) {
+    const double delta = pb - pa;
+    return (delta > -1e-7 && delta < 1e-7) ? 0.5 : (iso - pa) / delta;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_mc_vertex_backward(const uint32_t* band_index, uint32_t B, const uint8_t* band,
+                           const double* phi, orc_grid sc, float iso, int cpu,
+                           const double* g_tri, uint64_t capacity, double* g_phi) {
+    Soup soup;
+    soup_edges(band_index, B, band, phi, sc, iso, cpu, capacity, &soup);
+    const double isod = iso;
+    for (size_t k = 0; k < soup.a.size(); ++k) {
+        const double pa = phi[soup.a[k]], pb = phi[soup.b[k]];
+        const double delta = pb - pa;
+        if (delta > -1e-7 && delta < 1e-7) continue;
+        // the soup slot of this vertex: soup order is the emitter's
+        const double gt = g_tri[3 * k] * soup.dc[3 * k] + g_tri[3 * k + 1] * soup.dc[3 * k + 1] +
+                          g_tri[3 * k + 2] * soup.dc[3 * k + 2];
+        g_phi[soup.a[k]] += gt * (isod - pb) / (delta * delta);
+        g_phi[soup.b[k]] -= gt * (isod - pa) / (delta * delta);
+    }
+    return 0;
+}
+
+int orc_field_zb05_backward(const uint32_t* band_index, uint32_t B, const float* pos,
+                            uint32_t n, const uint32_t* ranges, orc_grid sp, orc_grid sc,
+                            float radius_sq, const double* g_phi, double* g_pos) {
+    std::fill(g_pos, g_pos + 3ull * n, 0.0);
+    field_zb05_backward_impl(band_index, B, pos, ranges, sp, sc, radius_sq, g_phi, g_pos);
+    return 0;
+}
+
+}  // extern "C"
+
+struct orc_adj {
+    int cpu = 0;
+    orc_grid sp{}, sc{};
+    float radius_sq = 0.f, particle_radius = 0.f, iso = 0.f;
+    double R0 = 0, wg = 0, ws = 0;
+    uint32_t n = 0, B = 0;
+    uint64_t V = 0, K = 0;
+    std::vector<double> pos;  // hash-sorted, FP64
+    std::vector<uint32_t> perm, inv, ranges, band_index;
+    std::vector<uint8_t> band;
+    std::vector<double> phi, sdf;
+    Soup soup;
+    std::vector<uint32_t> vs_off, vs_list;  // band position -> soup vertices on its edges
+};
+
+namespace {
+
+uint32_t band_pos(const orc_adj* A, uint32_t vid) {
+    return (uint32_t)(std::lower_bound(A->band_index.begin(), A->band_index.end(), vid) -
+                      A->band_index.begin());
+}
+
+double phi_at(const orc_adj* A, uint32_t vid) {
+    double x[3], sw, s[3];
+    zb05_sums(A->pos.data(), A->ranges.data(), A->sp, A->sc, (double)A->radius_sq, vid, x, &sw,
+              s);
+    if (!(sw > 0.0)) return 0.0;
+    const double d0 = x[0] - s[0] / sw, d1 = x[1] - s[1] / sw, d2 = x[2] - s[2] / sw;
+    return std::sqrt(d0 * d0 + d1 * d1 + d2 * d2) - (double)A->particle_radius;
+}
+
+double soup_term(const orc_adj* A, uint64_t k, double pa, double pb) {
+    const double t = lerp_t(pa, pb, (double)A->iso);
+    double r2 = 0;
+    for (int a = 0; a < 3; ++a) {
+        const double y = A->soup.ca[3 * k + a] + t * A->soup.dc[3 * k + a];
+        r2 += y * y;
+    }
+    const double d = std::sqrt(r2) - A->R0;
+    return d * d;
+}
+
+}  // namespace
+
+extern "C" {
+
+orc_adj* orc_adj_create(int method, const float* pos, const float* dens, uint32_t n, orc_grid sp,
+                        orc_grid sc, float h, float radius_sq, float particle_radius, float mass,
+                        float iso, double sphere_radius, double w_grid, double w_soup) {
+    orc_adj* A = new orc_adj();
+    A->cpu = method == 1;
+    A->sp = sp;
+    A->sc = sc;
+    A->radius_sq = radius_sq;
+    A->particle_radius = particle_radius;
+    A->iso = iso;
+    A->R0 = sphere_radius;
+    A->wg = w_grid;
+    A->ws = w_soup;
+    A->n = n;
+    const uint64_t C = (uint64_t)sp.rx * sp.ry * sp.rz;
+    A->V = (uint64_t)sc.rx * sc.ry * sc.rz;
+    std::vector<uint32_t> keys(n), flags(n);
+    std::vector<float> occ(C), ps(3ull * n), ds(n);
+    A->perm.resize(n);
+    A->ranges.resize(2 * C);
+    orc_hash_sort(pos, dens, n, sp, keys.data(), A->perm.data(), ps.data(), ds.data(),
+                  A->ranges.data(), occ.data());
+    A->inv.resize(n);
+    for (uint32_t i = 0; i < n; ++i) A->inv[A->perm[i]] = i;
+    if (method == 1) {
+        orc_surface_colorfield(ps.data(), ds.data(), n, A->ranges.data(), sp, h, mass,
+                               flags.data(), nullptr, nullptr);
+    } else {
+        std::vector<uint8_t> surf(C);
+        orc_surface_occupancy(occ.data(), A->ranges.data(), sp, keys.data(), n, surf.data(),
+                              flags.data(), nullptr);
+    }
+    A->band.resize(A->V);
+    orc_band(ps.data(), flags.data(), n, sc, h, A->band.data());
+    A->band_index.resize(A->V);
+    A->B = orc_compact(A->band.data(), A->V, A->band_index.data());
+    A->band_index.resize(A->B);
+    A->pos.assign(ps.begin(), ps.end());
+    A->phi.assign(A->V, 0.0);
+    field_zb05_impl(A->band_index.data(), A->B, A->pos.data(), A->ranges.data(), sp, sc,
+                    radius_sq, particle_radius, A->phi.data(), nullptr);
+    A->sdf.resize(A->V);
+    for (uint64_t v = 0; v < A->V; ++v) {
+        const uint32_t ix = v % sc.rx, iy = (v / sc.rx) % sc.ry, iz = v / ((uint64_t)sc.rx * sc.ry);
+        const double x = vpos(ix, sc.minx, sc.cell), y = vpos(iy, sc.miny, sc.cell),
+                     z = vpos(iz, sc.minz, sc.cell);
+        A->sdf[v] = std::sqrt(x * x + y * y + z * z) - sphere_radius;
+    }
+    soup_edges(A->band_index.data(), A->B, A->band.data(), A->phi.data(), sc, iso, A->cpu,
+               ~0ull, &A->soup);
+    A->K = A->soup.a.size();
+    A->vs_off.assign(A->B + 1, 0);
+    for (uint64_t k = 0; k < A->K; ++k) {
+        ++A->vs_off[band_pos(A, A->soup.a[k]) + 1];
+        ++A->vs_off[band_pos(A, A->soup.b[k]) + 1];
+    }
+    for (uint32_t b = 0; b < A->B; ++b) A->vs_off[b + 1] += A->vs_off[b];
+    A->vs_list.resize(A->vs_off[A->B]);
+    std::vector<uint32_t> cur(A->vs_off.begin(), A->vs_off.end() - 1);
+    for (uint64_t k = 0; k < A->K; ++k) {
+        A->vs_list[cur[band_pos(A, A->soup.a[k])]++] = (uint32_t)k;
+        A->vs_list[cur[band_pos(A, A->soup.b[k])]++] = (uint32_t)k;
+    }
+    return A;
+}
+
+void orc_adj_destroy(orc_adj* A) { delete A; }
+
+// info[6] = {n, B, K (soup vertices), V, 0, 0}
+void orc_adj_info(const orc_adj* A, uint64_t* info) {
+    info[0] = A->n;
+    info[1] = A->B;
+    info[2] = A->K;
+    info[3] = A->V;
+    info[4] = 0;
+    info[5] = 0;
+}
+
+// phi (double[V]) and soup vertex positions (double[3K]) of the base state;
+// band index (u32[B]); permutation (u32[n], sorted -> caller index).
+void orc_adj_state(const orc_adj* A, double* phi, double* soup_pos, uint32_t* band_index,
+                   uint32_t* perm) {
+    if (phi) std::copy(A->phi.begin(), A->phi.end(), phi);
+    if (soup_pos)
+        for (uint64_t k = 0; k < A->K; ++k) {
+            const double t = lerp_t(A->phi[A->soup.a[k]], A->phi[A->soup.b[k]], (double)A->iso);
+            for (int a = 0; a < 3; ++a)
+                soup_pos[3 * k + a] = A->soup.ca[3 * k + a] + t * A->soup.dc[3 * k + a];
+        }
+    if (band_index) std::copy(A->band_index.begin(), A->band_index.end(), band_index);
+    if (perm) std::copy(A->perm.begin(), A->perm.end(), perm);
+}
+
+// L = w_grid/V sum_v (phi_v - sdf_v)^2 + w_soup/K sum_k (|y_k| - R0)^2
+double orc_adj_loss(const orc_adj* A) {
+    double lg = 0, ls = 0;
+    for (uint64_t v = 0; v < A->V; ++v) {
+        const double d = A->phi[v] - A->sdf[v];
+        lg += d * d;
+    }
+    for (uint64_t k = 0; k < A->K; ++k)
+        ls += soup_term(A, k, A->phi[A->soup.a[k]], A->phi[A->soup.b[k]]);
+    return A->wg * lg / (double)A->V + (A->K ? A->ws * ls / (double)A->K : 0.0);
+}
+
+// Upstream gradients of the loss: g_phi (double[V]), g_soup (double[3K]).
+void orc_adj_upstream(const orc_adj* A, double* g_phi, double* g_soup) {
+    for (uint64_t v = 0; v < A->V; ++v)
+        g_phi[v] = 2.0 * A->wg * (A->phi[v] - A->sdf[v]) / (double)A->V;
+    for (uint64_t k = 0; k < A->K; ++k) {
+        const double t = lerp_t(A->phi[A->soup.a[k]], A->phi[A->soup.b[k]], (double)A->iso);
+        double y[3], r2 = 0;
+        for (int a = 0; a < 3; ++a) {
+            y[a] = A->soup.ca[3 * k + a] + t * A->soup.dc[3 * k + a];
+            r2 += y[a] * y[a];
+        }
+        const double r = std::sqrt(r2);
+        const double c = r > 0 ? 2.0 * A->ws * (r - A->R0) / (double)A->K / r : 0.0;
+        for (int a = 0; a < 3; ++a) g_soup[3 * k + a] = c * y[a];
+    }
+}
+
+// Analytic dL/dx (double[3n], caller order).
+int orc_adj_grad(const orc_adj* A, double* grad) {
+    std::vector<double> g_phi(A->V), g_soup(3 * A->K), gs(3ull * A->n, 0.0);
+    orc_adj_upstream(A, g_phi.data(), g_soup.data());
+    const double isod = A->iso;
+    for (uint64_t k = 0; k < A->K; ++k) {
+        const double pa = A->phi[A->soup.a[k]], pb = A->phi[A->soup.b[k]];
+        const double delta = pb - pa;
+        if (delta > -1e-7 && delta < 1e-7) continue;
+        const double gt = g_soup[3 * k] * A->soup.dc[3 * k] +
+                          g_soup[3 * k + 1] * A->soup.dc[3 * k + 1] +
+                          g_soup[3 * k + 2] * A->soup.dc[3 * k + 2];
+        g_phi[A->soup.a[k]] += gt * (isod - pb) / (delta * delta);
+        g_phi[A->soup.b[k]] -= gt * (isod - pa) / (delta * delta);
+    }
+    // only band vertices carry a field
+    std::vector<double> g_band(A->V, 0.0);
+    for (uint32_t b = 0; b < A->B; ++b) g_band[A->band_index[b]] = g_phi[A->band_index[b]];
+    field_zb05_backward_impl(A->band_index.data(), A->B, A->pos.data(), A->ranges.data(), A->sp,
+                             A->sc, A->radius_sq, g_band.data(), gs.data());
+    for (uint32_t i = 0; i < A->n; ++i)
+        for (int a = 0; a < 3; ++a) grad[3ull * A->perm[i] + a] = gs[3ull * i + a];
+    return 0;
+}
+
+// Central difference (L(x + eps e) - L(x - eps e)) / (2 eps) for the caller's
+// particle j, coordinate axis, topology and cells frozen.  Only the band
+// vertices within reach of the particle and the soup vertices on their edges
+// are re-evaluated; all other terms cancel exactly.
+double orc_adj_fd(orc_adj* A, uint32_t j, int axis, double eps) {
+    const uint32_t s = A->inv[j];
+    const double base = A->pos[3ull * s + axis];
+    const double R = std::sqrt((double)A->radius_sq) + 2.0 * eps;
+    const orc_grid& sc = A->sc;
+    const float mins[3] = {sc.minx, sc.miny, sc.minz};
+    const uint32_t res[3] = {sc.rx, sc.ry, sc.rz};
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        const double p = A->pos[3ull * s + a];
+        lo[a] = std::max(0, (int)std::floor((p - R - mins[a]) / sc.cell) - 1);
+        hi[a] = std::min((int)res[a] - 1, (int)std::ceil((p + R - mins[a]) / sc.cell) + 1);
+    }
+    std::vector<uint32_t> verts;
+    for (int z = lo[2]; z <= hi[2]; ++z)
+        for (int y = lo[1]; y <= hi[1]; ++y)
+            for (int x = lo[0]; x <= hi[0]; ++x) {
+                const uint32_t vid = (uint32_t)(((uint64_t)z * sc.ry + y) * sc.rx + x);
+                if (A->band[vid]) verts.push_back(vid);
+            }
+    std::vector<uint32_t> soups;
+    for (uint32_t vid : verts) {
+        const uint32_t b = band_pos(A, vid);
+        for (uint32_t q = A->vs_off[b]; q < A->vs_off[b + 1]; ++q) soups.push_back(A->vs_list[q]);
+    }
+    std::sort(soups.begin(), soups.end());
+    soups.erase(std::unique(soups.begin(), soups.end()), soups.end());
+    double lg[2] = {0, 0}, ls[2] = {0, 0};
+    std::vector<double> local(verts.size());
+    for (int side = 0; side < 2; ++side) {
+        A->pos[3ull * s + axis] = base + (side == 0 ? eps : -eps);
+        for (size_t i = 0; i < verts.size(); ++i) {
+            local[i] = phi_at(A, verts[i]);
+            const double d = local[i] - A->sdf[verts[i]];
+            lg[side] += d * d;
+        }
+        auto phi_of = [&](uint32_t vid) {
+            const auto it = std::lower_bound(verts.begin(), verts.end(), vid);
+            return (it != verts.end() && *it == vid) ? local[it - verts.begin()] : A->phi[vid];
+        };
+        for (uint32_t k : soups) ls[side] += soup_term(A, k, phi_of(A->soup.a[k]), phi_of(A->soup.b[k]));
+    }
+    A->pos[3ull * s + axis] = base;
+    const double dl = A->wg * (lg[0] - lg[1]) / (double)A->V +
+                      (A->K ? A->ws * (ls[0] - ls[1]) / (double)A->K : 0.0);
+    return dl / (2.0 * eps);
 }
 
 }  // extern "C"

@@ -95,6 +95,38 @@ int orc_frame(int method, float* pos, float* dens, uint32_t n, orc_grid sp, orc_
 /* Case tables (for the table parity test). */
 uint64_t orc_mc_tri_packed(uint32_t c);
 
+/* ---- Adjoint (no reference counterpart), FP64, topology held fixed ------- */
+
+/* g_phi[V] += vertex term of the marching cubes over phi (soup order and the
+ * per-cube capacity rule of the device emitter): for a soup vertex on the edge
+ * (a, b), g . (c_b - c_a) * dt/dphi_a and dt/dphi_b; flat edges skipped. */
+int orc_mc_vertex_backward(const uint32_t* band_index, uint32_t B, const uint8_t* band,
+                           const double* phi, orc_grid sc, float iso, int cpu,
+                           const double* g_tri, uint64_t capacity, double* g_phi);
+
+/* g_pos[3n] (hash-sorted order) = sum over band vertices of g_phi[v] dphi_v/dp. */
+int orc_field_zb05_backward(const uint32_t* band_index, uint32_t B, const float* pos,
+                            uint32_t n, const uint32_t* ranges, orc_grid sp, orc_grid sc,
+                            float radius_sq, const double* g_phi, double* g_pos);
+
+/* The adjoint workload: a frame (method 0 ZB05 / 1 Akinci12) reconstructed in
+ * FP64 from FP32 particles, then the loss
+ *   L = w_grid/V sum_v (phi_v - (|x_v| - R0))^2 + w_soup/K sum_k (|y_k| - R0)^2
+ * with band, cells and soup topology frozen at the base positions. */
+typedef struct orc_adj orc_adj;
+orc_adj* orc_adj_create(int method, const float* pos, const float* dens, uint32_t n,
+                        orc_grid sp, orc_grid sc, float h, float radius_sq,
+                        float particle_radius, float mass, float iso, double sphere_radius,
+                        double w_grid, double w_soup);
+void orc_adj_destroy(orc_adj* A);
+void orc_adj_info(const orc_adj* A, uint64_t* info);
+void orc_adj_state(const orc_adj* A, double* phi, double* soup_pos, uint32_t* band_index,
+                   uint32_t* perm);
+double orc_adj_loss(const orc_adj* A);
+void orc_adj_upstream(const orc_adj* A, double* g_phi, double* g_soup);
+int orc_adj_grad(const orc_adj* A, double* grad);
+double orc_adj_fd(orc_adj* A, uint32_t j, int axis, double eps);
+
 #ifdef __cplusplus
 }
 #endif

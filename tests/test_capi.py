@@ -56,3 +56,11 @@ def test_error_paths_without_device_work():
         assert st == 5  # FSR_ERR_NO_DEVICE
     assert lib.fsr_reconstruct_frame(None, 0, None, None, None, None, None, None, 0, None, None,
                                      None, 0, None, None) == 1
+
+
+def test_adjoint_entry_points_reject_missing_state():
+    lib = _lib.load()
+    assert lib.fsr_set_adjoint(None, 1) == 1
+    assert lib.fsr_mc_vertex_backward(None, None, None, None) == 1
+    assert lib.fsr_field_zb05_backward(None, None, None, None) == 1
+    assert b"null context" in lib.fsr_last_error()
