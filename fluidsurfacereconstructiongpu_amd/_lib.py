@@ -73,7 +73,8 @@ class FrameStats(ctypes.Structure):
         ("num_mesh_vertices", c_uint32),
         ("overflow", c_uint32),
         ("band_exceptions", c_uint32),
-        ("reserved", c_uint32 * 7),
+        ("num_unique_vertices", c_uint32),
+        ("reserved", c_uint32 * 6),
     ]
 
 
@@ -150,6 +151,11 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         POINTER(GridInfo), c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p,
         c_uint32, c_void_p, c_void_p]
     lib.fsr_reconstruct_frame.restype = st
+    lib.fsr_reconstruct_frame_indexed.argtypes = [
+        c_void_p, c_int, POINTER(SimParam), POINTER(AnisoParam), POINTER(GridInfo),
+        POINTER(GridInfo), c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p,
+        c_uint32, c_void_p, c_uint32, c_void_p, c_void_p]
+    lib.fsr_reconstruct_frame_indexed.restype = st
     lib.fsr_frame_buffer.argtypes = [c_void_p, c_int, POINTER(c_void_p), POINTER(c_uint64)]
     lib.fsr_frame_buffer.restype = st
     P = c_void_p

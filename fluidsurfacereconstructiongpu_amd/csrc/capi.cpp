@@ -184,28 +184,40 @@ fsr_status fsr_grid_from_aabb(fsr_float3 aabb_min, fsr_float3 aabb_max, float sm
     return FSR_OK;
 }
 
-fsr_status fsr_reconstruct_frame(fsr_context* ctx, fsr_method method, const fsr_sim_param* sim,
-                                 const fsr_aniso_param* aniso, const fsr_grid_info* sp_grid,
-                                 const fsr_grid_info* sc_grid, float* pos, float* dens,
-                                 uint32_t num_particles, float* phi, float* tri_pos,
-                                 float* tri_nrm, uint32_t tri_capacity, fsr_frame_stats* stats,
-                                 void* stream) {
+}  // extern "C"
+
+namespace {
+
+// Output of the marching-cubes stage: a triangle soup, or a deduplicated mesh.
+struct MeshOut {
+    bool indexed = false;
+    float* tri_pos = nullptr;    // soup
+    float* tri_nrm = nullptr;
+    uint32_t tri_capacity = 0;   // soup vertices, or indices in indexed mode
+    float* vtx_pos = nullptr;    // indexed
+    float* vtx_nrm = nullptr;
+    uint32_t vtx_capacity = 0;
+    uint32_t* indices = nullptr;
+};
+
+fsr_status frame_impl(const char* who, fsr_context* ctx, fsr_method method,
+                      const fsr_sim_param* sim, const fsr_aniso_param* aniso,
+                      const fsr_grid_info* sp_grid, const fsr_grid_info* sc_grid, float* pos,
+                      float* dens, uint32_t num_particles, float* phi, const MeshOut& mo,
+                      fsr_frame_stats* stats, void* stream) {
     if (!ctx || !sim || !stats || !phi) {
-        set_error("fsr_reconstruct_frame: ctx, sim, phi and stats are required");
+        set_error(std::string(who) + ": ctx, sim, phi and stats are required");
         return FSR_ERR_INVALID_ARG;
     }
     if (num_particles && (!pos || !dens)) {
-        set_error("fsr_reconstruct_frame: particle buffers are null");
-        return FSR_ERR_INVALID_ARG;
-    }
-    if (tri_capacity && !tri_pos) {
-        set_error("fsr_reconstruct_frame: tri_pos is null but tri_capacity > 0");
+        set_error(std::string(who) + ": particle buffers are null");
         return FSR_ERR_INVALID_ARG;
     }
     if (method != FSR_METHOD_ZB05 && method != FSR_METHOD_AKINCI12 && method != FSR_METHOD_YU13) {
-        set_error("fsr_reconstruct_frame: unknown method");
+        set_error(std::string(who) + ": unknown method");
         return FSR_ERR_INVALID_ARG;
     }
+    const uint32_t tri_capacity = mo.tri_capacity;
     const bool yu13 = method == FSR_METHOD_YU13;
     fsr_aniso_param ap;
     ap.kr = 4.0f;
@@ -365,16 +377,29 @@ fsr_status fsr_reconstruct_frame(fsr_context* ctx, fsr_method method, const fsr_
     ma.valid_off = valid_off;
     ma.vert_off = vert_off;
     ma.valid_idx = valid_idx;
-    ma.tri_pos = tri_pos;
-    ma.tri_nrm = tri_nrm;
+    ma.tri_pos = mo.tri_pos;
+    ma.tri_nrm = mo.tri_nrm;
     ma.capacity = tri_capacity;
     ma.max_band = V;
     ma.st = stats;
-    FSR_TRY(mc_run(ctx, ma, s));
+    ma.word_off = static_cast<const uint32_t*>(ctx->arena.ptr[A_WORD_OFF]);
+    ma.edge_mask = nullptr;
+    ma.edge_off = nullptr;
+    ma.vtx_pos = mo.vtx_pos;
+    ma.vtx_nrm = mo.vtx_nrm;
+    ma.vtx_capacity = mo.vtx_capacity;
+    ma.indices = mo.indices;
+    if (mo.indexed) {
+        FSR_TRY(arena_typed(ctx, A_EDGE_MASK, V, &ma.edge_mask));
+        FSR_TRY(arena_typed(ctx, A_EDGE_OFF, V, &ma.edge_off));
+        FSR_TRY(mc_run_indexed(ctx, ma, s));
+    } else {
+        FSR_TRY(mc_run(ctx, ma, s));
+    }
     prof_mark(ctx, FSR_NUM_STAGES, s);
     AdjointState& st = ctx->adj;
     st = AdjointState();
-    if (ctx->save_adjoint && !yu13) {
+    if (ctx->save_adjoint && !yu13 && !mo.indexed) {
         st.valid = true;
         st.cpu_flavor = ma.cpu_flavor;
         st.n = n;
@@ -397,6 +422,51 @@ fsr_status fsr_reconstruct_frame(fsr_context* ctx, fsr_method method, const fsr_
         st.counters = counters;
     }
     return FSR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+fsr_status fsr_reconstruct_frame(fsr_context* ctx, fsr_method method, const fsr_sim_param* sim,
+                                 const fsr_aniso_param* aniso, const fsr_grid_info* sp_grid,
+                                 const fsr_grid_info* sc_grid, float* pos, float* dens,
+                                 uint32_t num_particles, float* phi, float* tri_pos,
+                                 float* tri_nrm, uint32_t tri_capacity, fsr_frame_stats* stats,
+                                 void* stream) {
+    if (tri_capacity && !tri_pos) {
+        set_error("fsr_reconstruct_frame: tri_pos is null but tri_capacity > 0");
+        return FSR_ERR_INVALID_ARG;
+    }
+    MeshOut mo;
+    mo.tri_pos = tri_pos;
+    mo.tri_nrm = tri_nrm;
+    mo.tri_capacity = tri_capacity;
+    return frame_impl("fsr_reconstruct_frame", ctx, method, sim, aniso, sp_grid, sc_grid, pos,
+                      dens, num_particles, phi, mo, stats, stream);
+}
+
+fsr_status fsr_reconstruct_frame_indexed(fsr_context* ctx, fsr_method method,
+                                         const fsr_sim_param* sim, const fsr_aniso_param* aniso,
+                                         const fsr_grid_info* sp_grid,
+                                         const fsr_grid_info* sc_grid, float* pos, float* dens,
+                                         uint32_t num_particles, float* phi, float* vtx_pos,
+                                         float* vtx_nrm, uint32_t vtx_capacity,
+                                         uint32_t* indices, uint32_t idx_capacity,
+                                         fsr_frame_stats* stats, void* stream) {
+    if ((vtx_capacity && !vtx_pos) || (idx_capacity && !indices)) {
+        set_error("fsr_reconstruct_frame_indexed: null mesh buffer with a nonzero capacity");
+        return FSR_ERR_INVALID_ARG;
+    }
+    MeshOut mo;
+    mo.indexed = true;
+    mo.tri_capacity = idx_capacity;
+    mo.vtx_pos = vtx_pos;
+    mo.vtx_nrm = vtx_nrm;
+    mo.vtx_capacity = vtx_capacity;
+    mo.indices = indices;
+    return frame_impl("fsr_reconstruct_frame_indexed", ctx, method, sim, aniso, sp_grid,
+                      sc_grid, pos, dens, num_particles, phi, mo, stats, stream);
 }
 
 fsr_status fsr_set_profiling(fsr_context* ctx, uint32_t ring_frames) {

@@ -133,7 +133,7 @@ enum ArenaSlot {
     A_CHUNK_CELL, A_NVERTS, A_VALID, A_VALID_OFF, A_VERT_OFF, A_VALID_IDX,
     A_COUNTERS, A_SCAN_PARTIALS, A_STATS, A_INVOLVE, A_INVOLVE_OFF,
     A_INVOLVED_IDX, A_MEAN, A_SMOOTH, A_GMAT, A_SIGMA, A_AUX0, A_AUX1, A_REC,
-    A_ADJ_VERT, A_ADJ_COEF, A_ADJ_AUX,
+    A_ADJ_VERT, A_ADJ_COEF, A_ADJ_AUX, A_EDGE_MASK, A_EDGE_OFF,
     A_NUM_SLOTS
 };
 
@@ -335,8 +335,18 @@ struct McArgs {
     uint32_t capacity;
     uint64_t max_band;
     fsr_frame_stats* st;
+    // indexed (deduplicated) mode: capacity bounds the index count
+    const uint32_t* word_off;  // band rank of each mask word
+    uint32_t* edge_mask;       // [B] bit a: the edge (v, v + e_a) carries a mesh vertex
+    uint32_t* edge_off;        // [B] first mesh vertex of band vertex b
+    float* vtx_pos;            // float3[vtx_capacity]
+    float* vtx_nrm;            // float3[vtx_capacity] (may be null)
+    uint32_t vtx_capacity;
+    uint32_t* indices;         // u32[capacity]
 };
 fsr_status mc_run(fsr_context* ctx, const McArgs& a, hipStream_t s);
+// classify, scans, compaction, then the deduplicated mesh
+fsr_status mc_run_indexed(fsr_context* ctx, const McArgs& a, hipStream_t s);
 // single steps of mc_run (classify reads a.st->num_band_vertices, emit reads
 // a.st->num_valid_cubes)
 fsr_status mc_classify(fsr_context* ctx, const McArgs& a, hipStream_t s);

@@ -192,7 +192,214 @@ __global__ __launch_bounds__(256) void k_mc_emit(McArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Deduplicated mesh: one vertex per grid edge (v, v + e_axis) that is crossed by
+// the iso-surface and belongs to an emitting cube (the case tables place a
+// vertex on exactly the crossed edges of a cube, so these are the soup's
+// distinct vertices), owned by its lower end v.  Vertices are numbered in
+// ascending (band rank of v, axis x, y, z) order; each soup slot becomes an
+// index into them.  The vertex is interpolated from the lower end to the upper
+// one, so a soup vertex whose cube walks the edge the other way differs from
+// it only by the rounding of the lerp.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t band_rank(const uint32_t* __restrict__ bits,
+                                              const uint32_t* __restrict__ word_off,
+                                              uint32_t wx, uint32_t ry, uint32_t x, uint32_t y,
+                                              uint32_t z, bool* in) {
+    const uint64_t w = ((uint64_t)z * ry + y) * wx + (x >> 5);
+    const uint32_t word = bits[w];
+    *in = (word >> (x & 31u)) & 1u;
+    return word_off[w] + __popc(word & ((1u << (x & 31u)) - 1u));
+}
+
+template <bool kCpu>
+__device__ __forceinline__ float corner_coord(uint32_t i, float mn, float cell) {
+    return kCpu ? mn + (float)i * cell : vertex_coord(i, mn, cell);
+}
+
+// Central-difference normal at a grid vertex, normalised as the emitter does.
+template <bool kCpu>
+__device__ __forceinline__ void corner_normal(const float* __restrict__ phi, int X, int Y, int Z,
+                                              uint32_t rx, uint32_t ry, uint32_t rz, float* n) {
+    float nx = sample(phi, X - 1, Y, Z, rx, ry, rz) - sample(phi, X + 1, Y, Z, rx, ry, rz);
+    float ny = sample(phi, X, Y - 1, Z, rx, ry, rz) - sample(phi, X, Y + 1, Z, rx, ry, rz);
+    float nz = sample(phi, X, Y, Z - 1, rx, ry, rz) - sample(phi, X, Y, Z + 1, rx, ry, rz);
+    if (kCpu) {
+        const float len = __builtin_sqrtf(nx * nx + ny * ny + nz * nz);
+        if (len > 1.0e-7f) {
+            const float fac = 1.0f / len;
+            nx *= fac; ny *= fac; nz *= fac;
+        }
+    } else {
+        const float rs = __builtin_amdgcn_rsqf(__builtin_fmaf(nz, nz, __builtin_fmaf(ny, ny, nx * nx)));
+        nx *= rs; ny *= rs; nz *= rs;
+    }
+    n[0] = nx; n[1] = ny; n[2] = nz;
+}
+
+__global__ __launch_bounds__(256) void k_mc_edges(McArgs a) {
+    const uint32_t B = a.st->num_band_vertices;
+    const uint32_t res[3] = {a.sc.rx, a.sc.ry, a.sc.rz};
+    const uint64_t rxy = (uint64_t)res[0] * res[1];
+    const uint64_t stride[3] = {1u, res[0], rxy};
+    const uint32_t wx = row_words(res[0]);
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < B; b += gridDim.x * blockDim.x) {
+        const uint32_t vid = a.band_index[b];
+        const uint32_t c[3] = {vid % res[0], (uint32_t)((vid / res[0]) % res[1]),
+                               (uint32_t)(vid / rxy)};
+        const bool below = a.phi[vid] <= a.iso;
+        uint32_t mask = 0;
+        for (int ax = 0; ax < 3; ++ax) {
+            if (c[ax] + 1 >= res[ax]) continue;
+            if ((a.phi[vid + stride[ax]] <= a.iso) == below) continue;
+            const int p = ax == 0 ? 1 : 0, q = ax == 2 ? 1 : 2;
+            bool used = false;
+            for (uint32_t d = 0; d < 4 && !used; ++d) {
+                uint32_t u[3] = {c[0], c[1], c[2]};
+                const uint32_t dp = d & 1u, dq = d >> 1;
+                if (u[p] < dp || u[q] < dq) continue;
+                u[p] -= dp;
+                u[q] -= dq;
+                if (u[0] + 1 >= res[0] || u[1] + 1 >= res[1] || u[2] + 1 >= res[2]) continue;
+                bool in;
+                const uint32_t cb = band_rank(a.bits, a.word_off, wx, res[1], u[0], u[1], u[2], &in);
+                used = in && a.nverts[cb] > 0;
+            }
+            if (used) mask |= 1u << ax;
+        }
+        a.edge_mask[b] = mask;
+    }
+}
+
+template <bool kCpu>
+__global__ __launch_bounds__(256) void k_mc_vertices(McArgs a) {
+    const uint32_t B = a.st->num_band_vertices;
+    const uint32_t U = a.st->num_unique_vertices;
+    const uint32_t rx = a.sc.rx, ry = a.sc.ry, rz = a.sc.rz;
+    const uint64_t rxy = (uint64_t)rx * ry;
+    const float sign = (!kCpu && a.iso < 0.0f) ? -1.0f : 1.0f;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && U > a.vtx_capacity) a.st->overflow = 1u;
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < B; b += gridDim.x * blockDim.x) {
+        const uint32_t mask = a.edge_mask[b];
+        if (!mask) continue;
+        const uint32_t base = a.edge_off[b];
+        const uint32_t vid = a.band_index[b];
+        const int x = (int)(vid % rx), y = (int)((vid / rx) % ry), z = (int)(vid / rxy);
+        const float va = a.phi[vid];
+        const float pa[3] = {corner_coord<kCpu>(x, a.sc.minx, a.sc.cell),
+                             corner_coord<kCpu>(y, a.sc.miny, a.sc.cell),
+                             corner_coord<kCpu>(z, a.sc.minz, a.sc.cell)};
+        float na[3];
+        corner_normal<kCpu>(a.phi, x, y, z, rx, ry, rz, na);
+        uint32_t slot = base;
+        for (int ax = 0; ax < 3; ++ax) {
+            if (!((mask >> ax) & 1u)) continue;
+            const uint32_t k = slot++;
+            if (k >= a.vtx_capacity) continue;
+            const int X = x + (ax == 0), Y = y + (ax == 1), Z = z + (ax == 2);
+            const float vb = a.phi[((uint64_t)Z * ry + Y) * rx + X];
+            const float pb[3] = {corner_coord<kCpu>(X, a.sc.minx, a.sc.cell),
+                                 corner_coord<kCpu>(Y, a.sc.miny, a.sc.cell),
+                                 corner_coord<kCpu>(Z, a.sc.minz, a.sc.cell)};
+            float nb[3];
+            corner_normal<kCpu>(a.phi, X, Y, Z, rx, ry, rz, nb);
+            const float delta = vb - va;
+            float t;
+            if (kCpu) t = ((double)delta > -1.0e-7 && (double)delta < 1.0e-7) ? 0.5f : (a.iso - va) / delta;
+            else t = (delta > -1.0e-7f && delta < 1.0e-7f) ? 0.5f : (a.iso - va) / delta;
+            float p[3], n[3];
+            for (int i = 0; i < 3; ++i) {
+                if (kCpu) {
+                    p[i] = pa[i] + (pb[i] - pa[i]) * t;
+                    n[i] = na[i] + (nb[i] - na[i]) * t;
+                } else {
+                    p[i] = __builtin_fmaf(t, pb[i] - pa[i], pa[i]);
+                    n[i] = __builtin_fmaf(t, nb[i] - na[i], na[i]);
+                }
+            }
+            if (kCpu) {
+                const float len = __builtin_sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+                if (len > 1.0e-7f) {
+                    const float fac = 1.0f / len;
+                    n[0] *= fac; n[1] *= fac; n[2] *= fac;
+                }
+            } else {
+                const float rs = __builtin_amdgcn_rsqf(
+                    __builtin_fmaf(n[2], n[2], __builtin_fmaf(n[1], n[1], n[0] * n[0])));
+                n[0] = sign * (n[0] * rs); n[1] = sign * (n[1] * rs); n[2] = sign * (n[2] * rs);
+            }
+            const uint64_t o = 3ull * k;
+            a.vtx_pos[o + 0] = p[0];
+            a.vtx_pos[o + 1] = p[1];
+            a.vtx_pos[o + 2] = p[2];
+            if (a.vtx_nrm) {
+                a.vtx_nrm[o + 0] = n[0];
+                a.vtx_nrm[o + 1] = n[1];
+                a.vtx_nrm[o + 2] = n[2];
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mc_indices(McArgs a) {
+    const uint32_t Q = a.st->num_valid_cubes;
+    const uint32_t rx = a.sc.rx, ry = a.sc.ry;
+    const uint64_t rxy = (uint64_t)rx * ry;
+    const uint32_t wx = row_words(rx);
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < Q; q += gridDim.x * blockDim.x) {
+        const uint32_t b = a.valid_idx[q];
+        const uint32_t vid = a.band_index[b];
+        const uint32_t off = a.vert_off[b];
+        const uint32_t x = vid % rx, y = (uint32_t)((vid / rx) % ry), z = (uint32_t)(vid / rxy);
+        const uint32_t f = cube_flag(a.phi, vid, rx, rxy, a.iso);
+        const uint32_t nv = case_vertices(f);
+        if ((uint64_t)off + nv > a.capacity) continue;
+        const uint64_t tri = cMcTri[f];
+        for (uint32_t k = 0; k < nv; ++k) {
+            const uint32_t e = (uint32_t)(tri >> (4 * k)) & 0xFu;
+            const uint32_t s0 = (e < 8) ? (e & 3u) + (e & 4u) : e - 8u;
+            const uint32_t s1 = (e < 8) ? (((e & 3u) + 1u) & 3u) + (e & 4u) : e - 4u;
+            const uint32_t o0[3] = {s0 >= 4 ? 1u : 0u,
+                                    (s0 == 2 || s0 == 3 || s0 == 6 || s0 == 7) ? 1u : 0u,
+                                    (s0 == 1 || s0 == 2 || s0 == 5 || s0 == 6) ? 1u : 0u};
+            const uint32_t o1[3] = {s1 >= 4 ? 1u : 0u,
+                                    (s1 == 2 || s1 == 3 || s1 == 6 || s1 == 7) ? 1u : 0u,
+                                    (s1 == 1 || s1 == 2 || s1 == 5 || s1 == 6) ? 1u : 0u};
+            const int ax = (o0[0] != o1[0]) ? 0 : ((o0[1] != o1[1]) ? 1 : 2);
+            const bool lo0 = o0[ax] < o1[ax];
+            const uint32_t* lo = lo0 ? o0 : o1;
+            bool in;
+            const uint32_t ob = band_rank(a.bits, a.word_off, wx, ry, x + lo[0], y + lo[1],
+                                          z + lo[2], &in);
+            a.indices[off + k] = a.edge_off[ob] + __popc(a.edge_mask[ob] & ((1u << ax) - 1u));
+        }
+    }
+}
+
 }  // namespace
+
+fsr_status mc_run_indexed(fsr_context* ctx, const McArgs& a, hipStream_t s) {
+    FSR_TRY(mc_classify(ctx, a, s));
+    FSR_TRY(scan_exclusive(ctx, SCAN_U32, a.valid, a.valid_off, a.max_band,
+                           &a.st->num_band_vertices, &a.st->num_valid_cubes, s));
+    FSR_TRY(scan_exclusive(ctx, SCAN_U32, a.nverts, a.vert_off, a.max_band,
+                           &a.st->num_band_vertices, &a.st->num_mesh_vertices, s));
+    FSR_TRY(mc_compact(ctx, a, s));
+    const int blocks = persistent_blocks(ctx, 256, 8, TUNE_MC);
+    hipLaunchKernelGGL(k_mc_edges, dim3(blocks), dim3(256), 0, s, a);
+    FSR_LAUNCH_CHECK("k_mc_edges");
+    FSR_TRY(scan_exclusive(ctx, SCAN_POPC, a.edge_mask, a.edge_off, a.max_band,
+                           &a.st->num_band_vertices, &a.st->num_unique_vertices, s));
+    if (a.cpu_flavor)
+        hipLaunchKernelGGL(k_mc_vertices<true>, dim3(blocks), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_mc_vertices<false>, dim3(blocks), dim3(256), 0, s, a);
+    FSR_LAUNCH_CHECK("k_mc_vertices");
+    hipLaunchKernelGGL(k_mc_indices, dim3(blocks), dim3(256), 0, s, a);
+    FSR_LAUNCH_CHECK("k_mc_indices");
+    return FSR_OK;
+}
 
 fsr_status mc_classify(fsr_context* ctx, const McArgs& a, hipStream_t s) {
     const int blocks = persistent_blocks(ctx, 256, 8, TUNE_MC);

@@ -180,6 +180,47 @@ class Reconstructor:
         V = fr.sc.count
         return FrameResult(self._phi[:V], self._tri, self._nrm, stats)
 
+    def run_indexed(self, fr: Frame, vtx_capacity: int | None = None,
+                    idx_capacity: int | None = None, pos=None, dens=None):
+        """One frame with the deduplicated mesh (fsr_reconstruct_frame_indexed);
+        synchronises, grows the mesh buffers and reruns once on overflow.
+        Returns (phi, vertices (U, 3), normals (U, 3), indices (3T,), stats)."""
+        torch = self.torch
+        if pos is None:
+            pos = torch.from_numpy(fr.pos).to(self.device)
+            dens = torch.from_numpy(fr.dens).to(self.device)
+        vcap = vtx_capacity or max(1024, 2 * fr.n)
+        icap = idx_capacity or max(3 * 1024, 6 * fr.n)
+        p0, d0 = pos.clone(), dens.clone()
+        V = fr.sc.count
+        self._ensure(V, 1)
+        sim = self.params(fr)
+        ap = self.aniso_param()
+        spg, scg = grid_info(fr.sp), grid_info(fr.sc)
+        for attempt in range(2):
+            vtx = torch.empty((max(vcap, 1), 3), dtype=torch.float32, device=self.device)
+            nrm = torch.empty((max(vcap, 1), 3), dtype=torch.float32, device=self.device)
+            idx = torch.empty(max(icap, 1), dtype=torch.int32, device=self.device)
+            s = torch.cuda.current_stream(self.device)
+            _lib.check(self.ctx.lib.fsr_reconstruct_frame_indexed(
+                self.ctx.handle, METHODS[self.method], ctypes.byref(sim), ctypes.byref(ap),
+                ctypes.byref(spg), ctypes.byref(scg), ctypes.c_void_p(pos.data_ptr()),
+                ctypes.c_void_p(dens.data_ptr()), pos.shape[0],
+                ctypes.c_void_p(self._phi.data_ptr()), ctypes.c_void_p(vtx.data_ptr()),
+                ctypes.c_void_p(nrm.data_ptr()), vcap, ctypes.c_void_p(idx.data_ptr()), icap,
+                ctypes.c_void_p(self._stats.data_ptr()), ctypes.c_void_p(s.cuda_stream)))
+            torch.cuda.synchronize(self.device)
+            stats = stats_dict(self._stats)
+            if not stats["overflow"] or attempt == 1:
+                break
+            vcap = stats["num_unique_vertices"]
+            icap = stats["num_mesh_vertices"]
+            pos.copy_(p0)
+            dens.copy_(d0)
+        self.pos_sorted, self.dens_sorted = pos, dens
+        U, T3 = stats["num_unique_vertices"], stats["num_mesh_vertices"]
+        return self._phi[:V], vtx[:U], nrm[:U], idx[:T3], stats
+
     def buffer(self, name: str, dtype, count: int):
         """Copy of an intermediate buffer of the last frame (numpy)."""
         torch = self.torch

@@ -115,7 +115,8 @@ typedef struct {
     uint32_t num_mesh_vertices;      /* 3T: soup vertices required */
     uint32_t overflow;               /* nonzero: num_mesh_vertices exceeded the capacity */
     uint32_t band_exceptions;        /* surface particles handled by the exact band path */
-    uint32_t reserved[7];
+    uint32_t num_unique_vertices;    /* U: vertices of the deduplicated mesh (indexed mode) */
+    uint32_t reserved[6];
 } fsr_frame_stats;
 
 /* ------------------------------------------------------------------------- */
@@ -175,6 +176,27 @@ fsr_status fsr_reconstruct_frame(fsr_context* ctx, fsr_method method,
                                  float* phi, float* tri_pos, float* tri_nrm,
                                  uint32_t tri_capacity, fsr_frame_stats* stats,
                                  void* stream);
+
+/* fsr_reconstruct_frame with a deduplicated mesh instead of the soup (no
+ * counterpart in the reference, whose generateTriangles, ReconstructionCUDA.cu:
+ * 383-463, emits a soup): one vertex per grid edge that is crossed by the
+ * iso-surface and used by an emitted cube, numbered in ascending order of
+ * (lower grid vertex, axis x, y, z), interpolated from the lower end of its
+ * edge; and indices[3T] into those vertices in the soup's order (triangle k =
+ * indices[3k..3k+2]).
+ *   vtx_pos / vtx_nrm [out] float3[vtx_capacity] (vtx_nrm may be NULL);
+ *   indices           [out] u32[idx_capacity].
+ * stats->num_unique_vertices = U and stats->num_mesh_vertices = 3T; overflow is
+ * set when U > vtx_capacity or 3T > idx_capacity (then nothing beyond the
+ * capacities is written).  Frames of this entry point keep no adjoint state. */
+fsr_status fsr_reconstruct_frame_indexed(fsr_context* ctx, fsr_method method,
+                                         const fsr_sim_param* sim, const fsr_aniso_param* aniso,
+                                         const fsr_grid_info* sp_grid,
+                                         const fsr_grid_info* sc_grid, float* pos, float* dens,
+                                         uint32_t num_particles, float* phi, float* vtx_pos,
+                                         float* vtx_nrm, uint32_t vtx_capacity,
+                                         uint32_t* indices, uint32_t idx_capacity,
+                                         fsr_frame_stats* stats, void* stream);
 
 /* Intermediate buffers of the last frame run on `ctx`, for inspection and
  * parity checks.  The pointer stays valid until the next call that grows the

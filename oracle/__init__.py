@@ -78,6 +78,10 @@ def load():
     lib.orc_field_aniso.argtypes = [u32p, ctypes.c_uint32, f64p, f32p, u32p, u32p, u32p, f64p,
                                     OGrid, OGrid, ctypes.c_float, ctypes.c_void_p, f64p,
                                     ctypes.c_void_p]
+    lib.orc_mc_indexed.argtypes = [u32p, ctypes.c_uint32, u8p, f64p, OGrid, ctypes.c_float,
+                                   ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64, u64p]
+    lib.orc_mc_indexed.restype = ctypes.c_uint64
     lib.orc_mc_vertex_backward.argtypes = [u32p, ctypes.c_uint32, u8p, f64p, OGrid,
                                            ctypes.c_float, ctypes.c_int, f64p, ctypes.c_uint64,
                                            f64p]
@@ -301,6 +305,25 @@ def reference_mc_tables():
         edge_vertices=arr(lib.ref_mc_edge_vertices(), ctypes.c_uint32, 24).reshape(12, 2),
         corner_offsets=arr(lib.ref_mc_corner_offsets(), ctypes.c_int32, 24).reshape(8, 3),
     )
+
+
+def mc_indexed(index, band_u8, phi, sc, iso, cpu=False):
+    """Deduplicated mesh over `phi`: (vertices (U, 3), normals (U, 3), indices (3T,))."""
+    lib = load()
+    index = np.ascontiguousarray(index, np.uint32)
+    B = index.shape[0]
+    phi = np.ascontiguousarray(phi, np.float64)
+    band_u8 = np.ascontiguousarray(band_u8, np.uint8)
+    nidx = np.zeros(1, np.uint64)
+    U = int(lib.orc_mc_indexed(index, B, band_u8, phi, ogrid(sc), iso, int(cpu), None, None, 0,
+                               None, 0, nidx))
+    T3 = int(nidx[0])
+    pos = np.empty((U, 3), np.float64)
+    nrm = np.empty((U, 3), np.float64)
+    idx = np.empty(T3, np.uint32)
+    lib.orc_mc_indexed(index, B, band_u8, phi, ogrid(sc), iso, int(cpu), pos.ctypes.data,
+                       nrm.ctypes.data, U, idx.ctypes.data, T3, nidx)
+    return pos, nrm, idx
 
 
 def mc_vertex_backward(index, band_u8, phi, sc, iso, g_tri, cpu=False, g_phi=None):

@@ -1118,3 +1118,139 @@ double orc_adj_fd(orc_adj* A, uint32_t j, int axis, double eps) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Deduplicated marching-cubes mesh (no reference counterpart; the reference
+// emits a soup).  One vertex per grid edge that is crossed and used by a valid
+// cube, numbered by (lower grid vertex ascending, axis x, y, z), interpolated
+// from the lower end in FP64; indices in the soup's order.
+// ---------------------------------------------------------------------------
+extern "C" {
+
+uint64_t orc_mc_indexed(const uint32_t* band_index, uint32_t B, const uint8_t* band,
+                        const double* phi, orc_grid sc, float iso, int cpu, double* vtx_pos,
+                        double* vtx_nrm, uint64_t vtx_capacity, uint32_t* indices,
+                        uint64_t idx_capacity, uint64_t* num_indices) {
+    const uint64_t rx = sc.rx, ry = sc.ry, rz = sc.rz, rxy = rx * ry;
+    const uint64_t res[3] = {rx, ry, rz};
+    const uint64_t stride[3] = {1, rx, rxy};
+    const double isod = iso;
+    std::vector<uint32_t> nverts(B);
+    orc_mc(band_index, B, band, phi, sc, iso, cpu, nverts.data(), nullptr, nullptr, 0);
+    auto bpos = [&](uint64_t vid) -> int64_t {
+        const uint32_t* it = std::lower_bound(band_index, band_index + B, (uint32_t)vid);
+        return (it != band_index + B && *it == vid) ? (int64_t)(it - band_index) : -1;
+    };
+    auto cube_valid = [&](uint64_t x, uint64_t y, uint64_t z) {
+        if (x + 1 >= rx || y + 1 >= ry || z + 1 >= rz) return false;
+        const int64_t b = bpos((z * ry + y) * rx + x);
+        return b >= 0 && nverts[b] > 0;
+    };
+    // vertex ids per (band position, axis)
+    std::vector<int64_t> id(3ull * B, -1);
+    uint64_t U = 0;
+    for (uint32_t b = 0; b < B; ++b) {
+        const uint64_t vid = band_index[b];
+        const uint64_t c[3] = {vid % rx, (vid / rx) % ry, vid / rxy};
+        const bool below = phi[vid] <= isod;
+        for (int ax = 0; ax < 3; ++ax) {
+            if (c[ax] + 1 >= res[ax]) continue;
+            if ((phi[vid + stride[ax]] <= isod) == below) continue;
+            const int p = ax == 0 ? 1 : 0, q = ax == 2 ? 1 : 2;
+            bool used = false;
+            for (int d = 0; d < 4 && !used; ++d) {
+                uint64_t u[3] = {c[0], c[1], c[2]};
+                const uint64_t dp = d & 1, dq = d >> 1;
+                if (u[p] < dp || u[q] < dq) continue;
+                u[p] -= dp;
+                u[q] -= dq;
+                used = cube_valid(u[0], u[1], u[2]);
+            }
+            if (used) id[3ull * b + ax] = (int64_t)U++;
+        }
+    }
+    auto sample = [&](int64_t X, int64_t Y, int64_t Z) -> double {
+        if (X < 0 || Y < 0 || Z < 0 || X >= (int64_t)rx || Y >= (int64_t)ry || Z >= (int64_t)rz)
+            return 0.0;
+        return phi[((uint64_t)Z * ry + Y) * rx + X];
+    };
+    auto normal = [&](int64_t X, int64_t Y, int64_t Z, double n[3]) {
+        n[0] = sample(X - 1, Y, Z) - sample(X + 1, Y, Z);
+        n[1] = sample(X, Y - 1, Z) - sample(X, Y + 1, Z);
+        n[2] = sample(X, Y, Z - 1) - sample(X, Y, Z + 1);
+        const double len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        if (cpu) {
+            if (len > 1e-7) for (int a = 0; a < 3; ++a) n[a] /= len;
+        } else {
+            for (int a = 0; a < 3; ++a) n[a] *= 1.0 / len;
+        }
+    };
+    const float mins[3] = {sc.minx, sc.miny, sc.minz};
+    const double sign = (!cpu && iso < 0.0f) ? -1.0 : 1.0;
+    if (vtx_pos && U <= vtx_capacity) {
+        for (uint32_t b = 0; b < B; ++b) {
+            const uint64_t vid = band_index[b];
+            const int64_t c[3] = {(int64_t)(vid % rx), (int64_t)((vid / rx) % ry),
+                                  (int64_t)(vid / rxy)};
+            for (int ax = 0; ax < 3; ++ax) {
+                const int64_t k = id[3ull * b + ax];
+                if (k < 0) continue;
+                int64_t e[3] = {c[0], c[1], c[2]};
+                e[ax] += 1;
+                const double va = phi[vid], vb = phi[vid + stride[ax]];
+                const double delta = vb - va;
+                const double t = (delta > -1e-7 && delta < 1e-7) ? 0.5 : (isod - va) / delta;
+                double na[3], nb[3], nn[3];
+                normal(c[0], c[1], c[2], na);
+                normal(e[0], e[1], e[2], nb);
+                for (int a = 0; a < 3; ++a) {
+                    const double pa = cpu ? (double)(mins[a] + (float)c[a] * sc.cell)
+                                          : (double)vpos((uint32_t)c[a], mins[a], sc.cell);
+                    const double pb = cpu ? (double)(mins[a] + (float)e[a] * sc.cell)
+                                          : (double)vpos((uint32_t)e[a], mins[a], sc.cell);
+                    vtx_pos[3 * k + a] = pa + t * (pb - pa);
+                    nn[a] = na[a] + t * (nb[a] - na[a]);
+                }
+                const double len = std::sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+                if (cpu) {
+                    if (len > 1e-7) for (int a = 0; a < 3; ++a) nn[a] /= len;
+                } else {
+                    for (int a = 0; a < 3; ++a) nn[a] = sign * (nn[a] * (1.0 / len));
+                }
+                if (vtx_nrm) for (int a = 0; a < 3; ++a) vtx_nrm[3 * k + a] = nn[a];
+            }
+        }
+    }
+    uint64_t total = 0;
+    for (uint32_t b = 0; b < B; ++b) total += nverts[b];
+    if (num_indices) *num_indices = total;
+    if (indices && total <= idx_capacity) {
+        uint64_t off = 0;
+        for (uint32_t b = 0; b < B; ++b) {
+            const uint32_t nv = nverts[b];
+            if (!nv) continue;
+            const uint64_t vid = band_index[b];
+            const uint64_t x = vid % rx, y = (vid / rx) % ry, z = vid / rxy;
+            uint32_t f = 0;
+            for (int c = 0; c < 8; ++c) {
+                const uint64_t v = ((z + kCornerOff[c][2]) * ry + (y + kCornerOff[c][1])) * rx +
+                                   (x + kCornerOff[c][0]);
+                if (phi[v] <= isod) f |= 1u << c;
+            }
+            for (uint32_t k = 0; k < nv; ++k) {
+                const int e = (int)((kTri[f] >> (4 * k)) & 0xF);
+                const int s0 = kEdgeCorner[e][0], s1 = kEdgeCorner[e][1];
+                int ax = 0;
+                while (kCornerOff[s0][ax] == kCornerOff[s1][ax]) ++ax;
+                const int lo = kCornerOff[s0][ax] < kCornerOff[s1][ax] ? s0 : s1;
+                const uint64_t lv = ((z + kCornerOff[lo][2]) * ry + (y + kCornerOff[lo][1])) * rx +
+                                    (x + kCornerOff[lo][0]);
+                indices[off + k] = (uint32_t)id[3ull * bpos(lv) + ax];
+            }
+            off += nv;
+        }
+    }
+    return U;
+}
+
+}  // extern "C"

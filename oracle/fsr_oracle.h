@@ -95,6 +95,16 @@ int orc_frame(int method, float* pos, float* dens, uint32_t n, orc_grid sp, orc_
 /* Case tables (for the table parity test). */
 uint64_t orc_mc_tri_packed(uint32_t c);
 
+/* Deduplicated mesh (no reference counterpart): one vertex per crossed grid
+ * edge used by a valid cube, ordered by (lower grid vertex, axis), interpolated
+ * from the lower end in FP64; indices in the soup's order.  Returns U; writes
+ * the vertices when U <= vtx_capacity and the indices when 3T <= idx_capacity;
+ * *num_indices = 3T. */
+uint64_t orc_mc_indexed(const uint32_t* band_index, uint32_t B, const uint8_t* band,
+                        const double* phi, orc_grid sc, float iso, int cpu, double* vtx_pos,
+                        double* vtx_nrm, uint64_t vtx_capacity, uint32_t* indices,
+                        uint64_t idx_capacity, uint64_t* num_indices);
+
 /* ---- Adjoint (no reference counterpart), FP64, topology held fixed ------- */
 
 /* g_phi[V] += vertex term of the marching cubes over phi (soup order and the
