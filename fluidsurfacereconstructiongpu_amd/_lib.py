@@ -62,6 +62,13 @@ class AnisoParam(ctypes.Structure):
     _fields_ = [("kr", c_float), ("ks", c_float), ("kn", c_float), ("reserved", c_uint32)]
 
 
+class XyzHeader(ctypes.Structure):
+    """fsr_xyz_header: the five header lines of a .xyz frame."""
+
+    _fields_ = [("aabb_min", Float3), ("aabb_max", Float3), ("kernel_radius", c_float),
+                ("particle_radius", c_float), ("particle_mass", c_float)]
+
+
 class FrameStats(ctypes.Structure):
     _fields_ = [
         ("num_particles", c_uint32),
@@ -201,6 +208,18 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.fsr_set_profiling.restype = st
     lib.fsr_stage_times.argtypes = [c_void_p, POINTER(c_float), c_uint32, POINTER(c_uint32)]
     lib.fsr_stage_times.restype = st
+    lib.fsr_xyz_info.argtypes = [c_char_p, POINTER(XyzHeader), POINTER(c_uint32)]
+    lib.fsr_xyz_info.restype = st
+    lib.fsr_xyz_read.argtypes = [c_char_p, POINTER(XyzHeader), c_void_p, c_void_p, c_uint32,
+                                 POINTER(c_uint32)]
+    lib.fsr_xyz_read.restype = st
+    lib.fsr_obj_write_soup.argtypes = [c_char_p, c_void_p, c_void_p, c_uint32]
+    lib.fsr_obj_write_soup.restype = st
+    lib.fsr_obj_write_indexed.argtypes = [c_char_p, c_void_p, c_void_p, c_uint32, c_void_p,
+                                          c_uint32]
+    lib.fsr_obj_write_indexed.restype = st
+    lib.fsr_sim_param_for.argtypes = [c_int, POINTER(XyzHeader), POINTER(SimParam)]
+    lib.fsr_sim_param_for.restype = st
     lib.fsr_set_adjoint.argtypes = [c_void_p, c_int]
     lib.fsr_set_adjoint.restype = st
     lib.fsr_mc_vertex_backward.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]

@@ -28,9 +28,10 @@
 // is read exactly by the vertices of virtual cells q - 1 .. q + 1.  Neighbour
 // lists are not stored: the gather walks the same cell ranges the forward used.
 // Per pair: t by the cell-local expansion (4 ops), t^2, t^3, the dot product
-// (3), and three accumulations (7): 18 VALU ops.  Results are written in the
-// caller's particle order through the sort permutation; particles outside the
-// hash grid get 0.
+// (3), and three accumulations (7): 18 VALU ops.  Vertices that contribute
+// exactly zero to every particle of the chunk are culled at staging.  Results
+// are written in the caller's particle order through the sort permutation;
+// particles outside the hash grid get 0.
 #include "common.hpp"
 #include "mc_tables.h"
 
@@ -193,16 +194,67 @@ struct FieldBackArgs {
     float* g_pos;                 // float3[N], caller order
 };
 
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fminf(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+struct GradSums {
+    float g1x, g1y, g1z, g2x, g2y, g2z, sk;
+};
+
+// n staged vertices (a multiple of 8) against this lane's particle.
+__device__ __forceinline__ void gather_pairs(const float4* __restrict__ qx,
+                                             const float4* __restrict__ qu, uint32_t n,
+                                             float px, float py, float pz, float ax, float ay,
+                                             float az, float c0, GradSums& g) {
+    for (uint32_t t0 = 0; t0 < n; t0 += 8u) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const float4 xv = qx[t0 + t];
+            const float4 uv = qu[t0 + t];
+            float tt = __builtin_fmaf(az, xv.z, c0 - xv.w);
+            tt = __builtin_fmaf(ay, xv.y, tt);
+            tt = __builtin_fmaf(ax, xv.x, tt);
+            const float tp = fmaxf(tt, 0.0f);
+            const float t2 = tp * tp;
+            const float w = t2 * tp;
+            const float kd = __builtin_fmaf(uv.x, px, __builtin_fmaf(uv.y, py, __builtin_fmaf(uv.z, pz, uv.w)));
+            const float kk = kd * t2;
+            g.g1x = __builtin_fmaf(w, uv.x, g.g1x);
+            g.g1y = __builtin_fmaf(w, uv.y, g.g1y);
+            g.g1z = __builtin_fmaf(w, uv.z, g.g1z);
+            g.g2x = __builtin_fmaf(kk, xv.x, g.g2x);
+            g.g2y = __builtin_fmaf(kk, xv.y, g.g2y);
+            g.g2z = __builtin_fmaf(kk, xv.z, g.g2z);
+            g.sk += kk;
+        }
+    }
+}
+
+// Vertices whose coefficient is zero, or that lie at least 1.00005 R from the
+// bounding box of the chunk's particles, add exactly 0 to every lane and are
+// dropped at staging (order-preserving compaction, as in the forward kernel).
 __global__ __launch_bounds__(kAdjThreads) void k_field_zb05_backward(FieldBackArgs a) {
-    __shared__ float4 stage_x[kAdjThreads / 64][64];
-    __shared__ float4 stage_u[kAdjThreads / 64][64];
+    __shared__ float4 queue_x[kAdjThreads / 64][128];
+    __shared__ float4 queue_u[kAdjThreads / 64][128];
     __shared__ uint32_t row_start[kAdjThreads / 64][9];
     __shared__ uint32_t row_pref[kAdjThreads / 64][10];
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t nch = *a.num_chunks;
     const float inv_r2 = 1.0f / a.radius_sq;
+    const float cull_r2 = a.radius_sq * 1.0001f;
     const uint32_t rxy = a.sc.rx * a.sc.ry;
+    float4* qx = queue_x[wave];
+    float4* qu = queue_u[wave];
     for (;;) {
         uint32_t ch = 0;
         if (lane == 0) ch = atomicAdd(a.next, 1u);
@@ -213,14 +265,14 @@ __global__ __launch_bounds__(kAdjThreads) void k_field_zb05_backward(FieldBackAr
         const fsr_index_range own = a.ranges[c];
         const uint32_t i = own.start + 64u * k + lane;
         const bool live = i < own.end;
-        const uint32_t qx = c % a.sp.rx, qy = (c / a.sp.rx) % a.sp.ry, qz = c / (a.sp.rx * a.sp.ry);
+        const uint32_t qcx = c % a.sp.rx, qcy = (c / a.sp.rx) % a.sp.ry, qcz = c / (a.sp.rx * a.sp.ry);
 
         // 9 runs of band slots: virtual cells (q + radius) + {-1, 0, 1}^3
         uint32_t len = 0, rs = 0;
         if (lane < 9) {
-            const uint32_t vy = qy + a.radius + (lane % 3) - 1;
-            const uint32_t vz = qz + a.radius + (lane / 3) - 1;
-            const uint32_t v0 = (vz * a.vcy + vy) * a.vcx + qx + a.radius - 1;
+            const uint32_t vy = qcy + a.radius + (lane % 3) - 1;
+            const uint32_t vz = qcz + a.radius + (lane / 3) - 1;
+            const uint32_t v0 = (vz * a.vcy + vy) * a.vcx + qcx + a.radius - 1;
             rs = a.cell_off[v0];
             len = a.cell_off[v0 + 2] + a.cell_cnt[v0 + 2] - rs;
         }
@@ -232,9 +284,9 @@ __global__ __launch_bounds__(kAdjThreads) void k_field_zb05_backward(FieldBackAr
         }
         wave_lds_sync();
 
-        const float ox = __builtin_fmaf((float)qx, a.sp.cell, a.sp.minx);
-        const float oy = __builtin_fmaf((float)qy, a.sp.cell, a.sp.miny);
-        const float oz = __builtin_fmaf((float)qz, a.sp.cell, a.sp.minz);
+        const float ox = __builtin_fmaf((float)qcx, a.sp.cell, a.sp.minx);
+        const float oy = __builtin_fmaf((float)qcy, a.sp.cell, a.sp.miny);
+        const float oz = __builtin_fmaf((float)qcz, a.sp.cell, a.sp.minz);
         float px = 0.f, py = 0.f, pz = 0.f;
         if (live) {
             const float4 p = a.sorted[i];
@@ -242,59 +294,74 @@ __global__ __launch_bounds__(kAdjThreads) void k_field_zb05_backward(FieldBackAr
         }
         const float ax = 2.0f * px * inv_r2, ay = 2.0f * py * inv_r2, az = 2.0f * pz * inv_r2;
         const float c0 = 1.0f - __builtin_fmaf(pz, pz, __builtin_fmaf(py, py, px * px)) * inv_r2;
+        const float bx0 = wave_min(live ? px : 3.0e38f), bx1 = wave_max(live ? px : -3.0e38f);
+        const float by0 = wave_min(live ? py : 3.0e38f), by1 = wave_max(live ? py : -3.0e38f);
+        const float bz0 = wave_min(live ? pz : 3.0e38f), bz1 = wave_max(live ? pz : -3.0e38f);
 
-        float g1x = 0.f, g1y = 0.f, g1z = 0.f, g2x = 0.f, g2y = 0.f, g2z = 0.f, sk = 0.f;
+        GradSums g = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        uint32_t fill = 0;
         for (uint32_t base = 0; base < total; base += 64u) {
-            const uint32_t g = base + lane;
-            float4 sx = make_float4(0.f, 0.f, 0.f, 1.0e30f);
-            float4 su = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (g < total) {
+            const uint32_t gi = base + lane;
+            bool keep = false;
+            float4 sx, su;
+            if (gi < total) {
                 int j = 0;
-                while (j + 1 < 9 && row_pref[wave][j + 1] <= g) ++j;
-                const uint32_t slot = row_start[wave][j] + (g - row_pref[wave][j]);
-                const uint32_t vid = a.cell_band[slot];
-                const uint32_t ix = vid % a.sc.rx, iy = (vid / a.sc.rx) % a.sc.ry, iz = vid / rxy;
-                const float xx = vertex_coord(ix, a.sc.minx, a.sc.cell) - ox;
-                const float xy = vertex_coord(iy, a.sc.miny, a.sc.cell) - oy;
-                const float xz = vertex_coord(iz, a.sc.minz, a.sc.cell) - oz;
+                while (j + 1 < 9 && row_pref[wave][j + 1] <= gi) ++j;
+                const uint32_t slot = row_start[wave][j] + (gi - row_pref[wave][j]);
                 const float4 cf = a.coef[slot];
-                sx = make_float4(xx, xy, xz,
-                                 __builtin_fmaf(xz, xz, __builtin_fmaf(xy, xy, xx * xx)) * inv_r2);
-                // c - u . x' (the pair adds u . p')
-                su = make_float4(cf.x, cf.y, cf.z,
-                                 cf.w - __builtin_fmaf(cf.z, xz, __builtin_fmaf(cf.y, xy, cf.x * xx)));
+                if (cf.x != 0.0f || cf.y != 0.0f || cf.z != 0.0f || cf.w != 0.0f) {
+                    const uint32_t vid = a.cell_band[slot];
+                    const uint32_t ix = vid % a.sc.rx, iy = (vid / a.sc.rx) % a.sc.ry, iz = vid / rxy;
+                    const float xx = vertex_coord(ix, a.sc.minx, a.sc.cell) - ox;
+                    const float xy = vertex_coord(iy, a.sc.miny, a.sc.cell) - oy;
+                    const float xz = vertex_coord(iz, a.sc.minz, a.sc.cell) - oz;
+                    const float dx = fmaxf(fmaxf(bx0 - xx, xx - bx1), 0.0f);
+                    const float dy = fmaxf(fmaxf(by0 - xy, xy - by1), 0.0f);
+                    const float dz = fmaxf(fmaxf(bz0 - xz, xz - bz1), 0.0f);
+                    keep = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < cull_r2;
+                    sx = make_float4(xx, xy, xz,
+                                     __builtin_fmaf(xz, xz, __builtin_fmaf(xy, xy, xx * xx)) * inv_r2);
+                    // c - u . x' (the pair adds u . p')
+                    su = make_float4(cf.x, cf.y, cf.z,
+                                     cf.w - __builtin_fmaf(cf.z, xz, __builtin_fmaf(cf.y, xy, cf.x * xx)));
+                }
             }
-            stage_x[wave][lane] = sx;
-            stage_u[wave][lane] = su;
+            const uint64_t m = __ballot(keep);
+            if (keep) {
+                const uint32_t o = fill + mask_rank(m);
+                qx[o] = sx;
+                qu[o] = su;
+            }
+            fill += (uint32_t)__popcll(m);
+            if (fill >= 64u) {
+                wave_lds_sync();
+                gather_pairs(qx, qu, 64u, px, py, pz, ax, ay, az, c0, g);
+                wave_lds_sync();
+                const uint32_t rem = fill - 64u;
+                if (lane < rem) {
+                    qx[lane] = qx[64u + lane];
+                    qu[lane] = qu[64u + lane];
+                }
+                wave_lds_sync();
+                fill = rem;
+            }
+        }
+        if (fill) {
+            const uint32_t n = (fill + 7u) & ~7u;
+            if (lane >= fill && lane < n) {
+                qx[lane] = make_float4(0.f, 0.f, 0.f, 1.0e30f);
+                qu[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
             wave_lds_sync();
-#pragma unroll 8
-            for (int t = 0; t < 64; ++t) {
-                const float4 xv = stage_x[wave][t];
-                const float4 uv = stage_u[wave][t];
-                float tt = __builtin_fmaf(az, xv.z, c0 - xv.w);
-                tt = __builtin_fmaf(ay, xv.y, tt);
-                tt = __builtin_fmaf(ax, xv.x, tt);
-                const float tp = fmaxf(tt, 0.0f);
-                const float t2 = tp * tp;
-                const float w = t2 * tp;
-                const float kd = __builtin_fmaf(uv.x, px, __builtin_fmaf(uv.y, py, __builtin_fmaf(uv.z, pz, uv.w)));
-                const float kk = kd * t2;
-                g1x = __builtin_fmaf(w, uv.x, g1x);
-                g1y = __builtin_fmaf(w, uv.y, g1y);
-                g1z = __builtin_fmaf(w, uv.z, g1z);
-                g2x = __builtin_fmaf(kk, xv.x, g2x);
-                g2y = __builtin_fmaf(kk, xv.y, g2y);
-                g2z = __builtin_fmaf(kk, xv.z, g2z);
-                sk += kk;
-            }
+            gather_pairs(qx, qu, n, px, py, pz, ax, ay, az, c0, g);
             wave_lds_sync();
         }
         if (live) {
             const float s6 = 6.0f * inv_r2;
             const uint64_t o = 3ull * a.perm[i];
-            a.g_pos[o + 0] = -__builtin_fmaf(s6, __builtin_fmaf(-sk, px, g2x), g1x);
-            a.g_pos[o + 1] = -__builtin_fmaf(s6, __builtin_fmaf(-sk, py, g2y), g1y);
-            a.g_pos[o + 2] = -__builtin_fmaf(s6, __builtin_fmaf(-sk, pz, g2z), g1z);
+            a.g_pos[o + 0] = -__builtin_fmaf(s6, __builtin_fmaf(-g.sk, px, g.g2x), g.g1x);
+            a.g_pos[o + 1] = -__builtin_fmaf(s6, __builtin_fmaf(-g.sk, py, g.g2y), g.g1y);
+            a.g_pos[o + 2] = -__builtin_fmaf(s6, __builtin_fmaf(-g.sk, pz, g.g2z), g.g1z);
         }
     }
 }

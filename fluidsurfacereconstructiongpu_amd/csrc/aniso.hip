@@ -42,6 +42,13 @@ namespace {
 constexpr int kAnisoThreads = 256;
 constexpr int kMaxRows = 25;
 constexpr float kSigma = 1.5666814710608448f;  // 315 / (64 pi)
+// Candidate culling of the field kernel: a particle whose support ellipsoid
+// (radius 1 / sigma_min(G)) stays 1% (in squared radius) away from the bounding
+// box of a chunk's vertices has |G r|^2 >= 1.01 at every vertex of the chunk;
+// the quadratic form's FP32 rounding there is below 1e-4, so its weight is
+// exactly 0 and dropping it leaves the sums bit-identical.
+constexpr float kCullMargin = 1.01f;
+constexpr float kNoCull = 3.0e38f;
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) {
     return v < lo ? lo : (v > hi ? hi : v);
@@ -197,9 +204,10 @@ __global__ __launch_bounds__(256) void k_involved(
     if (c < C && involve[c] > 0) {
         involved_idx[involve_off[c] + (i - ranges[c].start)] = i;
     } else {
-        // M = I, unscaled kernel (anisotropicWOpt)
+        // M = I, unscaled kernel (anisotropicWOpt): its support (|r| < 1 in world
+        // units) covers the whole candidate block, so it is never culled
         rec[2ull * i + 0] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-        rec[2ull * i + 1] = make_float4(0.0f, 0.0f, kSigma * (mass / sorted[i].w), 0.0f);
+        rec[2ull * i + 1] = make_float4(0.0f, 0.0f, kSigma * (mass / sorted[i].w), kNoCull);
     }
 }
 
@@ -323,11 +331,13 @@ __global__ __launch_bounds__(kAnisoThreads) void k_covariance(CovArgs a, ChunkLi
         const uint32_t gi = a.involve_off[c] + (i - own.start);
         float G[3][3];
         float4 sig;
+        float gmin;  // smallest singular value of G: the support is |r| < 1 / gmin
         if (cnt < a.min_neighbors) {
             const float gv = a.inv_h / a.kn;
             for (int p = 0; p < 3; ++p)
                 for (int q = 0; q < 3; ++q) G[p][q] = p == q ? gv : 0.0f;
             sig = make_float4(0.f, 0.f, 0.f, 0.f);
+            gmin = gv;
         } else {
             const float iw = 1.0f / sw;
             const float a00 = (a.h2 + c00) * iw, a11 = (a.h2 + c11) * iw, a22 = (a.h2 + c22) * iw;
@@ -344,6 +354,7 @@ __global__ __launch_bounds__(kAnisoThreads) void k_covariance(CovArgs a, ChunkLi
             s2 = fmaxf(s2, floor_v);
             const float scale = cbrtf(s0 * s1 * s2) * a.inv_h;
             const float g0 = scale / s0, g1 = scale / s1, g2 = scale / s2;
+            gmin = fminf(g0, fminf(g1, g2));
             for (int p = 0; p < 3; ++p)
                 for (int q = 0; q < 3; ++q)
                     G[p][q] = V[0][p] * g0 * V[0][q] + V[1][p] * g1 * V[1][q] + V[2][p] * g2 * V[2][q];
@@ -364,7 +375,8 @@ __global__ __launch_bounds__(kAnisoThreads) void k_covariance(CovArgs a, ChunkLi
                 M[p][q] = G[0][p] * G[0][q] + G[1][p] * G[1][q] + G[2][p] * G[2][q];
         const float coef = kSigma * det * (a.mass / a.sorted[i].w);
         a.rec[2ull * i + 0] = make_float4(M[0][0], M[1][1], M[2][2], M[0][1]);
-        a.rec[2ull * i + 1] = make_float4(M[0][2], M[1][2], coef, 0.0f);
+        // squared culling radius of the field kernel, with a 1% margin
+        a.rec[2ull * i + 1] = make_float4(M[0][2], M[1][2], coef, kCullMargin / (gmin * gmin));
         }
     }
 }
@@ -381,14 +393,61 @@ struct AnisoFieldArgs {
     const uint32_t* num_chunks;
 };
 
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fminf(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+struct VertexTerms {
+    float xx, yy, zz, xy2, xz2, yz2, mx2, my2, mz2;
+};
+
+// n staged candidates (a multiple of 8) against this lane's vertex.
+__device__ __forceinline__ float aniso_pairs(const float4* __restrict__ q0s,
+                                             const float4* __restrict__ q1s,
+                                             const float4* __restrict__ q2s, uint32_t n,
+                                             const VertexTerms& v, float sum) {
+    for (uint32_t t0 = 0; t0 < n; t0 += 8u) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const float4 q0 = q0s[t0 + t];
+            const float4 q1 = q1s[t0 + t];
+            const float4 q2 = q2s[t0 + t];
+            // |G(p - v)|^2 = v^T M v - 2 v.(M p) + p^T M p
+            float d2 = __builtin_fmaf(q0.x, v.xx, q2.y);
+            d2 = __builtin_fmaf(q0.y, v.yy, d2);
+            d2 = __builtin_fmaf(q0.z, v.zz, d2);
+            d2 = __builtin_fmaf(q0.w, v.xy2, d2);
+            d2 = __builtin_fmaf(q1.x, v.xz2, d2);
+            d2 = __builtin_fmaf(q1.y, v.yz2, d2);
+            d2 = __builtin_fmaf(q1.z, v.mx2, d2);
+            d2 = __builtin_fmaf(q1.w, v.my2, d2);
+            d2 = __builtin_fmaf(q2.x, v.mz2, d2);
+            const float x = fmaxf(1.0f - d2, 0.0f);
+            sum = __builtin_fmaf(q2.z, x * x * x, sum);
+        }
+    }
+    return sum;
+}
+
 __global__ __launch_bounds__(kAnisoThreads) void k_field_aniso(AnisoFieldArgs a) {
-    __shared__ float4 stage[kAnisoThreads / 64][3][64];
+    __shared__ float4 queue[kAnisoThreads / 64][3][128];
     __shared__ uint32_t rstart[kAnisoThreads / 64][kMaxRows];
     __shared__ uint32_t rpref[kAnisoThreads / 64][kMaxRows];
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t nch = *a.num_chunks;
     const int R = a.bc.radius;
+    float4* q0s = queue[wave][0];
+    float4* q1s = queue[wave][1];
+    float4* q2s = queue[wave][2];
     for (;;) {
         const uint32_t ch = next_chunk(a.next);
         if (ch >= nch) break;
@@ -407,7 +466,8 @@ __global__ __launch_bounds__(kAnisoThreads) void k_field_aniso(AnisoFieldArgs a)
         const float oz = __builtin_fmaf((float)cz, a.sp.cell, a.sp.minz);
         uint32_t vid = 0;
         float vx = 0.f, vy = 0.f, vz = 0.f;
-        if (lane < cnt) {
+        const bool live = lane < cnt;
+        if (live) {
             vid = a.bc.cell_band[first + lane];
             const uint32_t ix = vid % a.sc.rx;
             const uint32_t iy = (vid / a.sc.rx) % a.sc.ry;
@@ -416,20 +476,29 @@ __global__ __launch_bounds__(kAnisoThreads) void k_field_aniso(AnisoFieldArgs a)
             vy = vertex_coord(iy, a.sc.miny, a.sc.cell) - oy;
             vz = vertex_coord(iz, a.sc.minz, a.sc.cell) - oz;
         }
-        const float xx = vx * vx, yy = vy * vy, zz = vz * vz;
-        const float xy2 = 2.0f * vx * vy, xz2 = 2.0f * vx * vz, yz2 = 2.0f * vy * vz;
-        const float mx2 = -2.0f * vx, my2 = -2.0f * vy, mz2 = -2.0f * vz;
+        VertexTerms vt;
+        vt.xx = vx * vx; vt.yy = vy * vy; vt.zz = vz * vz;
+        vt.xy2 = 2.0f * vx * vy; vt.xz2 = 2.0f * vx * vz; vt.yz2 = 2.0f * vy * vz;
+        vt.mx2 = -2.0f * vx; vt.my2 = -2.0f * vy; vt.mz2 = -2.0f * vz;
+        const float bx0 = wave_min(live ? vx : 3.0e38f), bx1 = wave_max(live ? vx : -3.0e38f);
+        const float by0 = wave_min(live ? vy : 3.0e38f), by1 = wave_max(live ? vy : -3.0e38f);
+        const float bz0 = wave_min(live ? vz : 3.0e38f), bz1 = wave_max(live ? vz : -3.0e38f);
         float sum = 0.f;
+        uint32_t fill = 0;
         for (uint32_t base = 0; base < total; base += 64u) {
             const uint32_t g = base + lane;
-            // padding candidate: |G r|^2 = 2 (outside the support) and zero weight
-            float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0, s2 = make_float4(0.f, 2.0f, 0.f, 0.f);
+            bool keep = false;
+            float4 s0, s1, s2;
             if (g < total) {
                 const uint32_t j = candidate_index(g, nrows, rstart[wave], rpref[wave]);
                 const float4 p4 = a.smooth[j];
                 const float4 r0 = a.rec[2ull * j + 0];
                 const float4 r1 = a.rec[2ull * j + 1];
                 const float px = p4.x - ox, py = p4.y - oy, pz = p4.z - oz;
+                const float dx = fmaxf(fmaxf(bx0 - px, px - bx1), 0.0f);
+                const float dy = fmaxf(fmaxf(by0 - py, py - by1), 0.0f);
+                const float dz = fmaxf(fmaxf(bz0 - pz, pz - bz1), 0.0f);
+                keep = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < r1.w;
                 const float M00 = r0.x, M11 = r0.y, M22 = r0.z, M01 = r0.w, M02 = r1.x, M12 = r1.y;
                 const float bx = M00 * px + M01 * py + M02 * pz;
                 const float by = M01 * px + M11 * py + M12 * pz;
@@ -439,31 +508,41 @@ __global__ __launch_bounds__(kAnisoThreads) void k_field_aniso(AnisoFieldArgs a)
                 s1 = make_float4(M02, M12, bx, by);
                 s2 = make_float4(bz, cc, r1.z, 0.f);
             }
-            stage[wave][0][lane] = s0;
-            stage[wave][1][lane] = s1;
-            stage[wave][2][lane] = s2;
-            wave_lds_sync();
-#pragma unroll 4
-            for (int t = 0; t < 64; ++t) {
-                const float4 q0 = stage[wave][0][t];
-                const float4 q1 = stage[wave][1][t];
-                const float4 q2 = stage[wave][2][t];
-                // |G(p - v)|^2 = v^T M v - 2 v.(M p) + p^T M p
-                float d2 = __builtin_fmaf(q0.x, xx, q2.y);
-                d2 = __builtin_fmaf(q0.y, yy, d2);
-                d2 = __builtin_fmaf(q0.z, zz, d2);
-                d2 = __builtin_fmaf(q0.w, xy2, d2);
-                d2 = __builtin_fmaf(q1.x, xz2, d2);
-                d2 = __builtin_fmaf(q1.y, yz2, d2);
-                d2 = __builtin_fmaf(q1.z, mx2, d2);
-                d2 = __builtin_fmaf(q1.w, my2, d2);
-                d2 = __builtin_fmaf(q2.x, mz2, d2);
-                const float x = fmaxf(1.0f - d2, 0.0f);
-                sum = __builtin_fmaf(q2.z, x * x * x, sum);
+            const uint64_t m = __ballot(keep);
+            if (keep) {
+                const uint32_t o = fill + mask_rank(m);
+                q0s[o] = s0;
+                q1s[o] = s1;
+                q2s[o] = s2;
+            }
+            fill += (uint32_t)__popcll(m);
+            if (fill >= 64u) {
+                wave_lds_sync();
+                sum = aniso_pairs(q0s, q1s, q2s, 64u, vt, sum);
+                wave_lds_sync();
+                const uint32_t rem = fill - 64u;
+                if (lane < rem) {
+                    q0s[lane] = q0s[64u + lane];
+                    q1s[lane] = q1s[64u + lane];
+                    q2s[lane] = q2s[64u + lane];
+                }
+                wave_lds_sync();
+                fill = rem;
+            }
+        }
+        if (fill) {
+            const uint32_t n = (fill + 7u) & ~7u;
+            if (lane >= fill && lane < n) {
+                // padding: |G r|^2 = 2 (outside the support) and zero weight
+                q0s[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+                q1s[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+                q2s[lane] = make_float4(0.f, 2.0f, 0.f, 0.f);
             }
             wave_lds_sync();
+            sum = aniso_pairs(q0s, q1s, q2s, n, vt, sum);
+            wave_lds_sync();
         }
-        if (lane < cnt) a.phi[vid] = sum - 0.5f;
+        if (live) a.phi[vid] = sum - 0.5f;
     }
 }
 

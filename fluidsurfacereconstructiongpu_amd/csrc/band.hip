@@ -176,60 +176,66 @@ __global__ void k_axis_maps(GridDev sc, GridDev sp, int radius, AxisMaps m) {
     }
 }
 
-// Thread per scalar row (y, z): counts, per x-virtual-cell, the band bits of
-// that row and adds them to the cell's count.
-__global__ __launch_bounds__(256) void k_cell_count_rows(const uint32_t* __restrict__ bits,
-                                                         GridDev sc, uint32_t wx, AxisMaps m,
-                                                         uint32_t vcx, uint32_t vcy,
-                                                         uint32_t* __restrict__ cnt) {
-    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= sc.ry * sc.rz) return;
-    const uint32_t y = row % sc.ry, z = row / sc.ry;
-    const uint32_t* rb = bits + (uint64_t)row * wx;
-    const uint32_t vbase = (m.vc[2][z] * vcy + m.vc[1][y]) * vcx;
-    for (uint32_t w = 0; w < wx; ++w) {
-        uint32_t b = rb[w];
-        while (b) {
-            const uint32_t x = w * 32u + __builtin_ctz(b);
-            const uint32_t cx = m.vc[0][x];
-            // all bits of this word inside the same x cell at once
-            const uint32_t hi = m.hi[0][cx];
-            const uint32_t end = hi < (w + 1) * 32u ? hi - w * 32u : 32u;
-            const uint32_t sel = end >= 32u ? b : (b & ((1u << end) - 1u));
-            atomicAdd(&cnt[vbase + cx], (uint32_t)__popc(sel));
-            b &= ~sel;
-        }
-    }
-}
-
-__global__ __launch_bounds__(256) void k_cell_fill_rows(const uint32_t* __restrict__ bits,
-                                                        GridDev sc, uint32_t wx, AxisMaps m,
-                                                        uint32_t vcx, uint32_t vcy,
-                                                        const uint32_t* __restrict__ cell_off,
-                                                        uint32_t* __restrict__ cursor,
-                                                        uint32_t* __restrict__ cell_band) {
-    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= sc.ry * sc.rz) return;
-    const uint32_t y = row % sc.ry, z = row / sc.ry;
-    const uint32_t* rb = bits + (uint64_t)row * wx;
-    const uint32_t vbase = (m.vc[2][z] * vcy + m.vc[1][y]) * vcx;
-    const uint32_t lin0 = row * sc.rx;
-    for (uint32_t w = 0; w < wx; ++w) {
-        uint32_t b = rb[w];
-        while (b) {
-            const uint32_t x = w * 32u + __builtin_ctz(b);
-            const uint32_t cx = m.vc[0][x];
-            const uint32_t hi = m.hi[0][cx];
-            const uint32_t end = hi < (w + 1) * 32u ? hi - w * 32u : 32u;
-            uint32_t sel = end >= 32u ? b : (b & ((1u << end) - 1u));
-            b &= ~sel;
-            const uint32_t v = vbase + cx;
-            uint32_t o = cell_off[v] + atomicAdd(&cursor[v], (uint32_t)__popc(sel));
-            while (sel) {
-                cell_band[o++] = lin0 + w * 32u + __builtin_ctz(sel);
-                sel &= sel - 1u;
+// One wave per virtual cell: the cell's scalar rows (y, z) in row-major order,
+// each row's band bits in [xlo, xhi).  Pass 0 writes the cell's band vertex
+// count; pass 1 (kFill) writes the cell's vertex ids in ascending order, so a
+// 64-vertex chunk is a compact slab of the cell (1-2 z-layers), which keeps the
+// candidate culling of the field kernels effective.  No atomics: the list is
+// deterministic.
+template <bool kFill>
+__global__ __launch_bounds__(256) void k_cell_rows(const uint32_t* __restrict__ bits, GridDev sc,
+                                                   uint32_t wx, AxisMaps m, uint32_t vcx,
+                                                   uint32_t vcy, uint32_t nvc,
+                                                   uint32_t* __restrict__ cnt,
+                                                   const uint32_t* __restrict__ cell_off,
+                                                   uint32_t* __restrict__ cell_band) {
+    const uint32_t lane = __lane_id();
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t v = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; v < nvc; v += nwaves) {
+        const uint32_t cx = v % vcx, cy = (v / vcx) % vcy, cz = v / (vcx * vcy);
+        const uint32_t xlo = m.lo[0][cx], xhi = m.hi[0][cx];
+        const uint32_t ylo = m.lo[1][cy], yhi = m.hi[1][cy];
+        const uint32_t zlo = m.lo[2][cz], zhi = m.hi[2][cz];
+        const uint32_t ny = yhi - ylo;
+        const uint32_t nrows = (xhi > xlo) ? ny * (zhi - zlo) : 0u;
+        const uint32_t base = kFill ? cell_off[v] : 0u;
+        const uint32_t w0 = xlo >> 5, w1 = (xhi + 31u) >> 5;
+        uint32_t running = 0;
+        for (uint32_t r0 = 0; r0 < nrows; r0 += 64u) {
+            const uint32_t r = r0 + lane;
+            uint32_t c = 0;
+            const uint32_t* rb = bits;
+            uint32_t lin0 = 0;
+            if (r < nrows) {
+                const uint32_t y = ylo + r % ny, z = zlo + r / ny;
+                const uint64_t row = (uint64_t)z * sc.ry + y;
+                rb = bits + row * wx;
+                lin0 = (uint32_t)(row * sc.rx);
+                for (uint32_t w = w0; w < w1; ++w) {
+                    uint32_t word = rb[w];
+                    const uint32_t lo = w * 32u, hi = lo + 32u;
+                    if (xlo > lo) word &= ~((1u << (xlo - lo)) - 1u);
+                    if (xhi < hi) word &= (1u << (xhi - lo)) - 1u;
+                    c += __popc(word);
+                }
             }
+            const uint32_t incl = wave_inclusive_sum(c);
+            if (kFill && c) {
+                uint32_t o = base + running + incl - c;
+                for (uint32_t w = w0; w < w1; ++w) {
+                    uint32_t word = rb[w];
+                    const uint32_t lo = w * 32u, hi = lo + 32u;
+                    if (xlo > lo) word &= ~((1u << (xlo - lo)) - 1u);
+                    if (xhi < hi) word &= (1u << (xhi - lo)) - 1u;
+                    while (word) {
+                        cell_band[o++] = lin0 + lo + __builtin_ctz(word);
+                        word &= word - 1u;
+                    }
+                }
+            }
+            running += __shfl(incl, 63, 64);
         }
+        if (!kFill && lane == 0) cnt[v] = running;
     }
 }
 
@@ -300,12 +306,11 @@ fsr_status band_compact(fsr_context* ctx, const uint32_t* bits, const GridDev& s
     // cell-major work list
     const uint32_t vcx = sp.rx + 2 * radius, vcy = sp.ry + 2 * radius, vcz = sp.rz + 2 * radius;
     const uint64_t nvc = (uint64_t)vcx * vcy * vcz;
-    uint32_t *maps, *cnt, *cell_off, *cursor, *chunk_off, *cell_band, *chunk_cell;
+    uint32_t *maps, *cnt, *cell_off, *chunk_off, *cell_band, *chunk_cell;
     const uint64_t map_len = (uint64_t)sc.rx + sc.ry + sc.rz + 2ull * (vcx + vcy + vcz);
     FSR_TRY(arena_typed(ctx, A_AXIS_MAPS, map_len, &maps));
     FSR_TRY(arena_typed(ctx, A_CELL_CNT, nvc, &cnt));
     FSR_TRY(arena_typed(ctx, A_CELL_OFF, nvc, &cell_off));
-    FSR_TRY(arena_typed(ctx, A_CELL_CHUNKS, nvc, &cursor));
     FSR_TRY(arena_typed(ctx, A_CHUNK_OFF, nvc, &chunk_off));
     FSR_TRY(arena_typed(ctx, A_CELL_BAND, V, &cell_band));
     const uint64_t max_chunks = V / 64 + nvc + 1;
@@ -321,21 +326,20 @@ fsr_status band_compact(fsr_context* ctx, const uint32_t* bits, const GridDev& s
     m.lo[2] = m.hi[1] + vcy;
     m.hi[2] = m.lo[2] + vcz;
     FSR_TRY(fill_u32(m.lo[0], 0u, 2ull * (vcx + vcy + vcz), s));
-    FSR_TRY(fill_u32(cnt, 0u, nvc, s));
-    FSR_TRY(fill_u32(cursor, 0u, nvc, s));
     const uint32_t maxres = std::max(sc.rx, std::max(sc.ry, sc.rz));
     hipLaunchKernelGGL(k_axis_maps, dim3((maxres + 255) / 256, 3), dim3(256), 0, s, sc, sp, radius, m);
     FSR_LAUNCH_CHECK("k_axis_maps");
-    const uint32_t rows = sc.ry * sc.rz;
-    hipLaunchKernelGGL(k_cell_count_rows, dim3((rows + 255) / 256), dim3(256), 0, s, bits, sc, wx,
-                       m, vcx, vcy, cnt);
-    FSR_LAUNCH_CHECK("k_cell_count_rows");
+    // one wave per virtual cell, 4 per block
+    const uint32_t cell_blocks = (uint32_t)std::min<uint64_t>((nvc + 3) / 4, 65535u * 4u);
+    hipLaunchKernelGGL(k_cell_rows<false>, dim3(cell_blocks), dim3(256), 0, s, bits, sc, wx, m,
+                       vcx, vcy, (uint32_t)nvc, cnt, (const uint32_t*)nullptr, cell_band);
+    FSR_LAUNCH_CHECK("k_cell_rows(count)");
     FSR_TRY(scan_exclusive(ctx, SCAN_U32, cnt, cell_off, nvc, nullptr, nullptr, s));
     FSR_TRY(scan_exclusive(ctx, SCAN_CHUNKS64, cnt, chunk_off, nvc, nullptr,
                            &counters->num_chunks, s));
-    hipLaunchKernelGGL(k_cell_fill_rows, dim3((rows + 255) / 256), dim3(256), 0, s, bits, sc, wx,
-                       m, vcx, vcy, cell_off, cursor, cell_band);
-    FSR_LAUNCH_CHECK("k_cell_fill_rows");
+    hipLaunchKernelGGL(k_cell_rows<true>, dim3(cell_blocks), dim3(256), 0, s, bits, sc, wx, m,
+                       vcx, vcy, (uint32_t)nvc, cnt, cell_off, cell_band);
+    FSR_LAUNCH_CHECK("k_cell_rows(fill)");
     hipLaunchKernelGGL(k_chunk_cells, dim3((uint32_t)((nvc + 255) / 256)), dim3(256), 0, s, cnt,
                        chunk_off, (uint32_t)nvc, chunk_cell);
     FSR_LAUNCH_CHECK("k_chunk_cells");

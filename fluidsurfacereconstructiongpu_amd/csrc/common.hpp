@@ -129,7 +129,7 @@ enum ArenaSlot {
     A_PACKED = 0, A_SORTED, A_KEYS0, A_KEYS1, A_VALS0, A_VALS1, A_RADIX_HIST,
     A_RANGES, A_OCCUPANCY, A_SURF_CELL, A_SURF_FLAGS, A_SBITS, A_BAND_TMP,
     A_BAND_BITS, A_EXCEPTIONS, A_WORD_OFF, A_BAND_INDEX, A_AXIS_MAPS,
-    A_CELL_CNT, A_CELL_OFF, A_CELL_CHUNKS, A_CHUNK_OFF, A_CELL_BAND,
+    A_CELL_CNT, A_CELL_OFF, A_CHUNK_OFF, A_CELL_BAND,
     A_CHUNK_CELL, A_NVERTS, A_VALID, A_VALID_OFF, A_VERT_OFF, A_VALID_IDX,
     A_COUNTERS, A_SCAN_PARTIALS, A_STATS, A_INVOLVE, A_INVOLVE_OFF,
     A_INVOLVED_IDX, A_MEAN, A_SMOOTH, A_GMAT, A_SIGMA, A_AUX0, A_AUX1, A_REC,

@@ -24,6 +24,16 @@
 // make |x - xbar| free of the cancellation the absolute-coordinate sum has.
 // Candidates are visited in the reference's order (z, y, x cells, then range
 // order).  Work is distributed dynamically, one chunk per atomic fetch.
+//
+// Culling: a chunk's vertices are a compact slab of its cell (the cell-major
+// list is in ascending vertex id), so many block candidates lie farther than R
+// from the chunk's bounding box.  At staging every candidate is tested against
+// that box with R^2 (1 + 1e-4); survivors are compacted in order (ballot rank)
+// into a 128-entry LDS queue and the inner loop runs over full groups of 64.
+// A culled candidate is at least 1.00005 R from every vertex of the chunk, so
+// its computed t is negative (the rounding of t is below 1e-5 there) and its
+// weight exactly 0: the sums, and so phi, are bit-identical to evaluating the
+// whole block, in the same order.
 #include "common.hpp"
 
 namespace fsr {
@@ -42,15 +52,53 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fminf(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+struct Sums {
+    float w, x, y, z;
+};
+
+// n staged candidates (a multiple of 16) against this lane's vertex.
+__device__ __forceinline__ void accumulate(const float4* __restrict__ q, uint32_t n, float ax,
+                                           float ay, float az, float c0, Sums& s) {
+    for (uint32_t t0 = 0; t0 < n; t0 += 16u) {
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const float4 p = q[t0 + t];
+            float tt = __builtin_fmaf(az, p.z, c0 - p.w);
+            tt = __builtin_fmaf(ay, p.y, tt);
+            tt = __builtin_fmaf(ax, p.x, tt);
+            const float w = fmaxf(tt, 0.0f);
+            const float w3 = w * w * w;
+            s.w += w3;
+            s.x = __builtin_fmaf(w3, p.x, s.x);
+            s.y = __builtin_fmaf(w3, p.y, s.y);
+            s.z = __builtin_fmaf(w3, p.z, s.z);
+        }
+    }
+}
+
 __global__ __launch_bounds__(kFieldThreads) void k_field_zb05(FieldArgs a, BandCompactOut bc) {
-    __shared__ float4 stage[kFieldThreads / 64][64];
+    __shared__ float4 queue[kFieldThreads / 64][128];
     __shared__ uint32_t row_start[kFieldThreads / 64][9];
     __shared__ uint32_t row_pref[kFieldThreads / 64][10];
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t num_chunks = a.counters->num_chunks;
     const float inv_r2 = 1.0f / a.radius_sq;
+    const float cull_r2 = a.radius_sq * 1.0001f;
     const int r = bc.radius;
+    float4* q = queue[wave];
 
     for (;;) {
         uint32_t ch = 0;
@@ -113,48 +161,62 @@ __global__ __launch_bounds__(kFieldThreads) void k_field_zb05(FieldArgs a, BandC
         }
         const float ax = 2.0f * vx * inv_r2, ay = 2.0f * vy * inv_r2, az = 2.0f * vz * inv_r2;
         const float c0 = 1.0f - __builtin_fmaf(vz, vz, __builtin_fmaf(vy, vy, vx * vx)) * inv_r2;
+        // bounding box of the chunk's vertices
+        const bool live = lane < cnt;
+        const float bx0 = wave_min(live ? vx : 3.0e38f), bx1 = wave_max(live ? vx : -3.0e38f);
+        const float by0 = wave_min(live ? vy : 3.0e38f), by1 = wave_max(live ? vy : -3.0e38f);
+        const float bz0 = wave_min(live ? vz : 3.0e38f), bz1 = wave_max(live ? vz : -3.0e38f);
 
-        float sw = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+        Sums sum = {0.f, 0.f, 0.f, 0.f};
+        uint32_t fill = 0;
         for (uint32_t base = 0; base < total; base += 64u) {
             const uint32_t g = base + lane;
-            float4 st = make_float4(0.f, 0.f, 0.f, 1.0e30f);
+            bool keep = false;
+            float4 st;
             if (g < total) {
                 int j = 0;
                 while (j + 1 < nrows && row_pref[wave][j + 1] <= g) ++j;
                 const float4 p = a.sorted[row_start[wave][j] + (g - row_pref[wave][j])];
                 const float px = p.x - ox, py = p.y - oy, pz = p.z - oz;
+                const float dx = fmaxf(fmaxf(bx0 - px, px - bx1), 0.0f);
+                const float dy = fmaxf(fmaxf(by0 - py, py - by1), 0.0f);
+                const float dz = fmaxf(fmaxf(bz0 - pz, pz - bz1), 0.0f);
+                keep = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < cull_r2;
                 st = make_float4(px, py, pz,
                                  __builtin_fmaf(pz, pz, __builtin_fmaf(py, py, px * px)) * inv_r2);
             }
-            stage[wave][lane] = st;
-            wave_lds_sync();
-#pragma unroll 16
-            for (int t = 0; t < 64; ++t) {
-                const float4 p = stage[wave][t];
-                float tt = __builtin_fmaf(az, p.z, c0 - p.w);
-                tt = __builtin_fmaf(ay, p.y, tt);
-                tt = __builtin_fmaf(ax, p.x, tt);
-                const float w = fmaxf(tt, 0.0f);
-                const float w3 = w * w * w;
-                sw += w3;
-                sx = __builtin_fmaf(w3, p.x, sx);
-                sy = __builtin_fmaf(w3, p.y, sy);
-                sz = __builtin_fmaf(w3, p.z, sz);
+            const uint64_t m = __ballot(keep);
+            if (keep) q[fill + mask_rank(m)] = st;
+            fill += (uint32_t)__popcll(m);
+            if (fill >= 64u) {
+                wave_lds_sync();
+                accumulate(q, 64u, ax, ay, az, c0, sum);
+                wave_lds_sync();
+                const uint32_t rem = fill - 64u;
+                if (lane < rem) q[lane] = q[64u + lane];
+                wave_lds_sync();
+                fill = rem;
             }
+        }
+        if (fill) {
+            const uint32_t n = (fill + 15u) & ~15u;
+            if (lane >= fill && lane < n) q[lane] = make_float4(0.f, 0.f, 0.f, 1.0e30f);
+            wave_lds_sync();
+            accumulate(q, n, ax, ay, az, c0, sum);
             wave_lds_sync();
         }
         if (lane < cnt) {
-            float4 keep = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (sw > 0.0f) {
-                const float inv = 1.0f / sw;
-                const float dx = vx - sx * inv, dy = vy - sy * inv, dz = vz - sz * inv;
+            float4 keep_out = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (sum.w > 0.0f) {
+                const float inv = 1.0f / sum.w;
+                const float dx = vx - sum.x * inv, dy = vy - sum.y * inv, dz = vz - sum.z * inv;
                 a.phi[vid] =
                     __builtin_sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx))) -
                     a.particle_radius;
-                keep = make_float4(dx, dy, dz, sw);
+                keep_out = make_float4(dx, dy, dz, sum.w);
             }
             // for the adjoint: x - xbar and sum w, at the vertex's cell-major slot
-            if (a.saved) a.saved[first + lane] = keep;
+            if (a.saved) a.saved[first + lane] = keep_out;
         }
     }
 }

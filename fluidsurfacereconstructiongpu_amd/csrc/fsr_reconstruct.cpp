@@ -1,0 +1,237 @@
+// fsr_reconstruct: the reference's frame-sequence driver on libfsr.
+//
+// Mirrors ReconstructorGPU::reconstruct (GPU/ReconstructorGPU.cpp:25-79): for
+// every frame in [from, to) it reads <directory><pattern % frame>.xyz, derives
+// the grids and constants as the chosen reconstructor class does, times the
+// reconstruction from spatial hashing to the end of marching cubes, writes the
+// mesh as <...>.obj, and at the end appends the per-frame times to
+// "<algorithm type>.time" and, with --config, writes "<algorithm type>.config"
+// (GPU/ReconstructorGPU.cpp:315-393: mean triangle count, surface particle and
+// surface vertex ratios, cumulative stage times).
+//
+//   fsr_reconstruct <directory/> <pattern> <from> <to>
+//                   [--method zb05|akinci12|yu13] [--no-obj] [--indexed]
+//                   [--config] [--device N]
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "../../include/fsr.h"
+
+namespace {
+
+#define CHECK_FSR(expr)                                                                  \
+    do {                                                                                 \
+        fsr_status _s = (expr);                                                          \
+        if (_s != FSR_OK) {                                                              \
+            std::fprintf(stderr, "%s failed: %s (%s)\n", #expr, fsr_status_string(_s),   \
+                         fsr_last_error());                                              \
+            std::exit(1);                                                                \
+        }                                                                                \
+    } while (0)
+
+#define CHECK_HIP(expr)                                                                  \
+    do {                                                                                 \
+        hipError_t _e = (expr);                                                          \
+        if (_e != hipSuccess) {                                                          \
+            std::fprintf(stderr, "%s failed: %s\n", #expr, hipGetErrorString(_e));       \
+            std::exit(1);                                                                \
+        }                                                                                \
+    } while (0)
+
+// Device buffer that only grows.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void* get(size_t want) {
+        if (want > bytes) {
+            if (p) CHECK_HIP(hipFree(p));
+            CHECK_HIP(hipMalloc(&p, want));
+            bytes = want;
+        }
+        return p;
+    }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+const char* algorithm_type(fsr_method m) {
+    // getAlgorithmType of the three reconstructor classes
+    switch (m) {
+        case FSR_METHOD_ZB05: return "Our Algorithm using ZB05 kernel";
+        case FSR_METHOD_AKINCI12: return "Akinci12 using ZB05 kernel";
+        default: return "Our Algorithm using anisotropic kernel";
+    }
+}
+
+void usage() {
+    std::fprintf(stderr,
+                 "usage: fsr_reconstruct <directory/> <pattern> <from> <to> "
+                 "[--method zb05|akinci12|yu13] [--no-obj] [--indexed] [--config] [--device N]\n");
+    std::exit(2);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 5) usage();
+    const std::string dir = argv[1];
+    const std::string pattern = argv[2];
+    const long from = std::strtol(argv[3], nullptr, 10), to = std::strtol(argv[4], nullptr, 10);
+    fsr_method method = FSR_METHOD_ZB05;
+    bool save_obj = true, indexed = false, save_cfg = false;
+    int device = 0;
+    for (int i = 5; i < argc; ++i) {
+        const std::string a = argv[i];
+        if (a == "--method" && i + 1 < argc) {
+            const std::string m = argv[++i];
+            if (m == "zb05") method = FSR_METHOD_ZB05;
+            else if (m == "akinci12") method = FSR_METHOD_AKINCI12;
+            else if (m == "yu13") method = FSR_METHOD_YU13;
+            else usage();
+        } else if (a == "--no-obj") {
+            save_obj = false;
+        } else if (a == "--indexed") {
+            indexed = true;
+        } else if (a == "--config") {
+            save_cfg = true;
+        } else if (a == "--device" && i + 1 < argc) {
+            device = std::atoi(argv[++i]);
+        } else {
+            usage();
+        }
+    }
+    if (from >= to) {
+        std::printf("Wrong settings occur for frame range.\n");
+        return 1;
+    }
+    fsr_context* ctx = nullptr;
+    CHECK_FSR(fsr_create(&ctx, device));
+    CHECK_FSR(fsr_set_profiling(ctx, 1));
+    DevBuf d_pos, d_dens, d_phi, d_tri, d_nrm, d_idx, d_stats;
+    std::vector<double> times;
+    double sum_tri = 0, sum_spr = 0, sum_svr = 0, sum_es = 0, sum_sf = 0, sum_mc = 0;
+    uint32_t cap = 0, vcap = 0;
+    for (long frame = from; frame < to; ++frame) {
+        char base[512];
+        std::snprintf(base, sizeof(base), pattern.c_str(), (unsigned)frame);
+        const std::string stem = dir + base;
+        std::printf("\n*************************** Frame %ld ***************************\n", frame);
+        fsr_xyz_header h;
+        uint32_t n = 0;
+        CHECK_FSR(fsr_xyz_info((stem + ".xyz").c_str(), &h, &n));
+        std::vector<float> pos(3ull * n), dens(n);
+        CHECK_FSR(fsr_xyz_read((stem + ".xyz").c_str(), &h, pos.data(), dens.data(), n, &n));
+        fsr_sim_param sim;
+        CHECK_FSR(fsr_sim_param_for(method, &h, &sim));
+        fsr_grid_info sp, sc;
+        CHECK_FSR(fsr_grid_from_aabb(h.aabb_min, h.aabb_max, sim.smoothing_radius,
+                                     sim.spatial_cell_size_scale, sim.sc_sp_grid_res_ratio, &sp,
+                                     &sc));
+        const uint64_t V = (uint64_t)sc.resolution.x * sc.resolution.y * sc.resolution.z;
+        std::printf("Number of particles: %g k.\n", n / 1000.0);
+        std::printf("Scalar Field Resolution: %u * %u * %u\n", sc.resolution.x, sc.resolution.y,
+                    sc.resolution.z);
+        if (!cap) cap = 6u * n + 3072u;
+        if (!vcap) vcap = 2u * n + 1024u;
+        float* pp = static_cast<float*>(d_pos.get(12ull * (n ? n : 1)));
+        float* pd = static_cast<float*>(d_dens.get(4ull * (n ? n : 1)));
+        float* phi = static_cast<float*>(d_phi.get(4ull * V));
+        fsr_frame_stats* st = static_cast<fsr_frame_stats*>(d_stats.get(sizeof(fsr_frame_stats)));
+        fsr_frame_stats hs;
+        double ms = 0;
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            CHECK_HIP(hipMemcpy(pp, pos.data(), 12ull * n, hipMemcpyHostToDevice));
+            CHECK_HIP(hipMemcpy(pd, dens.data(), 4ull * n, hipMemcpyHostToDevice));
+            CHECK_HIP(hipDeviceSynchronize());
+            const auto t0 = std::chrono::steady_clock::now();
+            if (indexed) {
+                CHECK_FSR(fsr_reconstruct_frame_indexed(
+                    ctx, method, &sim, nullptr, &sp, &sc, pp, pd, n, phi,
+                    static_cast<float*>(d_tri.get(12ull * vcap)),
+                    static_cast<float*>(d_nrm.get(12ull * vcap)), vcap,
+                    static_cast<uint32_t*>(d_idx.get(4ull * cap)), cap, st, nullptr));
+            } else {
+                CHECK_FSR(fsr_reconstruct_frame(ctx, method, &sim, nullptr, &sp, &sc, pp, pd, n,
+                                                phi, static_cast<float*>(d_tri.get(12ull * cap)),
+                                                static_cast<float*>(d_nrm.get(12ull * cap)), cap,
+                                                st, nullptr));
+            }
+            CHECK_HIP(hipMemcpy(&hs, st, sizeof(hs), hipMemcpyDeviceToHost));
+            const auto t1 = std::chrono::steady_clock::now();
+            ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+            if (!hs.overflow) break;
+            cap = hs.num_mesh_vertices + 3072u;
+            vcap = hs.num_unique_vertices + 1024u;
+        }
+        float stage[FSR_NUM_STAGES];
+        uint32_t nf = 0;
+        CHECK_FSR(fsr_stage_times(ctx, stage, 1, &nf));
+        times.push_back(ms);
+        std::printf("Reconstructing Frame%ld took %g milliseconds.\n", frame, ms);
+        const uint32_t T3 = hs.num_mesh_vertices;
+        sum_tri += T3 / 3;
+        sum_spr += n ? (double)hs.num_surface_particles / n : 0.0;
+        sum_svr += V ? (double)hs.num_band_vertices / (double)V : 0.0;
+        if (nf) {
+            sum_es += stage[FSR_STAGE_SURFACE] + stage[FSR_STAGE_BAND];
+            sum_sf += stage[FSR_STAGE_FILL] + stage[FSR_STAGE_FIELD];
+            sum_mc += stage[FSR_STAGE_MC];
+        }
+        if (save_obj && T3) {
+            const std::string path = stem + ".obj";
+            if (indexed) {
+                const uint32_t U = hs.num_unique_vertices;
+                std::vector<float> vp(3ull * U), vn(3ull * U);
+                std::vector<uint32_t> idx(T3);
+                CHECK_HIP(hipMemcpy(vp.data(), d_tri.p, 12ull * U, hipMemcpyDeviceToHost));
+                CHECK_HIP(hipMemcpy(vn.data(), d_nrm.p, 12ull * U, hipMemcpyDeviceToHost));
+                CHECK_HIP(hipMemcpy(idx.data(), d_idx.p, 4ull * T3, hipMemcpyDeviceToHost));
+                CHECK_FSR(fsr_obj_write_indexed(path.c_str(), vp.data(), vn.data(), U, idx.data(),
+                                                T3));
+            } else {
+                std::vector<float> tp(3ull * T3), tn(3ull * T3);
+                CHECK_HIP(hipMemcpy(tp.data(), d_tri.p, 12ull * T3, hipMemcpyDeviceToHost));
+                CHECK_HIP(hipMemcpy(tn.data(), d_nrm.p, 12ull * T3, hipMemcpyDeviceToHost));
+                CHECK_FSR(fsr_obj_write_soup(path.c_str(), tp.data(), tn.data(), T3));
+            }
+            std::printf("Writing to %s\n", path.c_str());
+        }
+    }
+    const double nfr = (double)times.size();
+    {
+        const std::string path = dir + algorithm_type(method) + ".time";
+        std::ofstream f(path, std::ios::out | std::ios::app);
+        if (f) {
+            for (double t : times) f << t << ' ';
+            f << std::endl;
+        } else {
+            std::fprintf(stderr, "Failed to save the file: %s\n", path.c_str());
+        }
+    }
+    if (save_cfg) {
+        const std::string path = dir + algorithm_type(method) + ".config";
+        std::ofstream f(path);
+        const double es = sum_es / nfr, sf = sum_sf / nfr, tri = sum_mc / nfr;
+        if (f) {
+            f << (long long)(sum_tri / nfr) << std::endl;
+            f << sum_spr / nfr << std::endl;
+            f << sum_svr / nfr << std::endl;
+            f << es << std::endl;
+            f << es + sf << std::endl;
+            f << tri << std::endl;
+            f << es + sf + tri << std::endl;
+        } else {
+            std::fprintf(stderr, "Failed to save the file: %s\n", path.c_str());
+        }
+    }
+    fsr_destroy(ctx);
+    return 0;
+}

@@ -126,7 +126,8 @@ __global__ void k_involve_from_list(const uint32_t* __restrict__ list, uint32_t 
     }
 }
 
-// field records (M = G^T G, coefficient) of every particle from the G list
+// field records (M = G^T G, coefficient) of every particle from the G list;
+// the caller's G carries no support bound, so no record is culled (w = 3e38)
 __global__ void k_rec_from_g(const fsr_index_range* __restrict__ ranges,
                              const uint32_t* __restrict__ involve,
                              const uint32_t* __restrict__ involve_off,
@@ -150,10 +151,10 @@ __global__ void k_rec_from_g(const fsr_index_range* __restrict__ ranges,
                               a[0][2] * a[1][0] * a[2][1] - a[0][0] * a[1][2] * a[2][1] -
                               a[0][1] * a[1][0] * a[2][2] - a[0][2] * a[1][1] * a[2][0];
             rec[2ull * j] = make_float4(M[0][0], M[1][1], M[2][2], M[0][1]);
-            rec[2ull * j + 1] = make_float4(M[0][2], M[1][2], sigma * det * mr, 0.0f);
+            rec[2ull * j + 1] = make_float4(M[0][2], M[1][2], sigma * det * mr, 3.0e38f);
         } else {
             rec[2ull * j] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-            rec[2ull * j + 1] = make_float4(0.0f, 0.0f, sigma * mr, 0.0f);
+            rec[2ull * j + 1] = make_float4(0.0f, 0.0f, sigma * mr, 3.0e38f);
         }
     }
 }

@@ -445,6 +445,49 @@ fsr_status fsr_stage_times(fsr_context* ctx, float* ms, uint32_t max_frames,
                            uint32_t* num_frames);
 
 /* ------------------------------------------------------------------------- */
+/* Frame files (host only, no device work): the reference's .xyz input and    */
+/* .obj output, and the constants its reconstructor classes set               */
+/* ------------------------------------------------------------------------- */
+
+/* Header of a .xyz frame (FSR/data_format.txt). */
+typedef struct {
+    fsr_float3 aabb_min;
+    fsr_float3 aabb_max;
+    float kernel_radius;
+    float particle_radius;
+    float particle_mass;
+} fsr_xyz_header;
+
+/* Reads the header and counts the particle lines of a .xyz file
+ * (readParticlesFromFile, GPU/ReconstructorGPU.cpp:200-290). */
+fsr_status fsr_xyz_info(const char* path, fsr_xyz_header* header, uint32_t* num_particles);
+
+/* Reads the header and up to `capacity` particles into host buffers:
+ * pos float3[capacity], dens float[capacity].  FSR_ERR_CAPACITY when the file
+ * holds more. */
+fsr_status fsr_xyz_read(const char* path, fsr_xyz_header* header, float* pos, float* dens,
+                        uint32_t capacity, uint32_t* num_particles);
+
+/* Writes a triangle soup (host float3[num_vertices] positions and normals,
+ * normals may be NULL) as the reference's .obj
+ * (saveFluidSurfaceObjToFile, GPU/ReconstructorGPUOurs.cpp:20-100). */
+fsr_status fsr_obj_write_soup(const char* path, const float* pos, const float* nrm,
+                              uint32_t num_vertices);
+
+/* Writes a deduplicated mesh (host buffers from fsr_reconstruct_frame_indexed)
+ * in the same .obj dialect: v, vn, then "f a//a b//b c//c" (1-based). */
+fsr_status fsr_obj_write_indexed(const char* path, const float* pos, const float* nrm,
+                                 uint32_t num_vertices, const uint32_t* indices,
+                                 uint32_t num_indices);
+
+/* The simulation parameters a reconstructor class derives from a .xyz header
+ * and its own constants (readParticlesFromFile, GPU/ReconstructorGPU.cpp:226-245;
+ * onInitialization of ReconstructorGPUOursZB05.cpp:85-97,
+ * ReconstructorGPUAkinci12.cpp:93-106, ReconstructorGPUOursYu13.cpp:21,78-99). */
+fsr_status fsr_sim_param_for(fsr_method method, const fsr_xyz_header* header,
+                             fsr_sim_param* out);
+
+/* ------------------------------------------------------------------------- */
 /* Adjoint (no counterpart in the reference, which has no backward pass):     */
 /* dL/d(particle positions) through the ZB05 field                            */
 /* (updateScalarFieldValueZB05, ReconstructionCUDA.cu:230-289) and the        */
