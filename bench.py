@@ -289,6 +289,9 @@ def main():
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 device copies of MIB MiB (scale factor of the HBM "
+                         "byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus %d is not supported "
@@ -297,6 +300,18 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
+                          "copies": 5}))
+        return
 
     from fluidsurfacereconstructiongpu_amd import datasets as ds
     from fluidsurfacereconstructiongpu_amd.reconstructor import Reconstructor, stats_dict

@@ -187,6 +187,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
                                                                 c_uint32, P, P],
         "fsr_computation_of_scalar_field_grid": [P, P, c_uint32, P, P, P, P, P, c_uint32, P, P, P,
                                                  P, P, P],
+        "fsr_estimate_surface_vertices": [P, P, P, P, c_uint32, P, P, P],
+        "fsr_update_scalar_grid_values_tc01": [P, P, c_uint32, P, P, c_uint32, P, P, P, P, P],
     }
     for name, sig in stage_sigs.items():
         f = getattr(lib, name)

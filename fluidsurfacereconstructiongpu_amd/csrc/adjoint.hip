@@ -68,6 +68,8 @@ __device__ __forceinline__ uint32_t case_vertices(uint32_t flag) {
 // ---------------------------------------------------------------------------
 struct McBackArgs {
     const uint32_t* band_index;
+    const uint32_t* valid_idx;
+    uint32_t* touched;          // bit mask over the grid (band mask layout)
     const uint32_t* bits;
     const uint32_t* word_off;
     const uint32_t* nverts;
@@ -82,6 +84,33 @@ struct McBackArgs {
     int cpu_flavor;
 };
 
+// Marks the 8 corners of every emitted cube, so that the gather below skips,
+// after one bit test, the band vertices no soup vertex depends on.
+__global__ __launch_bounds__(kAdjThreads) void k_mc_touch(McBackArgs a) {
+    const uint32_t Q = a.st->num_valid_cubes;
+    const uint32_t rx = a.sc.rx, ry = a.sc.ry;
+    const uint64_t rxy = (uint64_t)rx * ry;
+    const uint32_t wx = row_words(rx);
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < Q; q += gridDim.x * blockDim.x) {
+        const uint32_t cb = a.valid_idx[q];
+        if ((uint64_t)a.vert_off[cb] + a.nverts[cb] > a.capacity) continue;
+        const uint32_t vid = a.band_index[cb];
+        const uint32_t x = vid % rx, y = (vid / rx) % ry, z = (uint32_t)(vid / rxy);
+        // corners x and x + 1 of the 4 (y, z) rows; both bits may share a word
+        for (uint32_t d = 0; d < 4; ++d) {
+            const uint64_t row = (uint64_t)(z + (d >> 1)) * ry + (y + (d & 1u));
+            const uint64_t w0 = row * wx + (x >> 5);
+            const uint32_t bit = x & 31u;
+            if (bit < 31u) {
+                atomicOr(&a.touched[w0], 3u << bit);
+            } else {
+                atomicOr(&a.touched[w0], 1u << 31);
+                atomicOr(&a.touched[w0 + 1], 1u);
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(kAdjThreads) void k_mc_vertex_backward(McBackArgs a) {
     const uint32_t B = a.st->num_band_vertices;
     const uint32_t rx = a.sc.rx, ry = a.sc.ry, rz = a.sc.rz;
@@ -90,6 +119,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_mc_vertex_backward(McBackArgs a
     for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < B; b += gridDim.x * blockDim.x) {
         const uint32_t vid = a.band_index[b];
         const uint32_t x = vid % rx, y = (vid / rx) % ry, z = (uint32_t)(vid / rxy);
+        if (!((a.touched[((uint64_t)z * ry + y) * wx + (x >> 5)] >> (x & 31u)) & 1u)) continue;
         float acc = 0.0f;
         for (uint32_t cv = 0; cv < 8; ++cv) {
             // the cube whose corner cv is this vertex
@@ -182,7 +212,7 @@ struct FieldBackArgs {
     const uint32_t* chunk_cell;
     const uint32_t* chunk_off;
     const uint32_t* num_chunks;
-    uint32_t* next;
+    uint32_t* next;               // per-XCD fetch counters (next_item_xcd)
     const uint32_t* cell_off;     // virtual cell -> first cell-major slot
     const uint32_t* cell_cnt;
     const uint32_t* cell_band;    // cell-major slot -> vertex id
@@ -255,10 +285,9 @@ __global__ __launch_bounds__(kAdjThreads) void k_field_zb05_backward(FieldBackAr
     const uint32_t rxy = a.sc.rx * a.sc.ry;
     float4* qx = queue_x[wave];
     float4* qu = queue_u[wave];
+    uint32_t step = 0;
     for (;;) {
-        uint32_t ch = 0;
-        if (lane == 0) ch = atomicAdd(a.next, 1u);
-        ch = __shfl(ch, 0, 64);
+        const uint32_t ch = next_item_xcd(a.next, nch, step);
         if (ch >= nch) break;
         const uint32_t c = a.chunk_cell[ch];
         const uint32_t k = ch - a.chunk_off[c];
@@ -374,6 +403,10 @@ fsr_status mc_vertex_backward(fsr_context* ctx, const float* g_tri_pos, float* g
     if (!st.tri_capacity) return FSR_OK;  // no soup vertices were emitted
     McBackArgs a;
     a.band_index = st.bc.band_index;
+    a.valid_idx = st.valid_idx;
+    const uint64_t nwords = (uint64_t)row_words(st.sc.rx) * st.sc.ry * st.sc.rz;
+    FSR_TRY(arena_typed(ctx, A_ADJ_TOUCH, nwords, &a.touched));
+    FSR_TRY(fill_u32(a.touched, 0u, nwords, s));
     a.bits = st.bits;
     a.word_off = st.word_off;
     a.nverts = st.nverts;
@@ -387,6 +420,8 @@ fsr_status mc_vertex_backward(fsr_context* ctx, const float* g_tri_pos, float* g
     a.capacity = st.tri_capacity;
     a.cpu_flavor = st.cpu_flavor;
     const int blocks = persistent_blocks(ctx, kAdjThreads, 8, TUNE_MC);
+    hipLaunchKernelGGL(k_mc_touch, dim3(blocks), dim3(kAdjThreads), 0, s, a);
+    FSR_LAUNCH_CHECK("k_mc_touch");
     hipLaunchKernelGGL(k_mc_vertex_backward, dim3(blocks), dim3(kAdjThreads), 0, s, a);
     FSR_LAUNCH_CHECK("k_mc_vertex_backward");
     return FSR_OK;
@@ -415,8 +450,8 @@ fsr_status field_zb05_backward(fsr_context* ctx, const float* g_phi, float* g_po
     uint32_t* chunk_off = aux + C;
     uint32_t* chunk_cell = aux + 2ull * C;
     uint32_t* total = &st.counters->aux[8];
-    uint32_t* next = &st.counters->aux[9];
-    FSR_TRY(fill_u32(next, 0u, 1, s));
+    uint32_t* next = st.counters->xcd_next;
+    FSR_TRY(reset_xcd(st.counters, s));
     FSR_TRY(cell_chunk_list(ctx, st.ranges, nullptr, C, cnt, chunk_off, chunk_cell, total, s));
     FieldBackArgs a;
     a.sorted = st.sorted;

@@ -107,18 +107,12 @@ __device__ __forceinline__ uint32_t candidate_index(uint32_t g, int nrows,
     return row_start[j] + (g - row_pref[j]);
 }
 
-// Next chunk of the dynamic work list (one atomic per wave).
-__device__ __forceinline__ uint32_t next_chunk(uint32_t* counter) {
-    uint32_t ch = 0;
-    if (__lane_id() == 0) ch = atomicAdd(counter, 1u);
-    return __shfl(ch, 0, 64);
-}
 
 struct ChunkList {
     const uint32_t* chunk_cell;  // chunk -> cell
     const uint32_t* chunk_off;   // cell -> first chunk
     const uint32_t* num_chunks;  // device count
-    uint32_t* next;              // fetch counter
+    uint32_t* next;              // per-XCD fetch counters (next_item_xcd)
 };
 
 // ---------------------------------------------------------------- K10 -------
@@ -133,8 +127,9 @@ __global__ __launch_bounds__(kAnisoThreads) void k_smooth(
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t nch = *cl.num_chunks;
     const float inv_r2 = 1.0f / (radius * radius);
+    uint32_t step = 0;
     for (;;) {
-        const uint32_t ch = next_chunk(cl.next);
+        const uint32_t ch = next_item_xcd(cl.next, nch, step);
         if (ch >= nch) break;
         const uint32_t c = cl.chunk_cell[ch];
         const uint32_t k = ch - cl.chunk_off[c];
@@ -277,8 +272,9 @@ __global__ __launch_bounds__(kAnisoThreads) void k_covariance(CovArgs a, ChunkLi
     const uint32_t nch = *cl.num_chunks;
     const float r = a.radius;
     const float inv_r = 1.0f / r;
+    uint32_t step = 0;
     for (;;) {
-        const uint32_t ch = next_chunk(cl.next);
+        const uint32_t ch = next_item_xcd(cl.next, nch, step);
         if (ch >= nch) break;
         const uint32_t c = cl.chunk_cell[ch];
         const uint32_t k = ch - cl.chunk_off[c];
@@ -448,8 +444,9 @@ __global__ __launch_bounds__(kAnisoThreads) void k_field_aniso(AnisoFieldArgs a)
     float4* q0s = queue[wave][0];
     float4* q1s = queue[wave][1];
     float4* q2s = queue[wave][2];
+    uint32_t step = 0;
     for (;;) {
-        const uint32_t ch = next_chunk(a.next);
+        const uint32_t ch = next_item_xcd(a.next, nch, step);
         if (ch >= nch) break;
         const uint32_t v = a.bc.chunk_cell[ch];
         const int cx = (int)(v % a.bc.vcx) - R;
@@ -614,7 +611,8 @@ fsr_status aniso_smooth(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_chunks_of_cells, dim3((C + 255) / 256), dim3(256), 0, s, ps.cnt,
                        ps.chunk_off, C, ps.chunk_cell);
     FSR_LAUNCH_CHECK("k_chunks_of_cells");
-    ChunkList cl{ps.chunk_cell, ps.chunk_off, &a.counters->aux[0], &a.counters->aux[1]};
+    ChunkList cl{ps.chunk_cell, ps.chunk_off, &a.counters->aux[0], a.counters->xcd_next};
+    FSR_TRY(reset_xcd(a.counters, s));
     hipLaunchKernelGGL(k_smooth, dim3(blocks), dim3(kAnisoThreads), 0, s, a.sorted, a.ranges,
                        a.sp, a.aniso_radius, a.lambda, cl, a.mean, a.smooth);
     FSR_LAUNCH_CHECK("k_smooth");
@@ -647,7 +645,7 @@ fsr_status aniso_cov(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_chunks_of_cells, dim3((C + 255) / 256), dim3(256), 0, s, ps.cnt2,
                        ps.chunk_off2, C, ps.chunk_cell2);
     FSR_LAUNCH_CHECK("k_chunks_of_cells");
-    ChunkList cl2{ps.chunk_cell2, ps.chunk_off2, &a.counters->aux[2], &a.counters->aux[3]};
+    ChunkList cl2{ps.chunk_cell2, ps.chunk_off2, &a.counters->aux[2], a.counters->xcd_next};
     CovArgs ca;
     ca.sorted = a.sorted;
     ca.mean = a.mean;
@@ -664,6 +662,7 @@ fsr_status aniso_cov(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
     ca.gmat = a.gmat;
     ca.sigma = a.sigma;
     ca.rec = a.rec;
+    FSR_TRY(reset_xcd(a.counters, s));
     hipLaunchKernelGGL(k_covariance, dim3(blocks), dim3(kAnisoThreads), 0, s, ca, cl2);
     FSR_LAUNCH_CHECK("k_covariance");
     return FSR_OK;
@@ -686,8 +685,9 @@ fsr_status aniso_field(fsr_context* ctx, const AnisoArgs& a, const BandCompactOu
     fa.sp = a.sp;
     fa.sc = a.sc;
     fa.phi = phi;
-    fa.next = &a.counters->field_next_chunk;
+    fa.next = a.counters->xcd_next;
     fa.num_chunks = &a.counters->num_chunks;
+    FSR_TRY(reset_xcd(a.counters, s));
     hipLaunchKernelGGL(k_field_aniso, dim3(blocks), dim3(kAnisoThreads), 0, s, fa);
     FSR_LAUNCH_CHECK("k_field_aniso");
     return FSR_OK;

@@ -1,8 +1,9 @@
 // Batched frames captured in one HIP graph, and the launch-shape autotuner.
 //
 // A batch owns `lanes` lanes: each lane is a context of its own (scratch arena)
-// with its own stream.  Frame f runs on lane f % lanes, so up to `lanes` frames
-// are in flight at once and small frames fill the GPU together.  The first run
+// with its own stream.  Frames are spread over the lanes largest first onto the
+// least-loaded lane, so up to `lanes` frames are in flight at once, small frames
+// fill the GPU together, and the lanes finish close together.  The first run
 // of a frame list goes eagerly (it grows the lane arenas to the frames' sizes);
 // it is then captured once into a graph — fork from the batch's origin stream
 // to the lanes, the frames in order on their lanes, join back — and later runs
@@ -46,14 +47,41 @@ void release_graph(fsr_batch* b) {
     b->key.clear();
 }
 
+// Lane of every frame: largest first, each onto the lane with the least work so
+// far (longest-processing-time rule), with N + V/16 as the cost of a frame
+// (particle gathers plus the dense-grid passes).  Returns frames in enqueue
+// order with their lanes.
+std::vector<std::pair<uint32_t, uint32_t>> schedule(const fsr_frame_desc* frames, uint32_t n,
+                                                    uint32_t L) {
+    std::vector<uint32_t> order(n);
+    std::vector<double> cost(n);
+    for (uint32_t f = 0; f < n; ++f) {
+        order[f] = f;
+        const fsr_grid_info& g = frames[f].sc_grid;
+        cost[f] = (double)frames[f].num_particles +
+                  (double)g.resolution.x * g.resolution.y * g.resolution.z / 16.0;
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t c) { return cost[a] > cost[c]; });
+    std::vector<double> load(L, 0.0);
+    std::vector<std::pair<uint32_t, uint32_t>> out;
+    out.reserve(n);
+    for (uint32_t f : order) {
+        const uint32_t l = (uint32_t)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[l] += cost[f];
+        out.emplace_back(f, l);
+    }
+    return out;
+}
+
 fsr_status enqueue(fsr_batch* b, fsr_method method, const fsr_aniso_param* aniso,
                    const fsr_frame_desc* frames, uint32_t n, uint64_t* ts) {
     const uint32_t L = (uint32_t)b->lanes.size();
     FSR_HIP(hipEventRecord(b->fork, b->origin));
     for (uint32_t l = 0; l < L; ++l) FSR_HIP(hipStreamWaitEvent(b->streams[l], b->fork, 0));
-    for (uint32_t f = 0; f < n; ++f) {
+    for (const auto& fl : schedule(frames, n, L)) {
+        const uint32_t f = fl.first, l = fl.second;
         const fsr_frame_desc& d = frames[f];
-        const uint32_t l = f % L;
         hipStream_t s = b->streams[l];
         if (ts) FSR_TRY(device_timestamp(ts + 2ull * f, s));
         if (d.src_pos && d.num_particles) {

@@ -417,6 +417,7 @@ fsr_status frame_impl(const char* who, fsr_context* ctx, fsr_method method,
         st.word_off = static_cast<const uint32_t*>(ctx->arena.ptr[A_WORD_OFF]);
         st.nverts = nverts;
         st.vert_off = vert_off;
+        st.valid_idx = valid_idx;
         st.tri_capacity = tri_capacity;
         st.stats = stats;
         st.counters = counters;

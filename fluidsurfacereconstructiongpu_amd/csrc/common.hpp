@@ -115,10 +115,61 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
+// XCD-aware dynamic work fetch for persistent kernels.  The work list [0, total)
+// is cut into 8 contiguous segments; workgroups are dispatched round-robin over
+// the 8 XCDs, so blockIdx % 8 names the XCD, and a wave takes items from its
+// XCD's segment first (neighbouring items share particle data, which then stays
+// in that XCD's L2), then from the other segments.  `step` (0 at start) is the
+// wave's position in that order.  The 8 counters next8[32 k] sit on separate
+// 128-byte lines (sharing one line across XCDs serialises the fetches) and start
+// at 0.  Returns total when every item is taken.  Wave-uniform.
+__device__ __forceinline__ uint32_t next_item_xcd(uint32_t* next8, uint32_t total,
+                                                  uint32_t& step) {
+    const uint32_t xcd = blockIdx.x & 7u;
+    const uint32_t seg = (total + 7u) >> 3;
+    while (step < 8u) {
+        const uint32_t k = (xcd + step) & 7u;
+        const uint32_t lo = k * seg;
+        const uint32_t hi = min(lo + seg, total);
+        if (lo < hi) {
+            uint32_t idx = 0;
+            if (__lane_id() == 0) idx = atomicAdd(&next8[32u * k], 1u);
+            idx = __shfl(idx, 0, 64);
+            if (lo + idx < hi) return lo + idx;
+        }
+        ++step;
+    }
+    return total;
+}
+
 // Adds a per-lane count into a global counter with one atomic per wave.
 __device__ __forceinline__ void wave_count_add(uint32_t* counter, uint32_t v) {
     uint32_t s = wave_sum(v);
     if (__lane_id() == 0 && s) atomicAdd(counter, s);
+}
+
+// Adds two per-thread counts into two global counters with one atomic each per
+// block (thousands of waves adding to one address serialise in L2).  Every
+// thread of the block must call it: it synchronises the block.
+__device__ __forceinline__ void block_count_add2(uint32_t* c0, uint32_t v0, uint32_t* c1,
+                                                 uint32_t v1) {
+    __shared__ uint32_t part[2][16];
+    const uint32_t s0 = wave_sum(v0), s1 = wave_sum(v1);
+    const uint32_t w = threadIdx.x >> 6;
+    if (__lane_id() == 0) {
+        part[0][w] = s0;
+        part[1][w] = s1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t0 = 0, t1 = 0;
+        for (uint32_t k = 0; k < (blockDim.x + 63u) / 64u; ++k) {
+            t0 += part[0][k];
+            t1 += part[1][k];
+        }
+        if (t0) atomicAdd(c0, t0);
+        if (t1) atomicAdd(c1, t1);
+    }
 }
 #endif
 
@@ -133,7 +184,7 @@ enum ArenaSlot {
     A_CHUNK_CELL, A_NVERTS, A_VALID, A_VALID_OFF, A_VERT_OFF, A_VALID_IDX,
     A_COUNTERS, A_SCAN_PARTIALS, A_STATS, A_INVOLVE, A_INVOLVE_OFF,
     A_INVOLVED_IDX, A_MEAN, A_SMOOTH, A_GMAT, A_SIGMA, A_AUX0, A_AUX1, A_REC,
-    A_ADJ_VERT, A_ADJ_COEF, A_ADJ_AUX, A_EDGE_MASK, A_EDGE_OFF,
+    A_ADJ_VERT, A_ADJ_COEF, A_ADJ_AUX, A_ADJ_TOUCH, A_EDGE_MASK, A_EDGE_OFF,
     A_NUM_SLOTS
 };
 
@@ -164,6 +215,8 @@ struct Counters {
     uint32_t num_exceptions;
     uint32_t mc_next;
     uint32_t aux[12];
+    uint32_t pad[12];
+    uint32_t xcd_next[8 * 32];  // per-XCD fetch counters of the field kernels, 128 B apart
 };
 
 }  // namespace fsr
@@ -191,6 +244,7 @@ struct AdjointState {
     const uint32_t* word_off = nullptr;      // band rank of each mask word
     const uint32_t* nverts = nullptr;        // soup vertices of the cube at band slot b
     const uint32_t* vert_off = nullptr;      // their first soup index
+    const uint32_t* valid_idx = nullptr;     // band slots of the valid cubes
     uint32_t tri_capacity = 0;
     const fsr_frame_stats* stats = nullptr;
     Counters* counters = nullptr;
@@ -244,6 +298,10 @@ fsr_status scan_exclusive(fsr_context* ctx, ScanInput f, const uint32_t* in,
                           uint32_t* out, uint64_t n_max, const uint32_t* d_count,
                           uint32_t* d_total, hipStream_t s);
 fsr_status fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
+// Zeroes the per-XCD fetch counters before a persistent kernel that uses them.
+inline fsr_status reset_xcd(Counters* c, hipStream_t s) {
+    return fill_u32(c->xcd_next, 0u, sizeof(c->xcd_next) / 4, s);
+}
 
 // sort.hip ------------------------------------------------------------------
 struct HashOut {

@@ -23,7 +23,8 @@
 // Coordinates relative to the cell keep the expansion well conditioned and
 // make |x - xbar| free of the cancellation the absolute-coordinate sum has.
 // Candidates are visited in the reference's order (z, y, x cells, then range
-// order).  Work is distributed dynamically, one chunk per atomic fetch.
+// order).  Work is distributed dynamically, one chunk per atomic fetch, each XCD
+// starting on its own contiguous eighth of the chunk list.
 //
 // Culling: a chunk's vertices are a compact slab of its cell (the cell-major
 // list is in ascending vertex id), so many block candidates lie farther than R
@@ -72,7 +73,7 @@ struct Sums {
 __device__ __forceinline__ void accumulate(const float4* __restrict__ q, uint32_t n, float ax,
                                            float ay, float az, float c0, Sums& s) {
     for (uint32_t t0 = 0; t0 < n; t0 += 16u) {
-#pragma unroll
+#pragma unroll 8
         for (int t = 0; t < 16; ++t) {
             const float4 p = q[t0 + t];
             float tt = __builtin_fmaf(az, p.z, c0 - p.w);
@@ -100,10 +101,9 @@ __global__ __launch_bounds__(kFieldThreads) void k_field_zb05(FieldArgs a, BandC
     const int r = bc.radius;
     float4* q = queue[wave];
 
+    uint32_t step = 0;
     for (;;) {
-        uint32_t ch = 0;
-        if (lane == 0) ch = atomicAdd(&a.counters->field_next_chunk, 1u);
-        ch = __shfl(ch, 0, 64);
+        const uint32_t ch = next_item_xcd(a.counters->xcd_next, num_chunks, step);
         if (ch >= num_chunks) break;
 
         const uint32_t v = bc.chunk_cell[ch];
@@ -169,12 +169,12 @@ __global__ __launch_bounds__(kFieldThreads) void k_field_zb05(FieldArgs a, BandC
 
         Sums sum = {0.f, 0.f, 0.f, 0.f};
         uint32_t fill = 0;
+        int j = 0;  // row of this lane's candidate; g only grows, so the search resumes
         for (uint32_t base = 0; base < total; base += 64u) {
             const uint32_t g = base + lane;
             bool keep = false;
             float4 st;
             if (g < total) {
-                int j = 0;
                 while (j + 1 < nrows && row_pref[wave][j + 1] <= g) ++j;
                 const float4 p = a.sorted[row_start[wave][j] + (g - row_pref[wave][j])];
                 const float px = p.x - ox, py = p.y - oy, pz = p.z - oz;
@@ -225,6 +225,7 @@ __global__ __launch_bounds__(kFieldThreads) void k_field_zb05(FieldArgs a, BandC
 
 fsr_status field_zb05(fsr_context* ctx, const FieldArgs& a, hipStream_t s) {
     const int blocks = persistent_blocks(ctx, kFieldThreads, 8, TUNE_FIELD);
+    FSR_TRY(reset_xcd(a.counters, s));
     hipLaunchKernelGGL(k_field_zb05, dim3(blocks), dim3(kFieldThreads), 0, s, a, *a.bc);
     FSR_LAUNCH_CHECK("k_field_zb05");
     return FSR_OK;

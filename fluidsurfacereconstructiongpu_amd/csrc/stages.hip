@@ -161,6 +161,87 @@ __global__ void k_rec_from_g(const fsr_index_range* __restrict__ ranges,
 
 inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + 255) / 256 ? (n + 255) / 256 : 1); }
 
+__device__ __forceinline__ int clampi_s(int v, int lo, int hi) {
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// estimateSurfaceVertices (ReconstructionCUDA.cu:123-175): a density-grid cell
+// (not on the last layer) whose 8 corners straddle 0.5 (getVertexFlag,
+// ReconstructionCUDA.cuh:491-511: bit i set when corner i <= 0.5) marks every
+// scalar vertex of [c * ratio - ext, (c + 1) * ratio + ext], clamped per axis.
+__global__ void k_estimate_surface_vertices(const float* __restrict__ dens, GridDev dg,
+                                            uint32_t* __restrict__ is_surface, GridDev sg,
+                                            int ratio, int ext) {
+    const uint32_t C = dg.rx * dg.ry * dg.rz;
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const uint32_t x = c % dg.rx, y = (c / dg.rx) % dg.ry, z = c / (dg.rx * dg.ry);
+    if (x >= dg.rx - 1 || y >= dg.ry - 1 || z >= dg.rz - 1) return;
+    const uint64_t rxy = (uint64_t)dg.rx * dg.ry;
+    const float v[8] = {dens[c], dens[c + rxy], dens[c + rxy + dg.rx], dens[c + dg.rx],
+                        dens[c + 1], dens[c + 1 + rxy], dens[c + 1 + rxy + dg.rx],
+                        dens[c + 1 + dg.rx]};
+    uint32_t flag = 0;
+    for (int i = 0; i < 8; ++i) flag |= (v[i] <= 0.5f) ? (1u << i) : 0u;
+    if (flag == 0 || flag == 255) return;
+    const int x0 = clampi_s((int)x * ratio - ext, 0, (int)sg.rx - 1);
+    const int x1 = clampi_s(((int)x + 1) * ratio + ext, 0, (int)sg.rx - 1);
+    const int y0 = clampi_s((int)y * ratio - ext, 0, (int)sg.ry - 1);
+    const int y1 = clampi_s(((int)y + 1) * ratio + ext, 0, (int)sg.ry - 1);
+    const int z0 = clampi_s((int)z * ratio - ext, 0, (int)sg.rz - 1);
+    const int z1 = clampi_s(((int)z + 1) * ratio + ext, 0, (int)sg.rz - 1);
+    for (int zz = z0; zz <= z1; ++zz)
+        for (int yy = y0; yy <= y1; ++yy)
+            for (int xx = x0; xx <= x1; ++xx)
+                is_surface[((uint64_t)zz * sg.ry + yy) * sg.rx + xx] = 1u;
+}
+
+// updateScalarFieldValueTC01 (ReconstructionCUDA.cu:178-227) with kernelTC01
+// (ReconstructionCUDA.cuh:252-260): phi(x) = sum over the clamped 3^3 block of
+// (d - 1/2)^2 with d = |x - p|^2 / (2 R^2), for |x - p|^2 <= R^2.  One thread per
+// listed vertex; the value is written unconditionally.
+__global__ void k_field_tc01(const uint32_t* __restrict__ sv_index, uint32_t num_sv,
+                             const fsr_index_range* __restrict__ ranges,
+                             const float* __restrict__ particles, float* __restrict__ phi,
+                             GridDev sp, GridDev sc, float radius_sq) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= num_sv) return;
+    const uint32_t vid = sv_index[k];
+    const uint32_t ix = vid % sc.rx, iy = (vid / sc.rx) % sc.ry, iz = vid / (sc.rx * sc.ry);
+    const float vx = vertex_coord(ix, sc.minx, sc.cell);
+    const float vy = vertex_coord(iy, sc.miny, sc.cell);
+    const float vz = vertex_coord(iz, sc.minz, sc.cell);
+    auto cell = [](float q, uint32_t res) {
+        if (!(q == q)) return 0;
+        const float hi = (float)res + 4.0f;
+        q = q < -4.0f ? -4.0f : (q > hi ? hi : q);
+        return (int)q;
+    };
+    const int cx = cell(cell_coord(vx, sp.minx, sp.cell), sp.rx);
+    const int cy = cell(cell_coord(vy, sp.miny, sp.cell), sp.ry);
+    const int cz = cell(cell_coord(vz, sp.minz, sp.cell), sp.rz);
+    const int x0 = clampi_s(cx - 1, 0, sp.rx - 1), x1 = clampi_s(cx + 1, 0, sp.rx - 1);
+    const int y0 = clampi_s(cy - 1, 0, sp.ry - 1), y1 = clampi_s(cy + 1, 0, sp.ry - 1);
+    const int z0 = clampi_s(cz - 1, 0, sp.rz - 1), z1 = clampi_s(cz + 1, 0, sp.rz - 1);
+    float val = 0.f;
+    for (int zz = z0; zz <= z1; ++zz)
+        for (int yy = y0; yy <= y1; ++yy)
+            for (int xx = x0; xx <= x1; ++xx) {
+                const fsr_index_range r = ranges[((uint32_t)zz * sp.ry + yy) * sp.rx + xx];
+                if (r.start == 0xFFFFFFFFu) continue;
+                for (uint32_t i = r.start; i < r.end; ++i) {
+                    const float dx = vx - particles[3ull * i];
+                    const float dy = vy - particles[3ull * i + 1];
+                    const float dz = vz - particles[3ull * i + 2];
+                    const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                    if (d2 > radius_sq) continue;
+                    const float d = d2 / radius_sq * 0.5f;
+                    val += d * d - d + 0.25f;
+                }
+            }
+    phi[vid] = val;
+}
+
 }  // namespace
 
 // launchers used by the entry points below (defined in the other units)
@@ -728,6 +809,42 @@ fsr_status fsr_computation_of_scalar_field_grid(
     BandCompactOut bc;
     FSR_TRY(band_worklist_from_list(ctx, sv_index, num_sv, a.sc, a.sp, 2, ct, st, &bc, s));
     return aniso_field(ctx, a, bc, scalar_field, s);
+}
+
+fsr_status fsr_estimate_surface_vertices(fsr_context* ctx, const fsr_sim_param* sim,
+                                         const float* density_grid, const fsr_grid_info* sp,
+                                         uint32_t sc_search_ext, uint32_t* is_surface_grid,
+                                         const fsr_grid_info* sc, void* stream) {
+    FSR_TRY(need(ctx && sim && density_grid && is_surface_grid, "fsr_estimate_surface_vertices"));
+    FSR_TRY(grid_ok(sp, "fsr_estimate_surface_vertices"));
+    FSR_TRY(grid_ok(sc, "fsr_estimate_surface_vertices"));
+    FSR_HIP(hipSetDevice(ctx->device));
+    const uint64_t C = grid_count(*sp);
+    hipLaunchKernelGGL(k_estimate_surface_vertices, dim3(blocks_for(C)), dim3(256), 0,
+                       (hipStream_t)stream, density_grid, to_dev(*sp), is_surface_grid,
+                       to_dev(*sc), (int)sim->sc_sp_grid_res_ratio, (int)sc_search_ext);
+    FSR_LAUNCH_CHECK("k_estimate_surface_vertices");
+    return FSR_OK;
+}
+
+fsr_status fsr_update_scalar_grid_values_tc01(fsr_context* ctx, const uint32_t* sv_index,
+                                              uint32_t num_sv, const fsr_index_range* ranges,
+                                              const float* particles, uint32_t num_particles,
+                                              float* scalar_field, const fsr_grid_info* sp,
+                                              const fsr_grid_info* sc, const fsr_sim_param* sim,
+                                              void* stream) {
+    FSR_TRY(need(ctx && sim && ranges && scalar_field && (num_sv == 0 || sv_index) &&
+                     (num_particles == 0 || particles),
+                 "fsr_update_scalar_grid_values_tc01"));
+    FSR_TRY(grid_ok(sp, "fsr_update_scalar_grid_values_tc01"));
+    FSR_TRY(grid_ok(sc, "fsr_update_scalar_grid_values_tc01"));
+    FSR_HIP(hipSetDevice(ctx->device));
+    if (!num_sv) return FSR_OK;
+    hipLaunchKernelGGL(k_field_tc01, dim3(blocks_for(num_sv)), dim3(256), 0,
+                       (hipStream_t)stream, sv_index, num_sv, ranges, particles, scalar_field,
+                       to_dev(*sp), to_dev(*sc), sim->smoothing_radius_sq);
+    FSR_LAUNCH_CHECK("k_field_tc01");
+    return FSR_OK;
 }
 
 }  // extern "C"

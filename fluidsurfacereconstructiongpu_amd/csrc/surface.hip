@@ -73,11 +73,14 @@ __global__ __launch_bounds__(256) void k_cell_surface(const float* __restrict__ 
     }
 }
 
-// Band seeding for one particle (called by every lane; `surf` = its flag).
+// Band seeding for one particle (called by every thread of the block; `surf` =
+// its flag).
 __device__ __forceinline__ void seed_band(const SurfaceArgs& a, uint32_t i, float px, float py,
                                           float pz, bool surf) {
     if (!a.sbits) {  // detection only (uniform across the launch)
-        if (a.st) wave_count_add(&a.st->num_surface_particles, surf ? 1u : 0u);
+        if (a.st)
+            block_count_add2(&a.st->num_surface_particles, surf ? 1u : 0u,
+                             &a.st->band_exceptions, 0u);
         return;
     }
     const float qx = cell_coord(px, a.sc.minx, a.sc.cell);
@@ -100,8 +103,8 @@ __device__ __forceinline__ void seed_band(const SurfaceArgs& a, uint32_t i, floa
         base = __shfl(base, __ffsll((unsigned long long)m) - 1, 64);
         if (exc) a.exceptions[base + mask_rank(m)] = i;
     }
-    wave_count_add(&a.st->num_surface_particles, surf ? 1u : 0u);
-    wave_count_add(&a.st->band_exceptions, exc ? 1u : 0u);
+    block_count_add2(&a.st->num_surface_particles, surf ? 1u : 0u, &a.st->band_exceptions,
+                     exc ? 1u : 0u);
 }
 
 __global__ __launch_bounds__(256) void k_flags_from_cells(SurfaceArgs a,

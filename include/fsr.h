@@ -254,7 +254,8 @@ typedef struct fsr_batch fsr_batch;
 fsr_status fsr_batch_create(fsr_context* ctx, uint32_t lanes, fsr_batch** out);
 fsr_status fsr_batch_destroy(fsr_batch* batch);
 
-/* Runs the frames (frame f on lane f % lanes, lanes concurrently).  The first
+/* Runs the frames, lanes concurrently (frames assigned largest first to the
+ * least-loaded lane, by particles + scalar vertices / 16).  The first
  * call for a frame list runs it eagerly (sizing the lane arenas) and captures
  * the whole batch, forked over the lanes and joined on `stream`, into one HIP
  * graph; a later call with the same list (same descriptors) replays that graph
@@ -417,6 +418,31 @@ fsr_status fsr_computation_of_scalar_field_grid(
     uint32_t num_particles, const fsr_index_range* ranges, const uint32_t* sv_index,
     const uint32_t* num_involve, const uint32_t* num_involve_scan, float* scalar_field,
     void* stream);
+
+/* Launchers of the reference that its reconstructor classes never call, kept
+ * for a complete launcher set: */
+
+/* launchEstimateSurfaceVertices (ReconstructionCUDA.cuh:36-42, kernel
+ * ReconstructionCUDA.cu:123-175): every density-grid cell (sp, not on the last
+ * layer) whose corners straddle 0.5 sets is_surface_grid[V] = 1 on the scalar
+ * vertices [c * ratio - ext, (c + 1) * ratio + ext] (clamped), ratio =
+ * sim->sc_sp_grid_res_ratio.  Writes ones only (the caller zeroes). */
+fsr_status fsr_estimate_surface_vertices(fsr_context* ctx, const fsr_sim_param* sim,
+                                         const float* density_grid, const fsr_grid_info* sp,
+                                         uint32_t sc_search_ext, uint32_t* is_surface_grid,
+                                         const fsr_grid_info* sc, void* stream);
+
+/* The isotropic TC01 field (updateScalarFieldValueTC01, ReconstructionCUDA.cu:
+ * 178-227, kernelTC01 ReconstructionCUDA.cuh:252-260; the reference defines it
+ * but launches it nowhere): phi = sum over the clamped 3^3 block of
+ * (|x - p|^2 / (2 R^2) - 1/2)^2 for |x - p|^2 <= R^2, written at the num_sv
+ * listed vertices.  Particles are float3[n] in cell order. */
+fsr_status fsr_update_scalar_grid_values_tc01(fsr_context* ctx, const uint32_t* sv_index,
+                                              uint32_t num_sv, const fsr_index_range* ranges,
+                                              const float* particles, uint32_t num_particles,
+                                              float* scalar_field, const fsr_grid_info* sp,
+                                              const fsr_grid_info* sc, const fsr_sim_param* sim,
+                                              void* stream);
 
 /* ------------------------------------------------------------------------- */
 /* Stage timing (the reference's optional per-stage timers,                   */

@@ -78,6 +78,10 @@ def load():
     lib.orc_field_aniso.argtypes = [u32p, ctypes.c_uint32, f64p, f32p, u32p, u32p, u32p, f64p,
                                     OGrid, OGrid, ctypes.c_float, ctypes.c_void_p, f64p,
                                     ctypes.c_void_p]
+    lib.orc_estimate_surface_vertices.argtypes = [f32p, OGrid, ctypes.c_uint32, ctypes.c_uint32,
+                                                  OGrid, u32p]
+    lib.orc_field_tc01.argtypes = [u32p, ctypes.c_uint32, f32p, u32p, OGrid, OGrid,
+                                   ctypes.c_float, f64p]
     lib.orc_mc_indexed.argtypes = [u32p, ctypes.c_uint32, u8p, f64p, OGrid, ctypes.c_float,
                                    ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                    ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64, u64p]
@@ -305,6 +309,24 @@ def reference_mc_tables():
         edge_vertices=arr(lib.ref_mc_edge_vertices(), ctypes.c_uint32, 24).reshape(12, 2),
         corner_offsets=arr(lib.ref_mc_corner_offsets(), ctypes.c_int32, 24).reshape(8, 3),
     )
+
+
+def estimate_surface_vertices(dens_grid, dg, ratio, ext, sg):
+    """u32[V] ones on the vertices estimateSurfaceVertices marks."""
+    lib = load()
+    out = np.zeros(sg.count, np.uint32)
+    lib.orc_estimate_surface_vertices(np.ascontiguousarray(dens_grid, np.float32), ogrid(dg),
+                                      int(ratio), int(ext), ogrid(sg), out)
+    return out
+
+
+def field_tc01(index, pos_sorted, ranges, sp, sc, radius_sq, fill=0.0):
+    lib = load()
+    phi = np.full(sc.count, fill, np.float64)
+    index = np.ascontiguousarray(index, np.uint32)
+    lib.orc_field_tc01(index, index.shape[0], np.ascontiguousarray(pos_sorted, np.float32).ravel(),
+                       np.ascontiguousarray(ranges).ravel(), ogrid(sp), ogrid(sc), radius_sq, phi)
+    return phi
 
 
 def mc_indexed(index, band_u8, phi, sc, iso, cpu=False):

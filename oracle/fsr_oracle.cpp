@@ -1254,3 +1254,83 @@ uint64_t orc_mc_indexed(const uint32_t* band_index, uint32_t B, const uint8_t* b
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Launchers the reference defines but never calls.
+// ---------------------------------------------------------------------------
+extern "C" {
+
+// estimateSurfaceVertices (ReconstructionCUDA.cu:123-175) with getVertexFlag
+// (ReconstructionCUDA.cuh:491-511) on the density (occupancy) grid, iso 0.5.
+int orc_estimate_surface_vertices(const float* dens, orc_grid dg, uint32_t ratio, uint32_t ext,
+                                  orc_grid sg, uint32_t* is_surface) {
+    const int64_t rx = dg.rx, ry = dg.ry, rz = dg.rz;
+    for (int64_t z = 0; z + 1 < rz; ++z)
+        for (int64_t y = 0; y + 1 < ry; ++y)
+            for (int64_t x = 0; x + 1 < rx; ++x) {
+                uint32_t flag = 0;
+                for (int c = 0; c < 8; ++c) {
+                    const int64_t X = x + kCornerOff[c][0], Y = y + kCornerOff[c][1],
+                                  Z = z + kCornerOff[c][2];
+                    if (dens[(Z * ry + Y) * rx + X] <= 0.5f) flag |= 1u << c;
+                }
+                if (flag == 0 || flag == 255) continue;
+                const int64_t lo[3] = {x * (int64_t)ratio - (int64_t)ext,
+                                       y * (int64_t)ratio - (int64_t)ext,
+                                       z * (int64_t)ratio - (int64_t)ext};
+                const int64_t hi[3] = {(x + 1) * (int64_t)ratio + (int64_t)ext,
+                                       (y + 1) * (int64_t)ratio + (int64_t)ext,
+                                       (z + 1) * (int64_t)ratio + (int64_t)ext};
+                const int64_t res[3] = {sg.rx, sg.ry, sg.rz};
+                int64_t a[3], b[3];
+                for (int k = 0; k < 3; ++k) {
+                    a[k] = std::min(std::max(lo[k], (int64_t)0), res[k] - 1);
+                    b[k] = std::min(std::max(hi[k], (int64_t)0), res[k] - 1);
+                }
+                for (int64_t zz = a[2]; zz <= b[2]; ++zz)
+                    for (int64_t yy = a[1]; yy <= b[1]; ++yy)
+                        for (int64_t xx = a[0]; xx <= b[0]; ++xx)
+                            is_surface[(zz * sg.ry + yy) * sg.rx + xx] = 1;
+            }
+    return 0;
+}
+
+// updateScalarFieldValueTC01 (ReconstructionCUDA.cu:178-227) with kernelTC01
+// (ReconstructionCUDA.cuh:252-260) in FP64: (d - 1/2)^2, d = |x - p|^2 / (2 R^2),
+// over the clamped 3^3 block, for |x - p|^2 <= R^2.
+int orc_field_tc01(const uint32_t* band_index, uint32_t B, const float* pos,
+                   const uint32_t* ranges, orc_grid sp, orc_grid sc, float radius_sq,
+                   double* phi) {
+    const int rx = sp.rx, ry = sp.ry, rz = sp.rz;
+    const double R2 = radius_sq;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t b = 0; b < (int64_t)B; ++b) {
+        const uint32_t vid = band_index[b];
+        const uint32_t ix = vid % sc.rx, iy = (vid / sc.rx) % sc.ry, iz = vid / (sc.rx * sc.ry);
+        const float vxf = vpos(ix, sc.minx, sc.cell), vyf = vpos(iy, sc.miny, sc.cell),
+                    vzf = vpos(iz, sc.minz, sc.cell);
+        const int cx = trunc_q(cell_q(vxf, sp.minx, sp.cell), rx);
+        const int cy = trunc_q(cell_q(vyf, sp.miny, sp.cell), ry);
+        const int cz = trunc_q(cell_q(vzf, sp.minz, sp.cell), rz);
+        double val = 0;
+        for (int zz = clampi(cz - 1, 0, rz - 1); zz <= clampi(cz + 1, 0, rz - 1); ++zz)
+            for (int yy = clampi(cy - 1, 0, ry - 1); yy <= clampi(cy + 1, 0, ry - 1); ++yy)
+                for (int xx = clampi(cx - 1, 0, rx - 1); xx <= clampi(cx + 1, 0, rx - 1); ++xx) {
+                    const uint64_t c = ((uint64_t)zz * ry + yy) * rx + xx;
+                    if (ranges[2 * c] == 0xFFFFFFFFu) continue;
+                    for (uint32_t j = ranges[2 * c]; j < ranges[2 * c + 1]; ++j) {
+                        const double dx = (double)vxf - pos[3 * j];
+                        const double dy = (double)vyf - pos[3 * j + 1];
+                        const double dz = (double)vzf - pos[3 * j + 2];
+                        const double d2 = dx * dx + dy * dy + dz * dz;
+                        if (d2 > R2) continue;
+                        const double d = d2 / R2 * 0.5;
+                        val += d * d - d + 0.25;
+                    }
+                }
+        phi[vid] = val;
+    }
+    return 0;
+}
+
+}  // extern "C"

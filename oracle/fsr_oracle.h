@@ -95,6 +95,16 @@ int orc_frame(int method, float* pos, float* dens, uint32_t n, orc_grid sp, orc_
 /* Case tables (for the table parity test). */
 uint64_t orc_mc_tri_packed(uint32_t c);
 
+/* estimateSurfaceVertices (ReconstructionCUDA.cu:123-175), never launched by
+ * the reference: ones on is_surface[V] (the caller zeroes). */
+int orc_estimate_surface_vertices(const float* dens, orc_grid dg, uint32_t ratio, uint32_t ext,
+                                  orc_grid sg, uint32_t* is_surface);
+/* The isotropic TC01 field (ReconstructionCUDA.cu:178-227), never launched by
+ * the reference, FP64, at the listed vertices. */
+int orc_field_tc01(const uint32_t* band_index, uint32_t B, const float* pos,
+                   const uint32_t* ranges, orc_grid sp, orc_grid sc, float radius_sq,
+                   double* phi);
+
 /* Deduplicated mesh (no reference counterpart): one vertex per crossed grid
  * edge used by a valid cube, ordered by (lower grid vertex, axis), interpolated
  * from the lower end in FP64; indices in the soup's order.  Returns U; writes
