@@ -116,17 +116,54 @@ struct ChunkList {
 };
 
 // ---------------------------------------------------------------- K10 -------
+__device__ __forceinline__ float wave_minf(float v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fminf(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+__device__ __forceinline__ float wave_maxf(float v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+// Squared distance from (x, y, z) to the box [b0, b1].
+__device__ __forceinline__ float box_dist2(float x, float y, float z, const float* b0,
+                                           const float* b1) {
+    const float dx = fmaxf(fmaxf(b0[0] - x, x - b1[0]), 0.0f);
+    const float dy = fmaxf(fmaxf(b0[1] - y, y - b1[1]), 0.0f);
+    const float dz = fmaxf(fmaxf(b0[2] - z, z - b1[2]), 0.0f);
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+// Bounding box of the live lanes' points.
+__device__ __forceinline__ void wave_box(bool live, float x, float y, float z, float* b0,
+                                         float* b1) {
+    b0[0] = wave_minf(live ? x : 3.0e38f); b1[0] = wave_maxf(live ? x : -3.0e38f);
+    b0[1] = wave_minf(live ? y : 3.0e38f); b1[1] = wave_maxf(live ? y : -3.0e38f);
+    b0[2] = wave_minf(live ? z : 3.0e38f); b1[2] = wave_maxf(live ? z : -3.0e38f);
+}
+
+// Candidates farther than r * (1 + 5e-4) from the bounding box of a chunk's
+// points have weight exactly 0 (and lie outside the neighbour radius) at every
+// lane: they are dropped at staging, the rest compacted in order into a
+// 128-entry queue (as in the field kernels), so the sums are bit-identical.
+constexpr float kRadiusCull = 1.001f;
+
 __global__ __launch_bounds__(kAnisoThreads) void k_smooth(
     const float4* __restrict__ sorted, const fsr_index_range* __restrict__ ranges, GridDev sp,
     float radius, float lambda, ChunkList cl, float4* __restrict__ mean_out,
     float4* __restrict__ smooth_out) {
-    __shared__ float4 stage[kAnisoThreads / 64][64];
+    __shared__ float4 queue[kAnisoThreads / 64][128];
     __shared__ uint32_t rstart[kAnisoThreads / 64][kMaxRows];
     __shared__ uint32_t rpref[kAnisoThreads / 64][kMaxRows];
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x >> 6;
+    float4* q = queue[wave];
     const uint32_t nch = *cl.num_chunks;
     const float inv_r2 = 1.0f / (radius * radius);
+    const float cull2 = radius * radius * kRadiusCull;
     uint32_t step = 0;
     for (;;) {
         const uint32_t ch = next_item_xcd(cl.next, nch, step);
@@ -147,32 +184,57 @@ __global__ __launch_bounds__(kAnisoThreads) void k_smooth(
         const float px = self.x - ox, py = self.y - oy, pz = self.z - oz;
         const float ax = 2.0f * px * inv_r2, ay = 2.0f * py * inv_r2, az = 2.0f * pz * inv_r2;
         const float c0 = __builtin_fmaf(pz, pz, __builtin_fmaf(py, py, px * px)) * inv_r2;
+        float b0[3], b1[3];
+        wave_box(live, px, py, pz, b0, b1);
         float sw = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+        auto run = [&](uint32_t n) {
+            for (uint32_t t0 = 0; t0 < n; t0 += 8u) {
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const float4 qq = q[t0 + t];
+                    // s2 = |x - q|^2 / r^2
+                    float s2 = __builtin_fmaf(-az, qq.z, c0 + qq.w);
+                    s2 = __builtin_fmaf(-ay, qq.y, s2);
+                    s2 = __builtin_fmaf(-ax, qq.x, s2);
+                    s2 = fmaxf(s2, 0.0f);
+                    const float w = fmaxf(1.0f - s2 * __builtin_sqrtf(s2), 0.0f);
+                    sw += w;
+                    sx = __builtin_fmaf(w, qq.x, sx);
+                    sy = __builtin_fmaf(w, qq.y, sy);
+                    sz = __builtin_fmaf(w, qq.z, sz);
+                }
+            }
+        };
+        uint32_t fill = 0;
         for (uint32_t base = 0; base < total; base += 64u) {
             const uint32_t g = base + lane;
-            float4 st = make_float4(0.f, 0.f, 0.f, 1.0e30f);
+            bool keep = false;
+            float4 st;
             if (g < total) {
-                const float4 q = sorted[candidate_index(g, nrows, rstart[wave], rpref[wave])];
-                const float qx = q.x - ox, qy = q.y - oy, qz = q.z - oz;
+                const float4 cand = sorted[candidate_index(g, nrows, rstart[wave], rpref[wave])];
+                const float qx = cand.x - ox, qy = cand.y - oy, qz = cand.z - oz;
+                keep = box_dist2(qx, qy, qz, b0, b1) < cull2;
                 st = make_float4(qx, qy, qz,
                                  __builtin_fmaf(qz, qz, __builtin_fmaf(qy, qy, qx * qx)) * inv_r2);
             }
-            stage[wave][lane] = st;
-            wave_lds_sync();
-#pragma unroll 8
-            for (int t = 0; t < 64; ++t) {
-                const float4 q = stage[wave][t];
-                // s2 = |x - q|^2 / r^2
-                float s2 = __builtin_fmaf(-az, q.z, c0 + q.w);
-                s2 = __builtin_fmaf(-ay, q.y, s2);
-                s2 = __builtin_fmaf(-ax, q.x, s2);
-                s2 = fmaxf(s2, 0.0f);
-                const float w = fmaxf(1.0f - s2 * __builtin_sqrtf(s2), 0.0f);
-                sw += w;
-                sx = __builtin_fmaf(w, q.x, sx);
-                sy = __builtin_fmaf(w, q.y, sy);
-                sz = __builtin_fmaf(w, q.z, sz);
+            const uint64_t m = __ballot(keep);
+            if (keep) q[fill + mask_rank(m)] = st;
+            fill += (uint32_t)__popcll(m);
+            if (fill >= 64u) {
+                wave_lds_sync();
+                run(64u);
+                wave_lds_sync();
+                const uint32_t rem = fill - 64u;
+                if (lane < rem) q[lane] = q[64u + lane];
+                wave_lds_sync();
+                fill = rem;
             }
+        }
+        if (fill) {
+            const uint32_t n = (fill + 7u) & ~7u;
+            if (lane >= fill && lane < n) q[lane] = make_float4(0.f, 0.f, 0.f, 1.0e30f);
+            wave_lds_sync();
+            run(n);
             wave_lds_sync();
         }
         if (live && sw > 0.0f) {
@@ -264,14 +326,16 @@ struct CovArgs {
 };
 
 __global__ __launch_bounds__(kAnisoThreads) void k_covariance(CovArgs a, ChunkList cl) {
-    __shared__ float4 stage[kAnisoThreads / 64][64];
+    __shared__ float4 queue[kAnisoThreads / 64][128];
     __shared__ uint32_t rstart[kAnisoThreads / 64][kMaxRows];
     __shared__ uint32_t rpref[kAnisoThreads / 64][kMaxRows];
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x >> 6;
+    float4* q = queue[wave];
     const uint32_t nch = *cl.num_chunks;
     const float r = a.radius;
     const float inv_r = 1.0f / r;
+    const float cull2 = r * r * kRadiusCull;
     uint32_t step = 0;
     for (;;) {
         const uint32_t ch = next_item_xcd(cl.next, nch, step);
@@ -292,35 +356,60 @@ __global__ __launch_bounds__(kAnisoThreads) void k_covariance(CovArgs a, ChunkLi
         float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
         if (live) m = a.mean[i];
         const float mx = m.x - ox, my = m.y - oy, mz = m.z - oz;
+        float b0[3], b1[3];
+        wave_box(live, mx, my, mz, b0, b1);
         float sw = 0.f, c00 = 0.f, c01 = 0.f, c02 = 0.f, c11 = 0.f, c12 = 0.f, c22 = 0.f;
         uint32_t cnt = 0;
+        auto run = [&](uint32_t n) {
+            for (uint32_t t0 = 0; t0 < n; t0 += 4u) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const float4 qq = q[t0 + t];
+                    const float vx = qq.x - mx, vy = qq.y - my, vz = qq.z - mz;
+                    const float d2 = __builtin_fmaf(vz, vz, __builtin_fmaf(vy, vy, vx * vx));
+                    const float dist = __builtin_sqrtf(d2);
+                    cnt += dist <= r ? 1u : 0u;
+                    const float s = dist * inv_r;
+                    const float w = dist < r ? 1.0f - s * s * s : 0.0f;
+                    sw += w;
+                    const float wx = w * vx, wy = w * vy;
+                    c00 = __builtin_fmaf(wx, vx, c00);
+                    c01 = __builtin_fmaf(wx, vy, c01);
+                    c02 = __builtin_fmaf(wx, vz, c02);
+                    c11 = __builtin_fmaf(wy, vy, c11);
+                    c12 = __builtin_fmaf(wy, vz, c12);
+                    c22 = __builtin_fmaf(w * vz, vz, c22);
+                }
+            }
+        };
+        uint32_t fill = 0;
         for (uint32_t base = 0; base < total; base += 64u) {
             const uint32_t g = base + lane;
-            float4 st = make_float4(1.0e18f, 1.0e18f, 1.0e18f, 0.f);
+            bool keep = false;
+            float4 st;
             if (g < total) {
-                const float4 q = a.sorted[candidate_index(g, nrows, rstart[wave], rpref[wave])];
-                st = make_float4(q.x - ox, q.y - oy, q.z - oz, 0.f);
+                const float4 cand = a.sorted[candidate_index(g, nrows, rstart[wave], rpref[wave])];
+                st = make_float4(cand.x - ox, cand.y - oy, cand.z - oz, 0.f);
+                keep = box_dist2(st.x, st.y, st.z, b0, b1) < cull2;
             }
-            stage[wave][lane] = st;
+            const uint64_t bm = __ballot(keep);
+            if (keep) q[fill + mask_rank(bm)] = st;
+            fill += (uint32_t)__popcll(bm);
+            if (fill >= 64u) {
+                wave_lds_sync();
+                run(64u);
+                wave_lds_sync();
+                const uint32_t rem = fill - 64u;
+                if (lane < rem) q[lane] = q[64u + lane];
+                wave_lds_sync();
+                fill = rem;
+            }
+        }
+        if (fill) {
+            const uint32_t n = (fill + 3u) & ~3u;
+            if (lane >= fill && lane < n) q[lane] = make_float4(1.0e18f, 1.0e18f, 1.0e18f, 0.f);
             wave_lds_sync();
-#pragma unroll 4
-            for (int t = 0; t < 64; ++t) {
-                const float4 q = stage[wave][t];
-                const float vx = q.x - mx, vy = q.y - my, vz = q.z - mz;
-                const float d2 = __builtin_fmaf(vz, vz, __builtin_fmaf(vy, vy, vx * vx));
-                const float dist = __builtin_sqrtf(d2);
-                cnt += dist <= r ? 1u : 0u;
-                const float s = dist * inv_r;
-                const float w = dist < r ? 1.0f - s * s * s : 0.0f;
-                sw += w;
-                const float wx = w * vx, wy = w * vy;
-                c00 = __builtin_fmaf(wx, vx, c00);
-                c01 = __builtin_fmaf(wx, vy, c01);
-                c02 = __builtin_fmaf(wx, vz, c02);
-                c11 = __builtin_fmaf(wy, vy, c11);
-                c12 = __builtin_fmaf(wy, vz, c12);
-                c22 = __builtin_fmaf(w * vz, vz, c22);
-            }
+            run(n);
             wave_lds_sync();
         }
         if (live) {

@@ -184,7 +184,7 @@ enum ArenaSlot {
     A_CHUNK_CELL, A_NVERTS, A_VALID, A_VALID_OFF, A_VERT_OFF, A_VALID_IDX,
     A_COUNTERS, A_SCAN_PARTIALS, A_STATS, A_INVOLVE, A_INVOLVE_OFF,
     A_INVOLVED_IDX, A_MEAN, A_SMOOTH, A_GMAT, A_SIGMA, A_AUX0, A_AUX1, A_REC,
-    A_ADJ_VERT, A_ADJ_COEF, A_ADJ_AUX, A_ADJ_TOUCH, A_EDGE_MASK, A_EDGE_OFF,
+    A_ADJ_VERT, A_ADJ_COEF, A_ADJ_AUX, A_ADJ_TOUCH, A_EDGE_MASK, A_EDGE_OFF, A_SURF_CHUNKS,
     A_NUM_SLOTS
 };
 

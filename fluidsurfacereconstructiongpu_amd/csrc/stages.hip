@@ -165,6 +165,24 @@ __device__ __forceinline__ int clampi_s(int v, int lo, int hi) {
     return v < lo ? lo : (v > hi ? hi : v);
 }
 
+// Largest range end over the cells (the particles in cells form a prefix of the
+// cell-sorted array) into *max_end (zeroed by the caller).
+__global__ void k_out_of_grid(const fsr_index_range* __restrict__ ranges, uint32_t C, uint32_t n,
+                              uint32_t* max_end) {
+    uint32_t m = 0;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
+        const fsr_index_range r = ranges[c];
+        if (r.start != 0xFFFFFFFFu && r.end <= n) m = max(m, r.end);
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, d, 64));
+    if (__lane_id() == 0 && m) atomicMax(max_end, m);
+}
+
+__global__ void k_out_of_grid_finish(const uint32_t* max_end, uint32_t n, fsr_frame_stats* st) {
+    if (threadIdx.x == 0) st->num_out_of_grid = n - *max_end;
+}
+
 // estimateSurfaceVertices (ReconstructionCUDA.cu:123-175): a density-grid cell
 // (not on the last layer) whose 8 corners straddle 0.5 (getVertexFlag,
 // ReconstructionCUDA.cuh:491-511: bit i set when corner i <= 0.5) marks every
@@ -409,6 +427,17 @@ fsr_status fsr_extraction_of_surface_particles_for_akinci(
     hipLaunchKernelGGL(k_pack_f4, dim3(blocks_for(num_particles)), dim3(256), 0, s, particles,
                        densities, num_particles, p4);
     FSR_LAUNCH_CHECK("k_pack_f4");
+    fsr_frame_stats* st;
+    Counters* ct;
+    FSR_TRY(stage_state(ctx, &st, &ct, s));
+    // particles past the last cell range are outside the grid
+    const uint32_t C = (uint32_t)grid_count(*sp);
+    hipLaunchKernelGGL(k_out_of_grid, dim3(256), dim3(256), 0, s, ranges, C, num_particles,
+                       &ct->aux[5]);
+    FSR_LAUNCH_CHECK("k_out_of_grid");
+    hipLaunchKernelGGL(k_out_of_grid_finish, dim3(1), dim3(64), 0, s, &ct->aux[5], num_particles,
+                       st);
+    FSR_LAUNCH_CHECK("k_out_of_grid_finish");
     SurfaceArgs a;
     std::memset(&a, 0, sizeof(a));
     a.sorted = p4;
@@ -417,6 +446,8 @@ fsr_status fsr_extraction_of_surface_particles_for_akinci(
     a.sp = to_dev(*sp);
     a.h = sim->smoothing_radius;
     a.flags = surface_flags;
+    a.counters = ct;
+    a.st = st;
     return surface_colorfield(ctx, a, sim->particle_mass, s);
 }
 

@@ -73,16 +73,11 @@ __global__ __launch_bounds__(256) void k_cell_surface(const float* __restrict__ 
     }
 }
 
-// Band seeding for one particle (called by every thread of the block; `surf` =
-// its flag).
-__device__ __forceinline__ void seed_band(const SurfaceArgs& a, uint32_t i, float px, float py,
-                                          float pz, bool surf) {
-    if (!a.sbits) {  // detection only (uniform across the launch)
-        if (a.st)
-            block_count_add2(&a.st->num_surface_particles, surf ? 1u : 0u,
-                             &a.st->band_exceptions, 0u);
-        return;
-    }
+// Band seeding of one particle: its scalar cell is set in the surface-cell
+// mask, or the particle is appended to the exception list (exact path).
+// Wave-uniform call; returns whether the particle became an exception.
+__device__ __forceinline__ bool seed_particle(const SurfaceArgs& a, uint32_t i, float px,
+                                              float py, float pz, bool surf) {
     const float qx = cell_coord(px, a.sc.minx, a.sc.cell);
     const float qy = cell_coord(py, a.sc.miny, a.sc.cell);
     const float qz = cell_coord(pz, a.sc.minz, a.sc.cell);
@@ -103,8 +98,17 @@ __device__ __forceinline__ void seed_band(const SurfaceArgs& a, uint32_t i, floa
         base = __shfl(base, __ffsll((unsigned long long)m) - 1, 64);
         if (exc) a.exceptions[base + mask_rank(m)] = i;
     }
-    block_count_add2(&a.st->num_surface_particles, surf ? 1u : 0u, &a.st->band_exceptions,
-                     exc ? 1u : 0u);
+    return exc;
+}
+
+// Band seeding for one particle and the frame's surface / exception counts
+// (called by every thread of the block; `surf` = its flag).
+__device__ __forceinline__ void seed_band(const SurfaceArgs& a, uint32_t i, float px, float py,
+                                          float pz, bool surf) {
+    const bool exc = a.sbits ? seed_particle(a, i, px, py, pz, surf) : false;
+    if (a.st)
+        block_count_add2(&a.st->num_surface_particles, surf ? 1u : 0u, &a.st->band_exceptions,
+                         exc ? 1u : 0u);
 }
 
 __global__ __launch_bounds__(256) void k_flags_from_cells(SurfaceArgs a,
@@ -123,22 +127,24 @@ __global__ __launch_bounds__(256) void k_flags_from_cells(SurfaceArgs a,
     seed_band(a, i, p.x, p.y, p.z, surf);
 }
 
-__global__ __launch_bounds__(256) void k_colorfield(SurfaceArgs a, float mass) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i < a.n;
-    bool surf = false;
-    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (live) {
-        p = a.sorted[i];
+// Particles outside the hash grid (sorted last, in no cell): one thread each,
+// over the clamped block around their own clamped cell.  Grid-stride over the
+// device-side count; the surface / exception counts are added once per block.
+__global__ __launch_bounds__(256) void k_colorfield_tail(SurfaceArgs a, float mass) {
+    const uint32_t first = a.n - a.st->num_out_of_grid;
+    const float h = a.h;
+    const float inv_h = 1.0f / h;
+    const float gcoef = -45.0f / (3.14159265358979f * (h * h) * (h * h));
+    uint32_t n_surf = 0, n_exc = 0;
+    for (uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
+         i += gridDim.x * blockDim.x) {
+        const float4 p = a.sorted[i];
         const int cx = trunc_cell(cell_coord(p.x, a.sp.minx, a.sp.cell), a.sp.rx);
         const int cy = trunc_cell(cell_coord(p.y, a.sp.miny, a.sp.cell), a.sp.ry);
         const int cz = trunc_cell(cell_coord(p.z, a.sp.minz, a.sp.cell), a.sp.rz);
         const int x0 = clampi(cx - 1, 0, a.sp.rx - 1), x1 = clampi(cx + 1, 0, a.sp.rx - 1);
         const int y0 = clampi(cy - 1, 0, a.sp.ry - 1), y1 = clampi(cy + 1, 0, a.sp.ry - 1);
         const int z0 = clampi(cz - 1, 0, a.sp.rz - 1), z1 = clampi(cz + 1, 0, a.sp.rz - 1);
-        const float h = a.h;
-        const float inv_h = 1.0f / h;
-        const float gcoef = -45.0f / (3.14159265358979f * (h * h) * (h * h));
         uint32_t count = 0;
         float gx = 0.f, gy = 0.f, gz = 0.f;
         for (int zz = z0; zz <= z1; ++zz)
@@ -171,10 +177,171 @@ __global__ __launch_bounds__(256) void k_colorfield(SurfaceArgs a, float mass) {
                 }
             }
         const float g2 = __builtin_fmaf(gz, gz, __builtin_fmaf(gy, gy, gx * gx));
-        surf = count < 25u || __builtin_sqrtf(g2) > 3.0f;
+        const bool surf = count < 25u || __builtin_sqrtf(g2) > 3.0f;
         a.flags[i] = surf ? 1u : 0u;
+        const bool exc = a.sbits ? seed_particle(a, i, p.x, p.y, p.z, surf) : false;
+        n_surf += surf ? 1u : 0u;
+        n_exc += exc ? 1u : 0u;
     }
-    seed_band(a, i, p.x, p.y, p.z, surf);
+    block_count_add2(&a.st->num_surface_particles, n_surf, &a.st->band_exceptions, n_exc);
+}
+
+__device__ __forceinline__ float wave_minf(float v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fminf(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+__device__ __forceinline__ float wave_maxf(float v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct ColorChunks {
+    const uint32_t* chunk_cell;
+    const uint32_t* chunk_off;
+    const uint32_t* num_chunks;
+    uint32_t* next;  // per-XCD fetch counters
+};
+
+// The colour-field test of the particles of the hash cells, a wave per chunk of
+// up to 64 particles of one cell: the 9 x-runs of the cell's 3^3 block are
+// staged through LDS and read as broadcasts.  Candidates farther than
+// h * (1 + 5e-5) from the bounding box of the chunk's particles can only add to
+// the neighbour count (d^2 > 0, dist >= h): they are counted and dropped, the
+// others are compacted in order and evaluated with the per-thread kernel's
+// arithmetic, so counts, gradients and flags are identical to it.
+__global__ __launch_bounds__(256) void k_colorfield(SurfaceArgs a, float mass, ColorChunks cl) {
+    __shared__ float4 queue[4][128];
+    __shared__ uint32_t row_start[4][9];
+    __shared__ uint32_t row_pref[4][10];
+    const uint32_t lane = __lane_id();
+    const uint32_t wave = threadIdx.x >> 6;
+    float4* q = queue[wave];
+    const uint32_t nch = *cl.num_chunks;
+    const float h = a.h;
+    const float inv_h = 1.0f / h;
+    const float gcoef = -45.0f / (3.14159265358979f * (h * h) * (h * h));
+    const float cull2 = (h * h) * 1.0001f;
+    uint32_t n_surf = 0, n_exc = 0;
+    uint32_t step = 0;
+    for (;;) {
+        const uint32_t ch = next_item_xcd(cl.next, nch, step);
+        if (ch >= nch) break;
+        const uint32_t c = cl.chunk_cell[ch];
+        const uint32_t k = ch - cl.chunk_off[c];
+        const fsr_index_range own = a.ranges[c];
+        const uint32_t i = own.start + 64u * k + lane;
+        const bool live = i < own.end;
+        const int cx = (int)(c % a.sp.rx), cy = (int)((c / a.sp.rx) % a.sp.ry);
+        const int cz = (int)(c / (a.sp.rx * a.sp.ry));
+        const int x0 = clampi(cx - 1, 0, a.sp.rx - 1), x1 = clampi(cx + 1, 0, a.sp.rx - 1);
+        const int y0 = clampi(cy - 1, 0, a.sp.ry - 1), y1 = clampi(cy + 1, 0, a.sp.ry - 1);
+        const int z0 = clampi(cz - 1, 0, a.sp.rz - 1), z1 = clampi(cz + 1, 0, a.sp.rz - 1);
+        const int ny = y1 - y0 + 1;
+        const int nrows = ny * (z1 - z0 + 1);
+        uint32_t len = 0, rs = 0;
+        if ((int)lane < nrows) {
+            const int yy = y0 + (int)lane % ny, zz = z0 + (int)lane / ny;
+            const uint32_t rowc = ((uint32_t)zz * a.sp.ry + yy) * a.sp.rx;
+            uint32_t s0 = 0xFFFFFFFFu, e0 = 0;
+            for (int xx = x0; xx <= x1; ++xx) {
+                const fsr_index_range rg = a.ranges[rowc + xx];
+                if (rg.start != 0xFFFFFFFFu) {
+                    if (s0 == 0xFFFFFFFFu) s0 = rg.start;
+                    e0 = rg.end;
+                }
+            }
+            if (s0 != 0xFFFFFFFFu) { rs = s0; len = e0 - s0; }
+        }
+        const uint32_t incl = wave_inclusive_sum(len);
+        const uint32_t total = __shfl(incl, 63, 64);
+        if ((int)lane < nrows) {
+            row_start[wave][lane] = rs;
+            row_pref[wave][lane] = incl - len;
+        }
+        wave_lds_sync();
+        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (live) p = a.sorted[i];
+        const float bx0 = wave_minf(live ? p.x : 3.0e38f), bx1 = wave_maxf(live ? p.x : -3.0e38f);
+        const float by0 = wave_minf(live ? p.y : 3.0e38f), by1 = wave_maxf(live ? p.y : -3.0e38f);
+        const float bz0 = wave_minf(live ? p.z : 3.0e38f), bz1 = wave_maxf(live ? p.z : -3.0e38f);
+        uint32_t count = 0, kept = 0, fill = 0;
+        float gx = 0.f, gy = 0.f, gz = 0.f;
+        int j = 0;
+        auto run = [&](uint32_t n) {
+            for (uint32_t t = 0; t < n; ++t) {
+                const float4 qq = q[t];
+                const float rx = p.x - qq.x, ry = p.y - qq.y, rz = p.z - qq.z;
+                const float d2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
+                if (d2 > 0.0f) {
+                    ++count;
+                    const float dist = __builtin_sqrtf(d2);
+                    if (dist < h) {
+                        const float x = 1.0f - dist * inv_h;
+                        const float cc = qq.w * x * x * __builtin_amdgcn_rsqf(d2);
+                        gx = __builtin_fmaf(cc, rx, gx);
+                        gy = __builtin_fmaf(cc, ry, gy);
+                        gz = __builtin_fmaf(cc, rz, gz);
+                    }
+                }
+            }
+        };
+        for (uint32_t base = 0; base < total; base += 64u) {
+            const uint32_t g = base + lane;
+            bool keep = false;
+            float4 st;
+            if (g < total) {
+                while (j + 1 < nrows && row_pref[wave][j + 1] <= g) ++j;
+                const float4 cand = a.sorted[row_start[wave][j] + (g - row_pref[wave][j])];
+                const float dx = fmaxf(fmaxf(bx0 - cand.x, cand.x - bx1), 0.0f);
+                const float dy = fmaxf(fmaxf(by0 - cand.y, cand.y - by1), 0.0f);
+                const float dz = fmaxf(fmaxf(bz0 - cand.z, cand.z - bz1), 0.0f);
+                keep = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < cull2;
+                // (m / rho_j) * gcoef, in the per-thread kernel's operation order
+                st = make_float4(cand.x, cand.y, cand.z, mass / cand.w * gcoef);
+            }
+            const uint64_t m = __ballot(keep);
+            if (keep) q[fill + mask_rank(m)] = st;
+            const uint32_t added = (uint32_t)__popcll(m);
+            fill += added;
+            kept += added;
+            if (fill >= 64u) {
+                wave_lds_sync();
+                run(64u);
+                wave_lds_sync();
+                const uint32_t rem = fill - 64u;
+                if (lane < rem) q[lane] = q[64u + lane];
+                wave_lds_sync();
+                fill = rem;
+            }
+        }
+        if (fill) {
+            wave_lds_sync();
+            run(fill);
+            wave_lds_sync();
+        }
+        // every dropped candidate has d^2 > 0: it counts as a neighbour
+        count += total - kept;
+        bool surf = false;
+        if (live) {
+            const float g2 = __builtin_fmaf(gz, gz, __builtin_fmaf(gy, gy, gx * gx));
+            surf = count < 25u || __builtin_sqrtf(g2) > 3.0f;
+            a.flags[i] = surf ? 1u : 0u;
+        }
+        bool exc = false;
+        if (a.sbits) exc = seed_particle(a, i, p.x, p.y, p.z, surf);
+        n_surf += surf ? 1u : 0u;
+        n_exc += exc ? 1u : 0u;
+    }
+    block_count_add2(&a.st->num_surface_particles, n_surf, &a.st->band_exceptions, n_exc);
 }
 
 // band seeding from given flags (stage entry point of the band)
@@ -219,11 +386,22 @@ fsr_status surface_occupancy(fsr_context* ctx, const SurfaceArgs& a, uint8_t* su
 
 fsr_status surface_colorfield(fsr_context* ctx, const SurfaceArgs& a, float mass,
                               hipStream_t s) {
-    (void)ctx;
-    if (a.n) {
-        hipLaunchKernelGGL(k_colorfield, dim3((a.n + 255) / 256), dim3(256), 0, s, a, mass);
-        FSR_LAUNCH_CHECK("k_colorfield");
-    }
+    if (!a.n) return FSR_OK;
+    const uint32_t C = a.sp.rx * a.sp.ry * a.sp.rz;
+    const uint64_t max_chunks = (uint64_t)a.n / 64 + C + 1;
+    uint32_t* scratch;
+    FSR_TRY(arena_typed(ctx, A_SURF_CHUNKS, 2ull * C + max_chunks, &scratch));
+    uint32_t* total = &a.counters->aux[4];
+    FSR_TRY(cell_chunk_list(ctx, a.ranges, nullptr, C, scratch, scratch + C, scratch + 2ull * C,
+                            total, s));
+    ColorChunks cl{scratch + 2ull * C, scratch + C, total, a.counters->xcd_next};
+    FSR_TRY(reset_xcd(a.counters, s));
+    const int blocks = persistent_blocks(ctx, 256, 8, TUNE_FIELD);
+    hipLaunchKernelGGL(k_colorfield, dim3(blocks), dim3(256), 0, s, a, mass, cl);
+    FSR_LAUNCH_CHECK("k_colorfield");
+    // particles outside the grid (a device-side count)
+    hipLaunchKernelGGL(k_colorfield_tail, dim3(64), dim3(256), 0, s, a, mass);
+    FSR_LAUNCH_CHECK("k_colorfield_tail");
     return FSR_OK;
 }
 
