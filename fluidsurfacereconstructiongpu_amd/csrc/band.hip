@@ -130,22 +130,43 @@ __global__ __launch_bounds__(256) void k_band_exceptions(BandArgs a) {
     }
 }
 
+// Each word writes its set bits in ascending order at its scanned offset.  A
+// block's 256 words own one contiguous output range: it is assembled in LDS and
+// stored with consecutive lanes on consecutive entries (direct per-word stores
+// when the range exceeds the staging buffer).
 __global__ __launch_bounds__(256) void k_band_emit(const uint32_t* __restrict__ bits,
                                                    const uint32_t* __restrict__ word_off,
                                                    uint64_t nwords, uint32_t wx, uint32_t rx,
                                                    uint32_t* __restrict__ band_index) {
-    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= nwords) return;
-    uint32_t b = bits[w];
-    if (!b) return;
+    constexpr uint32_t kStage = 4096;
+    __shared__ uint32_t stage[kStage];
+    __shared__ uint32_t range[2];
+    const uint64_t first = (uint64_t)blockIdx.x * blockDim.x;
+    const uint64_t w = first + threadIdx.x;
+    const uint64_t last = min(first + blockDim.x, nwords) - 1;
+    const bool live = w < nwords;
+    uint32_t b = live ? bits[w] : 0u;
+    const uint32_t o = live ? word_off[w] : 0u;
+    if (w == first) range[0] = o;
+    if (w == last) range[1] = o + (uint32_t)__popc(b);
+    __syncthreads();
+    const uint32_t lo = range[0], total = range[1] - range[0];
     const uint64_t row = w / wx;
-    const uint32_t x0 = (uint32_t)(w % wx) * 32u;
-    const uint32_t base = (uint32_t)(row * rx) + x0;
-    uint32_t o = word_off[w];
-    while (b) {
-        const uint32_t t = __builtin_ctz(b);
-        band_index[o++] = base + t;
-        b &= b - 1u;
+    const uint32_t vbase = (uint32_t)(row * rx) + (uint32_t)(w % wx) * 32u;
+    if (total <= kStage) {
+        uint32_t k = o - lo;
+        while (b) {
+            stage[k++] = vbase + __builtin_ctz(b);
+            b &= b - 1u;
+        }
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t < total; t += blockDim.x) band_index[lo + t] = stage[t];
+    } else {
+        uint32_t k = o;
+        while (b) {
+            band_index[k++] = vbase + __builtin_ctz(b);
+            b &= b - 1u;
+        }
     }
 }
 
@@ -154,10 +175,32 @@ __global__ __launch_bounds__(256) void k_band_emit(const uint32_t* __restrict__ 
 //   vc_x[rx] vc_y[ry] vc_z[rz] lo_x[vcx] hi_x[vcx] lo_y[vcy] hi_y[vcy] lo_z[vcz] hi_z[vcz]
 struct AxisMaps {
     uint32_t *vc[3], *lo[3], *hi[3];
+    uint32_t* row_any;   // bit (z ry + y): scalar row (y, z) holds a band vertex
+    uint64_t row_words;  // words of row_any
 };
 
-__global__ void k_axis_maps(GridDev sc, GridDev sp, int radius, AxisMaps m) {
+// blockIdx.y = 0, 1, 2: the maps of axis x, y, z; blockIdx.y = 3: the row
+// occupancy bits, one wave per 64 rows (each wave writes its two words).
+__global__ __launch_bounds__(256) void k_axis_maps(GridDev sc, GridDev sp, int radius, AxisMaps m,
+                                                   const uint32_t* __restrict__ bits, uint32_t wx) {
     const uint32_t axis = blockIdx.y;
+    if (axis == 3) {
+        const uint64_t rows = (uint64_t)sc.ry * sc.rz;
+        for (uint64_t r0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); r0 < rows;
+             r0 += (uint64_t)gridDim.x * blockDim.x) {
+            const uint64_t r = r0 + __lane_id();
+            uint32_t any = 0;
+            if (r < rows)
+                for (uint32_t w = 0; w < wx; ++w) any |= bits[r * wx + w];
+            const uint64_t b = __ballot(any != 0);
+            if (__lane_id() == 0) {
+                const uint64_t k = r0 >> 5;
+                m.row_any[k] = (uint32_t)b;
+                if (k + 1 < m.row_words) m.row_any[k + 1] = (uint32_t)(b >> 32);
+            }
+        }
+        return;
+    }
     const uint32_t res = axis == 0 ? sc.rx : (axis == 1 ? sc.ry : sc.rz);
     const float smin = axis == 0 ? sc.minx : (axis == 1 ? sc.miny : sc.minz);
     const float pmin = axis == 0 ? sp.minx : (axis == 1 ? sp.miny : sp.minz);
@@ -176,67 +219,108 @@ __global__ void k_axis_maps(GridDev sc, GridDev sp, int radius, AxisMaps m) {
     }
 }
 
-// One wave per virtual cell: the cell's scalar rows (y, z) in row-major order,
-// each row's band bits in [xlo, xhi).  Pass 0 writes the cell's band vertex
-// count; pass 1 (kFill) writes the cell's vertex ids in ascending order, so a
-// 64-vertex chunk is a compact slab of the cell (1-2 z-layers), which keeps the
-// candidate culling of the field kernels effective.  No atomics: the list is
-// deterministic.
+// Band bits of one mask word restricted to the vertex range [xlo, xhi).
+__device__ __forceinline__ uint32_t clip_word(uint32_t word, uint32_t w, uint32_t xlo,
+                                              uint32_t xhi) {
+    const uint32_t lo = w * 32u, hi = lo + 32u;
+    if (xlo > lo) word &= ~((1u << (xlo - lo)) - 1u);
+    if (xhi < hi) word &= (1u << (xhi - lo)) - 1u;
+    return word;
+}
+
+// 32 row occupancy bits starting at row `bit0` (bits past the end read 0).
+__device__ __forceinline__ uint32_t row_bits32(const AxisMaps& m, uint64_t bit0) {
+    const uint64_t w = bit0 >> 5;
+    const uint32_t sh = (uint32_t)(bit0 & 31u);
+    const uint32_t lo = w < m.row_words ? m.row_any[w] : 0u;
+    if (!sh) return lo;
+    const uint32_t hi = w + 1 < m.row_words ? m.row_any[w + 1] : 0u;
+    return (lo >> sh) | (hi << (32u - sh));
+}
+
+// Occupied rows y in [ylo + yb, min(ylo + yb + 32, yhi)) of layer z, bit i = row ylo + yb + i.
+__device__ __forceinline__ uint32_t layer_rows(const AxisMaps& m, const GridDev& sc, uint32_t z,
+                                               uint32_t ylo, uint32_t yb, uint32_t ny) {
+    uint32_t b = row_bits32(m, (uint64_t)z * sc.ry + ylo + yb);
+    const uint32_t left = ny - yb;
+    if (left < 32u) b &= (1u << left) - 1u;
+    return b;
+}
+
+// Cell-major band list, built per slab: the slab (cx, cy, z) is the part of
+// virtual cell (cx, cy, cz = vc_z[z]) on scalar layer z.  Slabs are numbered
+// cell by cell (v = (cz vcy + cy) vcx + cx ascending) and by layer inside a
+// cell:
+//   slab(cx, cy, z) = vcx vcy lo_z[cz] + (cy vcx + cx) nz(cz) + (z - lo_z[cz]),
+// so the exclusive scan of the slab counts is each slab's offset in the cell
+// list, and a cell's vertices come out in ascending id order (layers z, rows y,
+// then x), i.e. a 64-vertex chunk is a compact slab of the cell, which keeps
+// the candidate culling of the field kernels effective.  One thread per slab
+// (consecutive threads = consecutive cx, which read the same row words); rows
+// without band vertices are skipped with the row occupancy bits.  Pass 0
+// writes the slab's band vertex count, pass 1 (kFill) its vertex ids at the
+// scanned offset.  No atomics: the list is deterministic.
 template <bool kFill>
-__global__ __launch_bounds__(256) void k_cell_rows(const uint32_t* __restrict__ bits, GridDev sc,
+__global__ __launch_bounds__(256) void k_slab_rows(const uint32_t* __restrict__ bits, GridDev sc,
                                                    uint32_t wx, AxisMaps m, uint32_t vcx,
-                                                   uint32_t vcy, uint32_t nvc,
-                                                   uint32_t* __restrict__ cnt,
-                                                   const uint32_t* __restrict__ cell_off,
+                                                   uint32_t vcy, uint32_t* __restrict__ slab,
                                                    uint32_t* __restrict__ cell_band) {
-    const uint32_t lane = __lane_id();
-    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t v = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; v < nvc; v += nwaves) {
-        const uint32_t cx = v % vcx, cy = (v / vcx) % vcy, cz = v / (vcx * vcy);
-        const uint32_t xlo = m.lo[0][cx], xhi = m.hi[0][cx];
+    const uint64_t total = (uint64_t)vcx * vcy * sc.rz;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t cx = (uint32_t)(t % vcx);
+        const uint32_t cy = (uint32_t)((t / vcx) % vcy);
+        const uint32_t z = (uint32_t)(t / ((uint64_t)vcx * vcy));
+        const uint32_t cz = m.vc[2][z];
+        const uint32_t zlo = m.lo[2][cz], nz = m.hi[2][cz] - zlo;
         const uint32_t ylo = m.lo[1][cy], yhi = m.hi[1][cy];
-        const uint32_t zlo = m.lo[2][cz], zhi = m.hi[2][cz];
-        const uint32_t ny = yhi - ylo;
-        const uint32_t nrows = (xhi > xlo) ? ny * (zhi - zlo) : 0u;
-        const uint32_t base = kFill ? cell_off[v] : 0u;
-        const uint32_t w0 = xlo >> 5, w1 = (xhi + 31u) >> 5;
-        uint32_t running = 0;
-        for (uint32_t r0 = 0; r0 < nrows; r0 += 64u) {
-            const uint32_t r = r0 + lane;
-            uint32_t c = 0;
-            const uint32_t* rb = bits;
-            uint32_t lin0 = 0;
-            if (r < nrows) {
-                const uint32_t y = ylo + r % ny, z = zlo + r / ny;
-                const uint64_t row = (uint64_t)z * sc.ry + y;
-                rb = bits + row * wx;
-                lin0 = (uint32_t)(row * sc.rx);
-                for (uint32_t w = w0; w < w1; ++w) {
-                    uint32_t word = rb[w];
-                    const uint32_t lo = w * 32u, hi = lo + 32u;
-                    if (xlo > lo) word &= ~((1u << (xlo - lo)) - 1u);
-                    if (xhi < hi) word &= (1u << (xhi - lo)) - 1u;
-                    c += __popc(word);
-                }
-            }
-            const uint32_t incl = wave_inclusive_sum(c);
-            if (kFill && c) {
-                uint32_t o = base + running + incl - c;
-                for (uint32_t w = w0; w < w1; ++w) {
-                    uint32_t word = rb[w];
-                    const uint32_t lo = w * 32u, hi = lo + 32u;
-                    if (xlo > lo) word &= ~((1u << (xlo - lo)) - 1u);
-                    if (xhi < hi) word &= (1u << (xhi - lo)) - 1u;
-                    while (word) {
-                        cell_band[o++] = lin0 + lo + __builtin_ctz(word);
-                        word &= word - 1u;
+        const uint32_t xlo = m.lo[0][cx], xhi = m.hi[0][cx];
+        const uint64_t idx = (uint64_t)vcx * vcy * zlo + ((uint64_t)cy * vcx + cx) * nz + (z - zlo);
+        const uint32_t ny = yhi > ylo ? yhi - ylo : 0u;
+        const uint32_t w0 = xlo >> 5;
+        const uint32_t nw = xhi > xlo ? ((xhi + 31u) >> 5) - w0 : 0u;
+        uint32_t o = kFill ? slab[idx] : 0u;
+        for (uint32_t yb = 0; nw && yb < ny; yb += 32u) {
+            uint32_t rows = layer_rows(m, sc, z, ylo, yb, ny);
+            while (rows) {
+                const uint64_t row = (uint64_t)z * sc.ry + ylo + yb + __builtin_ctz(rows);
+                rows &= rows - 1u;
+                for (uint32_t j = 0; j < nw; ++j) {
+                    const uint32_t w = w0 + j;
+                    uint32_t word = clip_word(bits[row * wx + w], w, xlo, xhi);
+                    if (!kFill) {
+                        o += __popc(word);
+                    } else {
+                        const uint32_t base = (uint32_t)(row * sc.rx) + w * 32u;
+                        while (word) {
+                            cell_band[o++] = base + __builtin_ctz(word);
+                            word &= word - 1u;
+                        }
                     }
                 }
             }
-            running += __shfl(incl, 63, 64);
         }
-        if (!kFill && lane == 0) cnt[v] = running;
+        if (!kFill) slab[idx] = o;
     }
+}
+
+// Per virtual cell: offset and count in the cell list from the scanned slabs.
+__global__ __launch_bounds__(256) void k_cells_from_slabs(AxisMaps m, uint32_t vcx, uint32_t vcy,
+                                                          uint32_t nvc, uint64_t nslabs,
+                                                          const uint32_t* __restrict__ slab_off,
+                                                          const uint32_t* __restrict__ slab_total,
+                                                          uint32_t* __restrict__ cnt,
+                                                          uint32_t* __restrict__ cell_off) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvc) return;
+    const uint32_t cx = v % vcx, cy = (v / vcx) % vcy, cz = v / (vcx * vcy);
+    const uint32_t zlo = m.lo[2][cz], nz = m.hi[2][cz] - zlo;
+    const uint64_t first = (uint64_t)vcx * vcy * zlo + ((uint64_t)cy * vcx + cx) * nz;
+    const uint32_t tot = *slab_total;
+    const uint32_t a = first < nslabs ? slab_off[first] : tot;
+    const uint32_t b = first + nz < nslabs ? slab_off[first + nz] : tot;
+    cell_off[v] = a;
+    cnt[v] = b - a;
 }
 
 __global__ __launch_bounds__(256) void k_chunk_cells(const uint32_t* __restrict__ cnt,
@@ -307,7 +391,10 @@ fsr_status band_compact(fsr_context* ctx, const uint32_t* bits, const GridDev& s
     const uint32_t vcx = sp.rx + 2 * radius, vcy = sp.ry + 2 * radius, vcz = sp.rz + 2 * radius;
     const uint64_t nvc = (uint64_t)vcx * vcy * vcz;
     uint32_t *maps, *cnt, *cell_off, *chunk_off, *cell_band, *chunk_cell;
-    const uint64_t map_len = (uint64_t)sc.rx + sc.ry + sc.rz + 2ull * (vcx + vcy + vcz);
+    const uint64_t rows = (uint64_t)sc.ry * sc.rz;
+    const uint64_t row_words = 2ull * ((rows + 63) / 64);
+    const uint64_t map_len =
+        (uint64_t)sc.rx + sc.ry + sc.rz + 2ull * (vcx + vcy + vcz) + row_words;
     FSR_TRY(arena_typed(ctx, A_AXIS_MAPS, map_len, &maps));
     FSR_TRY(arena_typed(ctx, A_CELL_CNT, nvc, &cnt));
     FSR_TRY(arena_typed(ctx, A_CELL_OFF, nvc, &cell_off));
@@ -325,21 +412,32 @@ fsr_status band_compact(fsr_context* ctx, const uint32_t* bits, const GridDev& s
     m.hi[1] = m.lo[1] + vcy;
     m.lo[2] = m.hi[1] + vcy;
     m.hi[2] = m.lo[2] + vcz;
+    m.row_any = m.hi[2] + vcz;
+    m.row_words = row_words;
     FSR_TRY(fill_u32(m.lo[0], 0u, 2ull * (vcx + vcy + vcz), s));
     const uint32_t maxres = std::max(sc.rx, std::max(sc.ry, sc.rz));
-    hipLaunchKernelGGL(k_axis_maps, dim3((maxres + 255) / 256, 3), dim3(256), 0, s, sc, sp, radius, m);
+    const uint32_t map_blocks = (uint32_t)std::max<uint64_t>(
+        (maxres + 255) / 256, std::min<uint64_t>((rows + 255) / 256, 1024));
+    hipLaunchKernelGGL(k_axis_maps, dim3(map_blocks, 4), dim3(256), 0, s, sc, sp, radius, m, bits,
+                       wx);
     FSR_LAUNCH_CHECK("k_axis_maps");
-    // one wave per virtual cell, 4 per block
-    const uint32_t cell_blocks = (uint32_t)std::min<uint64_t>((nvc + 3) / 4, 65535u * 4u);
-    hipLaunchKernelGGL(k_cell_rows<false>, dim3(cell_blocks), dim3(256), 0, s, bits, sc, wx, m,
-                       vcx, vcy, (uint32_t)nvc, cnt, (const uint32_t*)nullptr, cell_band);
-    FSR_LAUNCH_CHECK("k_cell_rows(count)");
-    FSR_TRY(scan_exclusive(ctx, SCAN_U32, cnt, cell_off, nvc, nullptr, nullptr, s));
+    // slab counts -> slab offsets -> per-cell offsets and counts -> slab fill
+    const uint64_t nslabs = (uint64_t)vcx * vcy * sc.rz;
+    uint32_t* slab;
+    FSR_TRY(arena_typed(ctx, A_SLAB, nslabs, &slab));
+    const uint32_t slab_blocks = (uint32_t)std::min<uint64_t>((nslabs + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_slab_rows<false>, dim3(slab_blocks), dim3(256), 0, s, bits, sc, wx, m,
+                       vcx, vcy, slab, cell_band);
+    FSR_LAUNCH_CHECK("k_slab_rows(count)");
+    FSR_TRY(scan_exclusive(ctx, SCAN_U32, slab, slab, nslabs, nullptr, &counters->aux[10], s));
+    hipLaunchKernelGGL(k_cells_from_slabs, dim3((uint32_t)((nvc + 255) / 256)), dim3(256), 0, s, m,
+                       vcx, vcy, (uint32_t)nvc, nslabs, slab, &counters->aux[10], cnt, cell_off);
+    FSR_LAUNCH_CHECK("k_cells_from_slabs");
     FSR_TRY(scan_exclusive(ctx, SCAN_CHUNKS64, cnt, chunk_off, nvc, nullptr,
                            &counters->num_chunks, s));
-    hipLaunchKernelGGL(k_cell_rows<true>, dim3(cell_blocks), dim3(256), 0, s, bits, sc, wx, m,
-                       vcx, vcy, (uint32_t)nvc, cnt, cell_off, cell_band);
-    FSR_LAUNCH_CHECK("k_cell_rows(fill)");
+    hipLaunchKernelGGL(k_slab_rows<true>, dim3(slab_blocks), dim3(256), 0, s, bits, sc, wx, m,
+                       vcx, vcy, slab, cell_band);
+    FSR_LAUNCH_CHECK("k_slab_rows(fill)");
     hipLaunchKernelGGL(k_chunk_cells, dim3((uint32_t)((nvc + 255) / 256)), dim3(256), 0, s, cnt,
                        chunk_off, (uint32_t)nvc, chunk_cell);
     FSR_LAUNCH_CHECK("k_chunk_cells");

@@ -244,23 +244,26 @@ fsr_status frame_impl(const char* who, fsr_context* ctx, fsr_method method,
 
     Counters* counters;
     FSR_TRY(arena_typed(ctx, A_COUNTERS, 1, &counters));
-    FSR_TRY(fill_u32(reinterpret_cast<uint32_t*>(counters), 0u, sizeof(Counters) / 4, s));
-    FSR_TRY(fill_u32(reinterpret_cast<uint32_t*>(stats), 0u, sizeof(fsr_frame_stats) / 4, s));
-
-    // 1. spatial hashing
-    prof_mark(ctx, FSR_STAGE_HASH, s);
-    HashOut ho;
-    FSR_TRY(spatial_hash(ctx, pos, dens, n, spg, stats, &ho, s));
-    prof_mark(ctx, FSR_STAGE_SURFACE, s);
-
-    // 2. surface particles and band seeds
     uint32_t *flags, *sbits, *tmp, *bits, *exc;
     FSR_TRY(arena_typed(ctx, A_SURF_FLAGS, n, &flags));
     FSR_TRY(arena_typed(ctx, A_SBITS, W, &sbits));
     FSR_TRY(arena_typed(ctx, A_BAND_TMP, W, &tmp));
     FSR_TRY(arena_typed(ctx, A_BAND_BITS, W, &bits));
     FSR_TRY(arena_typed(ctx, A_EXCEPTIONS, n, &exc));
-    FSR_TRY(fill_u32(sbits, 0u, W, s));
+    // counters, stats and the band seed mask start at zero; cleared in the
+    // launch that initialises the hash grid
+    FillList fills;
+    fills.add(counters, 0u, sizeof(Counters) / 4);
+    fills.add(stats, 0u, sizeof(fsr_frame_stats) / 4);
+    fills.add(sbits, 0u, W);
+
+    // 1. spatial hashing
+    prof_mark(ctx, FSR_STAGE_HASH, s);
+    HashOut ho;
+    FSR_TRY(spatial_hash(ctx, pos, dens, n, spg, stats, &ho, s, &fills));
+    prof_mark(ctx, FSR_STAGE_SURFACE, s);
+
+    // 2. surface particles and band seeds
     SurfaceArgs sa;
     sa.sorted = ho.sorted;
     sa.keys = ho.keys;

@@ -115,6 +115,12 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
+__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v |= __shfl_xor(v, d, 64);
+    return v;
+}
+
 // XCD-aware dynamic work fetch for persistent kernels.  The work list [0, total)
 // is cut into 8 contiguous segments; workgroups are dispatched round-robin over
 // the 8 XCDs, so blockIdx % 8 names the XCD, and a wave takes items from its
@@ -184,7 +190,7 @@ enum ArenaSlot {
     A_CHUNK_CELL, A_NVERTS, A_VALID, A_VALID_OFF, A_VERT_OFF, A_VALID_IDX,
     A_COUNTERS, A_SCAN_PARTIALS, A_STATS, A_INVOLVE, A_INVOLVE_OFF,
     A_INVOLVED_IDX, A_MEAN, A_SMOOTH, A_GMAT, A_SIGMA, A_AUX0, A_AUX1, A_REC,
-    A_ADJ_VERT, A_ADJ_COEF, A_ADJ_AUX, A_ADJ_TOUCH, A_EDGE_MASK, A_EDGE_OFF, A_SURF_CHUNKS,
+    A_ADJ_VERT, A_ADJ_COEF, A_ADJ_AUX, A_ADJ_TOUCH, A_EDGE_MASK, A_EDGE_OFF, A_SURF_CHUNKS, A_SLAB,
     A_NUM_SLOTS
 };
 
@@ -298,6 +304,22 @@ fsr_status scan_exclusive(fsr_context* ctx, ScanInput f, const uint32_t* in,
                           uint32_t* out, uint64_t n_max, const uint32_t* d_count,
                           uint32_t* d_total, hipStream_t s);
 fsr_status fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
+
+// Several independent u32 fills in one launch.
+constexpr int kMaxFillJobs = 8;
+struct FillList {
+    uint32_t* p[kMaxFillJobs];
+    uint32_t value[kMaxFillJobs];
+    uint64_t n[kMaxFillJobs];
+    int count = 0;
+    void add(void* ptr, uint32_t v, uint64_t words) {
+        p[count] = static_cast<uint32_t*>(ptr);
+        value[count] = v;
+        n[count] = words;
+        ++count;
+    }
+};
+fsr_status fill_multi(const FillList& l, hipStream_t s);
 // Zeroes the per-XCD fetch counters before a persistent kernel that uses them.
 inline fsr_status reset_xcd(Counters* c, hipStream_t s) {
     return fill_u32(c->xcd_next, 0u, sizeof(c->xcd_next) / 4, s);
@@ -311,9 +333,11 @@ struct HashOut {
     fsr_index_range* ranges;
     float* occupancy;
 };
+// `pre` (may be null): fills of the caller's buffers, issued in the same launch
+// as the hash grid's own initial fills.
 fsr_status spatial_hash(fsr_context* ctx, float* pos, float* dens, uint32_t n,
                         const fsr_grid_info& sp, fsr_frame_stats* st,
-                        HashOut* out, hipStream_t s);
+                        HashOut* out, hipStream_t s, const FillList* pre = nullptr);
 
 // surface.hip ---------------------------------------------------------------
 struct SurfaceArgs {

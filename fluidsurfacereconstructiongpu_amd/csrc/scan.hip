@@ -9,6 +9,8 @@
 // contiguous segment, one block scans the G partial sums, each block rescans
 // its segment in 1024-element tiles (uint4 loads, wave scans, one LDS exchange
 // per tile).  HBM traffic: read n, read n, write n words.
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace fsr {
@@ -149,6 +151,28 @@ __global__ void k_fill_f32(float* p, float v, uint64_t n) {
         p[i] = v;
 }
 
+struct FillArgs {
+    uint32_t* p[kMaxFillJobs];
+    uint64_t n[kMaxFillJobs];
+    uint32_t value[kMaxFillJobs];
+    uint32_t blk_end[kMaxFillJobs];  // cumulative block counts
+    int count;
+};
+
+// Blocks [blk_end[j-1], blk_end[j]) fill job j (block-uniform job lookup).
+__global__ __launch_bounds__(256) void k_fill_multi(FillArgs a) {
+    int j = 0;
+    while (j + 1 < a.count && blockIdx.x >= a.blk_end[j]) ++j;
+    const uint32_t b0 = j ? a.blk_end[j - 1] : 0u;
+    const uint32_t nb = a.blk_end[j] - b0;
+    uint32_t* p = a.p[j];
+    const uint32_t v = a.value[j];
+    const uint64_t n = a.n[j];
+    for (uint64_t i = (uint64_t)(blockIdx.x - b0) * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)nb * blockDim.x)
+        p[i] = v;
+}
+
 template <int F>
 fsr_status scan_impl(fsr_context* ctx, const uint32_t* in, uint32_t* out, uint64_t n_max,
                      const uint32_t* d_count, uint32_t* d_total, hipStream_t s) {
@@ -192,6 +216,25 @@ fsr_status fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s) {
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)blocks), dim3(256), 0, s, p, value, n);
     FSR_LAUNCH_CHECK("k_fill_u32");
+    return FSR_OK;
+}
+
+fsr_status fill_multi(const FillList& l, hipStream_t s) {
+    FillArgs a;
+    a.count = 0;
+    uint32_t blocks = 0;
+    for (int j = 0; j < l.count; ++j) {
+        if (l.n[j] == 0) continue;
+        const int k = a.count++;
+        a.p[k] = l.p[j];
+        a.n[k] = l.n[j];
+        a.value[k] = l.value[j];
+        blocks += (uint32_t)std::min<uint64_t>((l.n[j] + 255) / 256, 2048);
+        a.blk_end[k] = blocks;
+    }
+    if (!a.count) return FSR_OK;
+    hipLaunchKernelGGL(k_fill_multi, dim3(blocks), dim3(256), 0, s, a);
+    FSR_LAUNCH_CHECK("k_fill_multi");
     return FSR_OK;
 }
 

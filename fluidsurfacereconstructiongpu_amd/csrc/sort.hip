@@ -161,7 +161,7 @@ __global__ __launch_bounds__(256) void k_gather_ranges(
 
 fsr_status spatial_hash(fsr_context* ctx, float* pos, float* dens, uint32_t n,
                         const fsr_grid_info& spg, fsr_frame_stats* st, HashOut* out,
-                        hipStream_t s) {
+                        hipStream_t s, const FillList* pre) {
     const GridDev sp = to_dev(spg);
     const uint64_t C64 = grid_count(spg);
     if (C64 == 0 || C64 >= 0xFFFFFFFFull) {
@@ -185,8 +185,15 @@ fsr_status spatial_hash(fsr_context* ctx, float* pos, float* dens, uint32_t n,
     const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
     FSR_TRY(arena_typed(ctx, A_RADIX_HIST, 256ull * (tiles ? tiles : 1), &hist));
 
-    FSR_TRY(fill_u32(reinterpret_cast<uint32_t*>(ranges), 0xFFFFFFFFu, 2ull * C, s));
-    FSR_TRY(fill_u32(reinterpret_cast<uint32_t*>(occ), 0u, C, s));
+    FillList fills;
+    if (pre) fills = *pre;
+    if (fills.count + 2 > kMaxFillJobs) {
+        set_error("spatial_hash: too many fills");
+        return FSR_ERR_INVALID_ARG;
+    }
+    fills.add(ranges, 0xFFFFFFFFu, 2ull * C);
+    fills.add(occ, 0u, C);
+    FSR_TRY(fill_multi(fills, s));
     out->sorted = sorted;
     out->ranges = ranges;
     out->occupancy = occ;

@@ -17,6 +17,8 @@
 // mask of "surface scalar cells" that band.hip dilates, or, when the dilation
 // is not exact for it, it is appended to an exception list that band.hip
 // treats with the reference's per-particle box loop.
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace fsr {
@@ -75,7 +77,8 @@ __global__ __launch_bounds__(256) void k_cell_surface(const float* __restrict__ 
 
 // Band seeding of one particle: its scalar cell is set in the surface-cell
 // mask, or the particle is appended to the exception list (exact path).
-// Wave-uniform call; returns whether the particle became an exception.
+// Called with all 64 lanes of the wave active (lanes without a particle pass
+// surf = false); returns whether the particle became an exception.
 __device__ __forceinline__ bool seed_particle(const SurfaceArgs& a, uint32_t i, float px,
                                               float py, float pz, bool surf) {
     const float qx = cell_coord(px, a.sc.minx, a.sc.cell);
@@ -101,30 +104,29 @@ __device__ __forceinline__ bool seed_particle(const SurfaceArgs& a, uint32_t i, 
     return exc;
 }
 
-// Band seeding for one particle and the frame's surface / exception counts
-// (called by every thread of the block; `surf` = its flag).
-__device__ __forceinline__ void seed_band(const SurfaceArgs& a, uint32_t i, float px, float py,
-                                          float pz, bool surf) {
-    const bool exc = a.sbits ? seed_particle(a, i, px, py, pz, surf) : false;
-    if (a.st)
-        block_count_add2(&a.st->num_surface_particles, surf ? 1u : 0u, &a.st->band_exceptions,
-                         exc ? 1u : 0u);
-}
-
+// Surface flags from the surface hash cells (ZB05 occupancy method), band
+// seeding, and the frame's surface / exception counts.  Grid-stride with a
+// block-uniform trip count over a bounded grid: the two counters take one
+// atomic per block, and thousands of atomics on one address serialise.
 __global__ __launch_bounds__(256) void k_flags_from_cells(SurfaceArgs a,
                                                           const uint8_t* __restrict__ surf_cell) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i < a.n;
-    bool surf = false;
-    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (live) {
-        const uint32_t key = a.keys[i];
-        const uint32_t C = a.sp.rx * a.sp.ry * a.sp.rz;
-        surf = key < C && surf_cell[key] != 0;
-        a.flags[i] = surf ? 1u : 0u;
-        p = a.sorted[i];
+    const uint32_t C = a.sp.rx * a.sp.ry * a.sp.rz;
+    uint32_t n_surf = 0, n_exc = 0;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < a.n; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        bool surf = false;
+        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i < a.n) {
+            const uint32_t key = a.keys[i];
+            surf = key < C && surf_cell[key] != 0;
+            a.flags[i] = surf ? 1u : 0u;
+            p = a.sorted[i];
+        }
+        const bool exc = a.sbits ? seed_particle(a, i, p.x, p.y, p.z, surf) : false;
+        n_surf += surf ? 1u : 0u;
+        n_exc += exc ? 1u : 0u;
     }
-    seed_band(a, i, p.x, p.y, p.z, surf);
+    if (a.st) block_count_add2(&a.st->num_surface_particles, n_surf, &a.st->band_exceptions, n_exc);
 }
 
 // Particles outside the hash grid (sorted last, in no cell): one thread each,
@@ -136,49 +138,55 @@ __global__ __launch_bounds__(256) void k_colorfield_tail(SurfaceArgs a, float ma
     const float inv_h = 1.0f / h;
     const float gcoef = -45.0f / (3.14159265358979f * (h * h) * (h * h));
     uint32_t n_surf = 0, n_exc = 0;
-    for (uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
-         i += gridDim.x * blockDim.x) {
-        const float4 p = a.sorted[i];
-        const int cx = trunc_cell(cell_coord(p.x, a.sp.minx, a.sp.cell), a.sp.rx);
-        const int cy = trunc_cell(cell_coord(p.y, a.sp.miny, a.sp.cell), a.sp.ry);
-        const int cz = trunc_cell(cell_coord(p.z, a.sp.minz, a.sp.cell), a.sp.rz);
-        const int x0 = clampi(cx - 1, 0, a.sp.rx - 1), x1 = clampi(cx + 1, 0, a.sp.rx - 1);
-        const int y0 = clampi(cy - 1, 0, a.sp.ry - 1), y1 = clampi(cy + 1, 0, a.sp.ry - 1);
-        const int z0 = clampi(cz - 1, 0, a.sp.rz - 1), z1 = clampi(cz + 1, 0, a.sp.rz - 1);
-        uint32_t count = 0;
-        float gx = 0.f, gy = 0.f, gz = 0.f;
-        for (int zz = z0; zz <= z1; ++zz)
-            for (int yy = y0; yy <= y1; ++yy) {
-                const uint32_t row = ((uint32_t)zz * a.sp.ry + yy) * a.sp.rx;
-                uint32_t s = 0xFFFFFFFFu, e = 0;
-                for (int xx = x0; xx <= x1; ++xx) {
-                    const fsr_index_range r = a.ranges[row + xx];
-                    if (r.start != 0xFFFFFFFFu) {
-                        if (s == 0xFFFFFFFFu) s = r.start;
-                        e = r.end;
+    // block-uniform trip count: seed_particle needs whole waves
+    for (uint32_t base = first + blockIdx.x * blockDim.x; base < a.n;
+         base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        bool surf = false;
+        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i < a.n) {
+            p = a.sorted[i];
+            const int cx = trunc_cell(cell_coord(p.x, a.sp.minx, a.sp.cell), a.sp.rx);
+            const int cy = trunc_cell(cell_coord(p.y, a.sp.miny, a.sp.cell), a.sp.ry);
+            const int cz = trunc_cell(cell_coord(p.z, a.sp.minz, a.sp.cell), a.sp.rz);
+            const int x0 = clampi(cx - 1, 0, a.sp.rx - 1), x1 = clampi(cx + 1, 0, a.sp.rx - 1);
+            const int y0 = clampi(cy - 1, 0, a.sp.ry - 1), y1 = clampi(cy + 1, 0, a.sp.ry - 1);
+            const int z0 = clampi(cz - 1, 0, a.sp.rz - 1), z1 = clampi(cz + 1, 0, a.sp.rz - 1);
+            uint32_t count = 0;
+            float gx = 0.f, gy = 0.f, gz = 0.f;
+            for (int zz = z0; zz <= z1; ++zz)
+                for (int yy = y0; yy <= y1; ++yy) {
+                    const uint32_t row = ((uint32_t)zz * a.sp.ry + yy) * a.sp.rx;
+                    uint32_t s = 0xFFFFFFFFu, e = 0;
+                    for (int xx = x0; xx <= x1; ++xx) {
+                        const fsr_index_range r = a.ranges[row + xx];
+                        if (r.start != 0xFFFFFFFFu) {
+                            if (s == 0xFFFFFFFFu) s = r.start;
+                            e = r.end;
+                        }
                     }
-                }
-                if (s == 0xFFFFFFFFu) continue;
-                for (uint32_t j = s; j < e; ++j) {
-                    const float4 q = a.sorted[j];
-                    const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
-                    const float d2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
-                    if (d2 > 0.0f) {
-                        ++count;
-                        const float dist = __builtin_sqrtf(d2);
-                        if (dist < h) {
-                            const float x = 1.0f - dist * inv_h;
-                            const float c = mass / q.w * gcoef * x * x * __builtin_amdgcn_rsqf(d2);
-                            gx = __builtin_fmaf(c, rx, gx);
-                            gy = __builtin_fmaf(c, ry, gy);
-                            gz = __builtin_fmaf(c, rz, gz);
+                    if (s == 0xFFFFFFFFu) continue;
+                    for (uint32_t j = s; j < e; ++j) {
+                        const float4 q = a.sorted[j];
+                        const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
+                        const float d2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
+                        if (d2 > 0.0f) {
+                            ++count;
+                            const float dist = __builtin_sqrtf(d2);
+                            if (dist < h) {
+                                const float x = 1.0f - dist * inv_h;
+                                const float c = mass / q.w * gcoef * x * x * __builtin_amdgcn_rsqf(d2);
+                                gx = __builtin_fmaf(c, rx, gx);
+                                gy = __builtin_fmaf(c, ry, gy);
+                                gz = __builtin_fmaf(c, rz, gz);
+                            }
                         }
                     }
                 }
-            }
-        const float g2 = __builtin_fmaf(gz, gz, __builtin_fmaf(gy, gy, gx * gx));
-        const bool surf = count < 25u || __builtin_sqrtf(g2) > 3.0f;
-        a.flags[i] = surf ? 1u : 0u;
+            const float g2 = __builtin_fmaf(gz, gz, __builtin_fmaf(gy, gy, gx * gx));
+            surf = count < 25u || __builtin_sqrtf(g2) > 3.0f;
+            a.flags[i] = surf ? 1u : 0u;
+        }
         const bool exc = a.sbits ? seed_particle(a, i, p.x, p.y, p.z, surf) : false;
         n_surf += surf ? 1u : 0u;
         n_exc += exc ? 1u : 0u;
@@ -346,23 +354,33 @@ __global__ __launch_bounds__(256) void k_colorfield(SurfaceArgs a, float mass, C
 
 // band seeding from given flags (stage entry point of the band)
 __global__ __launch_bounds__(256) void k_seed_flags(SurfaceArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i < a.n;
-    bool surf = false;
-    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (live) {
-        surf = a.flags[i] != 0;
-        p = a.sorted[i];
+    uint32_t n_surf = 0, n_exc = 0;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < a.n; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        bool surf = false;
+        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i < a.n) {
+            surf = a.flags[i] != 0;
+            p = a.sorted[i];
+        }
+        const bool exc = a.sbits ? seed_particle(a, i, p.x, p.y, p.z, surf) : false;
+        n_surf += surf ? 1u : 0u;
+        n_exc += exc ? 1u : 0u;
     }
-    seed_band(a, i, p.x, p.y, p.z, surf);
+    if (a.st) block_count_add2(&a.st->num_surface_particles, n_surf, &a.st->band_exceptions, n_exc);
 }
 
 }  // namespace
 
+// Grid of the per-particle kernels that end in block counter atomics.
+static uint32_t count_blocks(fsr_context* ctx, uint32_t n) {
+    const uint32_t cap = (uint32_t)persistent_blocks(ctx, 256, 4, TUNE_BAND);
+    return std::max(1u, std::min((n + 255u) / 256u, cap));
+}
+
 fsr_status surface_seed_from_flags(fsr_context* ctx, const SurfaceArgs& a, hipStream_t s) {
-    (void)ctx;
     if (a.n) {
-        hipLaunchKernelGGL(k_seed_flags, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_seed_flags, dim3(count_blocks(ctx, a.n)), dim3(256), 0, s, a);
         FSR_LAUNCH_CHECK("k_seed_flags");
     }
     return FSR_OK;
@@ -377,7 +395,7 @@ fsr_status surface_occupancy(fsr_context* ctx, const SurfaceArgs& a, uint8_t* su
                        a.ranges, a.sp, surf_cell, involve);
     FSR_LAUNCH_CHECK("k_cell_surface");
     if (a.n) {
-        hipLaunchKernelGGL(k_flags_from_cells, dim3((a.n + 255) / 256), dim3(256), 0, s, a,
+        hipLaunchKernelGGL(k_flags_from_cells, dim3(count_blocks(ctx, a.n)), dim3(256), 0, s, a,
                            (const uint8_t*)surf_cell);
         FSR_LAUNCH_CHECK("k_flags_from_cells");
     }
