@@ -137,7 +137,8 @@ __global__ __launch_bounds__(256) void k_band_exceptions(BandArgs a) {
 __global__ __launch_bounds__(256) void k_band_emit(const uint32_t* __restrict__ bits,
                                                    const uint32_t* __restrict__ word_off,
                                                    uint64_t nwords, uint32_t wx, uint32_t rx,
-                                                   uint32_t* __restrict__ band_index) {
+                                                   uint32_t* __restrict__ band_index,
+                                                   uint64_t cap) {
     constexpr uint32_t kStage = 4096;
     __shared__ uint32_t stage[kStage];
     __shared__ uint32_t range[2];
@@ -156,15 +157,18 @@ __global__ __launch_bounds__(256) void k_band_emit(const uint32_t* __restrict__ 
     if (total <= kStage) {
         uint32_t k = o - lo;
         while (b) {
-            stage[k++] = vbase + __builtin_ctz(b);
+            if (k < kStage) stage[k] = vbase + __builtin_ctz(b);
+            ++k;
             b &= b - 1u;
         }
         __syncthreads();
-        for (uint32_t t = threadIdx.x; t < total; t += blockDim.x) band_index[lo + t] = stage[t];
+        for (uint32_t t = threadIdx.x; t < total; t += blockDim.x)
+            if ((uint64_t)lo + t < cap) band_index[lo + t] = stage[t];
     } else {
         uint32_t k = o;
         while (b) {
-            band_index[k++] = vbase + __builtin_ctz(b);
+            if (k < cap) band_index[k] = vbase + __builtin_ctz(b);
+            ++k;
             b &= b - 1u;
         }
     }
@@ -264,7 +268,7 @@ template <bool kFill>
 __global__ __launch_bounds__(256) void k_slab_rows(const uint32_t* __restrict__ bits, GridDev sc,
                                                    uint32_t wx, AxisMaps m, uint32_t vcx,
                                                    uint32_t vcy, uint32_t* __restrict__ slab,
-                                                   uint32_t* __restrict__ cell_band) {
+                                                   uint32_t* __restrict__ cell_band, uint64_t cap) {
     const uint64_t total = (uint64_t)vcx * vcy * sc.rz;
     for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
          t += (uint64_t)gridDim.x * blockDim.x) {
@@ -293,7 +297,8 @@ __global__ __launch_bounds__(256) void k_slab_rows(const uint32_t* __restrict__ 
                     } else {
                         const uint32_t base = (uint32_t)(row * sc.rx) + w * 32u;
                         while (word) {
-                            cell_band[o++] = base + __builtin_ctz(word);
+                            if (o < cap) cell_band[o] = base + __builtin_ctz(word);
+                            ++o;
                             word &= word - 1u;
                         }
                     }
@@ -325,12 +330,14 @@ __global__ __launch_bounds__(256) void k_cells_from_slabs(AxisMaps m, uint32_t v
 
 __global__ __launch_bounds__(256) void k_chunk_cells(const uint32_t* __restrict__ cnt,
                                                      const uint32_t* __restrict__ chunk_off,
-                                                     uint32_t nvc, uint32_t* __restrict__ chunk_cell) {
+                                                     uint32_t nvc, uint32_t* __restrict__ chunk_cell,
+                                                     uint64_t cap) {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nvc) return;
     const uint32_t k = (cnt[v] + 63u) >> 6;
     const uint32_t o = chunk_off[v];
-    for (uint32_t j = 0; j < k; ++j) chunk_cell[o + j] = v;
+    for (uint32_t j = 0; j < k; ++j)
+        if ((uint64_t)o + j < cap) chunk_cell[o + j] = v;
 }
 
 }  // namespace
@@ -383,7 +390,7 @@ fsr_status band_compact(fsr_context* ctx, const uint32_t* bits, const GridDev& s
     FSR_TRY(scan_exclusive(ctx, SCAN_POPC, bits, word_off, nwords, nullptr,
                            &st->num_band_vertices, s));
     hipLaunchKernelGGL(k_band_emit, dim3((uint32_t)((nwords + 255) / 256)), dim3(256), 0, s, bits,
-                       word_off, nwords, wx, sc.rx, band_index);
+                       word_off, nwords, wx, sc.rx, band_index, V);
     FSR_LAUNCH_CHECK("k_band_emit");
     out->band_index = band_index;
 
@@ -427,7 +434,7 @@ fsr_status band_compact(fsr_context* ctx, const uint32_t* bits, const GridDev& s
     FSR_TRY(arena_typed(ctx, A_SLAB, nslabs, &slab));
     const uint32_t slab_blocks = (uint32_t)std::min<uint64_t>((nslabs + 255) / 256, 16384);
     hipLaunchKernelGGL(k_slab_rows<false>, dim3(slab_blocks), dim3(256), 0, s, bits, sc, wx, m,
-                       vcx, vcy, slab, cell_band);
+                       vcx, vcy, slab, cell_band, V);
     FSR_LAUNCH_CHECK("k_slab_rows(count)");
     FSR_TRY(scan_exclusive(ctx, SCAN_U32, slab, slab, nslabs, nullptr, &counters->aux[10], s));
     hipLaunchKernelGGL(k_cells_from_slabs, dim3((uint32_t)((nvc + 255) / 256)), dim3(256), 0, s, m,
@@ -436,10 +443,10 @@ fsr_status band_compact(fsr_context* ctx, const uint32_t* bits, const GridDev& s
     FSR_TRY(scan_exclusive(ctx, SCAN_CHUNKS64, cnt, chunk_off, nvc, nullptr,
                            &counters->num_chunks, s));
     hipLaunchKernelGGL(k_slab_rows<true>, dim3(slab_blocks), dim3(256), 0, s, bits, sc, wx, m,
-                       vcx, vcy, slab, cell_band);
+                       vcx, vcy, slab, cell_band, V);
     FSR_LAUNCH_CHECK("k_slab_rows(fill)");
     hipLaunchKernelGGL(k_chunk_cells, dim3((uint32_t)((nvc + 255) / 256)), dim3(256), 0, s, cnt,
-                       chunk_off, (uint32_t)nvc, chunk_cell);
+                       chunk_off, (uint32_t)nvc, chunk_cell, max_chunks);
     FSR_LAUNCH_CHECK("k_chunk_cells");
     out->cell_off = cell_off;
     out->cell_cnt = cnt;

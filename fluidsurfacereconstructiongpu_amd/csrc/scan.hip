@@ -5,10 +5,14 @@
 // stays on the device, so later stages size themselves from it without a host
 // round trip.
 //
-// Three kernels on a fixed grid of G blocks (G <= 1024): each block reduces a
-// contiguous segment, one block scans the G partial sums, each block rescans
-// its segment in 1024-element tiles (uint4 loads, wave scans, one LDS exchange
-// per tile).  HBM traffic: read n, read n, write n words.
+// Two launches, no atomics and no waiting: each of G <= 1024 workgroups
+// reduces a contiguous segment of whole 4096-element tiles; then each
+// workgroup sums the partials of the segments before its own (at most 1023
+// words, from L2) and rescans its segment from that offset in tiles (uint4
+// loads, wave scans, one LDS exchange per tile); the last workgroup writes the
+// total.  HBM traffic: read n twice, write n words.  (A single-pass decoupled
+// look-back form measured no faster here: its per-tile ticket is a
+// same-address atomic, and those serialise at roughly 30 ns each.)
 #include <algorithm>
 
 #include "common.hpp"
@@ -18,7 +22,8 @@ namespace fsr {
 namespace {
 
 constexpr int kScanThreads = 256;
-constexpr int kScanTile = kScanThreads * 4;
+constexpr int kScanItems = 16;
+constexpr uint32_t kScanTile = kScanThreads * kScanItems;
 constexpr uint32_t kMaxScanBlocks = 1024;
 
 template <int F>
@@ -33,11 +38,6 @@ __device__ __forceinline__ uint64_t live_length(uint64_t n_max, const uint32_t* 
     if (!d_count) return n_max;
     uint64_t c = *d_count;
     return c < n_max ? c : n_max;
-}
-
-__device__ __forceinline__ uint64_t segment_len(uint64_t n, uint32_t g) {
-    uint64_t seg = (n + g - 1) / g;
-    return (seg + kScanTile - 1) / kScanTile * kScanTile;
 }
 
 // Block-wide exclusive scan of one value per thread; returns the block total.
@@ -60,77 +60,103 @@ __device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t* lds_wa
     return total;
 }
 
+__device__ __forceinline__ uint64_t segment_len(uint64_t n, uint32_t g) {
+    const uint64_t seg = (n + g - 1) / g;
+    return (seg + kScanTile - 1) / kScanTile * kScanTile;
+}
+
+// Two-pass form, pass 1: partials[b] = sum of segment b.
 template <int F>
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(
     const uint32_t* __restrict__ in, uint64_t n_max, const uint32_t* d_count,
-    uint32_t* __restrict__ partials) {
+    uint32_t* __restrict__ partials, int vec_ok) {
     __shared__ uint32_t lds[kScanThreads / 64];
     const uint64_t n = live_length(n_max, d_count);
     const uint64_t seg = segment_len(n, gridDim.x);
     const uint64_t lo = (uint64_t)blockIdx.x * seg;
     const uint64_t hi = lo + seg < n ? lo + seg : n;
     uint32_t sum = 0;
-    for (uint64_t i = lo + threadIdx.x; i < hi; i += kScanThreads) sum += xform<F>(in[i]);
+    for (uint64_t base = lo; base < hi; base += kScanTile) {
+        const uint64_t i0 = base + (uint64_t)kScanItems * threadIdx.x;
+        if (base + kScanTile <= hi && vec_ok) {
+#pragma unroll
+            for (int k = 0; k < kScanItems; k += 4) {
+                const uint4 q = *reinterpret_cast<const uint4*>(in + i0 + k);
+                sum += xform<F>(q.x) + xform<F>(q.y) + xform<F>(q.z) + xform<F>(q.w);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < kScanItems; ++k)
+                if (i0 + k < hi) sum += xform<F>(in[i0 + k]);
+        }
+    }
     uint32_t excl;
-    uint32_t total = block_exclusive(sum, lds, &excl);
+    const uint32_t total = block_exclusive(sum, lds, &excl);
     if (threadIdx.x == 0) partials[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(1024) void k_scan_partials(uint32_t* partials, uint32_t g,
-                                                        uint32_t* d_total) {
-    __shared__ uint32_t lds[16];
-    const uint32_t t = threadIdx.x;
-    const uint32_t lane = __lane_id();
-    const uint32_t wave = t >> 6;
-    uint32_t v = t < g ? partials[t] : 0u;
-    uint32_t inc = wave_inclusive_sum(v);
-    if (lane == 63) lds[wave] = inc;
-    __syncthreads();
-    uint32_t off = 0, total = 0;
-    for (uint32_t w = 0; w < 16; ++w) {
-        uint32_t s = lds[w];
-        if (w < wave) off += s;
-        total += s;
-    }
-    if (t < g) partials[t] = off + inc - v;
-    if (t == 0 && d_total) *d_total = total;
-}
-
+// Two-pass form, pass 2: offset = sum of the partials before this segment, then
+// the segment rescanned in tiles; the last workgroup writes the total.
 template <int F>
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(
-    const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint64_t n_max,
-    const uint32_t* d_count, const uint32_t* __restrict__ partials) {
+    const uint32_t* in, uint32_t* out, uint64_t n_max, const uint32_t* d_count,
+    const uint32_t* __restrict__ partials, uint32_t* d_total, int vec_ok) {
     __shared__ uint32_t lds[kScanThreads / 64];
+    const bool last = blockIdx.x == gridDim.x - 1;
+    if (!out && !last) return;
     const uint64_t n = live_length(n_max, d_count);
     const uint64_t seg = segment_len(n, gridDim.x);
     const uint64_t lo = (uint64_t)blockIdx.x * seg;
     const uint64_t hi = lo + seg < n ? lo + seg : n;
-    uint32_t running = partials[blockIdx.x];
+    uint32_t s = 0;
+    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kScanThreads) s += partials[i];
+    uint32_t excl;
+    uint32_t running = block_exclusive(s, lds, &excl);
+    if (!out) {
+        if (threadIdx.x == 0 && d_total) *d_total = running + partials[blockIdx.x];
+        return;
+    }
     for (uint64_t base = lo; base < hi; base += kScanTile) {
-        const uint64_t i0 = base + 4ull * threadIdx.x;
-        uint32_t v[4];
-        if (base + kScanTile <= hi) {
-            uint4 q = *reinterpret_cast<const uint4*>(in + i0);
-            v[0] = xform<F>(q.x); v[1] = xform<F>(q.y); v[2] = xform<F>(q.z); v[3] = xform<F>(q.w);
+        const uint64_t i0 = base + (uint64_t)kScanItems * threadIdx.x;
+        const bool full = base + kScanTile <= hi;
+        uint32_t v[kScanItems];
+        if (full && vec_ok) {
+#pragma unroll
+            for (int k = 0; k < kScanItems; k += 4) {
+                const uint4 q = *reinterpret_cast<const uint4*>(in + i0 + k);
+                v[k] = xform<F>(q.x);
+                v[k + 1] = xform<F>(q.y);
+                v[k + 2] = xform<F>(q.z);
+                v[k + 3] = xform<F>(q.w);
+            }
         } else {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) v[k] = (i0 + k < hi) ? xform<F>(in[i0 + k]) : 0u;
+            for (int k = 0; k < kScanItems; ++k) v[k] = (i0 + k < hi) ? xform<F>(in[i0 + k]) : 0u;
         }
-        uint32_t tsum = v[0] + v[1] + v[2] + v[3];
-        uint32_t excl;
-        uint32_t total = block_exclusive(tsum, lds, &excl);
-        uint32_t o0 = running + excl;
-        uint32_t r[4];
-        r[0] = o0; r[1] = o0 + v[0]; r[2] = r[1] + v[1]; r[3] = r[2] + v[2];
-        if (base + kScanTile <= hi) {
-            *reinterpret_cast<uint4*>(out + i0) = make_uint4(r[0], r[1], r[2], r[3]);
+        uint32_t tsum = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) tsum += v[k];
+        const uint32_t total = block_exclusive(tsum, lds, &excl);
+        uint32_t r = running + excl;
+        uint32_t o[kScanItems];
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            o[k] = r;
+            r += v[k];
+        }
+        if (full && vec_ok) {
+#pragma unroll
+            for (int k = 0; k < kScanItems; k += 4)
+                *reinterpret_cast<uint4*>(out + i0 + k) =
+                    make_uint4(o[k], o[k + 1], o[k + 2], o[k + 3]);
         } else {
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if (i0 + k < hi) out[i0 + k] = r[k];
+            for (int k = 0; k < kScanItems; ++k)
+                if (i0 + k < hi) out[i0 + k] = o[k];
         }
         running += total;
     }
+    if (last && threadIdx.x == 0 && d_total) *d_total = running;
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
@@ -176,20 +202,17 @@ __global__ __launch_bounds__(256) void k_fill_multi(FillArgs a) {
 template <int F>
 fsr_status scan_impl(fsr_context* ctx, const uint32_t* in, uint32_t* out, uint64_t n_max,
                      const uint32_t* d_count, uint32_t* d_total, hipStream_t s) {
-    uint64_t want = (n_max + 2047) / 2048;
-    uint32_t g = (uint32_t)(want < 1 ? 1 : (want > kMaxScanBlocks ? kMaxScanBlocks : want));
+    const uint64_t want = (n_max + kScanTile - 1) / kScanTile;
+    const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, kMaxScanBlocks));
     uint32_t* partials = nullptr;
     FSR_TRY(arena_typed(ctx, A_SCAN_PARTIALS, kMaxScanBlocks, &partials));
+    const int vec_ok = ((((uintptr_t)in) | ((uintptr_t)out)) & 15u) == 0 ? 1 : 0;
     hipLaunchKernelGGL(k_scan_reduce<F>, dim3(g), dim3(kScanThreads), 0, s, in, n_max, d_count,
-                       partials);
+                       partials, vec_ok);
     FSR_LAUNCH_CHECK("k_scan_reduce");
-    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, partials, g, d_total);
-    FSR_LAUNCH_CHECK("k_scan_partials");
-    if (out) {
-        hipLaunchKernelGGL(k_scan_down<F>, dim3(g), dim3(kScanThreads), 0, s, in, out, n_max,
-                           d_count, partials);
-        FSR_LAUNCH_CHECK("k_scan_down");
-    }
+    hipLaunchKernelGGL(k_scan_down<F>, dim3(g), dim3(kScanThreads), 0, s, in, out, n_max, d_count,
+                       partials, d_total, vec_ok);
+    FSR_LAUNCH_CHECK("k_scan_down");
     return FSR_OK;
 }
 
