@@ -273,6 +273,26 @@ def stats_dict_of(sf):
     return stats_dict(sf.rec._stats)
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_field_zb05_dambreak1M_256_hbm_traffic.md")
+
+
+def measured_traffic(args):
+    """HBM bytes per k_field_zb05 launch on the default workload, from the
+    FETCH_SIZE / WRITE_SIZE counter runs summarised in the profile above (each
+    counter scaled by what it reads for a device copy of known size); None for
+    any other workload."""
+    if args.method != "zb05" or args.particles != 1_000_000 or args.grid != 256:
+        return None
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch = "):
+                    return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -377,7 +397,8 @@ def main():
         t_roof = max(b_comp / HBM_PEAK, flops / FP32_PEAK)
         roofline = {"bound": "valu", "kernel": "k_field_zb05", "achieved": achieved_tflops,
                     "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
-                    "frac": achieved_tflops / (FP32_PEAK / 1e12), "traffic": None,
+                    "frac": achieved_tflops / (FP32_PEAK / 1e12),
+                    "traffic": measured_traffic(args),
                     "flops_per_launch": flops, "pairs_p27": p27,
                     "field_stage_roofline_frac": t_roof / ((field_ms + fill_ms) * 1e-3),
                     "field_stage_hbm_gbps": b_comp / ((field_ms + fill_ms) * 1e-3) / 1e9,
