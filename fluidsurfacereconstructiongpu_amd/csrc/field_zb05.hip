@@ -169,14 +169,24 @@ __global__ __launch_bounds__(kFieldThreads) void k_field_zb05(FieldArgs a, BandC
 
         Sums sum = {0.f, 0.f, 0.f, 0.f};
         uint32_t fill = 0;
+        // this lane's candidate of the next round is loaded one round ahead, so
+        // the load overlaps the culling and the accumulation of the current one
         int j = 0;  // row of this lane's candidate; g only grows, so the search resumes
+        float4 next = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (lane < total) {
+            while (j + 1 < nrows && row_pref[wave][j + 1] <= lane) ++j;
+            next = a.sorted[row_start[wave][j] + (lane - row_pref[wave][j])];
+        }
         for (uint32_t base = 0; base < total; base += 64u) {
             const uint32_t g = base + lane;
+            const float4 p = next;
+            if (g + 64u < total) {
+                while (j + 1 < nrows && row_pref[wave][j + 1] <= g + 64u) ++j;
+                next = a.sorted[row_start[wave][j] + (g + 64u - row_pref[wave][j])];
+            }
             bool keep = false;
             float4 st;
             if (g < total) {
-                while (j + 1 < nrows && row_pref[wave][j + 1] <= g) ++j;
-                const float4 p = a.sorted[row_start[wave][j] + (g - row_pref[wave][j])];
                 const float px = p.x - ox, py = p.y - oy, pz = p.z - oz;
                 const float dx = fmaxf(fmaxf(bx0 - px, px - bx1), 0.0f);
                 const float dy = fmaxf(fmaxf(by0 - py, py - by1), 0.0f);
