@@ -206,12 +206,17 @@ __global__ __launch_bounds__(kAnisoThreads) void k_smooth(
             }
         };
         uint32_t fill = 0;
+        // each lane's candidate is loaded one staging round ahead
+        float4 next = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (lane < total) next = sorted[candidate_index(lane, nrows, rstart[wave], rpref[wave])];
         for (uint32_t base = 0; base < total; base += 64u) {
             const uint32_t g = base + lane;
+            const float4 cand = next;
+            if (g + 64u < total)
+                next = sorted[candidate_index(g + 64u, nrows, rstart[wave], rpref[wave])];
             bool keep = false;
             float4 st;
             if (g < total) {
-                const float4 cand = sorted[candidate_index(g, nrows, rstart[wave], rpref[wave])];
                 const float qx = cand.x - ox, qy = cand.y - oy, qz = cand.z - oz;
                 keep = box_dist2(qx, qy, qz, b0, b1) < cull2;
                 st = make_float4(qx, qy, qz,
@@ -383,12 +388,17 @@ __global__ __launch_bounds__(kAnisoThreads) void k_covariance(CovArgs a, ChunkLi
             }
         };
         uint32_t fill = 0;
+        // each lane's candidate is loaded one staging round ahead
+        float4 next = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (lane < total) next = a.sorted[candidate_index(lane, nrows, rstart[wave], rpref[wave])];
         for (uint32_t base = 0; base < total; base += 64u) {
             const uint32_t g = base + lane;
+            const float4 cand = next;
+            if (g + 64u < total)
+                next = a.sorted[candidate_index(g + 64u, nrows, rstart[wave], rpref[wave])];
             bool keep = false;
             float4 st;
             if (g < total) {
-                const float4 cand = a.sorted[candidate_index(g, nrows, rstart[wave], rpref[wave])];
                 st = make_float4(cand.x - ox, cand.y - oy, cand.z - oz, 0.f);
                 keep = box_dist2(st.x, st.y, st.z, b0, b1) < cull2;
             }
@@ -571,15 +581,27 @@ __global__ __launch_bounds__(kAnisoThreads) void k_field_aniso(AnisoFieldArgs a)
         const float bz0 = wave_min(live ? vz : 3.0e38f), bz1 = wave_max(live ? vz : -3.0e38f);
         float sum = 0.f;
         uint32_t fill = 0;
+        // each lane's candidate (position, two record words) is loaded one
+        // staging round ahead
+        float4 np4 = make_float4(0.f, 0.f, 0.f, 0.f), nr0 = np4, nr1 = np4;
+        if (lane < total) {
+            const uint32_t j = candidate_index(lane, nrows, rstart[wave], rpref[wave]);
+            np4 = a.smooth[j];
+            nr0 = a.rec[2ull * j + 0];
+            nr1 = a.rec[2ull * j + 1];
+        }
         for (uint32_t base = 0; base < total; base += 64u) {
             const uint32_t g = base + lane;
+            const float4 p4 = np4, r0 = nr0, r1 = nr1;
+            if (g + 64u < total) {
+                const uint32_t j = candidate_index(g + 64u, nrows, rstart[wave], rpref[wave]);
+                np4 = a.smooth[j];
+                nr0 = a.rec[2ull * j + 0];
+                nr1 = a.rec[2ull * j + 1];
+            }
             bool keep = false;
             float4 s0, s1, s2;
             if (g < total) {
-                const uint32_t j = candidate_index(g, nrows, rstart[wave], rpref[wave]);
-                const float4 p4 = a.smooth[j];
-                const float4 r0 = a.rec[2ull * j + 0];
-                const float4 r1 = a.rec[2ull * j + 1];
                 const float px = p4.x - ox, py = p4.y - oy, pz = p4.z - oz;
                 const float dx = fmaxf(fmaxf(bx0 - px, px - bx1), 0.0f);
                 const float dy = fmaxf(fmaxf(by0 - py, py - by1), 0.0f);

@@ -302,13 +302,22 @@ __global__ __launch_bounds__(256) void k_colorfield(SurfaceArgs a, float mass, C
                 }
             }
         };
+        // each lane's candidate is loaded one staging round ahead
+        float4 next = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (lane < total) {
+            while (j + 1 < nrows && row_pref[wave][j + 1] <= lane) ++j;
+            next = a.sorted[row_start[wave][j] + (lane - row_pref[wave][j])];
+        }
         for (uint32_t base = 0; base < total; base += 64u) {
             const uint32_t g = base + lane;
+            const float4 cand = next;
+            if (g + 64u < total) {
+                while (j + 1 < nrows && row_pref[wave][j + 1] <= g + 64u) ++j;
+                next = a.sorted[row_start[wave][j] + (g + 64u - row_pref[wave][j])];
+            }
             bool keep = false;
             float4 st;
             if (g < total) {
-                while (j + 1 < nrows && row_pref[wave][j + 1] <= g) ++j;
-                const float4 cand = a.sorted[row_start[wave][j] + (g - row_pref[wave][j])];
                 const float dx = fmaxf(fmaxf(bx0 - cand.x, cand.x - bx1), 0.0f);
                 const float dy = fmaxf(fmaxf(by0 - cand.y, cand.y - by1), 0.0f);
                 const float dz = fmaxf(fmaxf(bz0 - cand.z, cand.z - bz1), 0.0f);
