@@ -222,6 +222,10 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.fsr_obj_write_indexed.restype = st
     lib.fsr_sim_param_for.argtypes = [c_int, POINTER(XyzHeader), POINTER(SimParam)]
     lib.fsr_sim_param_for.restype = st
+    lib.fsr_vis_write.argtypes = [c_void_p, c_char_p, c_int, POINTER(GridInfo), POINTER(GridInfo),
+                                  POINTER(SimParam), c_void_p, c_uint32, POINTER(FrameStats),
+                                  c_char_p]
+    lib.fsr_vis_write.restype = st
     lib.fsr_set_adjoint.argtypes = [c_void_p, c_int]
     lib.fsr_set_adjoint.restype = st
     lib.fsr_mc_vertex_backward.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]

@@ -11,6 +11,11 @@
 //   .obj  (GPU/ReconstructorGPUOurs.cpp:20-100): all "v x y z" lines, then all
 //         "vn" lines, then "f i//i j//j k//k" with 1-based indices, numbers in
 //         the default iostream format (6 significant digits).
+//   .vis  (saveMiddleDataToVisFile, GPU/ReconstructorGPUOursZB05.cpp:104-231 and
+//         its Akinci12 / Yu13 versions): the frame's intermediates for the
+//         reference's Visualizer, same number format; index lists are written
+//         "i j k " on one line, followed by a line break only when the list is
+//         not empty (the occupied-cell list: when the hash grid is not empty).
 #include <cerrno>
 #include <cstdio>
 #include <cstdlib>
@@ -214,6 +219,121 @@ fsr_status fsr_obj_write_indexed(const char* path, const float* pos, const float
     const bool ok = std::fclose(f) == 0;
     if (!ok) {
         set_error(std::string("fsr_obj_write_indexed: write error on ") + path);
+        return FSR_ERR_INVALID_ARG;
+    }
+    return FSR_OK;
+}
+
+fsr_status fsr_vis_write(fsr_context* ctx, const char* path, fsr_method method,
+                         const fsr_grid_info* spg, const fsr_grid_info* scg,
+                         const fsr_sim_param* sim, const float* sorted_pos,
+                         uint32_t n, const fsr_frame_stats* st, const char* obj_name) {
+    if (!ctx || !path || !spg || !scg || !sim || !st || !obj_name || (n && !sorted_pos)) {
+        set_error("fsr_vis_write: null argument");
+        return FSR_ERR_INVALID_ARG;
+    }
+    if (method != FSR_METHOD_ZB05 && method != FSR_METHOD_AKINCI12 && method != FSR_METHOD_YU13) {
+        set_error("fsr_vis_write: unknown method");
+        return FSR_ERR_INVALID_ARG;
+    }
+    const bool yu13 = method == FSR_METHOD_YU13;
+    const uint64_t C = (uint64_t)spg->resolution.x * spg->resolution.y * spg->resolution.z;
+    FSR_HIP(hipSetDevice(ctx->device));
+    FSR_HIP(hipDeviceSynchronize());
+    auto fetch = [&](fsr_buffer which, uint64_t count, size_t elem, std::vector<uint8_t>& out)
+        -> fsr_status {
+        out.resize(count * elem);
+        if (!count) return FSR_OK;
+        void* p = nullptr;
+        uint64_t have = 0;
+        FSR_TRY(fsr_frame_buffer(ctx, which, &p, &have));
+        if (!p || have < count) {
+            set_error("fsr_vis_write: the context holds no frame of this size");
+            return FSR_ERR_INVALID_ARG;
+        }
+        FSR_HIP(hipMemcpy(out.data(), p, count * elem, hipMemcpyDeviceToHost));
+        return FSR_OK;
+    };
+    std::vector<uint8_t> occ, band, flags, valid, smooth, involved;
+    std::vector<float> pos(3ull * n);
+    if (n) FSR_HIP(hipMemcpy(pos.data(), sorted_pos, 12ull * n, hipMemcpyDeviceToHost));
+    FSR_TRY(fetch(FSR_BUF_OCCUPANCY, C, 4, occ));
+    FSR_TRY(fetch(FSR_BUF_BAND_INDEX, st->num_band_vertices, 4, band));
+    FSR_TRY(fetch(FSR_BUF_SURFACE_FLAGS, n, 4, flags));
+    FSR_TRY(fetch(FSR_BUF_VALID_CUBES, st->num_valid_cubes, 4, valid));
+    if (yu13) {
+        FSR_TRY(fetch(FSR_BUF_SMOOTH_POS, n, 16, smooth));
+        FSR_TRY(fetch(FSR_BUF_INVOLVED_INDEX, st->num_involved_particles, 4, involved));
+    }
+    std::FILE* f = std::fopen(path, "w");
+    if (!f) {
+        set_error(std::string("fsr_vis_write: cannot write ") + path);
+        return FSR_ERR_INVALID_ARG;
+    }
+    char a[32], b[32], c[32];
+    auto put3 = [&](float x, float y, float z) {
+        put_float(a, x);
+        put_float(b, y);
+        put_float(c, z);
+        std::fprintf(f, "%s %s %s\n", a, b, c);
+    };
+    auto put1 = [&](float x) {
+        put_float(a, x);
+        std::fprintf(f, "%s\n", a);
+    };
+    auto grid = [&](const fsr_grid_info* g) {
+        put3(g->min_pos.x, g->min_pos.y, g->min_pos.z);
+        put3(g->max_pos.x, g->max_pos.y, g->max_pos.z);
+        std::fprintf(f, "%u %u %u\n", g->resolution.x, g->resolution.y, g->resolution.z);
+        put1(g->cell_size);
+    };
+    auto u32s = [](const std::vector<uint8_t>& v) {
+        return reinterpret_cast<const uint32_t*>(v.data());
+    };
+    auto f32s = [](const std::vector<uint8_t>& v) {
+        return reinterpret_cast<const float*>(v.data());
+    };
+    // index list: the count, then "i j k " and a line break when not empty
+    auto list = [&](const uint32_t* idx, uint64_t count) {
+        std::fprintf(f, "%llu\n", (unsigned long long)count);
+        for (uint64_t i = 0; i < count; ++i) std::fprintf(f, "%u ", idx[i]);
+        if (count) std::fputc('\n', f);
+    };
+    grid(spg);
+    grid(scg);
+    put1(sim->particle_radius);
+    std::fprintf(f, "%u\n", n);
+    for (uint32_t i = 0; i < n; ++i) put3(pos[3ull * i], pos[3ull * i + 1], pos[3ull * i + 2]);
+    // occupied hash cells
+    uint64_t nocc = 0;
+    for (uint64_t i = 0; i < C; ++i) nocc += f32s(occ)[i] > 0.0f ? 1u : 0u;
+    std::fprintf(f, "%llu\n", (unsigned long long)nocc);
+    for (uint64_t i = 0; i < C; ++i)
+        if (f32s(occ)[i] > 0.0f) std::fprintf(f, "%llu ", (unsigned long long)i);
+    if (C) std::fputc('\n', f);
+    // smoothed particles (Yu13 only)
+    if (yu13) {
+        std::fprintf(f, "%u\n", n);
+        for (uint32_t i = 0; i < n; ++i)
+            put3(f32s(smooth)[4ull * i], f32s(smooth)[4ull * i + 1], f32s(smooth)[4ull * i + 2]);
+    } else {
+        std::fprintf(f, "0\n");
+    }
+    list(u32s(band), st->num_band_vertices);
+    // surface particles: the ids whose flag is 1
+    std::vector<uint32_t> surf;
+    for (uint32_t i = 0; i < n; ++i)
+        if (u32s(flags)[i] == 1u) surf.push_back(i);
+    list(surf.data(), surf.size());
+    if (yu13)
+        list(u32s(involved), st->num_involved_particles);
+    else
+        std::fprintf(f, "0\n");
+    list(u32s(valid), st->num_valid_cubes);
+    std::fprintf(f, "%s\n", obj_name);
+    put1(sim->smoothing_radius);
+    if (std::fclose(f) != 0) {
+        set_error(std::string("fsr_vis_write: write error on ") + path);
         return FSR_ERR_INVALID_ARG;
     }
     return FSR_OK;

@@ -4,14 +4,15 @@
 // every frame in [from, to) it reads <directory><pattern % frame>.xyz, derives
 // the grids and constants as the chosen reconstructor class does, times the
 // reconstruction from spatial hashing to the end of marching cubes, writes the
-// mesh as <...>.obj, and at the end appends the per-frame times to
+// mesh as <...>.obj, with --vis writes the frame's intermediates as <...>.vis
+// (saveMiddleDataToVisFile), and at the end appends the per-frame times to
 // "<algorithm type>.time" and, with --config, writes "<algorithm type>.config"
 // (GPU/ReconstructorGPU.cpp:315-393: mean triangle count, surface particle and
 // surface vertex ratios, cumulative stage times).
 //
 //   fsr_reconstruct <directory/> <pattern> <from> <to>
 //                   [--method zb05|akinci12|yu13] [--no-obj] [--indexed]
-//                   [--config] [--device N]
+//                   [--vis] [--config] [--device N]
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -74,7 +75,8 @@ const char* algorithm_type(fsr_method m) {
 void usage() {
     std::fprintf(stderr,
                  "usage: fsr_reconstruct <directory/> <pattern> <from> <to> "
-                 "[--method zb05|akinci12|yu13] [--no-obj] [--indexed] [--config] [--device N]\n");
+                 "[--method zb05|akinci12|yu13] [--no-obj] [--indexed] [--vis] [--config] "
+                 "[--device N]\n");
     std::exit(2);
 }
 
@@ -86,7 +88,7 @@ int main(int argc, char** argv) {
     const std::string pattern = argv[2];
     const long from = std::strtol(argv[3], nullptr, 10), to = std::strtol(argv[4], nullptr, 10);
     fsr_method method = FSR_METHOD_ZB05;
-    bool save_obj = true, indexed = false, save_cfg = false;
+    bool save_obj = true, indexed = false, save_cfg = false, save_vis = false;
     int device = 0;
     for (int i = 5; i < argc; ++i) {
         const std::string a = argv[i];
@@ -102,6 +104,8 @@ int main(int argc, char** argv) {
             indexed = true;
         } else if (a == "--config") {
             save_cfg = true;
+        } else if (a == "--vis") {
+            save_vis = true;
         } else if (a == "--device" && i + 1 < argc) {
             device = std::atoi(argv[++i]);
         } else {
@@ -176,6 +180,13 @@ int main(int argc, char** argv) {
         CHECK_FSR(fsr_stage_times(ctx, stage, 1, &nf));
         times.push_back(ms);
         std::printf("Reconstructing Frame%ld took %g milliseconds.\n", frame, ms);
+        if (save_vis) {
+            const std::string path = stem + ".vis";
+            std::printf("Writing to %s...\n", path.c_str());
+            CHECK_FSR(fsr_vis_write(ctx, path.c_str(), method, &sp, &sc, &sim, pp, n, &hs,
+                                    (std::string(base) + ".obj").c_str()));
+            std::printf("Finish writing %s.\n", path.c_str());
+        }
         const uint32_t T3 = hs.num_mesh_vertices;
         sum_tri += T3 / 3;
         sum_spr += n ? (double)hs.num_surface_particles / n : 0.0;

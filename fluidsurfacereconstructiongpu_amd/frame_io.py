@@ -1,7 +1,9 @@
-"""Python side of the reference's frame files (.xyz in, .obj out) through the
-host-only entry points of libfsr (fsr_xyz_info / fsr_xyz_read /
-fsr_obj_write_soup / fsr_obj_write_indexed / fsr_sim_param_for).  The formats
-are those of FSR/data_format.txt and GPU/ReconstructorGPUOurs.cpp:20-100."""
+"""Python side of the reference's frame files (.xyz in, .obj and .vis out)
+through the entry points of libfsr (fsr_xyz_info / fsr_xyz_read /
+fsr_obj_write_soup / fsr_obj_write_indexed / fsr_vis_write /
+fsr_sim_param_for).  The formats are those of FSR/data_format.txt,
+GPU/ReconstructorGPUOurs.cpp:20-100 and saveMiddleDataToVisFile
+(GPU/ReconstructorGPUOursZB05.cpp:104-231)."""
 from __future__ import annotations
 
 import ctypes
@@ -75,4 +77,56 @@ def sim_param_for(method: str, header: dict) -> _lib.SimParam:
     h.particle_mass = header["particle_mass"]
     out = _lib.SimParam()
     _lib.check(lib.fsr_sim_param_for(METHODS[method], ctypes.byref(h), ctypes.byref(out)))
+    return out
+
+
+def write_vis(rec, path: str, fr, stats: dict, obj_name: str) -> None:
+    """The last frame of `rec` (a Reconstructor) as the reference's .vis file."""
+    from .reconstructor import grid_info
+    lib = _lib.load()
+    st = _lib.FrameStats()
+    for name, _ in _lib.FrameStats._fields_:
+        if name in stats:
+            setattr(st, name, int(stats[name]))
+    spg, scg = grid_info(fr.sp), grid_info(fr.sc)
+    sim = rec.params(fr)
+    _lib.check(lib.fsr_vis_write(rec.ctx.handle, path.encode(), METHODS[rec.method],
+                                 ctypes.byref(spg), ctypes.byref(scg), ctypes.byref(sim),
+                                 ctypes.c_void_p(rec.pos_sorted.data_ptr()), int(fr.n),
+                                 ctypes.byref(st), obj_name.encode()))
+
+
+def read_vis(path: str) -> dict:
+    """Parses a .vis file (for checks): grids, radius, particles, occupied
+    cells, smoothed particles, band vertices, surface particles, involved
+    particles, valid cubes, mesh file name, smoothing radius."""
+    with open(path) as f:
+        lines = f.read().split("\n")
+    it = iter(lines)
+
+    def nums(line, t=float):
+        return [t(x) for x in line.split()]
+
+    def grid():
+        return dict(min=nums(next(it)), max=nums(next(it)), res=nums(next(it), int),
+                    cell=float(next(it)))
+
+    def idx_list():
+        k = int(next(it))
+        return np.array(nums(next(it), int), np.int64) if k else np.zeros(0, np.int64), k
+
+    def points():
+        k = int(next(it))
+        return np.array([nums(next(it)) for _ in range(k)], np.float64).reshape(k, 3)
+
+    out = dict(sp=grid(), sc=grid(), particle_radius=float(next(it)))
+    out["particles"] = points()
+    out["occupied"], _ = idx_list()
+    out["smoothed"] = points()
+    out["band"], _ = idx_list()
+    out["surface"], _ = idx_list()
+    out["involved"], _ = idx_list()
+    out["valid_cubes"], _ = idx_list()
+    out["obj"] = next(it)
+    out["smoothing_radius"] = float(next(it))
     return out

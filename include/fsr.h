@@ -213,8 +213,8 @@ typedef enum {
     FSR_BUF_CUBE_NVERTS = 7,    /* u32[B]   soup vertices emitted by the cube at band entry b */
     FSR_BUF_VALID_CUBES = 8,    /* u32[Q]   band entries of valid cubes, ascending */
     FSR_BUF_INVOLVE_COUNTS = 9, /* u32[C]   Yu13 involved-particle count per cell */
-    FSR_BUF_MEAN_POS = 10,      /* float3[N] Yu13 weighted mean positions */
-    FSR_BUF_SMOOTH_POS = 11,    /* float3[N] Yu13 smoothed positions */
+    FSR_BUF_MEAN_POS = 10,      /* float4[N] Yu13 weighted mean positions (x, y, z, unused) */
+    FSR_BUF_SMOOTH_POS = 11,    /* float4[N] Yu13 smoothed positions (x, y, z, unused) */
     FSR_BUF_G_MATRICES = 12,    /* fsr_matrix[I] Yu13 G of the involved particles */
     FSR_BUF_SINGULAR = 13,      /* float3[I] Yu13 covariance singular values (before clamping) */
     FSR_BUF_INVOLVED_INDEX = 14 /* u32[I]   Yu13 involved particle ids (sorted order) */
@@ -505,6 +505,22 @@ fsr_status fsr_obj_write_soup(const char* path, const float* pos, const float* n
 fsr_status fsr_obj_write_indexed(const char* path, const float* pos, const float* nrm,
                                  uint32_t num_vertices, const uint32_t* indices,
                                  uint32_t num_indices);
+
+/* Writes the intermediates of the last frame run on ctx as the reference's
+ * .vis file for its Visualizer (saveMiddleDataToVisFile:
+ * GPU/ReconstructorGPUOursZB05.cpp:104-231, ReconstructorGPUAkinci12.cpp:290-413,
+ * ReconstructorGPUOursYu13.cpp:285-431): the two grids, the particle radius,
+ * the sorted particles, the occupied hash cells, the smoothed particles (Yu13),
+ * the band vertices, the surface particles, the involved particles (Yu13), the
+ * valid cubes (band entries), the mesh file name and the smoothing radius.
+ * `sorted_pos` is the frame's (device) position buffer after the frame, and
+ * `stats` a host copy of its fsr_frame_stats.  Blocking: copies to the host
+ * with the device synchronised, as the reference does. */
+fsr_status fsr_vis_write(fsr_context* ctx, const char* path, fsr_method method,
+                         const fsr_grid_info* sp_grid, const fsr_grid_info* sc_grid,
+                         const fsr_sim_param* sim, const float* sorted_pos,
+                         uint32_t num_particles, const fsr_frame_stats* stats,
+                         const char* obj_name);
 
 /* The simulation parameters a reconstructor class derives from a .xyz header
  * and its own constants (readParticlesFromFile, GPU/ReconstructorGPU.cpp:226-245;

@@ -64,3 +64,13 @@ def test_adjoint_entry_points_reject_missing_state():
     assert lib.fsr_mc_vertex_backward(None, None, None, None) == 1
     assert lib.fsr_field_zb05_backward(None, None, None, None) == 1
     assert b"null context" in lib.fsr_last_error()
+
+
+def test_vis_write_rejects_missing_arguments():
+    lib = _lib.load()
+    g = _lib.GridInfo()
+    sim = _lib.SimParam()
+    st = _lib.FrameStats()
+    assert lib.fsr_vis_write(None, b"/tmp/x.vis", 0, ctypes.byref(g), ctypes.byref(g),
+                             ctypes.byref(sim), None, 0, ctypes.byref(st), b"x.obj") == 1
+    assert b"null argument" in lib.fsr_last_error()

@@ -1,7 +1,8 @@
 """The frame-sequence driver (fsr_reconstruct, the reference's
 ReconstructorGPU::reconstruct flow) on two .xyz frames: its .obj output equals
 the library's soup for the same frame to the 6 significant digits the .obj
-dialect keeps, and the .time / .config files have the reference's layout."""
+dialect keeps, its .vis files hold the same frame's intermediates, and the
+.time / .config files have the reference's layout."""
 import os
 import subprocess
 
@@ -48,7 +49,7 @@ def test_driver_frames(torch_cuda, tmp_path, method):
                            base.particle_radius, base.mass, base.pos, base.dens)
         frames.append(fr)
     r = subprocess.run([frame_io.DRIVER, str(tmp_path) + "/", "frame_%06d", "0", "2", "--method",
-                        method, "--config"], capture_output=True, text=True, timeout=120)
+                        method, "--config", "--vis"], capture_output=True, text=True, timeout=120)
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0
     tris = []
@@ -64,6 +65,14 @@ def test_driver_frames(torch_cuda, tmp_path, method):
         ok = ~np.isnan(sn_).any(axis=1)
         np.testing.assert_allclose(vn[ok], sn_[ok], rtol=1e-5, atol=1e-6)
         assert np.array_equal(f.ravel(), np.arange(1, v.shape[0] + 1))
+        vis = frame_io.read_vis(str(tmp_path / ("frame_%06d.vis" % k)))
+        st = res.stats
+        assert vis["particles"].shape == (fr.n, 3)
+        assert vis["band"].shape[0] == st["num_band_vertices"]
+        assert vis["surface"].shape[0] == st["num_surface_particles"]
+        assert vis["valid_cubes"].shape[0] == st["num_valid_cubes"]
+        assert vis["obj"] == "frame_%06d.obj" % k
+        assert vis["sc"]["res"] == list(fr.sc.res)
     t = open(str(tmp_path / (ALGO[method] + ".time"))).read().split()
     assert len(t) == 2 and all(float(x) > 0 for x in t)
     cfg = open(str(tmp_path / (ALGO[method] + ".config"))).read().split()
