@@ -10,6 +10,12 @@
 // (GPU/ReconstructorGPU.cpp:315-393: mean triangle count, surface particle and
 // surface vertex ratios, cumulative stage times).
 //
+// Frames are pipelined (the reference runs them strictly one after another):
+// while frame k is reconstructed, a host thread parses frame k+1's .xyz into a
+// pinned buffer, and frame k+1 is uploaded on a copy stream while frame k's
+// mesh is written.  Per-frame times cover the reconstruction only, as the
+// reference's timer does; the end-to-end rate is printed at the end.
+//
 //   fsr_reconstruct <directory/> <pattern> <from> <to>
 //                   [--method zb05|akinci12|yu13] [--no-obj] [--indexed]
 //                   [--vis] [--config] [--device N]
@@ -17,6 +23,7 @@
 
 #include <chrono>
 #include <cstdio>
+#include <future>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -62,6 +69,47 @@ struct DevBuf {
         if (p) (void)hipFree(p);
     }
 };
+
+// Host copy of one parsed frame, in pinned memory that only grows.
+struct HostFrame {
+    std::string stem, base;
+    fsr_xyz_header h{};
+    uint32_t n = 0;
+    float* pos = nullptr;
+    float* dens = nullptr;
+    uint32_t cap = 0;
+    std::string error;  // empty when the frame was read
+    ~HostFrame() {
+        if (pos) (void)hipHostFree(pos);
+        if (dens) (void)hipHostFree(dens);
+    }
+};
+
+// Parses <dir><pattern % frame>.xyz into f (runs on the loader thread).
+void load_frame(HostFrame& f, const std::string& dir, const std::string& pattern, long frame) {
+    char base[512];
+    std::snprintf(base, sizeof(base), pattern.c_str(), (unsigned)frame);
+    f.base = base;
+    f.stem = dir + base;
+    f.error.clear();
+    const std::string path = f.stem + ".xyz";
+    uint32_t n = 0;
+    fsr_status st = fsr_xyz_info(path.c_str(), &f.h, &n);
+    if (st == FSR_OK && n > f.cap) {
+        if (f.pos) (void)hipHostFree(f.pos);
+        if (f.dens) (void)hipHostFree(f.dens);
+        f.pos = f.dens = nullptr;
+        f.cap = 0;
+        if (hipHostMalloc(reinterpret_cast<void**>(&f.pos), 12ull * n) != hipSuccess ||
+            hipHostMalloc(reinterpret_cast<void**>(&f.dens), 4ull * n) != hipSuccess) {
+            f.error = "hipHostMalloc failed";
+            return;
+        }
+        f.cap = n;
+    }
+    if (st == FSR_OK) st = fsr_xyz_read(path.c_str(), &f.h, f.pos, f.dens, f.cap, &f.n);
+    if (st != FSR_OK) f.error = std::string(fsr_status_string(st)) + " (" + fsr_last_error() + ")";
+}
 
 const char* algorithm_type(fsr_method m) {
     // getAlgorithmType of the three reconstructor classes
@@ -119,20 +167,51 @@ int main(int argc, char** argv) {
     fsr_context* ctx = nullptr;
     CHECK_FSR(fsr_create(&ctx, device));
     CHECK_FSR(fsr_set_profiling(ctx, 1));
-    DevBuf d_pos, d_dens, d_phi, d_tri, d_nrm, d_idx, d_stats;
+    hipStream_t compute = nullptr, copy = nullptr;
+    CHECK_HIP(hipStreamCreateWithFlags(&compute, hipStreamNonBlocking));
+    CHECK_HIP(hipStreamCreateWithFlags(&copy, hipStreamNonBlocking));
+    hipEvent_t uploaded[2];
+    for (auto& e : uploaded) CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HostFrame host[2];
+    DevBuf d_pos[2], d_dens[2], d_phi, d_tri, d_nrm, d_idx, d_stats;
     std::vector<double> times;
     double sum_tri = 0, sum_spr = 0, sum_svr = 0, sum_es = 0, sum_sf = 0, sum_mc = 0;
     uint32_t cap = 0, vcap = 0;
+    // frame `from`: read and uploaded before the loop
+    auto upload = [&](int slot) {
+        const HostFrame& f = host[slot];
+        const uint32_t n = f.n;
+        float* pp = static_cast<float*>(d_pos[slot].get(12ull * (n ? n : 1)));
+        float* pd = static_cast<float*>(d_dens[slot].get(4ull * (n ? n : 1)));
+        if (n) {
+            CHECK_HIP(hipMemcpyAsync(pp, f.pos, 12ull * n, hipMemcpyHostToDevice, copy));
+            CHECK_HIP(hipMemcpyAsync(pd, f.dens, 4ull * n, hipMemcpyHostToDevice, copy));
+        }
+        CHECK_HIP(hipEventRecord(uploaded[slot], copy));
+    };
+    auto check_loaded = [&](const HostFrame& f) {
+        if (!f.error.empty()) {
+            std::fprintf(stderr, "reading %s.xyz failed: %s\n", f.stem.c_str(), f.error.c_str());
+            std::exit(1);
+        }
+    };
+    load_frame(host[0], dir, pattern, from);
+    check_loaded(host[0]);
+    upload(0);
+    const auto wall0 = std::chrono::steady_clock::now();
     for (long frame = from; frame < to; ++frame) {
-        char base[512];
-        std::snprintf(base, sizeof(base), pattern.c_str(), (unsigned)frame);
-        const std::string stem = dir + base;
+        const int slot = (int)((frame - from) & 1);
+        HostFrame& hf = host[slot];
+        // the next frame is parsed while this one runs
+        std::future<void> next;
+        if (frame + 1 < to)
+            next = std::async(std::launch::async, load_frame, std::ref(host[slot ^ 1]), dir,
+                              pattern, frame + 1);
+        const std::string& stem = hf.stem;
+        const char* base = hf.base.c_str();
         std::printf("\n*************************** Frame %ld ***************************\n", frame);
-        fsr_xyz_header h;
-        uint32_t n = 0;
-        CHECK_FSR(fsr_xyz_info((stem + ".xyz").c_str(), &h, &n));
-        std::vector<float> pos(3ull * n), dens(n);
-        CHECK_FSR(fsr_xyz_read((stem + ".xyz").c_str(), &h, pos.data(), dens.data(), n, &n));
+        const fsr_xyz_header h = hf.h;
+        const uint32_t n = hf.n;
         fsr_sim_param sim;
         CHECK_FSR(fsr_sim_param_for(method, &h, &sim));
         fsr_grid_info sp, sc;
@@ -145,30 +224,34 @@ int main(int argc, char** argv) {
                     sc.resolution.z);
         if (!cap) cap = 6u * n + 3072u;
         if (!vcap) vcap = 2u * n + 1024u;
-        float* pp = static_cast<float*>(d_pos.get(12ull * (n ? n : 1)));
-        float* pd = static_cast<float*>(d_dens.get(4ull * (n ? n : 1)));
+        float* pp = static_cast<float*>(d_pos[slot].get(12ull * (n ? n : 1)));
+        float* pd = static_cast<float*>(d_dens[slot].get(4ull * (n ? n : 1)));
         float* phi = static_cast<float*>(d_phi.get(4ull * V));
         fsr_frame_stats* st = static_cast<fsr_frame_stats*>(d_stats.get(sizeof(fsr_frame_stats)));
         fsr_frame_stats hs;
         double ms = 0;
         for (int attempt = 0; attempt < 2; ++attempt) {
-            CHECK_HIP(hipMemcpy(pp, pos.data(), 12ull * n, hipMemcpyHostToDevice));
-            CHECK_HIP(hipMemcpy(pd, dens.data(), 4ull * n, hipMemcpyHostToDevice));
-            CHECK_HIP(hipDeviceSynchronize());
+            if (attempt) {
+                // the frame sorted its particles in place: upload them again
+                upload(slot);
+            }
+            CHECK_HIP(hipStreamWaitEvent(compute, uploaded[slot], 0));
+            CHECK_HIP(hipEventSynchronize(uploaded[slot]));
             const auto t0 = std::chrono::steady_clock::now();
             if (indexed) {
                 CHECK_FSR(fsr_reconstruct_frame_indexed(
                     ctx, method, &sim, nullptr, &sp, &sc, pp, pd, n, phi,
                     static_cast<float*>(d_tri.get(12ull * vcap)),
                     static_cast<float*>(d_nrm.get(12ull * vcap)), vcap,
-                    static_cast<uint32_t*>(d_idx.get(4ull * cap)), cap, st, nullptr));
+                    static_cast<uint32_t*>(d_idx.get(4ull * cap)), cap, st, compute));
             } else {
                 CHECK_FSR(fsr_reconstruct_frame(ctx, method, &sim, nullptr, &sp, &sc, pp, pd, n,
                                                 phi, static_cast<float*>(d_tri.get(12ull * cap)),
                                                 static_cast<float*>(d_nrm.get(12ull * cap)), cap,
-                                                st, nullptr));
+                                                st, compute));
             }
-            CHECK_HIP(hipMemcpy(&hs, st, sizeof(hs), hipMemcpyDeviceToHost));
+            CHECK_HIP(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, compute));
+            CHECK_HIP(hipStreamSynchronize(compute));
             const auto t1 = std::chrono::steady_clock::now();
             ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
             if (!hs.overflow) break;
@@ -180,6 +263,12 @@ int main(int argc, char** argv) {
         CHECK_FSR(fsr_stage_times(ctx, stage, 1, &nf));
         times.push_back(ms);
         std::printf("Reconstructing Frame%ld took %g milliseconds.\n", frame, ms);
+        // the next frame's upload runs while this frame's files are written
+        if (next.valid()) {
+            next.get();
+            check_loaded(host[slot ^ 1]);
+            upload(slot ^ 1);
+        }
         if (save_vis) {
             const std::string path = stem + ".vis";
             std::printf("Writing to %s...\n", path.c_str());
@@ -216,6 +305,10 @@ int main(int argc, char** argv) {
             std::printf("Writing to %s\n", path.c_str());
         }
     }
+    const double wall_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    std::printf("\n%zu frame(s) in %g ms end to end (%g frames/s; reading, uploading and writing "
+                "included).\n", times.size(), wall_ms, times.size() * 1000.0 / wall_ms);
     const double nfr = (double)times.size();
     {
         const std::string path = dir + algorithm_type(method) + ".time";
@@ -243,6 +336,9 @@ int main(int argc, char** argv) {
             std::fprintf(stderr, "Failed to save the file: %s\n", path.c_str());
         }
     }
+    for (auto& e : uploaded) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(copy);
+    (void)hipStreamDestroy(compute);
     fsr_destroy(ctx);
     return 0;
 }

@@ -2,7 +2,9 @@
 ReconstructorGPU::reconstruct flow) on two .xyz frames: its .obj output equals
 the library's soup for the same frame to the 6 significant digits the .obj
 dialect keeps, its .vis files hold the same frame's intermediates, and the
-.time / .config files have the reference's layout."""
+.time / .config files have the reference's layout.  The driver reads and
+uploads frame k+1 while frame k runs, so the second frame checks that
+pipeline."""
 import os
 import subprocess
 
@@ -52,6 +54,7 @@ def test_driver_frames(torch_cuda, tmp_path, method):
                         method, "--config", "--vis"], capture_output=True, text=True, timeout=120)
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0
+    assert "2 frame(s) in" in r.stdout and "end to end" in r.stdout
     tris = []
     for k, fr in enumerate(frames):
         rec = Reconstructor(method)
