@@ -209,17 +209,35 @@ __global__ __launch_bounds__(256) void k_axis_maps(GridDev sc, GridDev sp, int r
     const float smin = axis == 0 ? sc.minx : (axis == 1 ? sc.miny : sc.minz);
     const float pmin = axis == 0 ? sp.minx : (axis == 1 ? sp.miny : sp.minz);
     const int pres = (int)(axis == 0 ? sp.rx : (axis == 1 ? sp.ry : sp.rz));
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < res; i += gridDim.x * blockDim.x) {
-        auto vc_of = [&](uint32_t v) -> uint32_t {
-            const float q = cell_coord(vertex_coord(v, smin, sc.cell), pmin, sp.cell);
-            int c = trunc_clamped(q, pres + radius);
-            c = clampi(c, -radius, pres - 1 + radius);
-            return (uint32_t)(c + radius);
-        };
-        const uint32_t c = vc_of(i);
-        m.vc[axis][i] = c;
-        if (i == 0 || vc_of(i - 1) != c) m.lo[axis][c] = i;
-        if (i + 1 == res || vc_of(i + 1) != c) m.hi[axis][c] = i + 1;
+    const uint32_t nvc = (uint32_t)(pres + 2 * radius);
+    auto vc_of = [&](uint32_t v) -> uint32_t {
+        const float q = cell_coord(vertex_coord(v, smin, sc.cell), pmin, sp.cell);
+        int c = trunc_clamped(q, pres + radius);
+        c = clampi(c, -radius, pres - 1 + radius);
+        return (uint32_t)(c + radius);
+    };
+    // first vertex whose virtual cell is >= c (the map is non-decreasing:
+    // vertex positions, the cell coordinate, truncation and clamping all are)
+    auto first_at_least = [&](uint32_t c) -> uint32_t {
+        uint32_t lo = 0, hi = res;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (vc_of(mid) < c) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    };
+    // threads [0, res): the vertex -> cell map; [res, res + nvc): each virtual
+    // cell's vertex interval [lo, hi), empty (lo = hi) when no vertex maps to it
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < res + nvc;
+         i += gridDim.x * blockDim.x) {
+        if (i < res) {
+            m.vc[axis][i] = vc_of(i);
+        } else {
+            const uint32_t c = i - res;
+            m.lo[axis][c] = first_at_least(c);
+            m.hi[axis][c] = first_at_least(c + 1);
+        }
     }
 }
 
@@ -421,8 +439,7 @@ fsr_status band_compact(fsr_context* ctx, const uint32_t* bits, const GridDev& s
     m.hi[2] = m.lo[2] + vcz;
     m.row_any = m.hi[2] + vcz;
     m.row_words = row_words;
-    FSR_TRY(fill_u32(m.lo[0], 0u, 2ull * (vcx + vcy + vcz), s));
-    const uint32_t maxres = std::max(sc.rx, std::max(sc.ry, sc.rz));
+    const uint32_t maxres = std::max(sc.rx + vcx, std::max(sc.ry + vcy, sc.rz + vcz));
     const uint32_t map_blocks = (uint32_t)std::max<uint64_t>(
         (maxres + 255) / 256, std::min<uint64_t>((rows + 255) / 256, 1024));
     hipLaunchKernelGGL(k_axis_maps, dim3(map_blocks, 4), dim3(256), 0, s, sc, sp, radius, m, bits,

@@ -256,6 +256,9 @@ fsr_status frame_impl(const char* who, fsr_context* ctx, fsr_method method,
     fills.add(counters, 0u, sizeof(Counters) / 4);
     fills.add(stats, 0u, sizeof(fsr_frame_stats) / 4);
     fills.add(sbits, 0u, W);
+    // the dense field outside the band keeps the fill: 0, or the byte 100 for
+    // Yu13 (the reference's memset of the Yu13 grid)
+    fills.add(phi, yu13 ? 0x64646464u : 0u, V);
 
     // 1. spatial hashing
     prof_mark(ctx, FSR_STAGE_HASH, s);
@@ -307,13 +310,8 @@ fsr_status frame_impl(const char* who, fsr_context* ctx, fsr_method method,
     FSR_TRY(band_compact(ctx, bits, sc, sp, yu13 ? 2 : 1, counters, stats, &bc, s));
 
     // 4. field
+    // (the dense grid was filled in the frame's first launch)
     prof_mark(ctx, FSR_STAGE_FILL, s);
-    if (yu13) {
-        // the reference fills the Yu13 grid with the byte 100 (0x64646464)
-        FSR_TRY(fill_u32(reinterpret_cast<uint32_t*>(phi), 0x64646464u, V, s));
-    } else {
-        FSR_TRY(fill_f32(phi, 0.0f, V, s));
-    }
     prof_mark(ctx, FSR_STAGE_FIELD, s);
     if (yu13) {
         AnisoArgs aa;
@@ -363,10 +361,8 @@ fsr_status frame_impl(const char* who, fsr_context* ctx, fsr_method method,
     // 5. marching cubes
     prof_mark(ctx, FSR_STAGE_MC, s);
     McArgs ma;
-    uint32_t *nverts, *valid, *valid_off, *vert_off, *valid_idx;
+    uint32_t *nverts, *vert_off, *valid_idx;
     FSR_TRY(arena_typed(ctx, A_NVERTS, V, &nverts));
-    FSR_TRY(arena_typed(ctx, A_VALID, V, &valid));
-    FSR_TRY(arena_typed(ctx, A_VALID_OFF, V, &valid_off));
     FSR_TRY(arena_typed(ctx, A_VERT_OFF, V, &vert_off));
     FSR_TRY(arena_typed(ctx, A_VALID_IDX, V, &valid_idx));
     ma.band_index = bc.band_index;
@@ -376,8 +372,8 @@ fsr_status frame_impl(const char* who, fsr_context* ctx, fsr_method method,
     ma.iso = sim->iso_value;
     ma.cpu_flavor = method == FSR_METHOD_AKINCI12 ? 1 : 0;
     ma.nverts = nverts;
-    ma.valid = valid;
-    ma.valid_off = valid_off;
+    ma.valid = nullptr;      // the valid-cube list comes straight from the vertex counts
+    ma.valid_off = nullptr;
     ma.vert_off = vert_off;
     ma.valid_idx = valid_idx;
     ma.tri_pos = mo.tri_pos;

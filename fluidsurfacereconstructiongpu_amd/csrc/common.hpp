@@ -223,6 +223,8 @@ struct Counters {
     uint32_t aux[12];
     uint32_t pad[12];
     uint32_t xcd_next[8 * 32];  // per-XCD fetch counters of the field kernels, 128 B apart
+    uint32_t xcd_field[8 * 32]; // the same for the ZB05 field kernel, used once per frame,
+                                // so the frame's initial clear is its reset
 };
 
 }  // namespace fsr
@@ -304,6 +306,13 @@ fsr_status scan_exclusive(fsr_context* ctx, ScanInput f, const uint32_t* in,
                           uint32_t* out, uint64_t n_max, const uint32_t* d_count,
                           uint32_t* d_total, hipStream_t s);
 fsr_status fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
+
+// Marching-cubes offsets: vert_off = exclusive scan of the per-cube vertex
+// counts nv (live length *d_count, at most n_max) and valid_idx = the cubes
+// with nv > 0 in ascending order; the totals go to the two device counters.
+fsr_status scan_mc(fsr_context* ctx, const uint32_t* nv, uint64_t n_max, const uint32_t* d_count,
+                   uint32_t* vert_off, uint32_t* valid_idx, uint32_t* d_total_verts,
+                   uint32_t* d_total_valid, hipStream_t s);
 
 // Several independent u32 fills in one launch.
 constexpr int kMaxFillJobs = 8;
@@ -432,7 +441,6 @@ fsr_status mc_run_indexed(fsr_context* ctx, const McArgs& a, hipStream_t s);
 // single steps of mc_run (classify reads a.st->num_band_vertices, emit reads
 // a.st->num_valid_cubes)
 fsr_status mc_classify(fsr_context* ctx, const McArgs& a, hipStream_t s);
-fsr_status mc_compact(fsr_context* ctx, const McArgs& a, hipStream_t s);
 fsr_status mc_emit(fsr_context* ctx, const McArgs& a, hipStream_t s);
 
 // aniso.hip -----------------------------------------------------------------

@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kFieldThreads) void k_field_zb05(FieldArgs a, BandC
 
     uint32_t step = 0;
     for (;;) {
-        const uint32_t ch = next_item_xcd(a.counters->xcd_next, num_chunks, step);
+        const uint32_t ch = next_item_xcd(a.counters->xcd_field, num_chunks, step);
         if (ch >= num_chunks) break;
 
         const uint32_t v = bc.chunk_cell[ch];
@@ -234,8 +234,9 @@ __global__ __launch_bounds__(kFieldThreads) void k_field_zb05(FieldArgs a, BandC
 }  // namespace
 
 fsr_status field_zb05(fsr_context* ctx, const FieldArgs& a, hipStream_t s) {
+    // a.counters was cleared when the frame (or the stage call) began, and no
+    // other kernel fetches through xcd_field
     const int blocks = persistent_blocks(ctx, kFieldThreads, 8, TUNE_FIELD);
-    FSR_TRY(reset_xcd(a.counters, s));
     hipLaunchKernelGGL(k_field_zb05, dim3(blocks), dim3(kFieldThreads), 0, s, a, *a.bc);
     FSR_LAUNCH_CHECK("k_field_zb05");
     return FSR_OK;

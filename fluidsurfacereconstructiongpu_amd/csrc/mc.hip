@@ -1,4 +1,4 @@
-// Marching cubes over the band: classify, compact, emit a triangle soup.
+// Marching cubes over the band: classify, scan + compact, emit a triangle soup.
 //
 //   k_mc_classify  detectValidSurfaceCubes (ReconstructionCUDA.cu:328-364) and
 //                  detectionOfValidSurfaceCubesForAkinci (:974-1008): the cube
@@ -7,9 +7,12 @@
 //                  (ReconstructionCUDA.cuh:534-544); the cube is valid iff all 8
 //                  corners are band vertices (isAllSfVertex, .cuh:602-609) and its
 //                  case emits vertices.
-//   scans          valid flags and vertex counts (the reference's two thrust scans,
-//                  GPU/ReconstructorGPUOurs.cpp:195-235), totals kept on the device.
-//   k_mc_compact   compactValidSurfaceCubes (ReconstructionCUDA.cu:367-380).
+//   scan_mc        the reference's two thrust scans (valid flags and vertex
+//                  counts, GPU/ReconstructorGPUOurs.cpp:195-235) and
+//                  compactValidSurfaceCubes (ReconstructionCUDA.cu:367-380) in
+//                  one two-pass scan: a cube is valid iff it emits vertices, so
+//                  the vertex counts alone give the offsets and the valid-cube
+//                  list; totals stay on the device.
 //   k_mc_emit      generateTriangles (ReconstructionCUDA.cu:383-463): per crossed
 //                  edge t = (iso - phi0)/(phi1 - phi0) (0.5 if |delta| < 1e-7),
 //                  position lerp of the corner positions, normal lerp of the
@@ -78,16 +81,8 @@ __global__ __launch_bounds__(256) void k_mc_classify(McArgs a) {
             nv = all ? case_vertices(f) : 0u;
         }
         a.nverts[b] = nv;
-        a.valid[b] = nv ? 1u : 0u;
+        if (a.valid) a.valid[b] = nv ? 1u : 0u;
     }
-}
-
-__global__ __launch_bounds__(256) void k_mc_compact(McArgs a) {
-    const uint32_t B = a.st->num_band_vertices;
-    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < B; b += gridDim.x * blockDim.x)
-        if (a.valid[b]) a.valid_idx[a.valid_off[b]] = b;
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        a.st->overflow = a.st->num_mesh_vertices > a.capacity ? 1u : 0u;
 }
 
 __device__ __forceinline__ float sample(const float* __restrict__ phi, int x, int y, int z,
@@ -99,6 +94,8 @@ __device__ __forceinline__ float sample(const float* __restrict__ phi, int x, in
 template <bool kCpu>
 __global__ __launch_bounds__(256) void k_mc_emit(McArgs a) {
     const uint32_t Q = a.st->num_valid_cubes;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.st->num_mesh_vertices > a.capacity)
+        a.st->overflow = 1u;
     const uint32_t rx = a.sc.rx, ry = a.sc.ry, rz = a.sc.rz;
     const uint64_t rxy = (uint64_t)rx * ry;
     const float sign = (!kCpu && a.iso < 0.0f) ? -1.0f : 1.0f;
@@ -239,6 +236,8 @@ __device__ __forceinline__ void corner_normal(const float* __restrict__ phi, int
 }
 
 __global__ __launch_bounds__(256) void k_mc_edges(McArgs a) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.st->num_mesh_vertices > a.capacity)
+        a.st->overflow = 1u;
     const uint32_t B = a.st->num_band_vertices;
     const uint32_t res[3] = {a.sc.rx, a.sc.ry, a.sc.rz};
     const uint64_t rxy = (uint64_t)res[0] * res[1];
@@ -381,11 +380,8 @@ __global__ __launch_bounds__(256) void k_mc_indices(McArgs a) {
 
 fsr_status mc_run_indexed(fsr_context* ctx, const McArgs& a, hipStream_t s) {
     FSR_TRY(mc_classify(ctx, a, s));
-    FSR_TRY(scan_exclusive(ctx, SCAN_U32, a.valid, a.valid_off, a.max_band,
-                           &a.st->num_band_vertices, &a.st->num_valid_cubes, s));
-    FSR_TRY(scan_exclusive(ctx, SCAN_U32, a.nverts, a.vert_off, a.max_band,
-                           &a.st->num_band_vertices, &a.st->num_mesh_vertices, s));
-    FSR_TRY(mc_compact(ctx, a, s));
+    FSR_TRY(scan_mc(ctx, a.nverts, a.max_band, &a.st->num_band_vertices, a.vert_off, a.valid_idx,
+                    &a.st->num_mesh_vertices, &a.st->num_valid_cubes, s));
     const int blocks = persistent_blocks(ctx, 256, 8, TUNE_MC);
     hipLaunchKernelGGL(k_mc_edges, dim3(blocks), dim3(256), 0, s, a);
     FSR_LAUNCH_CHECK("k_mc_edges");
@@ -408,13 +404,6 @@ fsr_status mc_classify(fsr_context* ctx, const McArgs& a, hipStream_t s) {
     return FSR_OK;
 }
 
-fsr_status mc_compact(fsr_context* ctx, const McArgs& a, hipStream_t s) {
-    const int blocks = persistent_blocks(ctx, 256, 8, TUNE_MC);
-    hipLaunchKernelGGL(k_mc_compact, dim3(blocks), dim3(256), 0, s, a);
-    FSR_LAUNCH_CHECK("k_mc_compact");
-    return FSR_OK;
-}
-
 fsr_status mc_emit(fsr_context* ctx, const McArgs& a, hipStream_t s) {
     const int blocks = persistent_blocks(ctx, 256, 8, TUNE_MC);
     if (a.cpu_flavor)
@@ -427,11 +416,8 @@ fsr_status mc_emit(fsr_context* ctx, const McArgs& a, hipStream_t s) {
 
 fsr_status mc_run(fsr_context* ctx, const McArgs& a, hipStream_t s) {
     FSR_TRY(mc_classify(ctx, a, s));
-    FSR_TRY(scan_exclusive(ctx, SCAN_U32, a.valid, a.valid_off, a.max_band,
-                           &a.st->num_band_vertices, &a.st->num_valid_cubes, s));
-    FSR_TRY(scan_exclusive(ctx, SCAN_U32, a.nverts, a.vert_off, a.max_band,
-                           &a.st->num_band_vertices, &a.st->num_mesh_vertices, s));
-    FSR_TRY(mc_compact(ctx, a, s));
+    FSR_TRY(scan_mc(ctx, a.nverts, a.max_band, &a.st->num_band_vertices, a.vert_off, a.valid_idx,
+                    &a.st->num_mesh_vertices, &a.st->num_valid_cubes, s));
     return mc_emit(ctx, a, s);
 }
 

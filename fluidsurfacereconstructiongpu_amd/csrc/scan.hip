@@ -159,6 +159,118 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(
     if (last && threadIdx.x == 0 && d_total) *d_total = running;
 }
 
+// Marching-cubes offsets in two passes over the per-cube vertex counts nv[b]:
+// vert_off = exclusive scan of nv, and the ascending list of the cubes with
+// nv > 0 (the valid cubes) compacted in the same pass.  Pass 1: partials[b] =
+// sum of nv over segment b, partials[G + b] = its valid cubes.
+__global__ __launch_bounds__(kScanThreads) void k_mc_scan_reduce(
+    const uint32_t* __restrict__ nv, uint64_t n_max, const uint32_t* d_count,
+    uint32_t* __restrict__ partials) {
+    __shared__ uint32_t lds[kScanThreads / 64];
+    const uint64_t n = live_length(n_max, d_count);
+    const uint64_t seg = segment_len(n, gridDim.x);
+    const uint64_t lo = (uint64_t)blockIdx.x * seg;
+    const uint64_t hi = lo + seg < n ? lo + seg : n;
+    uint32_t sum = 0, cnt = 0;
+    for (uint64_t base = lo; base < hi; base += kScanTile) {
+        const uint64_t i0 = base + (uint64_t)kScanItems * threadIdx.x;
+        if (base + kScanTile <= hi) {
+#pragma unroll
+            for (int k = 0; k < kScanItems; k += 4) {
+                const uint4 q = *reinterpret_cast<const uint4*>(nv + i0 + k);
+                sum += q.x + q.y + q.z + q.w;
+                cnt += (q.x ? 1u : 0u) + (q.y ? 1u : 0u) + (q.z ? 1u : 0u) + (q.w ? 1u : 0u);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < kScanItems; ++k) {
+                const uint32_t v = (i0 + k < hi) ? nv[i0 + k] : 0u;
+                sum += v;
+                cnt += v ? 1u : 0u;
+            }
+        }
+    }
+    uint32_t excl;
+    const uint32_t total = block_exclusive(sum, lds, &excl);
+    const uint32_t total_cnt = block_exclusive(cnt, lds, &excl);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = total;
+        partials[gridDim.x + blockIdx.x] = total_cnt;
+    }
+}
+
+// Pass 2: each segment from the sums of the partials before it.
+__global__ __launch_bounds__(kScanThreads) void k_mc_scan_down(
+    const uint32_t* __restrict__ nv, uint64_t n_max, const uint32_t* d_count,
+    const uint32_t* __restrict__ partials, uint32_t* __restrict__ vert_off,
+    uint32_t* __restrict__ valid_idx, uint32_t* d_total_verts, uint32_t* d_total_valid) {
+    __shared__ uint32_t lds[kScanThreads / 64];
+    const uint64_t n = live_length(n_max, d_count);
+    const uint64_t seg = segment_len(n, gridDim.x);
+    const uint64_t lo = (uint64_t)blockIdx.x * seg;
+    const uint64_t hi = lo + seg < n ? lo + seg : n;
+    uint32_t s0 = 0, s1 = 0;
+    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kScanThreads) {
+        s0 += partials[i];
+        s1 += partials[gridDim.x + i];
+    }
+    uint32_t excl;
+    uint32_t run_v = block_exclusive(s0, lds, &excl);
+    uint32_t run_c = block_exclusive(s1, lds, &excl);
+    for (uint64_t base = lo; base < hi; base += kScanTile) {
+        const uint64_t i0 = base + (uint64_t)kScanItems * threadIdx.x;
+        const bool full = base + kScanTile <= hi;
+        uint32_t v[kScanItems];
+        if (full) {
+#pragma unroll
+            for (int k = 0; k < kScanItems; k += 4) {
+                const uint4 q = *reinterpret_cast<const uint4*>(nv + i0 + k);
+                v[k] = q.x;
+                v[k + 1] = q.y;
+                v[k + 2] = q.z;
+                v[k + 3] = q.w;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < kScanItems; ++k) v[k] = (i0 + k < hi) ? nv[i0 + k] : 0u;
+        }
+        uint32_t tv = 0, tc = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            tv += v[k];
+            tc += v[k] ? 1u : 0u;
+        }
+        uint32_t ev, ec;
+        const uint32_t totv = block_exclusive(tv, lds, &ev);
+        const uint32_t totc = block_exclusive(tc, lds, &ec);
+        uint32_t rv = run_v + ev, rc = run_c + ec;
+        uint32_t o[kScanItems];
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            o[k] = rv;
+            if (v[k] && i0 + k < hi) valid_idx[rc] = (uint32_t)(i0 + k);
+            rv += v[k];
+            rc += v[k] ? 1u : 0u;
+        }
+        if (full) {
+#pragma unroll
+            for (int k = 0; k < kScanItems; k += 4)
+                *reinterpret_cast<uint4*>(vert_off + i0 + k) =
+                    make_uint4(o[k], o[k + 1], o[k + 2], o[k + 3]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < kScanItems; ++k)
+                if (i0 + k < hi) vert_off[i0 + k] = o[k];
+        }
+        run_v += totv;
+        run_c += totc;
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+        *d_total_verts = run_v;
+        *d_total_valid = run_c;
+    }
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x)
@@ -185,18 +297,26 @@ struct FillArgs {
     int count;
 };
 
-// Blocks [blk_end[j-1], blk_end[j]) fill job j (block-uniform job lookup).
+// Blocks [blk_end[j-1], blk_end[j]) fill job j (block-uniform job lookup):
+// 16-byte stores from the first 16-byte boundary, single words around them.
 __global__ __launch_bounds__(256) void k_fill_multi(FillArgs a) {
     int j = 0;
     while (j + 1 < a.count && blockIdx.x >= a.blk_end[j]) ++j;
     const uint32_t b0 = j ? a.blk_end[j - 1] : 0u;
-    const uint32_t nb = a.blk_end[j] - b0;
+    const uint64_t nb = a.blk_end[j] - b0;
     uint32_t* p = a.p[j];
     const uint32_t v = a.value[j];
     const uint64_t n = a.n[j];
-    for (uint64_t i = (uint64_t)(blockIdx.x - b0) * blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)nb * blockDim.x)
-        p[i] = v;
+    const uint64_t t = (uint64_t)(blockIdx.x - b0) * blockDim.x + threadIdx.x;
+    const uint64_t stride = nb * blockDim.x;
+    const uint64_t head = min<uint64_t>(((16u - ((uintptr_t)p & 15u)) & 15u) >> 2, n);
+    const uint64_t n4 = (n - head) >> 2;
+    uint4* p4 = reinterpret_cast<uint4*>(p + head);
+    const uint4 q = make_uint4(v, v, v, v);
+    for (uint64_t i = t; i < n4; i += stride) p4[i] = q;
+    if (t < head) p[t] = v;
+    const uint64_t tail = head + 4 * n4;
+    if (t < n - tail) p[tail + t] = v;
 }
 
 template <int F>
@@ -205,7 +325,7 @@ fsr_status scan_impl(fsr_context* ctx, const uint32_t* in, uint32_t* out, uint64
     const uint64_t want = (n_max + kScanTile - 1) / kScanTile;
     const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, kMaxScanBlocks));
     uint32_t* partials = nullptr;
-    FSR_TRY(arena_typed(ctx, A_SCAN_PARTIALS, kMaxScanBlocks, &partials));
+    FSR_TRY(arena_typed(ctx, A_SCAN_PARTIALS, 2ull * kMaxScanBlocks, &partials));
     const int vec_ok = ((((uintptr_t)in) | ((uintptr_t)out)) & 15u) == 0 ? 1 : 0;
     hipLaunchKernelGGL(k_scan_reduce<F>, dim3(g), dim3(kScanThreads), 0, s, in, n_max, d_count,
                        partials, vec_ok);
@@ -233,6 +353,26 @@ fsr_status scan_exclusive(fsr_context* ctx, ScanInput f, const uint32_t* in, uin
     return FSR_ERR_INVALID_ARG;
 }
 
+fsr_status scan_mc(fsr_context* ctx, const uint32_t* nv, uint64_t n_max, const uint32_t* d_count,
+                   uint32_t* vert_off, uint32_t* valid_idx, uint32_t* d_total_verts,
+                   uint32_t* d_total_valid, hipStream_t s) {
+    if (((uintptr_t)nv | (uintptr_t)vert_off) & 15u) {
+        set_error("scan_mc: the vertex counts and offsets must be 16-byte aligned");
+        return FSR_ERR_INVALID_ARG;
+    }
+    const uint64_t want = (n_max + kScanTile - 1) / kScanTile;
+    const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, kMaxScanBlocks));
+    uint32_t* partials = nullptr;
+    FSR_TRY(arena_typed(ctx, A_SCAN_PARTIALS, 2ull * kMaxScanBlocks, &partials));
+    hipLaunchKernelGGL(k_mc_scan_reduce, dim3(g), dim3(kScanThreads), 0, s, nv, n_max, d_count,
+                       partials);
+    FSR_LAUNCH_CHECK("k_mc_scan_reduce");
+    hipLaunchKernelGGL(k_mc_scan_down, dim3(g), dim3(kScanThreads), 0, s, nv, n_max, d_count,
+                       partials, vert_off, valid_idx, d_total_verts, d_total_valid);
+    FSR_LAUNCH_CHECK("k_mc_scan_down");
+    return FSR_OK;
+}
+
 fsr_status fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s) {
     if (n == 0) return FSR_OK;
     uint64_t blocks = (n + 255) / 256;
@@ -252,7 +392,7 @@ fsr_status fill_multi(const FillList& l, hipStream_t s) {
         a.p[k] = l.p[j];
         a.n[k] = l.n[j];
         a.value[k] = l.value[j];
-        blocks += (uint32_t)std::min<uint64_t>((l.n[j] + 255) / 256, 2048);
+        blocks += (uint32_t)std::min<uint64_t>((l.n[j] + 1023) / 1024, 4096);
         a.blk_end[k] = blocks;
     }
     if (!a.count) return FSR_OK;
