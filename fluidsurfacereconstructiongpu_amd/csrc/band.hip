@@ -130,31 +130,71 @@ __global__ __launch_bounds__(256) void k_band_exceptions(BandArgs a) {
     }
 }
 
-// Each word writes its set bits in ascending order at its scanned offset.  A
-// block's 256 words own one contiguous output range: it is assembled in LDS and
+// Compaction of the band mask into the ascending band index (the reference's
+// scan + compactSurfaceVertex), in two launches: k_band_partials counts the
+// set bits of every 256-word block; k_band_emit gives each block its offset
+// (the sum of the counts of the blocks before it, at most 8 words per thread
+// from L2), scans its words' counts, writes each word's offset (word_off, the
+// band rank of the word's first vertex) and the block's vertices.  A block's
+// vertices own one contiguous output range: they are assembled in LDS and
 // stored with consecutive lanes on consecutive entries (direct per-word stores
-// when the range exceeds the staging buffer).
+// when the range exceeds the staging buffer).  The last block writes the band
+// size.
+__global__ __launch_bounds__(256) void k_band_partials(const uint32_t* __restrict__ bits,
+                                                       uint64_t nwords,
+                                                       uint32_t* __restrict__ partials) {
+    __shared__ uint32_t part[4];
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c = wave_sum(w < nwords ? (uint32_t)__popc(bits[w]) : 0u);
+    if (__lane_id() == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
 __global__ __launch_bounds__(256) void k_band_emit(const uint32_t* __restrict__ bits,
-                                                   const uint32_t* __restrict__ word_off,
+                                                   const uint32_t* __restrict__ partials,
                                                    uint64_t nwords, uint32_t wx, uint32_t rx,
+                                                   uint32_t* __restrict__ word_off,
                                                    uint32_t* __restrict__ band_index,
-                                                   uint64_t cap) {
+                                                   uint64_t cap, uint32_t* __restrict__ total,
+                                                   int scanned) {
     constexpr uint32_t kStage = 4096;
     __shared__ uint32_t stage[kStage];
-    __shared__ uint32_t range[2];
-    const uint64_t first = (uint64_t)blockIdx.x * blockDim.x;
-    const uint64_t w = first + threadIdx.x;
-    const uint64_t last = min(first + blockDim.x, nwords) - 1;
+    __shared__ uint32_t part[8];
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = w < nwords;
     uint32_t b = live ? bits[w] : 0u;
-    const uint32_t o = live ? word_off[w] : 0u;
-    if (w == first) range[0] = o;
-    if (w == last) range[1] = o + (uint32_t)__popc(b);
+    const uint32_t c = (uint32_t)__popc(b);
+    // offset of this block: the counts of the blocks before it (or, when
+    // `scanned`, partials already holds the exclusive offsets)
+    uint32_t before = 0;
+    if (scanned) {
+        before = threadIdx.x == 0 ? partials[blockIdx.x] : 0u;
+    } else {
+        for (uint32_t i = threadIdx.x; i < blockIdx.x; i += blockDim.x) before += partials[i];
+    }
+    before = wave_sum(before);
+    // exclusive scan of the words' counts inside the block
+    const uint32_t incl = wave_inclusive_sum(c);
+    const uint32_t wave = threadIdx.x >> 6;
+    if (__lane_id() == 63) {
+        part[wave] = incl;
+        part[4 + wave] = before;
+    }
     __syncthreads();
-    const uint32_t lo = range[0], total = range[1] - range[0];
+    uint32_t lo = part[4] + part[5] + part[6] + part[7];  // the block's first output slot
+    uint32_t wave_off = 0, block_total = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k < (int)wave) wave_off += part[k];
+        block_total += part[k];
+    }
+    const uint32_t o = lo + wave_off + incl - c;
+    if (live) word_off[w] = o;
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = lo + block_total;
     const uint64_t row = w / wx;
     const uint32_t vbase = (uint32_t)(row * rx) + (uint32_t)(w % wx) * 32u;
-    if (total <= kStage) {
+    if (block_total <= kStage) {
         uint32_t k = o - lo;
         while (b) {
             if (k < kStage) stage[k] = vbase + __builtin_ctz(b);
@@ -162,7 +202,7 @@ __global__ __launch_bounds__(256) void k_band_emit(const uint32_t* __restrict__ 
             b &= b - 1u;
         }
         __syncthreads();
-        for (uint32_t t = threadIdx.x; t < total; t += blockDim.x)
+        for (uint32_t t = threadIdx.x; t < block_total; t += blockDim.x)
             if ((uint64_t)lo + t < cap) band_index[lo + t] = stage[t];
     } else {
         uint32_t k = o;
@@ -184,10 +224,18 @@ struct AxisMaps {
 };
 
 // blockIdx.y = 0, 1, 2: the maps of axis x, y, z; blockIdx.y = 3: the row
-// occupancy bits, one wave per 64 rows (each wave writes its two words).
+// occupancy bits, one wave per 64 rows (each wave writes its two words);
+// blockIdx.y = 4: clears the per-cell band counts cnt[ncells].
 __global__ __launch_bounds__(256) void k_axis_maps(GridDev sc, GridDev sp, int radius, AxisMaps m,
-                                                   const uint32_t* __restrict__ bits, uint32_t wx) {
+                                                   const uint32_t* __restrict__ bits, uint32_t wx,
+                                                   uint32_t* __restrict__ cnt, uint64_t ncells) {
     const uint32_t axis = blockIdx.y;
+    if (axis == 4) {
+        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncells;
+             i += (uint64_t)gridDim.x * blockDim.x)
+            cnt[i] = 0u;
+        return;
+    }
     if (axis == 3) {
         const uint64_t rows = (uint64_t)sc.ry * sc.rz;
         for (uint64_t r0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); r0 < rows;
@@ -270,22 +318,24 @@ __device__ __forceinline__ uint32_t layer_rows(const AxisMaps& m, const GridDev&
 }
 
 // Cell-major band list, built per slab: the slab (cx, cy, z) is the part of
-// virtual cell (cx, cy, cz = vc_z[z]) on scalar layer z.  Slabs are numbered
-// cell by cell (v = (cz vcy + cy) vcx + cx ascending) and by layer inside a
-// cell:
+// virtual cell v = (cz vcy + cy) vcx + cx (cz = vc_z[z]) on scalar layer z.
+// Slabs are stored cell by cell and by layer inside a cell:
 //   slab(cx, cy, z) = vcx vcy lo_z[cz] + (cy vcx + cx) nz(cz) + (z - lo_z[cz]),
 // so the exclusive scan of the slab counts is each slab's offset in the cell
-// list, and a cell's vertices come out in ascending id order (layers z, rows y,
-// then x), i.e. a 64-vertex chunk is a compact slab of the cell, which keeps
-// the candidate culling of the field kernels effective.  One thread per slab
-// (consecutive threads = consecutive cx, which read the same row words); rows
-// without band vertices are skipped with the row occupancy bits.  Pass 0
-// writes the slab's band vertex count, pass 1 (kFill) its vertex ids at the
-// scanned offset.  No atomics: the list is deterministic.
+// list, and a cell's vertices come out in ascending id order (layers z, rows
+// y, then x), i.e. a 64-vertex chunk is a compact slab of the cell, which
+// keeps the candidate culling of the field kernels effective.  One thread per
+// slab (consecutive threads = consecutive cx, which read the same row words);
+// rows without band vertices are skipped with the row occupancy bits.  Pass 0
+// writes the slab's band vertex count and adds it to its cell's count (cnt,
+// cleared by k_axis_maps); pass 1 (kFill) writes the slab's vertex ids at its
+// scanned offset.  The list itself is deterministic.
 template <bool kFill>
 __global__ __launch_bounds__(256) void k_slab_rows(const uint32_t* __restrict__ bits, GridDev sc,
                                                    uint32_t wx, AxisMaps m, uint32_t vcx,
                                                    uint32_t vcy, uint32_t* __restrict__ slab,
+                                                   uint32_t* __restrict__ cnt,
+                                                   const uint32_t* __restrict__ slab_off,
                                                    uint32_t* __restrict__ cell_band, uint64_t cap) {
     const uint64_t total = (uint64_t)vcx * vcy * sc.rz;
     for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
@@ -295,13 +345,19 @@ __global__ __launch_bounds__(256) void k_slab_rows(const uint32_t* __restrict__ 
         const uint32_t z = (uint32_t)(t / ((uint64_t)vcx * vcy));
         const uint32_t cz = m.vc[2][z];
         const uint32_t zlo = m.lo[2][cz], nz = m.hi[2][cz] - zlo;
+        const uint64_t first = (uint64_t)vcx * vcy * zlo + ((uint64_t)cy * vcx + cx) * nz;
+        const uint64_t idx = first + (z - zlo);
+        const uint32_t v = (cz * vcy + cy) * vcx + cx;
+        uint32_t o = 0;
+        if (kFill) {
+            if (slab[idx] == 0u) continue;
+            o = slab_off[idx];
+        }
         const uint32_t ylo = m.lo[1][cy], yhi = m.hi[1][cy];
         const uint32_t xlo = m.lo[0][cx], xhi = m.hi[0][cx];
-        const uint64_t idx = (uint64_t)vcx * vcy * zlo + ((uint64_t)cy * vcx + cx) * nz + (z - zlo);
         const uint32_t ny = yhi > ylo ? yhi - ylo : 0u;
         const uint32_t w0 = xlo >> 5;
         const uint32_t nw = xhi > xlo ? ((xhi + 31u) >> 5) - w0 : 0u;
-        uint32_t o = kFill ? slab[idx] : 0u;
         for (uint32_t yb = 0; nw && yb < ny; yb += 32u) {
             uint32_t rows = layer_rows(m, sc, z, ylo, yb, ny);
             while (rows) {
@@ -323,39 +379,11 @@ __global__ __launch_bounds__(256) void k_slab_rows(const uint32_t* __restrict__ 
                 }
             }
         }
-        if (!kFill) slab[idx] = o;
+        if (!kFill) {
+            slab[idx] = o;
+            if (o) atomicAdd(&cnt[v], o);
+        }
     }
-}
-
-// Per virtual cell: offset and count in the cell list from the scanned slabs.
-__global__ __launch_bounds__(256) void k_cells_from_slabs(AxisMaps m, uint32_t vcx, uint32_t vcy,
-                                                          uint32_t nvc, uint64_t nslabs,
-                                                          const uint32_t* __restrict__ slab_off,
-                                                          const uint32_t* __restrict__ slab_total,
-                                                          uint32_t* __restrict__ cnt,
-                                                          uint32_t* __restrict__ cell_off) {
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nvc) return;
-    const uint32_t cx = v % vcx, cy = (v / vcx) % vcy, cz = v / (vcx * vcy);
-    const uint32_t zlo = m.lo[2][cz], nz = m.hi[2][cz] - zlo;
-    const uint64_t first = (uint64_t)vcx * vcy * zlo + ((uint64_t)cy * vcx + cx) * nz;
-    const uint32_t tot = *slab_total;
-    const uint32_t a = first < nslabs ? slab_off[first] : tot;
-    const uint32_t b = first + nz < nslabs ? slab_off[first + nz] : tot;
-    cell_off[v] = a;
-    cnt[v] = b - a;
-}
-
-__global__ __launch_bounds__(256) void k_chunk_cells(const uint32_t* __restrict__ cnt,
-                                                     const uint32_t* __restrict__ chunk_off,
-                                                     uint32_t nvc, uint32_t* __restrict__ chunk_cell,
-                                                     uint64_t cap) {
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nvc) return;
-    const uint32_t k = (cnt[v] + 63u) >> 6;
-    const uint32_t o = chunk_off[v];
-    for (uint32_t j = 0; j < k; ++j)
-        if ((uint64_t)o + j < cap) chunk_cell[o + j] = v;
 }
 
 }  // namespace
@@ -405,10 +433,20 @@ fsr_status band_compact(fsr_context* ctx, const uint32_t* bits, const GridDev& s
     uint32_t *word_off, *band_index;
     FSR_TRY(arena_typed(ctx, A_WORD_OFF, nwords, &word_off));
     FSR_TRY(arena_typed(ctx, A_BAND_INDEX, V, &band_index));
-    FSR_TRY(scan_exclusive(ctx, SCAN_POPC, bits, word_off, nwords, nullptr,
-                           &st->num_band_vertices, s));
-    hipLaunchKernelGGL(k_band_emit, dim3((uint32_t)((nwords + 255) / 256)), dim3(256), 0, s, bits,
-                       word_off, nwords, wx, sc.rx, band_index, V);
+    const uint32_t emit_blocks = (uint32_t)((nwords + 255) / 256);
+    uint32_t* partials;
+    FSR_TRY(arena_typed(ctx, A_BAND_PARTIALS, emit_blocks, &partials));
+    hipLaunchKernelGGL(k_band_partials, dim3(emit_blocks), dim3(256), 0, s, bits, nwords,
+                       partials);
+    FSR_LAUNCH_CHECK("k_band_partials");
+    // up to 4096 blocks each block sums the counts before it (at most 16 words
+    // per thread); beyond, the counts are scanned first
+    const int scanned = emit_blocks > 4096u ? 1 : 0;
+    if (scanned)
+        FSR_TRY(scan_exclusive(ctx, SCAN_U32, partials, partials, emit_blocks, nullptr, nullptr,
+                               s));
+    hipLaunchKernelGGL(k_band_emit, dim3(emit_blocks), dim3(256), 0, s, bits, partials, nwords,
+                       wx, sc.rx, word_off, band_index, V, &st->num_band_vertices, scanned);
     FSR_LAUNCH_CHECK("k_band_emit");
     out->band_index = band_index;
 
@@ -442,29 +480,25 @@ fsr_status band_compact(fsr_context* ctx, const uint32_t* bits, const GridDev& s
     const uint32_t maxres = std::max(sc.rx + vcx, std::max(sc.ry + vcy, sc.rz + vcz));
     const uint32_t map_blocks = (uint32_t)std::max<uint64_t>(
         (maxres + 255) / 256, std::min<uint64_t>((rows + 255) / 256, 1024));
-    hipLaunchKernelGGL(k_axis_maps, dim3(map_blocks, 4), dim3(256), 0, s, sc, sp, radius, m, bits,
-                       wx);
+    hipLaunchKernelGGL(k_axis_maps, dim3(map_blocks, 5), dim3(256), 0, s, sc, sp, radius, m, bits,
+                       wx, cnt, nvc);
     FSR_LAUNCH_CHECK("k_axis_maps");
-    // slab counts -> slab offsets -> per-cell offsets and counts -> slab fill
+    // slab counts (and cell counts) -> slab offsets; cell offsets, chunk
+    // offsets and the chunk list in one scan -> slab fill
     const uint64_t nslabs = (uint64_t)vcx * vcy * sc.rz;
-    uint32_t* slab;
-    FSR_TRY(arena_typed(ctx, A_SLAB, nslabs, &slab));
+    uint32_t *slab, *slab_off;
+    FSR_TRY(arena_typed(ctx, A_SLAB, 2 * nslabs, &slab));
+    slab_off = slab + nslabs;
     const uint32_t slab_blocks = (uint32_t)std::min<uint64_t>((nslabs + 255) / 256, 16384);
     hipLaunchKernelGGL(k_slab_rows<false>, dim3(slab_blocks), dim3(256), 0, s, bits, sc, wx, m,
-                       vcx, vcy, slab, cell_band, V);
+                       vcx, vcy, slab, cnt, (const uint32_t*)nullptr, cell_band, V);
     FSR_LAUNCH_CHECK("k_slab_rows(count)");
-    FSR_TRY(scan_exclusive(ctx, SCAN_U32, slab, slab, nslabs, nullptr, &counters->aux[10], s));
-    hipLaunchKernelGGL(k_cells_from_slabs, dim3((uint32_t)((nvc + 255) / 256)), dim3(256), 0, s, m,
-                       vcx, vcy, (uint32_t)nvc, nslabs, slab, &counters->aux[10], cnt, cell_off);
-    FSR_LAUNCH_CHECK("k_cells_from_slabs");
-    FSR_TRY(scan_exclusive(ctx, SCAN_CHUNKS64, cnt, chunk_off, nvc, nullptr,
-                           &counters->num_chunks, s));
+    FSR_TRY(scan_exclusive(ctx, SCAN_U32, slab, slab_off, nslabs, nullptr, nullptr, s));
+    FSR_TRY(scan_cells(ctx, cnt, nvc, cell_off, chunk_off, chunk_cell, max_chunks,
+                       &counters->num_chunks, s));
     hipLaunchKernelGGL(k_slab_rows<true>, dim3(slab_blocks), dim3(256), 0, s, bits, sc, wx, m,
-                       vcx, vcy, slab, cell_band, V);
+                       vcx, vcy, slab, cnt, slab_off, cell_band, V);
     FSR_LAUNCH_CHECK("k_slab_rows(fill)");
-    hipLaunchKernelGGL(k_chunk_cells, dim3((uint32_t)((nvc + 255) / 256)), dim3(256), 0, s, cnt,
-                       chunk_off, (uint32_t)nvc, chunk_cell, max_chunks);
-    FSR_LAUNCH_CHECK("k_chunk_cells");
     out->cell_off = cell_off;
     out->cell_cnt = cnt;
     out->chunk_off = chunk_off;

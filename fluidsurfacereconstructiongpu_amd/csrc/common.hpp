@@ -190,7 +190,7 @@ enum ArenaSlot {
     A_CHUNK_CELL, A_NVERTS, A_VALID, A_VALID_OFF, A_VERT_OFF, A_VALID_IDX,
     A_COUNTERS, A_SCAN_PARTIALS, A_STATS, A_INVOLVE, A_INVOLVE_OFF,
     A_INVOLVED_IDX, A_MEAN, A_SMOOTH, A_GMAT, A_SIGMA, A_AUX0, A_AUX1, A_REC,
-    A_ADJ_VERT, A_ADJ_COEF, A_ADJ_AUX, A_ADJ_TOUCH, A_EDGE_MASK, A_EDGE_OFF, A_SURF_CHUNKS, A_SLAB,
+    A_ADJ_VERT, A_ADJ_COEF, A_ADJ_AUX, A_ADJ_TOUCH, A_EDGE_MASK, A_EDGE_OFF, A_SURF_CHUNKS, A_SLAB, A_BAND_PARTIALS,
     A_NUM_SLOTS
 };
 
@@ -306,6 +306,13 @@ fsr_status scan_exclusive(fsr_context* ctx, ScanInput f, const uint32_t* in,
                           uint32_t* out, uint64_t n_max, const uint32_t* d_count,
                           uint32_t* d_total, hipStream_t s);
 fsr_status fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
+
+// Cell-major work list offsets from the per-cell band counts cnt[n]: cell_off
+// (exclusive scan), chunk_off (exclusive scan of the 64-vertex chunk counts),
+// chunk_cell (chunk -> cell, at most max_chunks entries) and the chunk total.
+fsr_status scan_cells(fsr_context* ctx, const uint32_t* cnt, uint64_t n, uint32_t* cell_off,
+                      uint32_t* chunk_off, uint32_t* chunk_cell, uint64_t max_chunks,
+                      uint32_t* d_num_chunks, hipStream_t s);
 
 // Marching-cubes offsets: vert_off = exclusive scan of the per-cube vertex
 // counts nv (live length *d_count, at most n_max) and valid_idx = the cubes

@@ -271,6 +271,80 @@ __global__ __launch_bounds__(kScanThreads) void k_mc_scan_down(
     }
 }
 
+// Cell-major work list offsets in two passes over the per-cell band counts:
+// cell_off = exclusive scan of cnt, chunk_off = exclusive scan of the cells'
+// 64-vertex chunk counts, and chunk_cell (chunk -> cell) written in the same
+// pass.  One cell per thread in 256-cell tiles and up to 4096 workgroups, so
+// that the cells with many chunks (the ring cells outside the hash grid) are
+// spread over many workgroups.  Pass 1: partials[b] / partials[G + b] = the
+// vertices / chunks of segment b.
+constexpr uint32_t kCellTile = kScanThreads;
+constexpr uint32_t kMaxCellBlocks = 4096;
+
+__device__ __forceinline__ uint32_t chunks64(uint32_t c) { return (c + 63u) >> 6; }
+
+__device__ __forceinline__ uint64_t cell_segment(uint64_t n, uint32_t g) {
+    const uint64_t seg = (n + g - 1) / g;
+    return (seg + kCellTile - 1) / kCellTile * kCellTile;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_cells_scan_reduce(
+    const uint32_t* __restrict__ cnt, uint64_t n, uint32_t* __restrict__ partials) {
+    __shared__ uint32_t lds[kScanThreads / 64];
+    const uint64_t seg = cell_segment(n, gridDim.x);
+    const uint64_t lo = (uint64_t)blockIdx.x * seg;
+    const uint64_t hi = lo + seg < n ? lo + seg : n;
+    uint32_t sum = 0, ch = 0;
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += kScanThreads) {
+        const uint32_t c = cnt[i];
+        sum += c;
+        ch += chunks64(c);
+    }
+    uint32_t excl;
+    const uint32_t tsum = block_exclusive(sum, lds, &excl);
+    const uint32_t tch = block_exclusive(ch, lds, &excl);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = tsum;
+        partials[gridDim.x + blockIdx.x] = tch;
+    }
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_cells_scan_down(
+    const uint32_t* __restrict__ cnt, uint64_t n, const uint32_t* __restrict__ partials,
+    uint32_t* __restrict__ cell_off, uint32_t* __restrict__ chunk_off,
+    uint32_t* __restrict__ chunk_cell, uint64_t max_chunks, uint32_t* d_num_chunks) {
+    __shared__ uint32_t lds[kScanThreads / 64];
+    const uint64_t seg = cell_segment(n, gridDim.x);
+    const uint64_t lo = (uint64_t)blockIdx.x * seg;
+    const uint64_t hi = lo + seg < n ? lo + seg : n;
+    uint32_t s0 = 0, s1 = 0;
+    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kScanThreads) {
+        s0 += partials[i];
+        s1 += partials[gridDim.x + i];
+    }
+    uint32_t excl;
+    uint32_t run_v = block_exclusive(s0, lds, &excl);
+    uint32_t run_c = block_exclusive(s1, lds, &excl);
+    for (uint64_t base = lo; base < hi; base += kCellTile) {
+        const uint64_t i = base + threadIdx.x;
+        const uint32_t c = i < hi ? cnt[i] : 0u;
+        const uint32_t nch = chunks64(c);
+        uint32_t ev, ec;
+        const uint32_t totv = block_exclusive(c, lds, &ev);
+        const uint32_t totc = block_exclusive(nch, lds, &ec);
+        if (i < hi) {
+            const uint32_t rc = run_c + ec;
+            cell_off[i] = run_v + ev;
+            chunk_off[i] = rc;
+            for (uint32_t j = 0; j < nch; ++j)
+                if ((uint64_t)rc + j < max_chunks) chunk_cell[rc + j] = (uint32_t)i;
+        }
+        run_v += totv;
+        run_c += totc;
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *d_num_chunks = run_c;
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x)
@@ -325,7 +399,7 @@ fsr_status scan_impl(fsr_context* ctx, const uint32_t* in, uint32_t* out, uint64
     const uint64_t want = (n_max + kScanTile - 1) / kScanTile;
     const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, kMaxScanBlocks));
     uint32_t* partials = nullptr;
-    FSR_TRY(arena_typed(ctx, A_SCAN_PARTIALS, 2ull * kMaxScanBlocks, &partials));
+    FSR_TRY(arena_typed(ctx, A_SCAN_PARTIALS, 2ull * kMaxCellBlocks, &partials));
     const int vec_ok = ((((uintptr_t)in) | ((uintptr_t)out)) & 15u) == 0 ? 1 : 0;
     hipLaunchKernelGGL(k_scan_reduce<F>, dim3(g), dim3(kScanThreads), 0, s, in, n_max, d_count,
                        partials, vec_ok);
@@ -363,13 +437,28 @@ fsr_status scan_mc(fsr_context* ctx, const uint32_t* nv, uint64_t n_max, const u
     const uint64_t want = (n_max + kScanTile - 1) / kScanTile;
     const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, kMaxScanBlocks));
     uint32_t* partials = nullptr;
-    FSR_TRY(arena_typed(ctx, A_SCAN_PARTIALS, 2ull * kMaxScanBlocks, &partials));
+    FSR_TRY(arena_typed(ctx, A_SCAN_PARTIALS, 2ull * kMaxCellBlocks, &partials));
     hipLaunchKernelGGL(k_mc_scan_reduce, dim3(g), dim3(kScanThreads), 0, s, nv, n_max, d_count,
                        partials);
     FSR_LAUNCH_CHECK("k_mc_scan_reduce");
     hipLaunchKernelGGL(k_mc_scan_down, dim3(g), dim3(kScanThreads), 0, s, nv, n_max, d_count,
                        partials, vert_off, valid_idx, d_total_verts, d_total_valid);
     FSR_LAUNCH_CHECK("k_mc_scan_down");
+    return FSR_OK;
+}
+
+fsr_status scan_cells(fsr_context* ctx, const uint32_t* cnt, uint64_t n, uint32_t* cell_off,
+                      uint32_t* chunk_off, uint32_t* chunk_cell, uint64_t max_chunks,
+                      uint32_t* d_num_chunks, hipStream_t s) {
+    const uint64_t want = (n + kCellTile - 1) / kCellTile;
+    const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, kMaxCellBlocks));
+    uint32_t* partials = nullptr;
+    FSR_TRY(arena_typed(ctx, A_SCAN_PARTIALS, 2ull * kMaxCellBlocks, &partials));
+    hipLaunchKernelGGL(k_cells_scan_reduce, dim3(g), dim3(kScanThreads), 0, s, cnt, n, partials);
+    FSR_LAUNCH_CHECK("k_cells_scan_reduce");
+    hipLaunchKernelGGL(k_cells_scan_down, dim3(g), dim3(kScanThreads), 0, s, cnt, n, partials,
+                       cell_off, chunk_off, chunk_cell, max_chunks, d_num_chunks);
+    FSR_LAUNCH_CHECK("k_cells_scan_down");
     return FSR_OK;
 }
 
