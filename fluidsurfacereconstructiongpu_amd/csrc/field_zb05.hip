@@ -89,7 +89,7 @@ __device__ __forceinline__ void accumulate(const float4* __restrict__ q, uint32_
     }
 }
 
-__global__ __launch_bounds__(kFieldThreads) void k_field_zb05(FieldArgs a, BandCompactOut bc) {
+__global__ __launch_bounds__(kFieldThreads, 5) void k_field_zb05(FieldArgs a, BandCompactOut bc) {
     __shared__ float4 queue[kFieldThreads / 64][128];
     __shared__ uint32_t row_start[kFieldThreads / 64][9];
     __shared__ uint32_t row_pref[kFieldThreads / 64][10];
