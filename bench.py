@@ -395,7 +395,7 @@ def main():
         b_comp = 8.0 * B + 8.0 * C + 12.0 * n_touched + 4.0 * V
         achieved_tflops = flops / (field_ms * 1e-3) / 1e12
         t_roof = max(b_comp / HBM_PEAK, flops / FP32_PEAK)
-        roofline = {"bound": "valu", "kernel": "k_field_zb05", "achieved": achieved_tflops,
+        roofline = {"bound": "mfma", "kernel": "k_field_zb05", "achieved": achieved_tflops,
                     "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
                     "frac": achieved_tflops / (FP32_PEAK / 1e12),
                     "traffic": measured_traffic(args),

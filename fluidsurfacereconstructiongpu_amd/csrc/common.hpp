@@ -140,7 +140,7 @@ __device__ __forceinline__ uint32_t next_item_xcd(uint32_t* next8, uint32_t tota
         if (lo < hi) {
             uint32_t idx = 0;
             if (__lane_id() == 0) idx = atomicAdd(&next8[32u * k], 1u);
-            idx = __shfl(idx, 0, 64);
+            idx = __builtin_amdgcn_readfirstlane(idx);
             if (lo + idx < hi) return lo + idx;
         }
         ++step;

@@ -12,29 +12,31 @@
 // MI355X mapping.  A wave owns a chunk of up to 64 band vertices of one hash
 // cell (the cell-major list built in band.hip), so all lanes share one
 // candidate block: the 9 x-runs of the 3^3 block (x-adjacent cells are
-// contiguous in the sorted particle array).  The wave stages 64 candidates at a
-// time in LDS (one coalesced 16-byte load per lane) and every lane then reads
-// them as broadcasts, so the inner loop is uniform, branch-free and has no
-// per-lane memory traffic:
-//   t = 1 - |x - p|^2/R^2 = c_x + a_x . p' - q_p      (4 VALU ops)
+// contiguous in the sorted particle array).  The wave stages candidates in LDS
+// (one coalesced 16-byte load per lane, one candidate per lane) and evaluates
+// them against the chunk's vertices in 16 x 16 tiles on the matrix cores:
+//   t = 1 - |x - p|^2/R^2 = c_x + a_x . p' - q_p
 // with p' = p - o and x' = x - o relative to the cell corner o, a = 2 x'/R^2,
-// c_x = 1 - |x'|^2/R^2 and q_p = |p'|^2/R^2 precomputed at staging; then
-// w = max(t, 0)^3 and the sums of w and w p' (7 ops): 11 VALU ops per pair.
-// Coordinates relative to the cell keep the expansion well conditioned and
-// make |x - xbar| free of the cancellation the absolute-coordinate sum has.
-// Candidates are visited in the reference's order (z, y, x cells, then range
-// order).  Work is distributed dynamically, one chunk per atomic fetch, each XCD
-// starting on its own contiguous eighth of the chunk list.
+// c_x = 1 - |x'|^2/R^2 and q_p = |p'|^2/R^2, is a K = 4 product (one
+// v_mfma_f32_16x16x4_f32 per tile), w = max(t, 0)^3 takes 3 VALU ops per
+// pair, and the sums of w and w p' are rank-1 updates of 4 x 4 blocks
+// (v_mfma_f32_4x4x1_16b_f32), so a pair costs 3 VALU ops where the scalar
+// form needs 11.  Coordinates relative to the cell keep the expansion well
+// conditioned and make |x - xbar| free of the cancellation the
+// absolute-coordinate sum has.  Candidates are staged in the reference's
+// order (z, y, x cells, then range order); each vertex's sums are formed as
+// four interleaved partial sums that are added at the end.  Work is
+// distributed dynamically, one chunk per atomic fetch, each XCD starting on
+// its own contiguous eighth of the chunk list.
 //
 // Culling: a chunk's vertices are a compact slab of its cell (the cell-major
 // list is in ascending vertex id), so many block candidates lie farther than R
 // from the chunk's bounding box.  At staging every candidate is tested against
 // that box with R^2 (1 + 1e-4); survivors are compacted in order (ballot rank)
-// into a 128-entry LDS queue and the inner loop runs over full groups of 64.
-// A culled candidate is at least 1.00005 R from every vertex of the chunk, so
-// its computed t is negative (the rounding of t is below 1e-5 there) and its
-// weight exactly 0: the sums, and so phi, are bit-identical to evaluating the
-// whole block, in the same order.
+// into a 128-entry LDS queue that is drained 64 at a time.  A culled candidate
+// is at least 1.00005 R from every vertex of the chunk, so its computed t is
+// negative (the rounding of t is below 1e-5 there) and its weight exactly 0:
+// culling leaves every sum unchanged.
 #include "common.hpp"
 
 namespace fsr {
@@ -53,53 +55,177 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ float wave_min(float v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v = fminf(v, __shfl_xor(v, d, 64));
+// Lane exchange i <-> i ^ m within each half-wave (m < 32), with no address
+// register: ds_swizzle in bit mode (and-mask 31, xor-mask m).
+template <int M>
+__device__ __forceinline__ float swz_xor(float v) {
+    return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x1F | (M << 10)));
+}
+
+// Minimum / maximum over the wave, returned in a scalar register.
+__device__ __forceinline__ float wave_min_u(float v) {
+    v = fminf(v, swz_xor<1>(v));
+    v = fminf(v, swz_xor<2>(v));
+    v = fminf(v, swz_xor<4>(v));
+    v = fminf(v, swz_xor<8>(v));
+    v = fminf(v, swz_xor<16>(v));
+    return fminf(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0)),
+                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32)));
+}
+
+__device__ __forceinline__ float wave_max_u(float v) {
+    v = fmaxf(v, swz_xor<1>(v));
+    v = fmaxf(v, swz_xor<2>(v));
+    v = fmaxf(v, swz_xor<4>(v));
+    v = fmaxf(v, swz_xor<8>(v));
+    v = fmaxf(v, swz_xor<16>(v));
+    return fmaxf(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0)),
+                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32)));
+}
+
+// Inclusive sum within each row of 16 lanes (DPP row shifts; lanes whose
+// source lies before the row start add 0).
+__device__ __forceinline__ uint32_t row16_inclusive_sum(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, true);
     return v;
 }
 
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
-    return v;
+// x, y, z of a sorted particle (one 12-byte load)
+__device__ __forceinline__ float3 load_xyz(const float4* p) {
+    return *reinterpret_cast<const float3*>(p);
 }
 
-struct Sums {
-    float w, x, y, z;
+// Division by a launch-constant divisor d with a multiply-high: the
+// round-up method of Granlund and Montgomery (1994), exact for every 32-bit
+// dividend.  m = floor(2^32 (2^l - d) / d) + 1 with l = ceil(log2 d).
+struct FastDiv {
+    uint32_t m, s1, s2;
 };
 
-// n staged candidates (a multiple of 16) against this lane's vertex.
-__device__ __forceinline__ void accumulate(const float4* __restrict__ q, uint32_t n, float ax,
-                                           float ay, float az, float c0, Sums& s) {
-    for (uint32_t t0 = 0; t0 < n; t0 += 16u) {
-#pragma unroll 8
-        for (int t = 0; t < 16; ++t) {
-            const float4 p = q[t0 + t];
-            float tt = __builtin_fmaf(az, p.z, c0 - p.w);
-            tt = __builtin_fmaf(ay, p.y, tt);
-            tt = __builtin_fmaf(ax, p.x, tt);
-            const float w = fmaxf(tt, 0.0f);
-            const float w3 = w * w * w;
-            s.w += w3;
-            s.x = __builtin_fmaf(w3, p.x, s.x);
-            s.y = __builtin_fmaf(w3, p.y, s.y);
-            s.z = __builtin_fmaf(w3, p.z, s.z);
+FastDiv make_fast_div(uint32_t d) {
+    const uint32_t l = d <= 1u ? 0u : 32u - (uint32_t)__builtin_clz(d - 1u);
+    FastDiv f;
+    f.m = (uint32_t)((((uint64_t)1 << l) - d) * ((uint64_t)1 << 32) / d + 1u);
+    f.s1 = l < 1u ? l : 1u;
+    f.s2 = l > 0u ? l - 1u : 0u;
+    return f;
+}
+
+__device__ __forceinline__ uint32_t fast_div(uint32_t n, const FastDiv& f) {
+    const uint32_t t = __umulhi(n, f.m);
+    return (t + ((n - t) >> f.s1)) >> f.s2;
+}
+
+// Divisors of the vertex id: x extent and xy extent of the scalar grid.
+struct VertexDiv {
+    FastDiv rx, rxy;
+};
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// LDS queue of one wave, structure of arrays: rows x', y', z', a row of ones
+// and a row of -q, kQueue entries each.  The first four rows start 144 floats
+// apart, so the 16-entry slices that one matrix operand reads from them fall
+// into distinct LDS banks.
+constexpr int kQueue = 128;
+constexpr int kRowStride = 144;
+constexpr int kRowX = 0, kRowY = 144, kRowZ = 288, kRowOne = 432, kRowQ = 576;
+constexpr int kQueueFloats = 704;
+
+// Sums of w and w p' for the NT 16-vertex tiles of one chunk over n staged
+// candidates (n a multiple of 16), on the matrix cores.  Per 16 candidates and
+// vertex tile:
+//   1. one v_mfma_f32_16x16x4_f32 gives t = c_x + a_x . p' - q_p for the
+//      16 x 16 pairs: A = (x', y', z', 1) of the candidates (rows),
+//      B = (a_x, a_y, a_z, c_x) of the vertices (columns), C = -q_p;
+//      lane l holds t of candidates 4(l/16)+r (r = 0..3) for vertex l%16;
+//   2. w = max(t, 0)^3 (3 VALU ops per value);
+//   3. four v_mfma_f32_4x4x1_16b_f32 (one per r) accumulate, in each 4 x 4
+//      block b = l/4, the outer product of (x', y', z', 1) of candidate
+//      4(b/4)+r with the weights of vertices 4(b%4)..+3, so lane l collects
+//      sum w p' (x, y, z) and sum w for vertex l%16 over the candidates
+//      congruent to l/16 in each group of 16.
+// The caller adds the four partial sums of a vertex (lanes l%16 + 16m).
+template <int NT>
+__device__ __forceinline__ void accumulate(const float* __restrict__ rows, uint32_t n,
+                                           const float (&b1)[4], v4f (&acc)[4]) {
+    const uint32_t lane = __lane_id();
+    const float* a1p = rows + (lane >> 4) * kRowStride + (lane & 15u);
+    const float* a2p = rows + (lane & 3u) * kRowStride + 4u * (lane >> 4);
+    const float* cqp = rows + kRowQ + 4u * (lane >> 4);
+#pragma unroll 2
+    for (uint32_t tb = 0; tb < n; tb += 16u) {
+        const float a1 = a1p[tb];
+        const float4 a2 = *reinterpret_cast<const float4*>(a2p + tb);
+        const float4 cq = *reinterpret_cast<const float4*>(cqp + tb);
+        const v4f c1 = v4f{cq.x, cq.y, cq.z, cq.w};
+        // the tiles' t products back to back, then the weights, then the
+        // rank-1 updates (independent across tiles, so no chain stalls)
+        v4f w[NT];
+#pragma unroll
+        for (int T = 0; T < NT; ++T) w[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1[T], c1, 0, 0, 0);
+#pragma unroll
+        for (int T = 0; T < NT; ++T) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                // max(t, 0) on the bit pattern: negative floats are negative integers
+                const float u = __int_as_float(max(__float_as_int(w[T][r]), 0));
+                w[T][r] = u * u * u;
+            }
         }
+#pragma unroll
+        for (int T = 0; T < NT; ++T) acc[T] = __builtin_amdgcn_mfma_f32_4x4x1f32(a2.x, w[T][0], acc[T], 0, 0, 0);
+#pragma unroll
+        for (int T = 0; T < NT; ++T) acc[T] = __builtin_amdgcn_mfma_f32_4x4x1f32(a2.y, w[T][1], acc[T], 0, 0, 0);
+#pragma unroll
+        for (int T = 0; T < NT; ++T) acc[T] = __builtin_amdgcn_mfma_f32_4x4x1f32(a2.z, w[T][2], acc[T], 0, 0, 0);
+#pragma unroll
+        for (int T = 0; T < NT; ++T) acc[T] = __builtin_amdgcn_mfma_f32_4x4x1f32(a2.w, w[T][3], acc[T], 0, 0, 0);
+        // keep that order in the schedule: NT matrix ops, 12 NT VALU ops, 4 NT
+        // matrix ops (VALU and matrix ops of one SIMD do not overlap, so
+        // fewer switches between them pay)
+        __builtin_amdgcn_sched_group_barrier(0x008, NT, 0);
+        __builtin_amdgcn_sched_group_barrier(0x002, 12 * NT, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 4 * NT, 0);
     }
 }
 
-__global__ __launch_bounds__(kFieldThreads, 5) void k_field_zb05(FieldArgs a, BandCompactOut bc) {
-    __shared__ float4 queue[kFieldThreads / 64][128];
-    __shared__ uint32_t row_start[kFieldThreads / 64][9];
-    __shared__ uint32_t row_pref[kFieldThreads / 64][10];
+__device__ __forceinline__ void accumulate_tiles(const float* rows, uint32_t n, int nt,
+                                                 const float (&b1)[4], v4f (&acc)[4]) {
+    switch (nt) {
+        case 1: accumulate<1>(rows, n, b1, acc); break;
+        case 2: accumulate<2>(rows, n, b1, acc); break;
+        case 3: accumulate<3>(rows, n, b1, acc); break;
+        default: accumulate<4>(rows, n, b1, acc); break;
+    }
+}
+
+__global__ __launch_bounds__(kFieldThreads, 5) void k_field_zb05(FieldArgs a, BandCompactOut bc,
+                                                              VertexDiv vd) {
+    // one LDS block per wave, so every per-lane LDS address is one base
+    // register plus an immediate offset
+    struct WaveLds {
+        float queue[kQueueFloats];
+        // per lane: x', y', z' of its vertex (for the end) and (a_x, a_y, a_z, c_x)
+        float vtx[7][64];
+        uint32_t row_start[12];
+        uint32_t row_pref[12];
+    };
+    __shared__ __align__(16) WaveLds lds[kFieldThreads / 64];
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t num_chunks = a.counters->num_chunks;
-    const float inv_r2 = 1.0f / a.radius_sq;
+    const float inv_r2 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(1.0f / a.radius_sq)));
     const float cull_r2 = a.radius_sq * 1.0001f;
     const int r = bc.radius;
-    float4* q = queue[wave];
+    uint32_t* row_start = lds[wave].row_start;
+    uint32_t* row_pref = lds[wave].row_pref;
+    float* q = lds[wave].queue;
+    q[kRowOne + lane] = 1.0f;
+    q[kRowOne + 64 + lane] = 1.0f;
 
     uint32_t step = 0;
     for (;;) {
@@ -135,11 +261,12 @@ __global__ __launch_bounds__(kFieldThreads, 5) void k_field_zb05(FieldArgs a, Ba
             }
             if (s != 0xFFFFFFFFu) { rs = s; len = e - s; }
         }
-        const uint32_t incl = wave_inclusive_sum(len);
-        const uint32_t total = __shfl(incl, 63, 64);
+        // nrows <= 9: the prefix over the first 16 lanes is the whole prefix
+        const uint32_t incl = row16_inclusive_sum(len);
+        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 15);
         if ((int)lane < nrows) {
-            row_start[wave][lane] = rs;
-            row_pref[wave][lane] = incl - len;
+            row_start[lane] = rs;
+            row_pref[lane] = incl - len;
         }
         wave_lds_sync();
 
@@ -152,78 +279,134 @@ __global__ __launch_bounds__(kFieldThreads, 5) void k_field_zb05(FieldArgs a, Ba
         float vx = 0.f, vy = 0.f, vz = 0.f;
         if (lane < cnt) {
             vid = bc.cell_band[first + lane];
-            const uint32_t ix = vid % a.sc.rx;
-            const uint32_t iy = (vid / a.sc.rx) % a.sc.ry;
-            const uint32_t iz = vid / (a.sc.rx * a.sc.ry);
+            const uint32_t iz = fast_div(vid, vd.rxy);
+            const uint32_t rem = vid - iz * (a.sc.rx * a.sc.ry);
+            const uint32_t iy = fast_div(rem, vd.rx);
+            const uint32_t ix = rem - iy * a.sc.rx;
             vx = vertex_coord(ix, a.sc.minx, a.sc.cell) - ox;
             vy = vertex_coord(iy, a.sc.miny, a.sc.cell) - oy;
             vz = vertex_coord(iz, a.sc.minz, a.sc.cell) - oz;
         }
-        const float ax = 2.0f * vx * inv_r2, ay = 2.0f * vy * inv_r2, az = 2.0f * vz * inv_r2;
-        const float c0 = 1.0f - __builtin_fmaf(vz, vz, __builtin_fmaf(vy, vy, vx * vx)) * inv_r2;
+        float* vt = &lds[wave].vtx[0][0];
+        vt[lane] = vx;
+        vt[64 + lane] = vy;
+        vt[128 + lane] = vz;
+        vt[192 + lane] = 2.0f * vx * inv_r2;
+        vt[256 + lane] = 2.0f * vy * inv_r2;
+        vt[320 + lane] = 2.0f * vz * inv_r2;
+        vt[384 + lane] = 1.0f - __builtin_fmaf(vz, vz, __builtin_fmaf(vy, vy, vx * vx)) * inv_r2;
         // bounding box of the chunk's vertices
         const bool live = lane < cnt;
-        const float bx0 = wave_min(live ? vx : 3.0e38f), bx1 = wave_max(live ? vx : -3.0e38f);
-        const float by0 = wave_min(live ? vy : 3.0e38f), by1 = wave_max(live ? vy : -3.0e38f);
-        const float bz0 = wave_min(live ? vz : 3.0e38f), bz1 = wave_max(live ? vz : -3.0e38f);
+        const float bx0 = wave_min_u(live ? vx : 3.0e38f), bx1 = wave_max_u(live ? vx : -3.0e38f);
+        const float by0 = wave_min_u(live ? vy : 3.0e38f), by1 = wave_max_u(live ? vy : -3.0e38f);
+        const float bz0 = wave_min_u(live ? vz : 3.0e38f), bz1 = wave_max_u(live ? vz : -3.0e38f);
+        wave_lds_sync();
+        // matrix operand of the vertex tiles: lane l carries component l/16 of
+        // (a_x, a_y, a_z, c_x) of vertex 16T + l%16
+        const int nt = (int)((cnt + 15u) >> 4);
+        float b1[4];
+        v4f acc[4];
+        {
+            const float* bsrc = vt + 192 + 64 * (lane >> 4) + (lane & 15u);
+#pragma unroll
+            for (int T = 0; T < 4; ++T) {
+                b1[T] = bsrc[16 * T];
+                acc[T] = v4f{0.f, 0.f, 0.f, 0.f};
+            }
+        }
 
-        Sums sum = {0.f, 0.f, 0.f, 0.f};
         uint32_t fill = 0;
         // this lane's candidate of the next round is loaded one round ahead, so
         // the load overlaps the culling and the accumulation of the current one
         int j = 0;  // row of this lane's candidate; g only grows, so the search resumes
-        float4 next = make_float4(0.f, 0.f, 0.f, 0.f);
+        float3 next = make_float3(0.f, 0.f, 0.f);
         if (lane < total) {
-            while (j + 1 < nrows && row_pref[wave][j + 1] <= lane) ++j;
-            next = a.sorted[row_start[wave][j] + (lane - row_pref[wave][j])];
+            while (j + 1 < nrows && row_pref[j + 1] <= lane) ++j;
+            next = load_xyz(a.sorted + row_start[j] + (lane - row_pref[j]));
         }
         for (uint32_t base = 0; base < total; base += 64u) {
             const uint32_t g = base + lane;
-            const float4 p = next;
+            const float3 p = next;
             if (g + 64u < total) {
-                while (j + 1 < nrows && row_pref[wave][j + 1] <= g + 64u) ++j;
-                next = a.sorted[row_start[wave][j] + (g + 64u - row_pref[wave][j])];
+                while (j + 1 < nrows && row_pref[j + 1] <= g + 64u) ++j;
+                next = load_xyz(a.sorted + row_start[j] + (g + 64u - row_pref[j]));
             }
             bool keep = false;
-            float4 st;
+            float px = 0.f, py = 0.f, pz = 0.f;
             if (g < total) {
-                const float px = p.x - ox, py = p.y - oy, pz = p.z - oz;
+                px = p.x - ox;
+                py = p.y - oy;
+                pz = p.z - oz;
                 const float dx = fmaxf(fmaxf(bx0 - px, px - bx1), 0.0f);
                 const float dy = fmaxf(fmaxf(by0 - py, py - by1), 0.0f);
                 const float dz = fmaxf(fmaxf(bz0 - pz, pz - bz1), 0.0f);
                 keep = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < cull_r2;
-                st = make_float4(px, py, pz,
-                                 __builtin_fmaf(pz, pz, __builtin_fmaf(py, py, px * px)) * inv_r2);
             }
             const uint64_t m = __ballot(keep);
-            if (keep) q[fill + mask_rank(m)] = st;
+            if (keep) {
+                const uint32_t slot = fill + mask_rank(m);
+                q[kRowX + slot] = px;
+                q[kRowY + slot] = py;
+                q[kRowZ + slot] = pz;
+                q[kRowQ + slot] = -(__builtin_fmaf(pz, pz, __builtin_fmaf(py, py, px * px)) * inv_r2);
+            }
             fill += (uint32_t)__popcll(m);
             if (fill >= 64u) {
                 wave_lds_sync();
-                accumulate(q, 64u, ax, ay, az, c0, sum);
+                accumulate_tiles(q, 64u, nt, b1, acc);
                 wave_lds_sync();
                 const uint32_t rem = fill - 64u;
-                if (lane < rem) q[lane] = q[64u + lane];
+                if (lane < rem) {
+                    q[kRowX + lane] = q[kRowX + 64u + lane];
+                    q[kRowY + lane] = q[kRowY + 64u + lane];
+                    q[kRowZ + lane] = q[kRowZ + 64u + lane];
+                    q[kRowQ + lane] = q[kRowQ + 64u + lane];
+                }
                 wave_lds_sync();
                 fill = rem;
             }
         }
         if (fill) {
+            // pad to a multiple of 16 with entries of weight 0 (t < 0 for every vertex)
             const uint32_t n = (fill + 15u) & ~15u;
-            if (lane >= fill && lane < n) q[lane] = make_float4(0.f, 0.f, 0.f, 1.0e30f);
+            if (lane >= fill && lane < n) {
+                q[kRowX + lane] = 0.f;
+                q[kRowY + lane] = 0.f;
+                q[kRowZ + lane] = 0.f;
+                q[kRowQ + lane] = -1.0e30f;
+            }
             wave_lds_sync();
-            accumulate(q, n, ax, ay, az, c0, sum);
+            accumulate_tiles(q, n, nt, b1, acc);
             wave_lds_sync();
         }
+        // add the four partial sums of each vertex; lane l then holds the sums
+        // of vertex 16T + l%16 in acc[T]
+#pragma unroll
+        for (int T = 0; T < 4; ++T) {
+            if (T < nt) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    float x = acc[T][c];
+                    x += swz_xor<16>(x);
+                    x += __shfl_xor(x, 32, 64);
+                    acc[T][c] = x;
+                }
+            }
+        }
+        const uint32_t my = lane >> 4;
+        const v4f s4 = my == 0 ? acc[0] : (my == 1 ? acc[1] : (my == 2 ? acc[2] : acc[3]));
         if (lane < cnt) {
             float4 keep_out = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (sum.w > 0.0f) {
-                const float inv = 1.0f / sum.w;
-                const float dx = vx - sum.x * inv, dy = vy - sum.y * inv, dz = vz - sum.z * inv;
+            const float sw = s4[3];
+            if (sw > 0.0f) {
+                const float inv = 1.0f / sw;
+                const float dx = vt[lane] - s4[0] * inv;
+                const float dy = vt[64 + lane] - s4[1] * inv;
+                const float dz = vt[128 + lane] - s4[2] * inv;
                 a.phi[vid] =
                     __builtin_sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx))) -
                     a.particle_radius;
-                keep_out = make_float4(dx, dy, dz, sum.w);
+                keep_out = make_float4(dx, dy, dz, sw);
             }
             // for the adjoint: x - xbar and sum w, at the vertex's cell-major slot
             if (a.saved) a.saved[first + lane] = keep_out;
@@ -237,7 +420,10 @@ fsr_status field_zb05(fsr_context* ctx, const FieldArgs& a, hipStream_t s) {
     // a.counters was cleared when the frame (or the stage call) began, and no
     // other kernel fetches through xcd_field
     const int blocks = persistent_blocks(ctx, kFieldThreads, 8, TUNE_FIELD);
-    hipLaunchKernelGGL(k_field_zb05, dim3(blocks), dim3(kFieldThreads), 0, s, a, *a.bc);
+    VertexDiv vd;
+    vd.rx = make_fast_div(a.sc.rx);
+    vd.rxy = make_fast_div(a.sc.rx * a.sc.ry);
+    hipLaunchKernelGGL(k_field_zb05, dim3(blocks), dim3(kFieldThreads), 0, s, a, *a.bc, vd);
     FSR_LAUNCH_CHECK("k_field_zb05");
     return FSR_OK;
 }
