@@ -37,6 +37,8 @@
 // is at least 1.00005 R from every vertex of the chunk, so its computed t is
 // negative (the rounding of t is below 1e-5 there) and its weight exactly 0:
 // culling leaves every sum unchanged.
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace fsr {
@@ -151,7 +153,7 @@ constexpr int kQueueFloats = 704;
 // The caller adds the four partial sums of a vertex (lanes l%16 + 16m).
 template <int NT>
 __device__ __forceinline__ void accumulate(const float* __restrict__ rows, uint32_t n,
-                                           const float (&b1)[4], v4f (&acc)[4]) {
+                                           const float (&b1)[NT], v4f (&acc)[NT]) {
     const uint32_t lane = __lane_id();
     const float* a1p = rows + (lane >> 4) * kRowStride + (lane & 15u);
     const float* a2p = rows + (lane & 3u) * kRowStride + 4u * (lane >> 4);
@@ -190,16 +192,6 @@ __device__ __forceinline__ void accumulate(const float* __restrict__ rows, uint3
         __builtin_amdgcn_sched_group_barrier(0x008, NT, 0);
         __builtin_amdgcn_sched_group_barrier(0x002, 12 * NT, 0);
         __builtin_amdgcn_sched_group_barrier(0x008, 4 * NT, 0);
-    }
-}
-
-__device__ __forceinline__ void accumulate_tiles(const float* rows, uint32_t n, int nt,
-                                                 const float (&b1)[4], v4f (&acc)[4]) {
-    switch (nt) {
-        case 1: accumulate<1>(rows, n, b1, acc); break;
-        case 2: accumulate<2>(rows, n, b1, acc); break;
-        case 3: accumulate<3>(rows, n, b1, acc); break;
-        default: accumulate<4>(rows, n, b1, acc); break;
     }
 }
 
@@ -301,89 +293,89 @@ __global__ __launch_bounds__(kFieldThreads, 5) void k_field_zb05(FieldArgs a, Ba
         const float by0 = wave_min_u(live ? vy : 3.0e38f), by1 = wave_max_u(live ? vy : -3.0e38f);
         const float bz0 = wave_min_u(live ? vz : 3.0e38f), bz1 = wave_max_u(live ? vz : -3.0e38f);
         wave_lds_sync();
-        // matrix operand of the vertex tiles: lane l carries component l/16 of
-        // (a_x, a_y, a_z, c_x) of vertex 16T + l%16
         const int nt = (int)((cnt + 15u) >> 4);
-        float b1[4];
-        v4f acc[4];
-        {
+        // The rest of the chunk, compiled once per tile count NT, so that the
+        // accumulators keep fixed registers across the staging loop.
+        auto run = [&](auto ntc) {
+            constexpr int NT = decltype(ntc)::value;
+            // matrix operand of the vertex tiles: lane l carries component l/16
+            // of (a_x, a_y, a_z, c_x) of vertex 16T + l%16
+            float b1[NT];
+            v4f acc[NT];
             const float* bsrc = vt + 192 + 64 * (lane >> 4) + (lane & 15u);
 #pragma unroll
-            for (int T = 0; T < 4; ++T) {
+            for (int T = 0; T < NT; ++T) {
                 b1[T] = bsrc[16 * T];
                 acc[T] = v4f{0.f, 0.f, 0.f, 0.f};
             }
-        }
-
-        uint32_t fill = 0;
-        // this lane's candidate of the next round is loaded one round ahead, so
-        // the load overlaps the culling and the accumulation of the current one
-        int j = 0;  // row of this lane's candidate; g only grows, so the search resumes
-        float3 next = make_float3(0.f, 0.f, 0.f);
-        if (lane < total) {
-            while (j + 1 < nrows && row_pref[j + 1] <= lane) ++j;
-            next = load_xyz(a.sorted + row_start[j] + (lane - row_pref[j]));
-        }
-        for (uint32_t base = 0; base < total; base += 64u) {
-            const uint32_t g = base + lane;
-            const float3 p = next;
-            if (g + 64u < total) {
-                while (j + 1 < nrows && row_pref[j + 1] <= g + 64u) ++j;
-                next = load_xyz(a.sorted + row_start[j] + (g + 64u - row_pref[j]));
+            // the queue is a ring of kQueue entries; entries [head, head + fill)
+            // wait for evaluation, and head is 0 or 64
+            uint32_t head = 0, fill = 0;
+            // this lane's candidate of the next round is loaded one round ahead,
+            // so the load overlaps the culling and the accumulation of the
+            // current one
+            int j = 0;  // row of this lane's candidate; g only grows, so the search resumes
+            float3 next = make_float3(0.f, 0.f, 0.f);
+            if (lane < total) {
+                while (j + 1 < nrows && row_pref[j + 1] <= lane) ++j;
+                next = load_xyz(a.sorted + row_start[j] + (lane - row_pref[j]));
             }
-            bool keep = false;
-            float px = 0.f, py = 0.f, pz = 0.f;
-            if (g < total) {
-                px = p.x - ox;
-                py = p.y - oy;
-                pz = p.z - oz;
-                const float dx = fmaxf(fmaxf(bx0 - px, px - bx1), 0.0f);
-                const float dy = fmaxf(fmaxf(by0 - py, py - by1), 0.0f);
-                const float dz = fmaxf(fmaxf(bz0 - pz, pz - bz1), 0.0f);
-                keep = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < cull_r2;
+            for (uint32_t base = 0; base < total; base += 64u) {
+                const uint32_t g = base + lane;
+                const float3 p = next;
+                if (g + 64u < total) {
+                    while (j + 1 < nrows && row_pref[j + 1] <= g + 64u) ++j;
+                    next = load_xyz(a.sorted + row_start[j] + (g + 64u - row_pref[j]));
+                }
+                bool keep = false;
+                float px = 0.f, py = 0.f, pz = 0.f;
+                if (g < total) {
+                    px = p.x - ox;
+                    py = p.y - oy;
+                    pz = p.z - oz;
+                    const float dx = fmaxf(fmaxf(bx0 - px, px - bx1), 0.0f);
+                    const float dy = fmaxf(fmaxf(by0 - py, py - by1), 0.0f);
+                    const float dz = fmaxf(fmaxf(bz0 - pz, pz - bz1), 0.0f);
+                    keep = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < cull_r2;
+                }
+                const uint64_t m = __ballot(keep);
+                if (keep) {
+                    const uint32_t slot = (head + fill + mask_rank(m)) & (kQueue - 1);
+                    q[kRowX + slot] = px;
+                    q[kRowY + slot] = py;
+                    q[kRowZ + slot] = pz;
+                    q[kRowQ + slot] =
+                        -(__builtin_fmaf(pz, pz, __builtin_fmaf(py, py, px * px)) * inv_r2);
+                }
+                fill += (uint32_t)__popcll(m);
+                if (fill >= 64u) {
+                    wave_lds_sync();
+                    accumulate<NT>(q + head, 64u, b1, acc);
+                    wave_lds_sync();
+                    head ^= 64u;
+                    fill -= 64u;
+                }
             }
-            const uint64_t m = __ballot(keep);
-            if (keep) {
-                const uint32_t slot = fill + mask_rank(m);
-                q[kRowX + slot] = px;
-                q[kRowY + slot] = py;
-                q[kRowZ + slot] = pz;
-                q[kRowQ + slot] = -(__builtin_fmaf(pz, pz, __builtin_fmaf(py, py, px * px)) * inv_r2);
-            }
-            fill += (uint32_t)__popcll(m);
-            if (fill >= 64u) {
-                wave_lds_sync();
-                accumulate_tiles(q, 64u, nt, b1, acc);
-                wave_lds_sync();
-                const uint32_t rem = fill - 64u;
-                if (lane < rem) {
-                    q[kRowX + lane] = q[kRowX + 64u + lane];
-                    q[kRowY + lane] = q[kRowY + 64u + lane];
-                    q[kRowZ + lane] = q[kRowZ + 64u + lane];
-                    q[kRowQ + lane] = q[kRowQ + 64u + lane];
+            if (fill) {
+                // pad to a multiple of 16 with entries of weight 0 (t < 0 for
+                // every vertex); head + 64 <= kQueue, so the pad does not wrap
+                const uint32_t n = (fill + 15u) & ~15u;
+                if (lane >= fill && lane < n) {
+                    q[kRowX + head + lane] = 0.f;
+                    q[kRowY + head + lane] = 0.f;
+                    q[kRowZ + head + lane] = 0.f;
+                    q[kRowQ + head + lane] = -1.0e30f;
                 }
                 wave_lds_sync();
-                fill = rem;
+                accumulate<NT>(q + head, n, b1, acc);
+                wave_lds_sync();
             }
-        }
-        if (fill) {
-            // pad to a multiple of 16 with entries of weight 0 (t < 0 for every vertex)
-            const uint32_t n = (fill + 15u) & ~15u;
-            if (lane >= fill && lane < n) {
-                q[kRowX + lane] = 0.f;
-                q[kRowY + lane] = 0.f;
-                q[kRowZ + lane] = 0.f;
-                q[kRowQ + lane] = -1.0e30f;
-            }
-            wave_lds_sync();
-            accumulate_tiles(q, n, nt, b1, acc);
-            wave_lds_sync();
-        }
-        // add the four partial sums of each vertex; lane l then holds the sums
-        // of vertex 16T + l%16 in acc[T]
+            // add the four partial sums of each vertex; lane l then holds the
+            // sums of vertex 16T + l%16 in acc[T]
+            v4f s4 = acc[0];
+            const uint32_t my = lane >> 4;
 #pragma unroll
-        for (int T = 0; T < 4; ++T) {
-            if (T < nt) {
+            for (int T = 0; T < NT; ++T) {
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
                     float x = acc[T][c];
@@ -391,25 +383,30 @@ __global__ __launch_bounds__(kFieldThreads, 5) void k_field_zb05(FieldArgs a, Ba
                     x += __shfl_xor(x, 32, 64);
                     acc[T][c] = x;
                 }
+                if (my == (uint32_t)T) s4 = acc[T];
             }
-        }
-        const uint32_t my = lane >> 4;
-        const v4f s4 = my == 0 ? acc[0] : (my == 1 ? acc[1] : (my == 2 ? acc[2] : acc[3]));
-        if (lane < cnt) {
-            float4 keep_out = make_float4(0.f, 0.f, 0.f, 0.f);
-            const float sw = s4[3];
-            if (sw > 0.0f) {
-                const float inv = 1.0f / sw;
-                const float dx = vt[lane] - s4[0] * inv;
-                const float dy = vt[64 + lane] - s4[1] * inv;
-                const float dz = vt[128 + lane] - s4[2] * inv;
-                a.phi[vid] =
-                    __builtin_sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx))) -
-                    a.particle_radius;
-                keep_out = make_float4(dx, dy, dz, sw);
+            if (lane < cnt) {
+                float4 keep_out = make_float4(0.f, 0.f, 0.f, 0.f);
+                const float sw = s4[3];
+                if (sw > 0.0f) {
+                    const float inv = 1.0f / sw;
+                    const float dx = vt[lane] - s4[0] * inv;
+                    const float dy = vt[64 + lane] - s4[1] * inv;
+                    const float dz = vt[128 + lane] - s4[2] * inv;
+                    a.phi[vid] =
+                        __builtin_sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx))) -
+                        a.particle_radius;
+                    keep_out = make_float4(dx, dy, dz, sw);
+                }
+                // for the adjoint: x - xbar and sum w, at the vertex's cell-major slot
+                if (a.saved) a.saved[first + lane] = keep_out;
             }
-            // for the adjoint: x - xbar and sum w, at the vertex's cell-major slot
-            if (a.saved) a.saved[first + lane] = keep_out;
+        };
+        switch (nt) {
+            case 1: run(std::integral_constant<int, 1>{}); break;
+            case 2: run(std::integral_constant<int, 2>{}); break;
+            case 3: run(std::integral_constant<int, 3>{}); break;
+            default: run(std::integral_constant<int, 4>{}); break;
         }
     }
 }
