@@ -69,7 +69,8 @@ __global__ __launch_bounds__(256) void k_mc_classify(McArgs a) {
         const uint32_t x = vid % rx, y = (vid / rx) % ry, z = (uint32_t)(vid / rxy);
         uint32_t nv = 0;
         if (x < rx - 1 && y < ry - 1 && z < rz - 1) {
-            const uint32_t f = cube_flag(a.phi, vid, rx, rxy, a.iso);
+            // the corners' band bits first: the field is read only for cubes
+            // whose 8 corners are band vertices
             bool all = true;
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
@@ -78,7 +79,7 @@ __global__ __launch_bounds__(256) void k_mc_classify(McArgs a) {
                 const uint32_t dz = (c == 1 || c == 2 || c == 5 || c == 6) ? 1u : 0u;
                 all = all && band_bit(a.bits, wx, ry, x + dx, y + dy, z + dz);
             }
-            nv = all ? case_vertices(f) : 0u;
+            if (all) nv = case_vertices(cube_flag(a.phi, vid, rx, rxy, a.iso));
         }
         a.nverts[b] = nv;
         if (a.valid) a.valid[b] = nv ? 1u : 0u;
@@ -91,101 +92,126 @@ __device__ __forceinline__ float sample(const float* __restrict__ phi, int x, in
     return phi[((uint64_t)z * ry + y) * rx + x];
 }
 
+// OR of a value over the 8 lanes of an aligned lane group (ds_swizzle, bit
+// mode, xor masks 1, 2, 4).
+__device__ __forceinline__ uint32_t group8_or(uint32_t v) {
+    v |= (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (1 << 10));
+    v |= (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (2 << 10));
+    v |= (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (4 << 10));
+    return v;
+}
+
+// Eight lanes per valid cube: lane c of the group loads corner c's value and
+// its six central-difference samples and forms the corner position and normal
+// (the arithmetic of the reference's per-cube loop, corner by corner); the
+// group ORs the corner bits into the cube index and exchanges the corner data
+// through LDS; lane c then writes the cube's vertices c and c + 8.
 template <bool kCpu>
 __global__ __launch_bounds__(256) void k_mc_emit(McArgs a) {
+    __shared__ float4 corner[256][2];  // per lane: (phi, x, y, z), (nx, ny, nz, 0)
     const uint32_t Q = a.st->num_valid_cubes;
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.st->num_mesh_vertices > a.capacity)
         a.st->overflow = 1u;
     const uint32_t rx = a.sc.rx, ry = a.sc.ry, rz = a.sc.rz;
     const uint64_t rxy = (uint64_t)rx * ry;
     const float sign = (!kCpu && a.iso < 0.0f) ? -1.0f : 1.0f;
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < Q; q += gridDim.x * blockDim.x) {
+    const uint32_t c = threadIdx.x & 7u;
+    const uint32_t g0 = threadIdx.x & ~7u;
+    const int dx = (c >= 4) ? 1 : 0;
+    const int dy = (c == 2 || c == 3 || c == 6 || c == 7) ? 1 : 0;
+    const int dz = (c == 1 || c == 2 || c == 5 || c == 6) ? 1 : 0;
+    const uint32_t groups = gridDim.x * (blockDim.x / 8u);
+    for (uint32_t q = blockIdx.x * (blockDim.x / 8u) + threadIdx.x / 8u; q < Q; q += groups) {
         const uint32_t b = a.valid_idx[q];
         const uint32_t vid = a.band_index[b];
         const uint32_t off = a.vert_off[b];
         const int x = (int)(vid % rx), y = (int)((vid / rx) % ry), z = (int)(vid / rxy);
-        float val[8], cpx[8], cpy[8], cpz[8], cnx[8], cny[8], cnz[8];
-        uint32_t f = 0;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int dx = (c >= 4) ? 1 : 0;
-            const int dy = (c == 2 || c == 3 || c == 6 || c == 7) ? 1 : 0;
-            const int dz = (c == 1 || c == 2 || c == 5 || c == 6) ? 1 : 0;
-            const int X = x + dx, Y = y + dy, Z = z + dz;
-            val[c] = a.phi[((uint64_t)Z * ry + Y) * rx + X];
-            f |= (val[c] <= a.iso) ? (1u << c) : 0u;
-            if (kCpu) {
-                cpx[c] = a.sc.minx + (float)X * a.sc.cell;
-                cpy[c] = a.sc.miny + (float)Y * a.sc.cell;
-                cpz[c] = a.sc.minz + (float)Z * a.sc.cell;
-            } else {
-                cpx[c] = vertex_coord((uint32_t)X, a.sc.minx, a.sc.cell);
-                cpy[c] = vertex_coord((uint32_t)Y, a.sc.miny, a.sc.cell);
-                cpz[c] = vertex_coord((uint32_t)Z, a.sc.minz, a.sc.cell);
-            }
-            float nx = sample(a.phi, X - 1, Y, Z, rx, ry, rz) - sample(a.phi, X + 1, Y, Z, rx, ry, rz);
-            float ny = sample(a.phi, X, Y - 1, Z, rx, ry, rz) - sample(a.phi, X, Y + 1, Z, rx, ry, rz);
-            float nz = sample(a.phi, X, Y, Z - 1, rx, ry, rz) - sample(a.phi, X, Y, Z + 1, rx, ry, rz);
-            if (kCpu) {
-                const float len = __builtin_sqrtf(nx * nx + ny * ny + nz * nz);
-                if (len > 1.0e-7f) {
-                    const float fac = 1.0f / len;
-                    nx *= fac; ny *= fac; nz *= fac;
-                }
-            } else {
-                const float rs = __builtin_amdgcn_rsqf(
-                    __builtin_fmaf(nz, nz, __builtin_fmaf(ny, ny, nx * nx)));
-                nx *= rs; ny *= rs; nz *= rs;
-            }
-            cnx[c] = nx; cny[c] = ny; cnz[c] = nz;
+        const int X = x + dx, Y = y + dy, Z = z + dz;
+        const float val = a.phi[((uint64_t)Z * ry + Y) * rx + X];
+        float cpx, cpy, cpz;
+        if (kCpu) {
+            cpx = a.sc.minx + (float)X * a.sc.cell;
+            cpy = a.sc.miny + (float)Y * a.sc.cell;
+            cpz = a.sc.minz + (float)Z * a.sc.cell;
+        } else {
+            cpx = vertex_coord((uint32_t)X, a.sc.minx, a.sc.cell);
+            cpy = vertex_coord((uint32_t)Y, a.sc.miny, a.sc.cell);
+            cpz = vertex_coord((uint32_t)Z, a.sc.minz, a.sc.cell);
         }
+        float nx = sample(a.phi, X - 1, Y, Z, rx, ry, rz) - sample(a.phi, X + 1, Y, Z, rx, ry, rz);
+        float ny = sample(a.phi, X, Y - 1, Z, rx, ry, rz) - sample(a.phi, X, Y + 1, Z, rx, ry, rz);
+        float nz = sample(a.phi, X, Y, Z - 1, rx, ry, rz) - sample(a.phi, X, Y, Z + 1, rx, ry, rz);
+        if (kCpu) {
+            const float len = __builtin_sqrtf(nx * nx + ny * ny + nz * nz);
+            if (len > 1.0e-7f) {
+                const float fac = 1.0f / len;
+                nx *= fac; ny *= fac; nz *= fac;
+            }
+        } else {
+            const float rs = __builtin_amdgcn_rsqf(
+                __builtin_fmaf(nz, nz, __builtin_fmaf(ny, ny, nx * nx)));
+            nx *= rs; ny *= rs; nz *= rs;
+        }
+        corner[threadIdx.x][0] = make_float4(val, cpx, cpy, cpz);
+        corner[threadIdx.x][1] = make_float4(nx, ny, nz, 0.0f);
+        const uint32_t f = group8_or((val <= a.iso) ? (1u << c) : 0u);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         const uint32_t nv = case_vertices(f);
-        if ((uint64_t)off + nv > a.capacity) continue;
-        const uint64_t tri = cMcTri[f];
-        for (uint32_t k = 0; k < nv; ++k) {
-            const uint32_t e = (uint32_t)(tri >> (4 * k)) & 0xFu;
-            // corners of edge e
-            const uint32_t s0 = (e < 8) ? (e & 3u) + (e & 4u) : e - 8u;
-            const uint32_t s1 = (e < 8) ? (((e & 3u) + 1u) & 3u) + (e & 4u) : e - 4u;
-            const float v0 = val[s0], v1 = val[s1];
-            const float delta = v1 - v0;
-            float t;
-            if (kCpu) t = ((double)delta > -1.0e-7 && (double)delta < 1.0e-7) ? 0.5f : (a.iso - v0) / delta;
-            else t = (delta > -1.0e-7f && delta < 1.0e-7f) ? 0.5f : (a.iso - v0) / delta;
-            float px, py, pz, nx, ny, nz;
-            if (kCpu) {
-                px = cpx[s0] + (cpx[s1] - cpx[s0]) * t;
-                py = cpy[s0] + (cpy[s1] - cpy[s0]) * t;
-                pz = cpz[s0] + (cpz[s1] - cpz[s0]) * t;
-                nx = cnx[s0] + (cnx[s1] - cnx[s0]) * t;
-                ny = cny[s0] + (cny[s1] - cny[s0]) * t;
-                nz = cnz[s0] + (cnz[s1] - cnz[s0]) * t;
-                const float len = __builtin_sqrtf(nx * nx + ny * ny + nz * nz);
-                if (len > 1.0e-7f) {
-                    const float fac = 1.0f / len;
-                    nx *= fac; ny *= fac; nz *= fac;
+        if ((uint64_t)off + nv <= a.capacity) {
+            const uint64_t tri = cMcTri[f];
+            for (uint32_t k = c; k < nv; k += 8u) {
+                const uint32_t e = (uint32_t)(tri >> (4 * k)) & 0xFu;
+                // corners of edge e
+                const uint32_t s0 = (e < 8) ? (e & 3u) + (e & 4u) : e - 8u;
+                const uint32_t s1 = (e < 8) ? (((e & 3u) + 1u) & 3u) + (e & 4u) : e - 4u;
+                const float4 p0 = corner[g0 + s0][0], p1 = corner[g0 + s1][0];
+                const float4 n0 = corner[g0 + s0][1], n1 = corner[g0 + s1][1];
+                const float v0 = p0.x, v1 = p1.x;
+                const float delta = v1 - v0;
+                float t;
+                if (kCpu) t = ((double)delta > -1.0e-7 && (double)delta < 1.0e-7) ? 0.5f : (a.iso - v0) / delta;
+                else t = (delta > -1.0e-7f && delta < 1.0e-7f) ? 0.5f : (a.iso - v0) / delta;
+                float px, py, pz, mx, my, mz;
+                if (kCpu) {
+                    px = p0.y + (p1.y - p0.y) * t;
+                    py = p0.z + (p1.z - p0.z) * t;
+                    pz = p0.w + (p1.w - p0.w) * t;
+                    mx = n0.x + (n1.x - n0.x) * t;
+                    my = n0.y + (n1.y - n0.y) * t;
+                    mz = n0.z + (n1.z - n0.z) * t;
+                    const float len = __builtin_sqrtf(mx * mx + my * my + mz * mz);
+                    if (len > 1.0e-7f) {
+                        const float fac = 1.0f / len;
+                        mx *= fac; my *= fac; mz *= fac;
+                    }
+                } else {
+                    px = __builtin_fmaf(t, p1.y - p0.y, p0.y);
+                    py = __builtin_fmaf(t, p1.z - p0.z, p0.z);
+                    pz = __builtin_fmaf(t, p1.w - p0.w, p0.w);
+                    mx = __builtin_fmaf(t, n1.x - n0.x, n0.x);
+                    my = __builtin_fmaf(t, n1.y - n0.y, n0.y);
+                    mz = __builtin_fmaf(t, n1.z - n0.z, n0.z);
+                    const float rs = __builtin_amdgcn_rsqf(
+                        __builtin_fmaf(mz, mz, __builtin_fmaf(my, my, mx * mx)));
+                    mx = sign * (mx * rs); my = sign * (my * rs); mz = sign * (mz * rs);
                 }
-            } else {
-                px = __builtin_fmaf(t, cpx[s1] - cpx[s0], cpx[s0]);
-                py = __builtin_fmaf(t, cpy[s1] - cpy[s0], cpy[s0]);
-                pz = __builtin_fmaf(t, cpz[s1] - cpz[s0], cpz[s0]);
-                nx = __builtin_fmaf(t, cnx[s1] - cnx[s0], cnx[s0]);
-                ny = __builtin_fmaf(t, cny[s1] - cny[s0], cny[s0]);
-                nz = __builtin_fmaf(t, cnz[s1] - cnz[s0], cnz[s0]);
-                const float rs = __builtin_amdgcn_rsqf(
-                    __builtin_fmaf(nz, nz, __builtin_fmaf(ny, ny, nx * nx)));
-                nx = sign * (nx * rs); ny = sign * (ny * rs); nz = sign * (nz * rs);
-            }
-            const uint64_t o = 3ull * (off + k);
-            a.tri_pos[o + 0] = px;
-            a.tri_pos[o + 1] = py;
-            a.tri_pos[o + 2] = pz;
-            if (a.tri_nrm) {
-                a.tri_nrm[o + 0] = nx;
-                a.tri_nrm[o + 1] = ny;
-                a.tri_nrm[o + 2] = nz;
+                const uint64_t o = 3ull * (off + k);
+                a.tri_pos[o + 0] = px;
+                a.tri_pos[o + 1] = py;
+                a.tri_pos[o + 2] = pz;
+                if (a.tri_nrm) {
+                    a.tri_nrm[o + 0] = mx;
+                    a.tri_nrm[o + 1] = my;
+                    a.tri_nrm[o + 2] = mz;
+                }
             }
         }
+        // the group's corner slots are rewritten by its next cube
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 

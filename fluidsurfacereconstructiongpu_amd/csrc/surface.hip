@@ -104,6 +104,11 @@ __device__ __forceinline__ bool seed_particle(const SurfaceArgs& a, uint32_t i, 
     return exc;
 }
 
+// Elements per thread and grid-stride step of the per-particle kernels below:
+// their loads are issued together, so one step waits for memory once per
+// dependent load rather than once per element.
+constexpr uint32_t kPerThread = 4;
+
 // Surface flags from the surface hash cells (ZB05 occupancy method), band
 // seeding, and the frame's surface / exception counts.  Grid-stride with a
 // block-uniform trip count over a bounded grid: the two counters take one
@@ -111,20 +116,35 @@ __device__ __forceinline__ bool seed_particle(const SurfaceArgs& a, uint32_t i, 
 __global__ __launch_bounds__(256) void k_flags_from_cells(SurfaceArgs a,
                                                           const uint8_t* __restrict__ surf_cell) {
     const uint32_t C = a.sp.rx * a.sp.ry * a.sp.rz;
+    const uint32_t span = blockDim.x * kPerThread;
     uint32_t n_surf = 0, n_exc = 0;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < a.n; base += gridDim.x * blockDim.x) {
-        const uint32_t i = base + threadIdx.x;
-        bool surf = false;
-        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (i < a.n) {
-            const uint32_t key = a.keys[i];
-            surf = key < C && surf_cell[key] != 0;
-            a.flags[i] = surf ? 1u : 0u;
-            p = a.sorted[i];
+    for (uint32_t base = blockIdx.x * span; base < a.n; base += gridDim.x * span) {
+        uint32_t key[kPerThread];
+        float4 p[kPerThread];
+#pragma unroll
+        for (uint32_t k = 0; k < kPerThread; ++k) {
+            const uint32_t i = base + k * blockDim.x + threadIdx.x;
+            key[k] = C;
+            p[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (i < a.n) {
+                key[k] = a.keys[i];
+                p[k] = a.sorted[i];
+            }
         }
-        const bool exc = a.sbits ? seed_particle(a, i, p.x, p.y, p.z, surf) : false;
-        n_surf += surf ? 1u : 0u;
-        n_exc += exc ? 1u : 0u;
+        bool surf[kPerThread];
+#pragma unroll
+        for (uint32_t k = 0; k < kPerThread; ++k) {
+            const uint32_t i = base + k * blockDim.x + threadIdx.x;
+            surf[k] = key[k] < C && surf_cell[key[k]] != 0;
+            if (i < a.n) a.flags[i] = surf[k] ? 1u : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kPerThread; ++k) {
+            const uint32_t i = base + k * blockDim.x + threadIdx.x;
+            const bool exc = a.sbits ? seed_particle(a, i, p[k].x, p[k].y, p[k].z, surf[k]) : false;
+            n_surf += surf[k] ? 1u : 0u;
+            n_exc += exc ? 1u : 0u;
+        }
     }
     if (a.st) block_count_add2(&a.st->num_surface_particles, n_surf, &a.st->band_exceptions, n_exc);
 }
@@ -363,18 +383,28 @@ __global__ __launch_bounds__(256) void k_colorfield(SurfaceArgs a, float mass, C
 
 // band seeding from given flags (stage entry point of the band)
 __global__ __launch_bounds__(256) void k_seed_flags(SurfaceArgs a) {
+    const uint32_t span = blockDim.x * kPerThread;
     uint32_t n_surf = 0, n_exc = 0;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < a.n; base += gridDim.x * blockDim.x) {
-        const uint32_t i = base + threadIdx.x;
-        bool surf = false;
-        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (i < a.n) {
-            surf = a.flags[i] != 0;
-            p = a.sorted[i];
+    for (uint32_t base = blockIdx.x * span; base < a.n; base += gridDim.x * span) {
+        bool surf[kPerThread];
+        float4 p[kPerThread];
+#pragma unroll
+        for (uint32_t k = 0; k < kPerThread; ++k) {
+            const uint32_t i = base + k * blockDim.x + threadIdx.x;
+            surf[k] = false;
+            p[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (i < a.n) {
+                surf[k] = a.flags[i] != 0;
+                p[k] = a.sorted[i];
+            }
         }
-        const bool exc = a.sbits ? seed_particle(a, i, p.x, p.y, p.z, surf) : false;
-        n_surf += surf ? 1u : 0u;
-        n_exc += exc ? 1u : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < kPerThread; ++k) {
+            const uint32_t i = base + k * blockDim.x + threadIdx.x;
+            const bool exc = a.sbits ? seed_particle(a, i, p[k].x, p[k].y, p[k].z, surf[k]) : false;
+            n_surf += surf[k] ? 1u : 0u;
+            n_exc += exc ? 1u : 0u;
+        }
     }
     if (a.st) block_count_add2(&a.st->num_surface_particles, n_surf, &a.st->band_exceptions, n_exc);
 }
@@ -384,7 +414,7 @@ __global__ __launch_bounds__(256) void k_seed_flags(SurfaceArgs a) {
 // Grid of the per-particle kernels that end in block counter atomics.
 static uint32_t count_blocks(fsr_context* ctx, uint32_t n) {
     const uint32_t cap = (uint32_t)persistent_blocks(ctx, 256, 4, TUNE_BAND);
-    return std::max(1u, std::min((n + 255u) / 256u, cap));
+    return std::max(1u, std::min((n + 256u * kPerThread - 1u) / (256u * kPerThread), cap));
 }
 
 fsr_status surface_seed_from_flags(fsr_context* ctx, const SurfaceArgs& a, hipStream_t s) {
