@@ -38,15 +38,28 @@ namespace fsr {
 
 namespace {
 
-__global__ __launch_bounds__(256) void k_dilate_x(const uint32_t* __restrict__ in,
+// Bits of four 0/1 bytes (byte k -> bit k).
+__device__ __forceinline__ uint32_t pack4(uint32_t x) {
+    return (x & 1u) | ((x >> 7) & 2u) | ((x >> 14) & 4u) | ((x >> 21) & 8u);
+}
+
+// The x pass reads the seed mask as bytes (one per scalar cell, rows padded to
+// whole words: word w covers bytes 32w .. 32w + 31) and writes bits.
+__global__ __launch_bounds__(256) void k_dilate_x(const uint8_t* __restrict__ in,
                                                   uint32_t* __restrict__ out, uint32_t wx,
                                                   uint64_t nwords, uint32_t rx) {
     const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= nwords) return;
     const uint32_t xw = (uint32_t)(w % wx);
-    const uint32_t cur = in[w];
-    const uint32_t prev = xw > 0 ? in[w - 1] : 0u;
-    const uint32_t next = xw + 1 < wx ? in[w + 1] : 0u;
+    const uint4* chunk = reinterpret_cast<const uint4*>(in + (w << 5));
+    const uint4 lo = chunk[0], hi = chunk[1];
+    const uint32_t cur = pack4(lo.x) | (pack4(lo.y) << 4) | (pack4(lo.z) << 8) |
+                         (pack4(lo.w) << 12) | (pack4(hi.x) << 16) | (pack4(hi.y) << 20) |
+                         (pack4(hi.z) << 24) | (pack4(hi.w) << 28);
+    // the two top bits of the previous word and the three low bits of the next
+    const uint32_t prev =
+        xw > 0 ? pack4(reinterpret_cast<const uint32_t*>(in + (w << 5))[-1]) << 28 : 0u;
+    const uint32_t next = xw + 1 < wx ? pack4(reinterpret_cast<const uint32_t*>(in + (w << 5))[8]) : 0u;
     uint32_t o = cur;
     o |= (cur >> 1) | (next << 31);
     o |= (cur >> 2) | (next << 30);
@@ -408,9 +421,11 @@ fsr_status band_build(fsr_context* ctx, const BandArgs& a, hipStream_t s) {
     const uint32_t wx = row_words(a.sc.rx);
     const uint64_t nwords = (uint64_t)wx * a.sc.ry * a.sc.rz;
     const uint32_t blocks = (uint32_t)((nwords + 255) / 256);
-    // S -> tmp (x), tmp -> S (y), S -> bits (z); `sbits` is consumed
+    // S (bytes) -> tmp (x), tmp -> S (y, its first nwords words), S -> bits
+    // (z); `sbits` is consumed
     uint32_t* sb = const_cast<uint32_t*>(a.sbits);
-    hipLaunchKernelGGL(k_dilate_x, dim3(blocks), dim3(256), 0, s, sb, a.tmp, wx, nwords, a.sc.rx);
+    hipLaunchKernelGGL(k_dilate_x, dim3(blocks), dim3(256), 0, s, (const uint8_t*)sb, a.tmp, wx,
+                       nwords, a.sc.rx);
     FSR_LAUNCH_CHECK("k_dilate_x");
     hipLaunchKernelGGL(k_dilate_rows, dim3(blocks), dim3(256), 0, s, a.tmp, sb, nwords,
                        (uint64_t)wx, a.sc.ry, (uint64_t)a.sc.ry);

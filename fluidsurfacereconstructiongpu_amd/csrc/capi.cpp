@@ -246,7 +246,7 @@ fsr_status frame_impl(const char* who, fsr_context* ctx, fsr_method method,
     FSR_TRY(arena_typed(ctx, A_COUNTERS, 1, &counters));
     uint32_t *flags, *sbits, *tmp, *bits, *exc;
     FSR_TRY(arena_typed(ctx, A_SURF_FLAGS, n, &flags));
-    FSR_TRY(arena_typed(ctx, A_SBITS, W, &sbits));
+    FSR_TRY(arena_typed(ctx, A_SBITS, 8ull * W, &sbits));  // byte seed mask
     FSR_TRY(arena_typed(ctx, A_BAND_TMP, W, &tmp));
     FSR_TRY(arena_typed(ctx, A_BAND_BITS, W, &bits));
     FSR_TRY(arena_typed(ctx, A_EXCEPTIONS, n, &exc));
@@ -255,7 +255,7 @@ fsr_status frame_impl(const char* who, fsr_context* ctx, fsr_method method,
     FillList fills;
     fills.add(counters, 0u, sizeof(Counters) / 4);
     fills.add(stats, 0u, sizeof(fsr_frame_stats) / 4);
-    fills.add(sbits, 0u, W);
+    fills.add(sbits, 0u, 8ull * W);
     // the dense field outside the band keeps the fill: 0, or the byte 100 for
     // Yu13 (the reference's memset of the Yu13 grid)
     fills.add(phi, yu13 ? 0x64646464u : 0u, V);

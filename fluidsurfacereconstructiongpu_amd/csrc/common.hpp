@@ -366,7 +366,9 @@ struct SurfaceArgs {
     GridDev sc;
     float h;
     uint32_t* flags;        // u32[N] surface flags (sorted order)
-    uint32_t* sbits;        // band-shaped mask of surface scalar cells
+    uint32_t* sbits;        // seed mask: one byte per scalar cell of the surface
+                            // particles, rows padded to whole band-mask words
+                            // (32 bytes per word, 8 words of storage each)
     uint32_t* exceptions;   // particles routed to the exact band path
     Counters* counters;
     fsr_frame_stats* st;

@@ -473,12 +473,12 @@ fsr_status fsr_estimation_of_surface_vertices(fsr_context* ctx, const fsr_sim_pa
     uint32_t *sbits, *tmp, *bits, *exc;
     float4* p4;
     const uint64_t nn = num_particles ? num_particles : 1;
-    FSR_TRY(arena_typed(ctx, A_SBITS, W, &sbits));
+    FSR_TRY(arena_typed(ctx, A_SBITS, 8ull * W, &sbits));  // byte seed mask
     FSR_TRY(arena_typed(ctx, A_BAND_TMP, W, &tmp));
     FSR_TRY(arena_typed(ctx, A_BAND_BITS, W, &bits));
     FSR_TRY(arena_typed(ctx, A_EXCEPTIONS, nn, &exc));
     FSR_TRY(arena_typed(ctx, A_SORTED, nn, &p4));
-    FSR_TRY(fill_u32(sbits, 0u, W, s));
+    FSR_TRY(fill_u32(sbits, 0u, 8ull * W, s));
     if (num_particles) {
         hipLaunchKernelGGL(k_pack_f4, dim3(blocks_for(num_particles)), dim3(256), 0, s, particles,
                            (const float*)nullptr, num_particles, p4);

@@ -76,7 +76,7 @@ __global__ __launch_bounds__(256) void k_cell_surface(const float* __restrict__ 
 }
 
 // Band seeding of one particle: its scalar cell is set in the surface-cell
-// mask, or the particle is appended to the exception list (exact path).
+// byte mask, or the particle is appended to the exception list (exact path).
 // Called with all 64 lanes of the wave active (lanes without a particle pass
 // surf = false); returns whether the particle became an exception.
 __device__ __forceinline__ bool seed_particle(const SurfaceArgs& a, uint32_t i, float px,
@@ -88,10 +88,11 @@ __device__ __forceinline__ bool seed_particle(const SurfaceArgs& a, uint32_t i, 
                          qy < (float)a.sc.ry && qz > -1.0f && qz < (float)a.sc.rz;
     const bool exc = surf && (a.all_exceptions || !in_grid);
     if (surf && !exc) {
+        // one byte per scalar cell, rows padded to whole mask words: a plain
+        // byte store (every writer stores 1), no read-modify-write
         const uint32_t ix = (uint32_t)(int)qx, iy = (uint32_t)(int)qy, iz = (uint32_t)(int)qz;
         const uint32_t wx = row_words(a.sc.rx);
-        const uint64_t w = ((uint64_t)iz * a.sc.ry + iy) * wx + (ix >> 5);
-        atomicOr(&a.sbits[w], 1u << (ix & 31u));
+        reinterpret_cast<uint8_t*>(a.sbits)[(((uint64_t)iz * a.sc.ry + iy) * wx << 5) + ix] = 1u;
     }
     const uint64_t m = __ballot(exc);
     if (m) {
