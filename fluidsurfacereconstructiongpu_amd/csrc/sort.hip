@@ -3,7 +3,8 @@
 //
 // Replaces launchSpatialGridBuilding (ReconstructionCUDA.cu:1437-1481):
 //   calcParticlesHashKernel (ReconstructionCUDA.cu:1379-1396)  -> k_hash
-//   thrust::sort_by_key     (ReconstructionCUDA.cu:1461-1466)  -> k_radix_hist / k_radix_scatter
+//   thrust::sort_by_key     (ReconstructionCUDA.cu:1461-1466)  -> k_hash (first histogram),
+//                                                                 k_radix_hist, k_radix_scatter
 //   findCellRangeKernel     (ReconstructionCUDA.cu:1399-1435)  -> k_gather_ranges
 // Semantics kept: cell id = z*rx*ry + y*rx + x with x = int((p.x-min.x)/cell)
 // in IEEE FP32 (ReconstructionCUDA.cuh:338-358); the sort is stable, so
@@ -15,10 +16,13 @@
 // reference indexes out of bounds for it.
 //
 // The sort: 8-bit digits, ceil(bits(C)/8) passes (2 for C < 65536).  A block
-// owns a 2048-key tile.  Pass = per-tile digit histogram in LDS -> one exclusive
-// scan over the digit-major [256 x tiles] table -> stable scatter in which each
-// 256-key round is ranked with eight wave ballots (lanes that share a digit),
-// per-wave digit counts exchanged through LDS and running per-digit offsets.
+// owns a 2048-key tile.  Pass = per-tile digit histogram in LDS (for the first
+// pass, made by k_hash) -> per digit, an exclusive scan over the tiles and the
+// digit total (one block per digit) -> stable scatter (which adds the totals of
+// the smaller digits): each wave ranks its 512 contiguous
+// keys with eight ballots per round and wave-private running digit counts, the
+// block combines the four waves' counts once, reorders the tile by digit in LDS
+// and writes each digit's run with consecutive threads.
 #include "common.hpp"
 
 namespace fsr {
@@ -29,33 +33,45 @@ constexpr int kSortThreads = 256;
 constexpr int kSortRounds = 8;
 constexpr int kSortTile = kSortThreads * kSortRounds;
 
-__global__ __launch_bounds__(256) void k_hash(const float* __restrict__ pos,
-                                              const float* __restrict__ dens, uint32_t n,
-                                              GridDev sp, float4* __restrict__ packed,
-                                              uint32_t* __restrict__ keys) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float px = pos[3ull * i + 0];
-    const float py = pos[3ull * i + 1];
-    const float pz = pos[3ull * i + 2];
-    const float d = dens[i];
-    packed[i] = make_float4(px, py, pz, d);
-    const float qx = cell_coord(px, sp.minx, sp.cell);
-    const float qy = cell_coord(py, sp.miny, sp.cell);
-    const float qz = cell_coord(pz, sp.minz, sp.cell);
+// Cell id per particle, the packed (x, y, z, density) copy, and the first
+// pass's digit histogram of the block's 2048-key tile.
+__global__ __launch_bounds__(kSortThreads) void k_hash(const float* __restrict__ pos,
+                                                       const float* __restrict__ dens, uint32_t n,
+                                                       GridDev sp, float4* __restrict__ packed,
+                                                       uint32_t* __restrict__ keys,
+                                                       uint32_t* __restrict__ hist, uint32_t tiles) {
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
     const uint32_t C = sp.rx * sp.ry * sp.rz;
-    const bool ok = qx > -1.0f && qx < (float)sp.rx && qy > -1.0f && qy < (float)sp.ry &&
-                    qz > -1.0f && qz < (float)sp.rz;
-    uint32_t key = C;
-    if (ok) {
-        // truncation toward zero, as int3(float3) does
-        const uint32_t ix = (uint32_t)(int)qx, iy = (uint32_t)(int)qy, iz = (uint32_t)(int)qz;
-        key = iz * sp.rx * sp.ry + iy * sp.rx + ix;
+    const uint32_t base = blockIdx.x * kSortTile;
+#pragma unroll
+    for (int r = 0; r < kSortRounds; ++r) {
+        const uint32_t i = base + r * kSortThreads + threadIdx.x;
+        if (i >= n) continue;
+        const float px = pos[3ull * i + 0];
+        const float py = pos[3ull * i + 1];
+        const float pz = pos[3ull * i + 2];
+        const float d = dens[i];
+        packed[i] = make_float4(px, py, pz, d);
+        const float qx = cell_coord(px, sp.minx, sp.cell);
+        const float qy = cell_coord(py, sp.miny, sp.cell);
+        const float qz = cell_coord(pz, sp.minz, sp.cell);
+        const bool ok = qx > -1.0f && qx < (float)sp.rx && qy > -1.0f && qy < (float)sp.ry &&
+                        qz > -1.0f && qz < (float)sp.rz;
+        uint32_t key = C;
+        if (ok) {
+            // truncation toward zero, as int3(float3) does
+            const uint32_t ix = (uint32_t)(int)qx, iy = (uint32_t)(int)qy, iz = (uint32_t)(int)qz;
+            key = iz * sp.rx * sp.ry + iy * sp.rx + ix;
+        }
+        keys[i] = key;
+        atomicAdd(&cnt[key & 255u], 1u);
     }
-    keys[i] = key;
+    __syncthreads();
+    hist[threadIdx.x * tiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
-template <bool kFirst>
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ keys,
                                                              uint32_t n, uint32_t shift,
                                                              uint32_t* __restrict__ hist,
@@ -73,30 +89,92 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __r
     hist[threadIdx.x * tiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
+// Per digit (one block each): exclusive prefix of the digit's per-tile counts
+// over the tiles, in place, and the digit's total in totals[digit].
+__global__ __launch_bounds__(kSortThreads) void k_digit_scan(uint32_t* __restrict__ hist,
+                                                             uint32_t tiles,
+                                                             uint32_t* __restrict__ totals) {
+    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t carry;
+    uint32_t* row = hist + (uint64_t)blockIdx.x * tiles;
+    const uint32_t t = threadIdx.x, wave = t >> 6;
+    if (t == 0) carry = 0;
+    for (uint32_t b0 = 0; b0 < tiles; b0 += kSortThreads) {
+        __syncthreads();
+        const uint32_t b = b0 + t;
+        const uint32_t c = b < tiles ? row[b] : 0u;
+        const uint32_t incl = wave_inclusive_sum(c);
+        if ((t & 63u) == 63u) wsum[wave] = incl;
+        __syncthreads();
+        uint32_t before = carry;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) before += (w < (int)wave) ? wsum[w] : 0u;
+        if (b < tiles) row[b] = before + incl - c;
+        __syncthreads();
+        if (t == kSortThreads - 1) carry = before + incl;
+    }
+    __syncthreads();
+    if (t == 0) totals[blockIdx.x] = carry;
+}
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Stable scatter of one 2048-key tile.  Wave w owns the contiguous keys
+// [512 w, 512 w + 512) of the tile, in 8 rounds of 64:
+//   1. per round, the lanes holding the same digit are found with 8 ballots;
+//      a key's rank in the wave's sub-tile is the wave's running count of its
+//      digit plus its rank among those lanes (wave-private LDS counters);
+//   2. one block step turns the four waves' digit counts into the start of
+//      every (wave, digit) run in the tile's digit order;
+//   3. keys and values are placed in LDS in that order and written out with
+//      consecutive threads on consecutive positions of each digit's run.
 template <bool kFirst>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n,
-    uint32_t shift, const uint32_t* __restrict__ offsets, uint32_t tiles) {
-    __shared__ uint32_t digit_base[256];
-    __shared__ uint32_t run[256];
-    __shared__ uint32_t wave_cnt[4][256];
-    __shared__ uint32_t wave_off[4][256];
+    uint32_t shift, const uint32_t* __restrict__ offsets, uint32_t tiles,
+    const uint32_t* __restrict__ totals) {
+    __shared__ uint32_t wcnt[4][256];   // counts, then run starts, of (wave, digit)
+    __shared__ uint32_t dstart[256];    // first tile slot of each digit
+    __shared__ uint32_t gbase[256];     // output position of the digit's first key of the tile
+    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t skey[kSortTile];
+    __shared__ uint32_t sval[kSortTile];
     const uint32_t t = threadIdx.x;
-    const uint32_t wave = t >> 6;
-    digit_base[t] = offsets[t * tiles + blockIdx.x];
-    run[t] = 0;
+    const uint32_t wave = t >> 6, lane = t & 63u;
 #pragma unroll
-    for (int w = 0; w < 4; ++w) wave_cnt[w][t] = 0;
+    for (int w = 0; w < 4; ++w) wcnt[w][t] = 0;
+    {
+        // output position of digit t's first key of this tile: the keys of
+        // all smaller digits, then digit t's keys of the tiles before this one
+        const uint32_t tot = totals[t];
+        const uint32_t incl = wave_inclusive_sum(tot);
+        if ((t & 63u) == 63u) wsum[t >> 6] = incl;
+        __syncthreads();
+        uint32_t before = 0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) before += (w < (int)(t >> 6)) ? wsum[w] : 0u;
+        gbase[t] = before + incl - tot + offsets[t * tiles + blockIdx.x];
+    }
     __syncthreads();
-    const uint32_t base = blockIdx.x * kSortTile;
-    for (int r = 0; r < kSortRounds; ++r) {
-        const uint32_t i = base + r * kSortThreads + t;
+    const uint32_t tile0 = blockIdx.x * kSortTile;
+    const uint32_t base = tile0 + wave * (kSortTile / 4);
+    uint32_t key[kSortRounds], val[kSortRounds], loc[kSortRounds];
+#pragma unroll
+    for (int j = 0; j < kSortRounds; ++j) {
+        const uint32_t i = base + j * 64u + lane;
         const bool valid = i < n;
-        const uint32_t key = valid ? keys_in[i] : 0u;
-        const uint32_t val = kFirst ? i : (valid ? vals_in[i] : 0u);
-        const uint32_t d = (key >> shift) & 255u;
-        // lanes of this wave holding the same digit
+        key[j] = valid ? keys_in[i] : 0u;
+        val[j] = kFirst ? i : (valid ? vals_in[i] : 0u);
+    }
+#pragma unroll
+    for (int j = 0; j < kSortRounds; ++j) {
+        const bool valid = base + j * 64u + lane < n;
+        const uint32_t d = (key[j] >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -105,25 +183,48 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
             peers &= bit ? m : ~m;
         }
         const uint32_t rank = mask_rank(peers);
-        if (valid && rank == 0) wave_cnt[wave][d] = (uint32_t)__popcll(peers);
+        const uint32_t pre = wcnt[wave][d];
+        wave_sync_lds();
+        if (valid && rank == 0) wcnt[wave][d] = pre + (uint32_t)__popcll(peers);
+        wave_sync_lds();
+        loc[j] = pre + rank;
+    }
+    __syncthreads();
+    {
+        // thread t: digit t.  Tile total of the digit, its start in the tile
+        // (exclusive scan over digits), and the start of each wave's run.
+        const uint32_t c0 = wcnt[0][t], c1 = wcnt[1][t], c2 = wcnt[2][t], c3 = wcnt[3][t];
+        const uint32_t tot = c0 + c1 + c2 + c3;
+        const uint32_t incl = wave_inclusive_sum(tot);
+        if (lane == 63) wsum[wave] = incl;
         __syncthreads();
-        {
-            const uint32_t c0 = wave_cnt[0][t], c1 = wave_cnt[1][t], c2 = wave_cnt[2][t],
-                           c3 = wave_cnt[3][t];
-            const uint32_t r0 = run[t];
-            wave_off[0][t] = r0;
-            wave_off[1][t] = r0 + c0;
-            wave_off[2][t] = r0 + c0 + c1;
-            wave_off[3][t] = r0 + c0 + c1 + c2;
-            run[t] = r0 + c0 + c1 + c2 + c3;
-            wave_cnt[0][t] = 0; wave_cnt[1][t] = 0; wave_cnt[2][t] = 0; wave_cnt[3][t] = 0;
+        uint32_t before = 0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) before += (w < (int)wave) ? wsum[w] : 0u;
+        const uint32_t st = before + incl - tot;
+        dstart[t] = st;
+        wcnt[0][t] = st;
+        wcnt[1][t] = st + c0;
+        wcnt[2][t] = st + c0 + c1;
+        wcnt[3][t] = st + c0 + c1 + c2;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kSortRounds; ++j) {
+        if (base + j * 64u + lane < n) {
+            const uint32_t slot = wcnt[wave][(key[j] >> shift) & 255u] + loc[j];
+            skey[slot] = key[j];
+            sval[slot] = val[j];
         }
-        __syncthreads();
-        if (valid) {
-            const uint32_t p = digit_base[d] + wave_off[wave][d] + rank;
-            keys_out[p] = key;
-            vals_out[p] = val;
-        }
+    }
+    __syncthreads();
+    const uint32_t count = min((uint32_t)kSortTile, n - tile0);
+    for (uint32_t q = t; q < count; q += kSortThreads) {
+        const uint32_t k = skey[q];
+        const uint32_t d = (k >> shift) & 255u;
+        const uint32_t p = gbase[d] + (q - dstart[d]);
+        keys_out[p] = k;
+        vals_out[p] = sval[q];
     }
 }
 
@@ -183,7 +284,9 @@ fsr_status spatial_hash(fsr_context* ctx, float* pos, float* dens, uint32_t n,
     FSR_TRY(arena_typed(ctx, A_RANGES, C, &ranges));
     FSR_TRY(arena_typed(ctx, A_OCCUPANCY, C, &occ));
     const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
-    FSR_TRY(arena_typed(ctx, A_RADIX_HIST, 256ull * (tiles ? tiles : 1), &hist));
+    // [256 x tiles] per-tile digit counts, then 256 digit totals
+    FSR_TRY(arena_typed(ctx, A_RADIX_HIST, 256ull * (tiles ? tiles : 1) + 256ull, &hist));
+    uint32_t* totals = hist + 256ull * (tiles ? tiles : 1);
 
     FillList fills;
     if (pre) fills = *pre;
@@ -204,7 +307,8 @@ fsr_status spatial_hash(fsr_context* ctx, float* pos, float* dens, uint32_t n,
         return FSR_OK;
     }
     const uint32_t blocks = (n + 255) / 256;
-    hipLaunchKernelGGL(k_hash, dim3(blocks), dim3(256), 0, s, pos, dens, n, sp, packed, k0);
+    hipLaunchKernelGGL(k_hash, dim3(tiles), dim3(kSortThreads), 0, s, pos, dens, n, sp, packed, k0,
+                       hist, tiles);
     FSR_LAUNCH_CHECK("k_hash");
 
     // key C (out of grid) must be representable
@@ -215,20 +319,20 @@ fsr_status spatial_hash(fsr_context* ctx, float* pos, float* dens, uint32_t n,
     uint32_t *kin = k0, *kout = k1, *vin = v0, *vout = v1;
     for (int p = 0; p < passes; ++p) {
         const uint32_t shift = 8u * p;
-        if (p == 0)
-            hipLaunchKernelGGL(k_radix_hist<true>, dim3(tiles), dim3(kSortThreads), 0, s, kin, n,
-                               shift, hist, tiles);
-        else
-            hipLaunchKernelGGL(k_radix_hist<false>, dim3(tiles), dim3(kSortThreads), 0, s, kin, n,
-                               shift, hist, tiles);
-        FSR_LAUNCH_CHECK("k_radix_hist");
-        FSR_TRY(scan_exclusive(ctx, SCAN_U32, hist, hist, 256ull * tiles, nullptr, nullptr, s));
+        // the first pass's histogram comes from k_hash
+        if (p > 0) {
+            hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kSortThreads), 0, s, kin, n, shift,
+                               hist, tiles);
+            FSR_LAUNCH_CHECK("k_radix_hist");
+        }
+        hipLaunchKernelGGL(k_digit_scan, dim3(256), dim3(kSortThreads), 0, s, hist, tiles, totals);
+        FSR_LAUNCH_CHECK("k_digit_scan");
         if (p == 0)
             hipLaunchKernelGGL(k_radix_scatter<true>, dim3(tiles), dim3(kSortThreads), 0, s, kin,
-                               vin, kout, vout, n, shift, hist, tiles);
+                               vin, kout, vout, n, shift, hist, tiles, totals);
         else
             hipLaunchKernelGGL(k_radix_scatter<false>, dim3(tiles), dim3(kSortThreads), 0, s, kin,
-                               vin, kout, vout, n, shift, hist, tiles);
+                               vin, kout, vout, n, shift, hist, tiles, totals);
         FSR_LAUNCH_CHECK("k_radix_scatter");
         uint32_t* t;
         t = kin; kin = kout; kout = t;
