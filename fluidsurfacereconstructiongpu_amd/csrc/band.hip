@@ -337,19 +337,17 @@ __device__ __forceinline__ uint32_t layer_rows(const AxisMaps& m, const GridDev&
 // so the exclusive scan of the slab counts is each slab's offset in the cell
 // list, and a cell's vertices come out in ascending id order (layers z, rows
 // y, then x), i.e. a 64-vertex chunk is a compact slab of the cell, which
-// keeps the candidate culling of the field kernels effective.  One thread per
-// slab (consecutive threads = consecutive cx, which read the same row words);
-// rows without band vertices are skipped with the row occupancy bits.  Pass 0
-// writes the slab's band vertex count and adds it to its cell's count (cnt,
-// cleared by k_axis_maps); pass 1 (kFill) writes the slab's vertex ids at its
-// scanned offset.  The list itself is deterministic.
-template <bool kFill>
+// keeps the candidate culling of the field kernels effective.  The count pass
+// takes one thread per slab (consecutive threads = consecutive cx, which read
+// the same row words); rows without band vertices are skipped with the row
+// occupancy bits.  It writes the slab's band vertex count and adds it to its
+// cell's count (cnt, cleared by k_axis_maps).  The fill pass (k_slab_fill)
+// writes the slab's vertex ids at its scanned offset.  The list itself is
+// deterministic.
 __global__ __launch_bounds__(256) void k_slab_rows(const uint32_t* __restrict__ bits, GridDev sc,
                                                    uint32_t wx, AxisMaps m, uint32_t vcx,
                                                    uint32_t vcy, uint32_t* __restrict__ slab,
-                                                   uint32_t* __restrict__ cnt,
-                                                   const uint32_t* __restrict__ slab_off,
-                                                   uint32_t* __restrict__ cell_band, uint64_t cap) {
+                                                   uint32_t* __restrict__ cnt) {
     const uint64_t total = (uint64_t)vcx * vcy * sc.rz;
     for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
          t += (uint64_t)gridDim.x * blockDim.x) {
@@ -362,10 +360,63 @@ __global__ __launch_bounds__(256) void k_slab_rows(const uint32_t* __restrict__ 
         const uint64_t idx = first + (z - zlo);
         const uint32_t v = (cz * vcy + cy) * vcx + cx;
         uint32_t o = 0;
-        if (kFill) {
-            if (slab[idx] == 0u) continue;
-            o = slab_off[idx];
+        const uint32_t ylo = m.lo[1][cy], yhi = m.hi[1][cy];
+        const uint32_t xlo = m.lo[0][cx], xhi = m.hi[0][cx];
+        const uint32_t ny = yhi > ylo ? yhi - ylo : 0u;
+        const uint32_t w0 = xlo >> 5;
+        const uint32_t nw = xhi > xlo ? ((xhi + 31u) >> 5) - w0 : 0u;
+        for (uint32_t yb = 0; nw && yb < ny; yb += 32u) {
+            uint32_t rows = layer_rows(m, sc, z, ylo, yb, ny);
+            while (rows) {
+                const uint64_t row = (uint64_t)z * sc.ry + ylo + yb + __builtin_ctz(rows);
+                rows &= rows - 1u;
+                for (uint32_t j = 0; j < nw; ++j) {
+                    const uint32_t w = w0 + j;
+                    o += __popc(clip_word(bits[row * wx + w], w, xlo, xhi));
+                }
+            }
         }
+        slab[idx] = o;
+        if (o) atomicAdd(&cnt[v], o);
+    }
+}
+
+// Fill pass in slab order: thread t of block b takes slab idx = 256 b + t, so
+// the block's slabs own one contiguous range of the cell list.  The slab's ids
+// are written to LDS (serial over its set bits) and the block then copies the
+// range out with consecutive threads on consecutive entries.  A slab idx
+// decodes as: layer q = idx / (vcx vcy) lies in the layer range of the slab's
+// z-cell cz = vc_z[q]; r = idx - vcx vcy lo_z[cz] = (cy vcx + cx) nz + (z - lo_z).
+__global__ __launch_bounds__(256) void k_slab_fill(const uint32_t* __restrict__ bits, GridDev sc,
+                                                   uint32_t wx, AxisMaps m, uint32_t vcx,
+                                                   uint32_t vcy, const uint32_t* __restrict__ slab,
+                                                   const uint32_t* __restrict__ slab_off,
+                                                   uint32_t* __restrict__ cell_band, uint64_t cap,
+                                                   uint64_t nslabs) {
+    constexpr uint32_t kStage = 8192;
+    __shared__ uint32_t stage[kStage];
+    __shared__ uint32_t range[2];
+    const uint64_t first = (uint64_t)blockIdx.x * blockDim.x;
+    const uint64_t idx = first + threadIdx.x;
+    const uint64_t last = min(first + blockDim.x, nslabs) - 1;
+    if (threadIdx.x == 0) {
+        range[0] = slab_off[first];
+        range[1] = slab_off[last] + slab[last];
+    }
+    __syncthreads();
+    const uint32_t lo = range[0], total = range[1] - range[0];
+    const bool staged = total <= kStage;
+    uint32_t n = idx < nslabs ? slab[idx] : 0u;
+    if (n) {
+        const uint64_t layer = (uint64_t)vcx * vcy;
+        const uint32_t q = (uint32_t)(idx / layer);
+        const uint32_t cz = m.vc[2][q];
+        const uint32_t zlo = m.lo[2][cz], nz = m.hi[2][cz] - zlo;
+        const uint64_t r = idx - layer * zlo;
+        const uint32_t c = (uint32_t)(r / nz);
+        const uint32_t z = zlo + (uint32_t)(r - (uint64_t)c * nz);
+        const uint32_t cx = c % vcx, cy = c / vcx;
+        uint32_t o = slab_off[idx];
         const uint32_t ylo = m.lo[1][cy], yhi = m.hi[1][cy];
         const uint32_t xlo = m.lo[0][cx], xhi = m.hi[0][cx];
         const uint32_t ny = yhi > ylo ? yhi - ylo : 0u;
@@ -379,23 +430,22 @@ __global__ __launch_bounds__(256) void k_slab_rows(const uint32_t* __restrict__ 
                 for (uint32_t j = 0; j < nw; ++j) {
                     const uint32_t w = w0 + j;
                     uint32_t word = clip_word(bits[row * wx + w], w, xlo, xhi);
-                    if (!kFill) {
-                        o += __popc(word);
-                    } else {
-                        const uint32_t base = (uint32_t)(row * sc.rx) + w * 32u;
-                        while (word) {
-                            if (o < cap) cell_band[o] = base + __builtin_ctz(word);
-                            ++o;
-                            word &= word - 1u;
-                        }
+                    const uint32_t base = (uint32_t)(row * sc.rx) + w * 32u;
+                    while (word) {
+                        const uint32_t id = base + __builtin_ctz(word);
+                        if (staged) stage[o - lo] = id;
+                        else if (o < cap) cell_band[o] = id;
+                        ++o;
+                        word &= word - 1u;
                     }
                 }
             }
         }
-        if (!kFill) {
-            slab[idx] = o;
-            if (o) atomicAdd(&cnt[v], o);
-        }
+    }
+    if (staged) {
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < total; k += blockDim.x)
+            if ((uint64_t)lo + k < cap) cell_band[lo + k] = stage[k];
     }
 }
 
@@ -505,15 +555,16 @@ fsr_status band_compact(fsr_context* ctx, const uint32_t* bits, const GridDev& s
     FSR_TRY(arena_typed(ctx, A_SLAB, 2 * nslabs, &slab));
     slab_off = slab + nslabs;
     const uint32_t slab_blocks = (uint32_t)std::min<uint64_t>((nslabs + 255) / 256, 16384);
-    hipLaunchKernelGGL(k_slab_rows<false>, dim3(slab_blocks), dim3(256), 0, s, bits, sc, wx, m,
-                       vcx, vcy, slab, cnt, (const uint32_t*)nullptr, cell_band, V);
-    FSR_LAUNCH_CHECK("k_slab_rows(count)");
+    hipLaunchKernelGGL(k_slab_rows, dim3(slab_blocks), dim3(256), 0, s, bits, sc, wx, m, vcx, vcy,
+                       slab, cnt);
+    FSR_LAUNCH_CHECK("k_slab_rows");
     FSR_TRY(scan_exclusive(ctx, SCAN_U32, slab, slab_off, nslabs, nullptr, nullptr, s));
     FSR_TRY(scan_cells(ctx, cnt, nvc, cell_off, chunk_off, chunk_cell, max_chunks,
                        &counters->num_chunks, s));
-    hipLaunchKernelGGL(k_slab_rows<true>, dim3(slab_blocks), dim3(256), 0, s, bits, sc, wx, m,
-                       vcx, vcy, slab, cnt, slab_off, cell_band, V);
-    FSR_LAUNCH_CHECK("k_slab_rows(fill)");
+    hipLaunchKernelGGL(k_slab_fill, dim3((uint32_t)((nslabs + 255) / 256)), dim3(256), 0, s, bits,
+                       sc, wx, m, vcx, vcy, (const uint32_t*)slab, (const uint32_t*)slab_off,
+                       cell_band, V, nslabs);
+    FSR_LAUNCH_CHECK("k_slab_fill");
     out->cell_off = cell_off;
     out->cell_cnt = cnt;
     out->chunk_off = chunk_off;

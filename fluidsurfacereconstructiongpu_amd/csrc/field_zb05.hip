@@ -158,7 +158,6 @@ __device__ __forceinline__ void accumulate(const float* __restrict__ rows, uint3
     const float* a1p = rows + (lane >> 4) * kRowStride + (lane & 15u);
     const float* a2p = rows + (lane & 3u) * kRowStride + 4u * (lane >> 4);
     const float* cqp = rows + kRowQ + 4u * (lane >> 4);
-#pragma unroll 2
     for (uint32_t tb = 0; tb < n; tb += 16u) {
         const float a1 = a1p[tb];
         const float4 a2 = *reinterpret_cast<const float4*>(a2p + tb);
