@@ -182,15 +182,24 @@ __global__ __launch_bounds__(kAdjThreads) void k_mc_vertex_backward(McBackArgs a
 // ---------------------------------------------------------------------------
 // per band vertex (cell-major slot): u = (g/W) n, c = (g/W) |x - xbar|
 // ---------------------------------------------------------------------------
+// Per cell-major slot: (u, c) (zero when the vertex carries no gradient) and
+// the vertex position, so that the gather below stages a vertex with two
+// 16-byte loads and no index arithmetic.
 __global__ __launch_bounds__(kAdjThreads) void k_field_coef(const uint32_t* __restrict__ cell_band,
                                                             const float4* __restrict__ saved,
                                                             const float* __restrict__ g_phi,
-                                                            const fsr_frame_stats* st,
-                                                            float4* __restrict__ coef) {
+                                                            const fsr_frame_stats* st, GridDev sc,
+                                                            float4* __restrict__ coef,
+                                                            float4* __restrict__ vpos) {
     const uint32_t B = st->num_band_vertices;
+    const uint32_t rxy = sc.rx * sc.ry;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < B; i += gridDim.x * blockDim.x) {
         const float4 s = saved[i];
-        const float g = g_phi[cell_band[i]];
+        const uint32_t vid = cell_band[i];
+        const float g = g_phi[vid];
+        const uint32_t ix = vid % sc.rx, iy = (vid / sc.rx) % sc.ry, iz = vid / rxy;
+        vpos[i] = make_float4(vertex_coord(ix, sc.minx, sc.cell), vertex_coord(iy, sc.miny, sc.cell),
+                              vertex_coord(iz, sc.minz, sc.cell), 0.0f);
         float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
         const float len = __builtin_sqrtf(__builtin_fmaf(s.z, s.z, __builtin_fmaf(s.y, s.y, s.x * s.x)));
         if (s.w > 0.0f && len > 0.0f && g != 0.0f) {
@@ -217,6 +226,7 @@ struct FieldBackArgs {
     const uint32_t* cell_cnt;
     const uint32_t* cell_band;    // cell-major slot -> vertex id
     const float4* coef;           // cell-major slot -> (u, c)
+    const float4* vpos;           // cell-major slot -> vertex position
     GridDev sp, sc;
     uint32_t vcx, vcy;
     int radius;
@@ -282,7 +292,6 @@ __global__ __launch_bounds__(kAdjThreads) void k_field_zb05_backward(FieldBackAr
     const uint32_t nch = *a.num_chunks;
     const float inv_r2 = 1.0f / a.radius_sq;
     const float cull_r2 = a.radius_sq * 1.0001f;
-    const uint32_t rxy = a.sc.rx * a.sc.ry;
     float4* qx = queue_x[wave];
     float4* qu = queue_u[wave];
     uint32_t step = 0;
@@ -329,21 +338,31 @@ __global__ __launch_bounds__(kAdjThreads) void k_field_zb05_backward(FieldBackAr
 
         GradSums g = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         uint32_t fill = 0;
+        // this lane's vertex of the next round is loaded one round ahead
+        int j = 0;  // run of this lane's vertex; gi only grows, so the search resumes
+        float4 ncf = make_float4(0.f, 0.f, 0.f, 0.f), nvp = ncf;
+        if (lane < total) {
+            while (j + 1 < 9 && row_pref[wave][j + 1] <= lane) ++j;
+            const uint32_t slot = row_start[wave][j] + (lane - row_pref[wave][j]);
+            ncf = a.coef[slot];
+            nvp = a.vpos[slot];
+        }
         for (uint32_t base = 0; base < total; base += 64u) {
             const uint32_t gi = base + lane;
+            const float4 cf = ncf, vp = nvp;
+            if (gi + 64u < total) {
+                while (j + 1 < 9 && row_pref[wave][j + 1] <= gi + 64u) ++j;
+                const uint32_t slot = row_start[wave][j] + (gi + 64u - row_pref[wave][j]);
+                ncf = a.coef[slot];
+                nvp = a.vpos[slot];
+            }
             bool keep = false;
             float4 sx, su;
             if (gi < total) {
-                int j = 0;
-                while (j + 1 < 9 && row_pref[wave][j + 1] <= gi) ++j;
-                const uint32_t slot = row_start[wave][j] + (gi - row_pref[wave][j]);
-                const float4 cf = a.coef[slot];
                 if (cf.x != 0.0f || cf.y != 0.0f || cf.z != 0.0f || cf.w != 0.0f) {
-                    const uint32_t vid = a.cell_band[slot];
-                    const uint32_t ix = vid % a.sc.rx, iy = (vid / a.sc.rx) % a.sc.ry, iz = vid / rxy;
-                    const float xx = vertex_coord(ix, a.sc.minx, a.sc.cell) - ox;
-                    const float xy = vertex_coord(iy, a.sc.miny, a.sc.cell) - oy;
-                    const float xz = vertex_coord(iz, a.sc.minz, a.sc.cell) - oz;
+                    const float xx = vp.x - ox;
+                    const float xy = vp.y - oy;
+                    const float xz = vp.z - oz;
                     const float dx = fmaxf(fmaxf(bx0 - xx, xx - bx1), 0.0f);
                     const float dy = fmaxf(fmaxf(by0 - xy, xy - by1), 0.0f);
                     const float dz = fmaxf(fmaxf(bz0 - xz, xz - bz1), 0.0f);
@@ -435,11 +454,12 @@ fsr_status field_zb05_backward(fsr_context* ctx, const float* g_phi, float* g_po
     const uint64_t V = (uint64_t)st.sc.rx * st.sc.ry * st.sc.rz;
     const uint32_t C = st.sp.rx * st.sp.ry * st.sp.rz;
     float4* coef;
-    FSR_TRY(arena_typed(ctx, A_ADJ_COEF, V, &coef));
+    FSR_TRY(arena_typed(ctx, A_ADJ_COEF, 2 * V, &coef));  // (u, c), then vertex positions
+    float4* vpos = coef + V;
     {
         const int blocks = persistent_blocks(ctx, kAdjThreads, 4, TUNE_MC);
         hipLaunchKernelGGL(k_field_coef, dim3(blocks), dim3(kAdjThreads), 0, s, st.bc.cell_band,
-                           st.saved, g_phi, st.stats, coef);
+                           st.saved, g_phi, st.stats, st.sc, coef, vpos);
         FSR_LAUNCH_CHECK("k_field_coef");
     }
     // particle chunks per hash cell
@@ -465,6 +485,7 @@ fsr_status field_zb05_backward(fsr_context* ctx, const float* g_phi, float* g_po
     a.cell_cnt = st.bc.cell_cnt;
     a.cell_band = st.bc.cell_band;
     a.coef = coef;
+    a.vpos = vpos;
     a.sp = st.sp;
     a.sc = st.sc;
     a.vcx = st.bc.vcx;
