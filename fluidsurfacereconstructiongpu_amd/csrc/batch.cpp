@@ -39,6 +39,8 @@ using namespace fsr;
 
 namespace {
 
+constexpr int kLaneFieldWorkgroupsPerCu = 2;
+
 void release_graph(fsr_batch* b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -133,6 +135,12 @@ fsr_status fsr_batch_create(fsr_context* ctx, uint32_t lanes, fsr_batch** out) {
         fsr_status st = fsr_create(&c, ctx->device);
         if (st != FSR_OK) return fail(st);
         std::copy(ctx->tune, ctx->tune + TUNE_NUM, c->tune);
+        // Lanes share the GPU: unless a launch shape was chosen, a lane's
+        // persistent field kernel takes 2 workgroups per CU (of the 8 a lone
+        // frame uses), so the field kernels of concurrent lanes run side by
+        // side instead of queueing behind each other's workgroups.  Measured
+        // on 64 random-shape frames: 2 per CU beats 1, 3, 4 and 8.
+        if (c->tune[TUNE_FIELD] == 0) c->tune[TUNE_FIELD] = kLaneFieldWorkgroupsPerCu;
         b->lanes.push_back(c);
         hipStream_t s;
         hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
