@@ -102,43 +102,173 @@ __device__ __forceinline__ int trunc_clamped(float q, int res) {
     return (int)q;
 }
 
-// The reference's per-particle band loop for the exception list.
-__global__ __launch_bounds__(256) void k_band_exceptions(BandArgs a) {
-    const uint32_t count = a.counters->num_exceptions;
-    const uint32_t wx = row_words(a.sc.rx);
-    const double limit = (double)a.h * 1.5;
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < count;
-         e += gridDim.x * blockDim.x) {
-        const float4 p = a.sorted[a.exceptions[e]];
-        const int i = trunc_clamped(cell_coord(p.x, a.sc.minx, a.sc.cell), a.sc.rx);
-        const int j = trunc_clamped(cell_coord(p.y, a.sc.miny, a.sc.cell), a.sc.ry);
-        const int k = trunc_clamped(cell_coord(p.z, a.sc.minz, a.sc.cell), a.sc.rz);
-        const int x0 = clampi(i - 3, 0, a.sc.rx - 1), x1 = clampi(i + 3, 0, a.sc.rx - 1);
-        const int y0 = clampi(j - 3, 0, a.sc.ry - 1), y1 = clampi(j + 3, 0, a.sc.ry - 1);
-        const int z0 = clampi(k - 3, 0, a.sc.rz - 1), z1 = clampi(k + 3, 0, a.sc.rz - 1);
-        for (int z = z0; z <= z1; ++z) {
-            const float vz = vertex_coord((uint32_t)z, a.sc.minz, a.sc.cell);
-            for (int y = y0; y <= y1; ++y) {
-                const float vy = vertex_coord((uint32_t)y, a.sc.miny, a.sc.cell);
-                const uint64_t rowbase = ((uint64_t)z * a.sc.ry + y) * wx;
-                uint32_t cur_w = 0xFFFFFFFFu, m = 0;
-                for (int x = x0; x < x1; ++x) {
-                    const float vx = vertex_coord((uint32_t)x, a.sc.minx, a.sc.cell);
-                    const float dx = vx - p.x, dy = vy - p.y, dz = vz - p.z;
-                    const float dist =
-                        __builtin_sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)));
-                    if ((double)dist < limit) {
-                        const uint32_t ww = (uint32_t)x >> 5;
-                        if (ww != cur_w) {
-                            if (m) atomicOr(&a.bits[rowbase + cur_w], m);
-                            cur_w = ww;
-                            m = 0;
-                        }
-                        m |= 1u << ((uint32_t)x & 31u);
-                    }
-                }
-                if (m) atomicOr(&a.bits[rowbase + cur_w], m);
+// Rows of one exception particle's box (the reference's per-particle loop):
+// calls emit(z, y, word, bits) for every row word with vertices in the band.
+// The x-run of a row has at most 6 vertices (x in [clamp(i-3), clamp(i+3)))
+// and so touches at most two words; its 6 tests are unrolled and branch-free,
+// so they run as independent chains.
+template <typename Emit>
+__device__ __forceinline__ void exception_rows(const BandArgs& a, const float4& p, int x0, int x1,
+                                               int y0, int y1, int z0, int z1, float limit,
+                                               Emit emit) {
+    const uint32_t w0 = (uint32_t)x0 >> 5;
+    for (int z = z0; z <= z1; ++z) {
+        const float vz = vertex_coord((uint32_t)z, a.sc.minz, a.sc.cell);
+        const float dz = vz - p.z;
+        for (int y = y0; y <= y1; ++y) {
+            const float vy = vertex_coord((uint32_t)y, a.sc.miny, a.sc.cell);
+            const float dy = vy - p.y;
+            uint32_t m0 = 0, m1 = 0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const int x = x0 + k;
+                const float vx = vertex_coord((uint32_t)x, a.sc.minx, a.sc.cell);
+                const float dx = vx - p.x;
+                // the reference's |v - p| (same operation order as fma(dz, dz, fma(dy, dy, dx dx)))
+                const float dist =
+                    __builtin_sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)));
+                const uint32_t bit = (x < x1 && dist < limit) ? 1u << ((uint32_t)x & 31u) : 0u;
+                if (((uint32_t)x >> 5) == w0) m0 |= bit;
+                else m1 |= bit;
             }
+            if (m0) emit(z, y, w0, m0);
+            if (m1) emit(z, y, w0 + 1u, m1);
+        }
+    }
+}
+
+// The same tests with wave-uniform loops (the box has at most 7 x 7 x 6
+// vertices), for the windowed path: every lane runs all 294 slots, invalid
+// slots test nothing, and each row ORs its nonzero bits into the LDS window.
+__device__ __forceinline__ void exception_rows_window(const BandArgs& a, const float4& p,
+                                                      bool live, int x0, int x1, int y0, int y1,
+                                                      int z0, int z1, float limit, uint32_t* w,
+                                                      int W0, int Y0, int Z0, int nw, int ny) {
+    const uint32_t w0 = (uint32_t)x0 >> 5;
+    for (int kz = 0; kz < 7; ++kz) {
+        const int z = z0 + kz;
+        const bool zok = live && z <= z1;
+        const float dz = vertex_coord((uint32_t)z, a.sc.minz, a.sc.cell) - p.z;
+#pragma unroll
+        for (int ky = 0; ky < 7; ++ky) {
+            const int y = y0 + ky;
+            const bool ok = zok && y <= y1;
+            const float dy = vertex_coord((uint32_t)y, a.sc.miny, a.sc.cell) - p.y;
+            uint32_t m0 = 0, m1 = 0;
+#pragma unroll
+            for (int kx = 0; kx < 6; ++kx) {
+                const int x = x0 + kx;
+                const float dx = vertex_coord((uint32_t)x, a.sc.minx, a.sc.cell) - p.x;
+                const float dist =
+                    __builtin_sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)));
+                const uint32_t bit = (x < x1 && dist < limit) ? 1u << ((uint32_t)x & 31u) : 0u;
+                if (((uint32_t)x >> 5) == w0) m0 |= bit;
+                else m1 |= bit;
+            }
+            if (ok && (m0 | m1)) {
+                const int base = ((z - Z0) * ny + (y - Y0)) * nw + ((int)w0 - W0);
+                if (m0) atomicOr(&w[base], m0);
+                if (m1) atomicOr(&w[base + 1], m1);  // m1 != 0: word w0 + 1 is in the window
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = min(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The reference's per-particle band loop for the exception list, one wave per
+// 64 exceptions.  Device-scope atomics are slow here, so a wave whose boxes
+// fit a window of kWinWords x kWinRows x kWinRows mask words (the common case:
+// the list is filled in sorted order, so a wave's particles are neighbours)
+// ORs its bits into that window in LDS and then adds each nonzero window word
+// to the mask with one atomic; other waves OR straight into the mask.
+constexpr int kWinWords = 3, kWinRows = 16;
+constexpr int kWin = kWinWords * kWinRows * kWinRows;
+
+__global__ __launch_bounds__(256) void k_band_exceptions(BandArgs a) {
+    __shared__ uint32_t win[4][kWin];
+    const bool by_flags = a.all_exceptions != 0 && a.flags != nullptr;
+    const uint32_t count = by_flags ? a.num_particles : a.counters->num_exceptions;
+    const uint32_t wx = row_words(a.sc.rx);
+    // The reference compares the FP32 distance with 1.5 h in double.  For a
+    // float d, (double)d < L holds exactly when d < the smallest float >= L.
+    const double L = (double)a.h * 1.5;
+    float limit = (float)L;
+    if ((double)limit < L) limit = __int_as_float(__float_as_int(limit) + 1);
+    const uint32_t lane = __lane_id();
+    uint32_t* w = win[threadIdx.x >> 6];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // wave-uniform loop: every lane of the wave runs the same iterations
+    for (uint32_t e0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); e0 < count; e0 += stride) {
+        const uint32_t e = e0 + lane;
+        bool live = e < count;
+        uint32_t pi = 0;
+        if (live) {
+            if (by_flags) {
+                live = a.flags[e] != 0u;
+                pi = e;
+            } else {
+                pi = a.exceptions[e];
+            }
+        }
+        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        int x0 = 0x7fffffff, x1 = -1, y0 = 0x7fffffff, y1 = -1, z0 = 0x7fffffff, z1 = -1;
+        if (live) {
+            p = a.sorted[pi];
+            const int i = trunc_clamped(cell_coord(p.x, a.sc.minx, a.sc.cell), a.sc.rx);
+            const int j = trunc_clamped(cell_coord(p.y, a.sc.miny, a.sc.cell), a.sc.ry);
+            const int k = trunc_clamped(cell_coord(p.z, a.sc.minz, a.sc.cell), a.sc.rz);
+            x0 = clampi(i - 3, 0, a.sc.rx - 1); x1 = clampi(i + 3, 0, a.sc.rx - 1);
+            y0 = clampi(j - 3, 0, a.sc.ry - 1); y1 = clampi(j + 3, 0, a.sc.ry - 1);
+            z0 = clampi(k - 3, 0, a.sc.rz - 1); z1 = clampi(k + 3, 0, a.sc.rz - 1);
+        }
+        // a wave without a surface particle has nothing to do (wave-uniform)
+        if (__ballot(live) == 0) continue;
+        // the wave's window: words [W0, W1] of rows [Y0, Y1] of layers [Z0, Z1]
+        // (over the live lanes; at least one is live)
+        const int W0 = wave_min_i(x0) >> 5, W1 = wave_max_i(x1) >> 5;
+        const int Y0 = wave_min_i(y0), Y1 = wave_max_i(y1);
+        const int Z0 = wave_min_i(z0), Z1 = wave_max_i(z1);
+        const int nw = W1 - W0 + 1, ny = Y1 - Y0 + 1, nz = Z1 - Z0 + 1;
+        if (nw <= kWinWords && ny <= kWinRows && nz <= kWinRows) {
+            const int n = nw * ny * nz;
+            for (int t = (int)lane; t < n; t += 64) w[t] = 0u;
+            wave_sync_lds();
+            // lanes without a particle test nothing (in-range dummy bounds)
+            exception_rows_window(a, p, live, live ? x0 : 0, live ? x1 : 0, live ? y0 : 0,
+                                  live ? y1 : 0, live ? z0 : 0, live ? z1 : 0, limit, w, W0, Y0,
+                                  Z0, nw, ny);
+            wave_sync_lds();
+            for (int t = (int)lane; t < n; t += 64) {
+                const uint32_t m = w[t];
+                if (m) {
+                    const int wi = t % nw, yi = (t / nw) % ny, zi = t / (nw * ny);
+                    const uint64_t row = (uint64_t)(Z0 + zi) * a.sc.ry + (uint32_t)(Y0 + yi);
+                    atomicOr(&a.bits[row * wx + (uint32_t)(W0 + wi)], m);
+                }
+            }
+            wave_sync_lds();
+        } else if (live) {
+            exception_rows(a, p, x0, x1, y0, y1, z0, z1, limit,
+                           [&](int z, int y, uint32_t word, uint32_t m) {
+                               atomicOr(&a.bits[((uint64_t)z * a.sc.ry + y) * wx + word], m);
+                           });
         }
     }
 }

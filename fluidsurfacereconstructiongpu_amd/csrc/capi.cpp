@@ -305,6 +305,9 @@ fsr_status frame_impl(const char* who, fsr_context* ctx, fsr_method method,
     ba.h = sim->smoothing_radius;
     ba.counters = counters;
     ba.st = stats;
+    ba.flags = flags;
+    ba.num_particles = n;
+    ba.all_exceptions = sa.all_exceptions;
     FSR_TRY(band_build(ctx, ba, s));
     BandCompactOut bc;
     FSR_TRY(band_compact(ctx, bits, sc, sp, yu13 ? 2 : 1, counters, stats, &bc, s));

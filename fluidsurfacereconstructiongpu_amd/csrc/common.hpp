@@ -391,6 +391,12 @@ struct BandArgs {
     float h;
     Counters* counters;
     fsr_frame_stats* st;
+    // When every surface particle takes the exact path, the exact pass walks
+    // the surface flags in particle order (sorted: neighbours share a wave)
+    // instead of the exception list, whose order is that of the appends.
+    const uint32_t* flags;
+    uint32_t num_particles;
+    int all_exceptions;
 };
 fsr_status band_build(fsr_context* ctx, const BandArgs& a, hipStream_t s);
 // Returns 1 when every vertex of the reference's band box lies within 1.5 h of

@@ -508,6 +508,9 @@ fsr_status fsr_estimation_of_surface_vertices(fsr_context* ctx, const fsr_sim_pa
     ba.h = sim->smoothing_radius;
     ba.counters = ct;
     ba.st = st;
+    ba.flags = surface_flags;
+    ba.num_particles = num_particles;
+    ba.all_exceptions = band_dilation_exact(*sc, sim->smoothing_radius) ? 0 : 1;
     FSR_TRY(band_build(ctx, ba, s));
     hipLaunchKernelGGL(k_bits_to_dense, dim3(blocks_for(V)), dim3(256), 0, s, bits, g.rx, wx, V,
                        is_surface_grid);

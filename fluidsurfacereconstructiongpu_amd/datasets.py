@@ -316,6 +316,18 @@ def small_block(nx=12, ny=10, nz=8, spacing=0.01, seed=BASE_SEED + 100, ratio=2,
     return _finish("block%dx%dx%d" % (nx, ny, nz), p, spacing, ("ratio", ratio), iso, method, seed)
 
 
+def scattered(n: int = 3000, extent: float = 1.6, spacing: float = 0.01,
+              seed: int = BASE_SEED + 101, ratio: int = 1, iso: float = -1e-4,
+              method: str = "zb05") -> Frame:
+    """Sparse particles, uniform in a cube of side `extent`, on the reference's
+    grid rule with scalar cell = hash cell / ratio: few particles per hash cell,
+    so 64 consecutive sorted particles span many cells (coarse-grid band path
+    with widely spread exceptions)."""
+    rng = SplitMix64(seed)
+    p = rng.uniform(3 * n).reshape(-1, 3) * extent
+    return _finish("scattered%d" % n, p, spacing, ("ratio", ratio), iso, method, seed)
+
+
 def sim_param_fields(fr: Frame, ratio: int = 2, lam: float = 0.5, scale: float = 1.0,
                      min_neighbors: int = 25) -> dict:
     f = np.float32

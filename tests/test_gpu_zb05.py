@@ -17,6 +17,8 @@ def _frames():
         "lattice16": lambda: ds.jittered_lattice(n_side=16, grid_res=64),
         "lattice40_128": lambda: ds.jittered_lattice(),
         "block_ratio2": lambda: ds.small_block(nx=20, ny=16, nz=12, spacing=0.01, ratio=2),
+        # coarse grid, every surface particle an exception, spread over many cells
+        "scattered_ratio1": lambda: ds.scattered(),
         "dambreak_mini": lambda: ds.dam_break(n_target=50_000, block=(60, 30, 30),
                                               sphere_center=(0.15, 0.28, 0.075),
                                               sphere_radius=0.04, grid_res=128),
