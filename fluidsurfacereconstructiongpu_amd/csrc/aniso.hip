@@ -33,6 +33,9 @@
 // v^T M v - 2 v.(M p) + p^T M p, where M p and p^T M p are formed once per staged
 // candidate; the per-pair cost is 9 fused multiply-adds.  Particles of
 // non-involved cells carry M = I and the unscaled kernel, as the reference has it.
+#include <cmath>
+#include <limits>
+
 #include "common.hpp"
 
 namespace fsr {
@@ -197,7 +200,9 @@ __global__ __launch_bounds__(kAnisoThreads) void k_smooth(
                     s2 = __builtin_fmaf(-ay, qq.y, s2);
                     s2 = __builtin_fmaf(-ax, qq.x, s2);
                     s2 = fmaxf(s2, 0.0f);
-                    const float w = fmaxf(1.0f - s2 * __builtin_sqrtf(s2), 0.0f);
+                    // hardware square root (within an ulp): the weight is a
+                    // value, not a decision
+                    const float w = fmaxf(1.0f - s2 * __builtin_amdgcn_sqrtf(s2), 0.0f);
                     sw += w;
                     sx = __builtin_fmaf(w, qq.x, sx);
                     sy = __builtin_fmaf(w, qq.y, sy);
@@ -320,6 +325,9 @@ struct CovArgs {
     const uint32_t* involve_off;
     GridDev sp;
     float radius;           // anisotropic radius r
+    // the reference's tests on the correctly rounded distance, as exact tests
+    // on d^2: sqrt(d2) <= r  <=>  d2 <= d2_le_r;  sqrt(d2) < r  <=>  d2 <= d2_lt_r
+    float d2_le_r, d2_lt_r;
     float h2;               // smoothing radius squared
     float inv_h;            // 1 / h
     float kr, kn;
@@ -372,10 +380,12 @@ __global__ __launch_bounds__(kAnisoThreads) void k_covariance(CovArgs a, ChunkLi
                     const float4 qq = q[t0 + t];
                     const float vx = qq.x - mx, vy = qq.y - my, vz = qq.z - mz;
                     const float d2 = __builtin_fmaf(vz, vz, __builtin_fmaf(vy, vy, vx * vx));
-                    const float dist = __builtin_sqrtf(d2);
-                    cnt += dist <= r ? 1u : 0u;
+                    // the count and the support test decide exactly (on d2);
+                    // the weight uses the hardware square root
+                    const float dist = __builtin_amdgcn_sqrtf(d2);
+                    cnt += d2 <= a.d2_le_r ? 1u : 0u;
                     const float s = dist * inv_r;
-                    const float w = dist < r ? 1.0f - s * s * s : 0.0f;
+                    const float w = d2 <= a.d2_lt_r ? 1.0f - s * s * s : 0.0f;
                     sw += w;
                     const float wx = w * vx, wy = w * vy;
                     c00 = __builtin_fmaf(wx, vx, c00);
@@ -677,6 +687,17 @@ __global__ __launch_bounds__(256) void k_chunks_of_cells(const uint32_t* __restr
 
 }  // namespace
 
+// Largest float t with sqrtf(t) <= r (strict: sqrtf(t) < r), sqrtf correctly
+// rounded; r > 0.
+static float sqrt_bound(float r, bool strict) {
+    auto ok = [&](float t) { return strict ? std::sqrt(t) < r : std::sqrt(t) <= r; };
+    const float inf = std::numeric_limits<float>::infinity();
+    float t = r * r;
+    while (t > 0.0f && !ok(t)) t = std::nextafter(t, 0.0f);
+    while (ok(std::nextafter(t, inf))) t = std::nextafter(t, inf);
+    return t;
+}
+
 fsr_status cell_chunk_list(fsr_context* ctx, const fsr_index_range* ranges,
                            const uint32_t* filter, uint32_t C, uint32_t* cnt,
                            uint32_t* chunk_off, uint32_t* chunk_cell, uint32_t* d_total,
@@ -764,6 +785,8 @@ fsr_status aniso_cov(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
     ca.involve_off = a.involve_off;
     ca.sp = a.sp;
     ca.radius = a.aniso_radius;
+    ca.d2_le_r = sqrt_bound(a.aniso_radius, false);
+    ca.d2_lt_r = sqrt_bound(a.aniso_radius, true);
     ca.h2 = a.h2;
     ca.inv_h = a.inv_h;
     ca.kr = a.kr;
