@@ -193,7 +193,7 @@ __global__ __launch_bounds__(256) void k_colorfield_tail(SurfaceArgs a, float ma
                         const float d2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
                         if (d2 > 0.0f) {
                             ++count;
-                            const float dist = __builtin_sqrtf(d2);
+                            const float dist = __builtin_amdgcn_sqrtf(d2);  // hardware square root
                             if (dist < h) {
                                 const float x = 1.0f - dist * inv_h;
                                 const float c = mass / q.w * gcoef * x * x * __builtin_amdgcn_rsqf(d2);
@@ -246,7 +246,9 @@ struct ColorChunks {
 // h * (1 + 5e-5) from the bounding box of the chunk's particles can only add to
 // the neighbour count (d^2 > 0, dist >= h): they are counted and dropped, the
 // others are compacted in order and evaluated with the per-thread kernel's
-// arithmetic, so counts, gradients and flags are identical to it.
+// arithmetic, so counts, gradients and flags are identical to it.  Both use
+// the hardware square root for the distance: it enters the gradient through
+// (1 - d/h)^2, which vanishes where the d < h test could flip.
 __global__ __launch_bounds__(256) void k_colorfield(SurfaceArgs a, float mass, ColorChunks cl) {
     __shared__ float4 queue[4][128];
     __shared__ uint32_t row_start[4][9];
@@ -312,7 +314,7 @@ __global__ __launch_bounds__(256) void k_colorfield(SurfaceArgs a, float mass, C
                 const float d2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
                 if (d2 > 0.0f) {
                     ++count;
-                    const float dist = __builtin_sqrtf(d2);
+                    const float dist = __builtin_amdgcn_sqrtf(d2);  // hardware square root
                     if (dist < h) {
                         const float x = 1.0f - dist * inv_h;
                         const float cc = qq.w * x * x * __builtin_amdgcn_rsqf(d2);
