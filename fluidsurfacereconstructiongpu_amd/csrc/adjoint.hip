@@ -57,12 +57,6 @@ __device__ __forceinline__ uint32_t corner_dz(uint32_t c) {
     return (c == 1 || c == 2 || c == 5 || c == 6) ? 1u : 0u;
 }
 
-__device__ __forceinline__ uint32_t case_vertices(uint32_t flag) {
-    const uint64_t x = cAdjTri[flag];
-    const uint64_t f = x & (x >> 1) & (x >> 2) & (x >> 3) & 0x1111111111111111ull;
-    return 16u - (uint32_t)__popcll(f);
-}
-
 // ---------------------------------------------------------------------------
 // vertex term: g_phi[v] += sum over soup vertices on edges at v
 // ---------------------------------------------------------------------------
