@@ -202,8 +202,6 @@ __global__ __launch_bounds__(kFieldThreads, 5) void k_field_zb05(FieldArgs a, Ba
         float queue[kQueueFloats];
         // per lane: x', y', z' of its vertex (for the end) and (a_x, a_y, a_z, c_x)
         float vtx[7][64];
-        uint32_t row_start[12];
-        uint32_t row_pref[12];
     };
     __shared__ __align__(16) WaveLds lds[kFieldThreads / 64];
     const uint32_t lane = __lane_id();
@@ -212,8 +210,6 @@ __global__ __launch_bounds__(kFieldThreads, 5) void k_field_zb05(FieldArgs a, Ba
     const float inv_r2 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(1.0f / a.radius_sq)));
     const float cull_r2 = a.radius_sq * 1.0001f;
     const int r = bc.radius;
-    uint32_t* row_start = lds[wave].row_start;
-    uint32_t* row_pref = lds[wave].row_pref;
     float* q = lds[wave].queue;
     q[kRowOne + lane] = 1.0f;
     q[kRowOne + 64 + lane] = 1.0f;
@@ -252,14 +248,13 @@ __global__ __launch_bounds__(kFieldThreads, 5) void k_field_zb05(FieldArgs a, Ba
             }
             if (s != 0xFFFFFFFFu) { rs = s; len = e - s; }
         }
-        // nrows <= 9: the prefix over the first 16 lanes is the whole prefix
+        // nrows <= 9: the prefix over the first 16 lanes is the whole prefix.
+        // Lane j < nrows keeps the first candidate index of row j (rp) and the
+        // offset from candidate index to sorted index in that row (ro).
         const uint32_t incl = row16_inclusive_sum(len);
         const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 15);
-        if ((int)lane < nrows) {
-            row_start[lane] = rs;
-            row_pref[lane] = incl - len;
-        }
-        wave_lds_sync();
+        const uint32_t rp = incl - len;
+        const uint32_t ro = rs - rp;
 
         // cell corner as the local origin
         const float ox = __builtin_fmaf((float)cx, a.sp.cell, a.sp.minx);
@@ -313,18 +308,37 @@ __global__ __launch_bounds__(kFieldThreads, 5) void k_field_zb05(FieldArgs a, Ba
             // this lane's candidate of the next round is loaded one round ahead,
             // so the load overlaps the culling and the accumulation of the
             // current one
-            int j = 0;  // row of this lane's candidate; g only grows, so the search resumes
+            //
+            // Row lookup: jc is a row whose first candidate index is at most
+            // the first index nb of the round being located (rows only grow
+            // with the index).  The rows that start inside the round are
+            // visited with wave-uniform reads of rp and ro; a lane takes the
+            // last one that starts at or before its index gg.  Empty rows
+            // share their first index with the next row, which is visited
+            // after them.
+            uint32_t jc = 0;
+            auto row_offset = [&](uint32_t nb, uint32_t gg) {
+                uint32_t off = (uint32_t)__builtin_amdgcn_readlane((int)ro, (int)jc);
+                for (uint32_t jj = jc + 1u; jj < (uint32_t)nrows; ++jj) {
+                    const uint32_t pj = (uint32_t)__builtin_amdgcn_readlane((int)rp, (int)jj);
+                    if (pj >= nb + 64u) break;
+                    if (gg >= pj) off = (uint32_t)__builtin_amdgcn_readlane((int)ro, (int)jj);
+                    jc = jj;
+                }
+                return off;
+            };
             float3 next = make_float3(0.f, 0.f, 0.f);
-            if (lane < total) {
-                while (j + 1 < nrows && row_pref[j + 1] <= lane) ++j;
-                next = load_xyz(a.sorted + row_start[j] + (lane - row_pref[j]));
+            {
+                const uint32_t off = row_offset(0u, lane);
+                if (lane < total) next = load_xyz(a.sorted + (off + lane));
             }
             for (uint32_t base = 0; base < total; base += 64u) {
                 const uint32_t g = base + lane;
                 const float3 p = next;
-                if (g + 64u < total) {
-                    while (j + 1 < nrows && row_pref[j + 1] <= g + 64u) ++j;
-                    next = load_xyz(a.sorted + row_start[j] + (g + 64u - row_pref[j]));
+                if (base + 64u < total) {
+                    const uint32_t gn = g + 64u;
+                    const uint32_t off = row_offset(base + 64u, gn);
+                    if (gn < total) next = load_xyz(a.sorted + (off + gn));
                 }
                 bool keep = false;
                 float px = 0.f, py = 0.f, pz = 0.f;
@@ -332,9 +346,11 @@ __global__ __launch_bounds__(kFieldThreads, 5) void k_field_zb05(FieldArgs a, Ba
                     px = p.x - ox;
                     py = p.y - oy;
                     pz = p.z - oz;
-                    const float dx = fmaxf(fmaxf(bx0 - px, px - bx1), 0.0f);
-                    const float dy = fmaxf(fmaxf(by0 - py, py - by1), 0.0f);
-                    const float dz = fmaxf(fmaxf(bz0 - pz, pz - bz1), 0.0f);
+                    // distance to the box per axis: p - clamp(p, lo, hi), whose
+                    // square equals that of max(lo - p, p - hi, 0) exactly
+                    const float dx = px - __builtin_amdgcn_fmed3f(px, bx0, bx1);
+                    const float dy = py - __builtin_amdgcn_fmed3f(py, by0, by1);
+                    const float dz = pz - __builtin_amdgcn_fmed3f(pz, bz0, bz1);
                     keep = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < cull_r2;
                 }
                 const uint64_t m = __ballot(keep);
