@@ -43,7 +43,9 @@ namespace fsr {
 namespace {
 
 constexpr int kAnisoThreads = 256;
+// rows of a 5^3 cell block, one per lane
 constexpr int kMaxRows = 25;
+static_assert(kMaxRows <= 64, "one block row per lane");
 constexpr float kSigma = 1.5666814710608448f;  // 315 / (64 pi)
 // Candidate culling of the field kernel: a particle whose support ellipsoid
 // (radius 1 / sigma_min(G)) stays 1% (in squared radius) away from the bounding
@@ -63,14 +65,33 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Rows (x-runs) of the clamped (2R+1)^3 block around cell (cx, cy, cz); lane k
-// computes row k.  Returns the total candidate count; row starts and
-// exclusive prefix sums are left in LDS.
+// Rows (x-runs) of the clamped (2R+1)^3 block around cell (cx, cy, cz), lane
+// k holding row k: rp = first candidate index of the row, ro = offset from
+// candidate index to sorted index.  Candidate indices are staged in rounds of
+// 64 in ascending order; offset(nb, gg) gives the offset of the row holding
+// index gg of the round that starts at nb.  jc, a row whose first index is at
+// most nb, only grows; the rows that start inside the round are visited with
+// wave-uniform reads, each lane taking the last one that starts at or before
+// its index (an empty row shares its first index with the next row, visited
+// after it).
+struct BlockRows {
+    uint32_t rp, ro, total, jc;
+    int nrows;
+    __device__ __forceinline__ uint32_t offset(uint32_t nb, uint32_t gg) {
+        uint32_t off = (uint32_t)__builtin_amdgcn_readlane((int)ro, (int)jc);
+        for (uint32_t jj = jc + 1u; jj < (uint32_t)nrows; ++jj) {
+            const uint32_t pj = (uint32_t)__builtin_amdgcn_readlane((int)rp, (int)jj);
+            if (pj >= nb + 64u) break;
+            if (gg >= pj) off = (uint32_t)__builtin_amdgcn_readlane((int)ro, (int)jj);
+            jc = jj;
+        }
+        return off;
+    }
+};
+
 template <int R>
-__device__ __forceinline__ uint32_t block_rows(const fsr_index_range* __restrict__ ranges,
-                                               const GridDev& sp, int cx, int cy, int cz,
-                                               uint32_t* row_start, uint32_t* row_pref,
-                                               int* nrows_out) {
+__device__ __forceinline__ BlockRows block_rows(const fsr_index_range* __restrict__ ranges,
+                                                const GridDev& sp, int cx, int cy, int cz) {
     const uint32_t lane = __lane_id();
     const int x0 = clampi(cx - R, 0, sp.rx - 1), x1 = clampi(cx + R, 0, sp.rx - 1);
     const int y0 = clampi(cy - R, 0, sp.ry - 1), y1 = clampi(cy + R, 0, sp.ry - 1);
@@ -92,24 +113,14 @@ __device__ __forceinline__ uint32_t block_rows(const fsr_index_range* __restrict
         if (s != 0xFFFFFFFFu) { rs = s; len = e - s; }
     }
     const uint32_t incl = wave_inclusive_sum(len);
-    const uint32_t total = __shfl(incl, 63, 64);
-    if ((int)lane < nrows) {
-        row_start[lane] = rs;
-        row_pref[lane] = incl - len;
-    }
-    *nrows_out = nrows;
-    wave_lds_sync();
-    return total;
+    BlockRows b;
+    b.total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+    b.rp = incl - len;
+    b.ro = rs - b.rp;
+    b.jc = 0;
+    b.nrows = nrows;
+    return b;
 }
-
-__device__ __forceinline__ uint32_t candidate_index(uint32_t g, int nrows,
-                                                    const uint32_t* row_start,
-                                                    const uint32_t* row_pref) {
-    int j = 0;
-    while (j + 1 < nrows && row_pref[j + 1] <= g) ++j;
-    return row_start[j] + (g - row_pref[j]);
-}
-
 
 struct ChunkList {
     const uint32_t* chunk_cell;  // chunk -> cell
@@ -134,9 +145,10 @@ __device__ __forceinline__ float wave_maxf(float v) {
 // Squared distance from (x, y, z) to the box [b0, b1].
 __device__ __forceinline__ float box_dist2(float x, float y, float z, const float* b0,
                                            const float* b1) {
-    const float dx = fmaxf(fmaxf(b0[0] - x, x - b1[0]), 0.0f);
-    const float dy = fmaxf(fmaxf(b0[1] - y, y - b1[1]), 0.0f);
-    const float dz = fmaxf(fmaxf(b0[2] - z, z - b1[2]), 0.0f);
+    // x - clamp(x, lo, hi): its square equals that of max(lo - x, x - hi, 0)
+    const float dx = x - __builtin_amdgcn_fmed3f(x, b0[0], b1[0]);
+    const float dy = y - __builtin_amdgcn_fmed3f(y, b0[1], b1[1]);
+    const float dz = z - __builtin_amdgcn_fmed3f(z, b0[2], b1[2]);
     return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
 }
 
@@ -159,8 +171,6 @@ __global__ __launch_bounds__(kAnisoThreads) void k_smooth(
     float radius, float lambda, ChunkList cl, float4* __restrict__ mean_out,
     float4* __restrict__ smooth_out) {
     __shared__ float4 queue[kAnisoThreads / 64][128];
-    __shared__ uint32_t rstart[kAnisoThreads / 64][kMaxRows];
-    __shared__ uint32_t rpref[kAnisoThreads / 64][kMaxRows];
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x >> 6;
     float4* q = queue[wave];
@@ -177,8 +187,8 @@ __global__ __launch_bounds__(kAnisoThreads) void k_smooth(
         const uint32_t i = own.start + 64u * k + lane;
         const bool live = i < own.end;
         const int cx = (int)(c % sp.rx), cy = (int)((c / sp.rx) % sp.ry), cz = (int)(c / (sp.rx * sp.ry));
-        int nrows;
-        const uint32_t total = block_rows<2>(ranges, sp, cx, cy, cz, rstart[wave], rpref[wave], &nrows);
+        BlockRows br = block_rows<2>(ranges, sp, cx, cy, cz);
+        const uint32_t total = br.total;
         const float ox = __builtin_fmaf((float)cx, sp.cell, sp.minx);
         const float oy = __builtin_fmaf((float)cy, sp.cell, sp.miny);
         const float oz = __builtin_fmaf((float)cz, sp.cell, sp.minz);
@@ -213,12 +223,17 @@ __global__ __launch_bounds__(kAnisoThreads) void k_smooth(
         uint32_t fill = 0;
         // each lane's candidate is loaded one staging round ahead
         float4 next = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (lane < total) next = sorted[candidate_index(lane, nrows, rstart[wave], rpref[wave])];
+        {
+            const uint32_t off = br.offset(0u, lane);
+            if (lane < total) next = sorted[off + lane];
+        }
         for (uint32_t base = 0; base < total; base += 64u) {
             const uint32_t g = base + lane;
             const float4 cand = next;
-            if (g + 64u < total)
-                next = sorted[candidate_index(g + 64u, nrows, rstart[wave], rpref[wave])];
+            if (base + 64u < total) {
+                const uint32_t off = br.offset(base + 64u, g + 64u);
+                if (g + 64u < total) next = sorted[off + g + 64u];
+            }
             bool keep = false;
             float4 st;
             if (g < total) {
@@ -340,8 +355,6 @@ struct CovArgs {
 
 __global__ __launch_bounds__(kAnisoThreads) void k_covariance(CovArgs a, ChunkList cl) {
     __shared__ float4 queue[kAnisoThreads / 64][128];
-    __shared__ uint32_t rstart[kAnisoThreads / 64][kMaxRows];
-    __shared__ uint32_t rpref[kAnisoThreads / 64][kMaxRows];
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x >> 6;
     float4* q = queue[wave];
@@ -360,9 +373,8 @@ __global__ __launch_bounds__(kAnisoThreads) void k_covariance(CovArgs a, ChunkLi
         const bool live = i < own.end;
         const int cx = (int)(c % a.sp.rx), cy = (int)((c / a.sp.rx) % a.sp.ry),
                   cz = (int)(c / (a.sp.rx * a.sp.ry));
-        int nrows;
-        const uint32_t total =
-            block_rows<2>(a.ranges, a.sp, cx, cy, cz, rstart[wave], rpref[wave], &nrows);
+        BlockRows br = block_rows<2>(a.ranges, a.sp, cx, cy, cz);
+        const uint32_t total = br.total;
         const float ox = __builtin_fmaf((float)cx, a.sp.cell, a.sp.minx);
         const float oy = __builtin_fmaf((float)cy, a.sp.cell, a.sp.miny);
         const float oz = __builtin_fmaf((float)cz, a.sp.cell, a.sp.minz);
@@ -400,12 +412,17 @@ __global__ __launch_bounds__(kAnisoThreads) void k_covariance(CovArgs a, ChunkLi
         uint32_t fill = 0;
         // each lane's candidate is loaded one staging round ahead
         float4 next = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (lane < total) next = a.sorted[candidate_index(lane, nrows, rstart[wave], rpref[wave])];
+        {
+            const uint32_t off = br.offset(0u, lane);
+            if (lane < total) next = a.sorted[off + lane];
+        }
         for (uint32_t base = 0; base < total; base += 64u) {
             const uint32_t g = base + lane;
             const float4 cand = next;
-            if (g + 64u < total)
-                next = a.sorted[candidate_index(g + 64u, nrows, rstart[wave], rpref[wave])];
+            if (base + 64u < total) {
+                const uint32_t off = br.offset(base + 64u, g + 64u);
+                if (g + 64u < total) next = a.sorted[off + g + 64u];
+            }
             bool keep = false;
             float4 st;
             if (g < total) {
@@ -544,8 +561,6 @@ __device__ __forceinline__ float aniso_pairs(const float4* __restrict__ q0s,
 
 __global__ __launch_bounds__(kAnisoThreads) void k_field_aniso(AnisoFieldArgs a) {
     __shared__ float4 queue[kAnisoThreads / 64][3][128];
-    __shared__ uint32_t rstart[kAnisoThreads / 64][kMaxRows];
-    __shared__ uint32_t rpref[kAnisoThreads / 64][kMaxRows];
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t nch = *a.num_chunks;
@@ -564,9 +579,8 @@ __global__ __launch_bounds__(kAnisoThreads) void k_field_aniso(AnisoFieldArgs a)
         const uint32_t k = ch - a.bc.chunk_off[v];
         const uint32_t first = a.bc.cell_off[v] + 64u * k;
         const uint32_t cnt = min(64u, a.bc.cell_cnt[v] - 64u * k);
-        int nrows;
-        const uint32_t total =
-            block_rows<2>(a.ranges, a.sp, cx, cy, cz, rstart[wave], rpref[wave], &nrows);
+        BlockRows br = block_rows<2>(a.ranges, a.sp, cx, cy, cz);
+        const uint32_t total = br.total;
         const float ox = __builtin_fmaf((float)cx, a.sp.cell, a.sp.minx);
         const float oy = __builtin_fmaf((float)cy, a.sp.cell, a.sp.miny);
         const float oz = __builtin_fmaf((float)cz, a.sp.cell, a.sp.minz);
@@ -594,8 +608,9 @@ __global__ __launch_bounds__(kAnisoThreads) void k_field_aniso(AnisoFieldArgs a)
         // each lane's candidate (position, two record words) is loaded one
         // staging round ahead
         float4 np4 = make_float4(0.f, 0.f, 0.f, 0.f), nr0 = np4, nr1 = np4;
+        const uint32_t off0 = br.offset(0u, lane);
         if (lane < total) {
-            const uint32_t j = candidate_index(lane, nrows, rstart[wave], rpref[wave]);
+            const uint32_t j = off0 + lane;
             np4 = a.smooth[j];
             nr0 = a.rec[2ull * j + 0];
             nr1 = a.rec[2ull * j + 1];
@@ -603,8 +618,9 @@ __global__ __launch_bounds__(kAnisoThreads) void k_field_aniso(AnisoFieldArgs a)
         for (uint32_t base = 0; base < total; base += 64u) {
             const uint32_t g = base + lane;
             const float4 p4 = np4, r0 = nr0, r1 = nr1;
+            const uint32_t off = base + 64u < total ? br.offset(base + 64u, g + 64u) : 0u;
             if (g + 64u < total) {
-                const uint32_t j = candidate_index(g + 64u, nrows, rstart[wave], rpref[wave]);
+                const uint32_t j = off + g + 64u;
                 np4 = a.smooth[j];
                 nr0 = a.rec[2ull * j + 0];
                 nr1 = a.rec[2ull * j + 1];
@@ -613,9 +629,9 @@ __global__ __launch_bounds__(kAnisoThreads) void k_field_aniso(AnisoFieldArgs a)
             float4 s0, s1, s2;
             if (g < total) {
                 const float px = p4.x - ox, py = p4.y - oy, pz = p4.z - oz;
-                const float dx = fmaxf(fmaxf(bx0 - px, px - bx1), 0.0f);
-                const float dy = fmaxf(fmaxf(by0 - py, py - by1), 0.0f);
-                const float dz = fmaxf(fmaxf(bz0 - pz, pz - bz1), 0.0f);
+                const float dx = px - __builtin_amdgcn_fmed3f(px, bx0, bx1);
+                const float dy = py - __builtin_amdgcn_fmed3f(py, by0, by1);
+                const float dz = pz - __builtin_amdgcn_fmed3f(pz, bz0, bz1);
                 keep = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < r1.w;
                 const float M00 = r0.x, M11 = r0.y, M22 = r0.z, M01 = r0.w, M02 = r1.x, M12 = r1.y;
                 const float bx = M00 * px + M01 * py + M02 * pz;
