@@ -279,8 +279,6 @@ __device__ __forceinline__ void gather_pairs(const float4* __restrict__ qx,
 __global__ __launch_bounds__(kAdjThreads) void k_field_zb05_backward(FieldBackArgs a) {
     __shared__ float4 queue_x[kAdjThreads / 64][128];
     __shared__ float4 queue_u[kAdjThreads / 64][128];
-    __shared__ uint32_t row_start[kAdjThreads / 64][9];
-    __shared__ uint32_t row_pref[kAdjThreads / 64][10];
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t nch = *a.num_chunks;
@@ -308,13 +306,28 @@ __global__ __launch_bounds__(kAdjThreads) void k_field_zb05_backward(FieldBackAr
             rs = a.cell_off[v0];
             len = a.cell_off[v0 + 2] + a.cell_cnt[v0 + 2] - rs;
         }
+        // lane j < 9 holds run j's first index (rp) and the offset from index
+        // to band slot in that run (ro)
         const uint32_t incl = wave_inclusive_sum(len);
-        const uint32_t total = __shfl(incl, 63, 64);
-        if (lane < 9) {
-            row_start[wave][lane] = rs;
-            row_pref[wave][lane] = incl - len;
-        }
-        wave_lds_sync();
+        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+        const uint32_t rp = incl - len;
+        const uint32_t ro = rs - rp;
+        // Run of index gg of the staging round that starts at nb: jc, a run
+        // whose first index is at most nb, only grows; the runs that start
+        // inside the round are visited with wave-uniform reads, and a lane
+        // takes the last one that starts at or before gg (an empty run shares
+        // its first index with the next run, visited after it).
+        uint32_t jc = 0;
+        auto run_offset = [&](uint32_t nb, uint32_t gg) {
+            uint32_t off = (uint32_t)__builtin_amdgcn_readlane((int)ro, (int)jc);
+            for (uint32_t jj = jc + 1u; jj < 9u; ++jj) {
+                const uint32_t pj = (uint32_t)__builtin_amdgcn_readlane((int)rp, (int)jj);
+                if (pj >= nb + 64u) break;
+                if (gg >= pj) off = (uint32_t)__builtin_amdgcn_readlane((int)ro, (int)jj);
+                jc = jj;
+            }
+            return off;
+        };
 
         const float ox = __builtin_fmaf((float)qcx, a.sp.cell, a.sp.minx);
         const float oy = __builtin_fmaf((float)qcy, a.sp.cell, a.sp.miny);
@@ -333,22 +346,25 @@ __global__ __launch_bounds__(kAdjThreads) void k_field_zb05_backward(FieldBackAr
         GradSums g = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         uint32_t fill = 0;
         // this lane's vertex of the next round is loaded one round ahead
-        int j = 0;  // run of this lane's vertex; gi only grows, so the search resumes
         float4 ncf = make_float4(0.f, 0.f, 0.f, 0.f), nvp = ncf;
-        if (lane < total) {
-            while (j + 1 < 9 && row_pref[wave][j + 1] <= lane) ++j;
-            const uint32_t slot = row_start[wave][j] + (lane - row_pref[wave][j]);
-            ncf = a.coef[slot];
-            nvp = a.vpos[slot];
+        {
+            const uint32_t off = run_offset(0u, lane);
+            if (lane < total) {
+                const uint32_t slot = off + lane;
+                ncf = a.coef[slot];
+                nvp = a.vpos[slot];
+            }
         }
         for (uint32_t base = 0; base < total; base += 64u) {
             const uint32_t gi = base + lane;
             const float4 cf = ncf, vp = nvp;
-            if (gi + 64u < total) {
-                while (j + 1 < 9 && row_pref[wave][j + 1] <= gi + 64u) ++j;
-                const uint32_t slot = row_start[wave][j] + (gi + 64u - row_pref[wave][j]);
-                ncf = a.coef[slot];
-                nvp = a.vpos[slot];
+            if (base + 64u < total) {
+                const uint32_t off = run_offset(base + 64u, gi + 64u);
+                if (gi + 64u < total) {
+                    const uint32_t slot = off + gi + 64u;
+                    ncf = a.coef[slot];
+                    nvp = a.vpos[slot];
+                }
             }
             bool keep = false;
             float4 sx, su;
@@ -357,9 +373,9 @@ __global__ __launch_bounds__(kAdjThreads) void k_field_zb05_backward(FieldBackAr
                     const float xx = vp.x - ox;
                     const float xy = vp.y - oy;
                     const float xz = vp.z - oz;
-                    const float dx = fmaxf(fmaxf(bx0 - xx, xx - bx1), 0.0f);
-                    const float dy = fmaxf(fmaxf(by0 - xy, xy - by1), 0.0f);
-                    const float dz = fmaxf(fmaxf(bz0 - xz, xz - bz1), 0.0f);
+                    const float dx = xx - __builtin_amdgcn_fmed3f(xx, bx0, bx1);
+                    const float dy = xy - __builtin_amdgcn_fmed3f(xy, by0, by1);
+                    const float dz = xz - __builtin_amdgcn_fmed3f(xz, bz0, bz1);
                     keep = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < cull_r2;
                     sx = make_float4(xx, xy, xz,
                                      __builtin_fmaf(xz, xz, __builtin_fmaf(xy, xy, xx * xx)) * inv_r2);

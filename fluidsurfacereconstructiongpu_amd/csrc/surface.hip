@@ -251,8 +251,6 @@ struct ColorChunks {
 // (1 - d/h)^2, which vanishes where the d < h test could flip.
 __global__ __launch_bounds__(256) void k_colorfield(SurfaceArgs a, float mass, ColorChunks cl) {
     __shared__ float4 queue[4][128];
-    __shared__ uint32_t row_start[4][9];
-    __shared__ uint32_t row_pref[4][10];
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x >> 6;
     float4* q = queue[wave];
@@ -292,13 +290,28 @@ __global__ __launch_bounds__(256) void k_colorfield(SurfaceArgs a, float mass, C
             }
             if (s0 != 0xFFFFFFFFu) { rs = s0; len = e0 - s0; }
         }
+        // lane j < nrows holds row j's first candidate index (rp) and the
+        // offset from candidate index to sorted index in that row (ro)
         const uint32_t incl = wave_inclusive_sum(len);
-        const uint32_t total = __shfl(incl, 63, 64);
-        if ((int)lane < nrows) {
-            row_start[wave][lane] = rs;
-            row_pref[wave][lane] = incl - len;
-        }
-        wave_lds_sync();
+        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+        const uint32_t rp = incl - len;
+        const uint32_t ro = rs - rp;
+        // Row of candidate index gg of the staging round that starts at nb:
+        // jc, a row whose first index is at most nb, only grows; the rows
+        // that start inside the round are visited with wave-uniform reads, and
+        // a lane takes the last one that starts at or before gg (an empty row
+        // shares its first index with the next row, visited after it).
+        uint32_t jc = 0;
+        auto row_offset = [&](uint32_t nb, uint32_t gg) {
+            uint32_t off = (uint32_t)__builtin_amdgcn_readlane((int)ro, (int)jc);
+            for (uint32_t jj = jc + 1u; jj < (uint32_t)nrows; ++jj) {
+                const uint32_t pj = (uint32_t)__builtin_amdgcn_readlane((int)rp, (int)jj);
+                if (pj >= nb + 64u) break;
+                if (gg >= pj) off = (uint32_t)__builtin_amdgcn_readlane((int)ro, (int)jj);
+                jc = jj;
+            }
+            return off;
+        };
         float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
         if (live) p = a.sorted[i];
         const float bx0 = wave_minf(live ? p.x : 3.0e38f), bx1 = wave_maxf(live ? p.x : -3.0e38f);
@@ -306,7 +319,6 @@ __global__ __launch_bounds__(256) void k_colorfield(SurfaceArgs a, float mass, C
         const float bz0 = wave_minf(live ? p.z : 3.0e38f), bz1 = wave_maxf(live ? p.z : -3.0e38f);
         uint32_t count = 0, kept = 0, fill = 0;
         float gx = 0.f, gy = 0.f, gz = 0.f;
-        int j = 0;
         auto run = [&](uint32_t n) {
             for (uint32_t t = 0; t < n; ++t) {
                 const float4 qq = q[t];
@@ -327,23 +339,24 @@ __global__ __launch_bounds__(256) void k_colorfield(SurfaceArgs a, float mass, C
         };
         // each lane's candidate is loaded one staging round ahead
         float4 next = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (lane < total) {
-            while (j + 1 < nrows && row_pref[wave][j + 1] <= lane) ++j;
-            next = a.sorted[row_start[wave][j] + (lane - row_pref[wave][j])];
+        {
+            const uint32_t off = row_offset(0u, lane);
+            if (lane < total) next = a.sorted[off + lane];
         }
         for (uint32_t base = 0; base < total; base += 64u) {
             const uint32_t g = base + lane;
             const float4 cand = next;
-            if (g + 64u < total) {
-                while (j + 1 < nrows && row_pref[wave][j + 1] <= g + 64u) ++j;
-                next = a.sorted[row_start[wave][j] + (g + 64u - row_pref[wave][j])];
+            if (base + 64u < total) {
+                const uint32_t off = row_offset(base + 64u, g + 64u);
+                if (g + 64u < total) next = a.sorted[off + g + 64u];
             }
             bool keep = false;
             float4 st;
             if (g < total) {
-                const float dx = fmaxf(fmaxf(bx0 - cand.x, cand.x - bx1), 0.0f);
-                const float dy = fmaxf(fmaxf(by0 - cand.y, cand.y - by1), 0.0f);
-                const float dz = fmaxf(fmaxf(bz0 - cand.z, cand.z - bz1), 0.0f);
+                // c - clamp(c, lo, hi): its square equals that of max(lo - c, c - hi, 0)
+                const float dx = cand.x - __builtin_amdgcn_fmed3f(cand.x, bx0, bx1);
+                const float dy = cand.y - __builtin_amdgcn_fmed3f(cand.y, by0, by1);
+                const float dz = cand.z - __builtin_amdgcn_fmed3f(cand.z, bz0, bz1);
                 keep = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < cull2;
                 // (m / rho_j) * gcoef, in the per-thread kernel's operation order
                 st = make_float4(cand.x, cand.y, cand.z, mass / cand.w * gcoef);
