@@ -605,50 +605,75 @@ __global__ __launch_bounds__(kAnisoThreads) void k_field_aniso(AnisoFieldArgs a)
         const float bz0 = wave_min(live ? vz : 3.0e38f), bz1 = wave_max(live ? vz : -3.0e38f);
         float sum = 0.f;
         uint32_t fill = 0;
-        // each lane's candidate (position, two record words) is loaded one
-        // staging round ahead
-        float4 np4 = make_float4(0.f, 0.f, 0.f, 0.f), nr0 = np4, nr1 = np4;
-        const uint32_t off0 = br.offset(0u, lane);
-        if (lane < total) {
-            const uint32_t j = off0 + lane;
-            np4 = a.smooth[j];
-            nr0 = a.rec[2ull * j + 0];
-            nr1 = a.rec[2ull * j + 1];
+        // Staging in two steps.  Each lane's candidate position (with its
+        // squared culling radius in w) is loaded one round ahead; after the
+        // box test, the two record words are loaded for the survivors only
+        // (a dropped candidate reads record 0, one shared line) and enter the
+        // queue one round later, so their load overlaps the next round's
+        // test.  Queue order stays candidate order.
+        float4 np4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        uint32_t nj;
+        {
+            nj = br.offset(0u, lane) + lane;
+            if (lane < total) np4 = a.smooth[nj];
         }
-        for (uint32_t base = 0; base < total; base += 64u) {
+        bool keep_p = false;
+        float px_p = 0.f, py_p = 0.f, pz_p = 0.f;
+        float4 r0_p = make_float4(0.f, 0.f, 0.f, 0.f), r1_p = r0_p;
+        for (uint32_t base = 0; base < total + 64u; base += 64u) {
             const uint32_t g = base + lane;
-            const float4 p4 = np4, r0 = nr0, r1 = nr1;
-            const uint32_t off = base + 64u < total ? br.offset(base + 64u, g + 64u) : 0u;
-            if (g + 64u < total) {
-                const uint32_t j = off + g + 64u;
-                np4 = a.smooth[j];
-                nr0 = a.rec[2ull * j + 0];
-                nr1 = a.rec[2ull * j + 1];
-            }
             bool keep = false;
+            float px = 0.f, py = 0.f, pz = 0.f;
+            float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0;
+            if (base < total) {
+                const float4 p4 = np4;
+                const uint32_t j = nj;
+                if (base + 64u < total) {
+                    nj = br.offset(base + 64u, g + 64u) + g + 64u;
+                    if (g + 64u < total) np4 = a.smooth[nj];
+                }
+                if (g < total) {
+                    px = p4.x - ox;
+                    py = p4.y - oy;
+                    pz = p4.z - oz;
+                    const float dx = px - __builtin_amdgcn_fmed3f(px, bx0, bx1);
+                    const float dy = py - __builtin_amdgcn_fmed3f(py, by0, by1);
+                    const float dz = pz - __builtin_amdgcn_fmed3f(pz, bz0, bz1);
+                    keep = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < p4.w;
+                }
+                const uint32_t jr = keep ? j : 0u;
+                r0 = a.rec[2ull * jr + 0];
+                r1 = a.rec[2ull * jr + 1];
+            }
+            if (base == 0) {
+                keep_p = keep;
+                px_p = px; py_p = py; pz_p = pz;
+                r0_p = r0; r1_p = r1;
+                continue;
+            }
+            // the previous round's survivors enter the queue
             float4 s0, s1, s2;
-            if (g < total) {
-                const float px = p4.x - ox, py = p4.y - oy, pz = p4.z - oz;
-                const float dx = px - __builtin_amdgcn_fmed3f(px, bx0, bx1);
-                const float dy = py - __builtin_amdgcn_fmed3f(py, by0, by1);
-                const float dz = pz - __builtin_amdgcn_fmed3f(pz, bz0, bz1);
-                keep = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < r1.w;
-                const float M00 = r0.x, M11 = r0.y, M22 = r0.z, M01 = r0.w, M02 = r1.x, M12 = r1.y;
-                const float bx = M00 * px + M01 * py + M02 * pz;
-                const float by = M01 * px + M11 * py + M12 * pz;
-                const float bz = M02 * px + M12 * py + M22 * pz;
-                const float cc = px * bx + py * by + pz * bz;
+            if (keep_p) {
+                const float M00 = r0_p.x, M11 = r0_p.y, M22 = r0_p.z, M01 = r0_p.w;
+                const float M02 = r1_p.x, M12 = r1_p.y;
+                const float bx = M00 * px_p + M01 * py_p + M02 * pz_p;
+                const float by = M01 * px_p + M11 * py_p + M12 * pz_p;
+                const float bz = M02 * px_p + M12 * py_p + M22 * pz_p;
+                const float cc = px_p * bx + py_p * by + pz_p * bz;
                 s0 = make_float4(M00, M11, M22, M01);
                 s1 = make_float4(M02, M12, bx, by);
-                s2 = make_float4(bz, cc, r1.z, 0.f);
+                s2 = make_float4(bz, cc, r1_p.z, 0.f);
             }
-            const uint64_t m = __ballot(keep);
-            if (keep) {
+            const uint64_t m = __ballot(keep_p);
+            if (keep_p) {
                 const uint32_t o = fill + mask_rank(m);
                 q0s[o] = s0;
                 q1s[o] = s1;
                 q2s[o] = s2;
             }
+            keep_p = keep;
+            px_p = px; py_p = py; pz_p = pz;
+            r0_p = r0; r1_p = r1;
             fill += (uint32_t)__popcll(m);
             if (fill >= 64u) {
                 wave_lds_sync();
@@ -678,6 +703,16 @@ __global__ __launch_bounds__(kAnisoThreads) void k_field_aniso(AnisoFieldArgs a)
         }
         if (live) a.phi[vid] = sum - 0.5f;
     }
+}
+
+// The field kernel tests a candidate's box distance against the candidate's
+// squared culling radius (w of its second record word) before it reads the
+// records: a copy of that radius goes into the w of the smoothed position.
+__global__ __launch_bounds__(256) void k_cull_radius_to_w(const float4* __restrict__ rec,
+                                                          uint32_t n, float4* __restrict__ smooth) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    reinterpret_cast<float*>(smooth)[4ull * i + 3] = rec[2ull * i + 1].w;
 }
 
 __global__ __launch_bounds__(256) void k_cell_counts(const fsr_index_range* __restrict__ ranges,
@@ -837,6 +872,11 @@ fsr_status aniso_field(fsr_context* ctx, const AnisoArgs& a, const BandCompactOu
     fa.phi = phi;
     fa.next = a.counters->xcd_next;
     fa.num_chunks = &a.counters->num_chunks;
+    if (a.n) {
+        hipLaunchKernelGGL(k_cull_radius_to_w, dim3((a.n + 255) / 256), dim3(256), 0, s, a.rec,
+                           a.n, a.smooth);
+        FSR_LAUNCH_CHECK("k_cull_radius_to_w");
+    }
     FSR_TRY(reset_xcd(a.counters, s));
     hipLaunchKernelGGL(k_field_aniso, dim3(blocks), dim3(kAnisoThreads), 0, s, fa);
     FSR_LAUNCH_CHECK("k_field_aniso");

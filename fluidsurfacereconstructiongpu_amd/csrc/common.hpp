@@ -474,7 +474,7 @@ struct AnisoArgs {
     float mass;
     uint32_t min_neighbors;
     float4* mean;                 // [N] xyz
-    float4* smooth;               // [N] xyz
+    float4* smooth;               // [N] xyz; w = squared culling radius once the field starts
     uint32_t* involved_idx;       // [I]
     fsr_matrix* gmat;             // [I]
     float4* sigma;                // [I] covariance eigenvalues, descending

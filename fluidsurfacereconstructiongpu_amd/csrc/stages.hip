@@ -836,6 +836,7 @@ fsr_status fsr_computation_of_scalar_field_grid(
                            num_involve_scan, g, densities, sim->particle_mass, C, a.rec);
         FSR_LAUNCH_CHECK("k_rec_from_g");
     }
+    a.n = n;
     a.ranges = ranges;
     a.sp = to_dev(*sp);
     a.sc = to_dev(*sc);
