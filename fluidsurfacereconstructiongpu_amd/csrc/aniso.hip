@@ -42,6 +42,11 @@ namespace fsr {
 
 namespace {
 
+// Workgroups of 4 waves.  The neighbour kernels ask for 6 (smoothing,
+// covariance) and 5 (field) resident workgroups per CU, i.e. waves per SIMD:
+// the register limit that implies (80 and 96 VGPRs) leaves the first two
+// without spills and the field with 4 spilled registers, and each ran faster
+// than at the 4 waves per SIMD its unbounded register use allowed.
 constexpr int kAnisoThreads = 256;
 // rows of a 5^3 cell block, one per lane
 constexpr int kMaxRows = 25;
@@ -166,7 +171,7 @@ __device__ __forceinline__ void wave_box(bool live, float x, float y, float z, f
 // 128-entry queue (as in the field kernels), so the sums are bit-identical.
 constexpr float kRadiusCull = 1.001f;
 
-__global__ __launch_bounds__(kAnisoThreads) void k_smooth(
+__global__ __launch_bounds__(kAnisoThreads, 6) void k_smooth(
     const float4* __restrict__ sorted, const fsr_index_range* __restrict__ ranges, GridDev sp,
     float radius, float lambda, ChunkList cl, float4* __restrict__ mean_out,
     float4* __restrict__ smooth_out) {
@@ -353,7 +358,7 @@ struct CovArgs {
     float4* rec;            // [2N] field records
 };
 
-__global__ __launch_bounds__(kAnisoThreads) void k_covariance(CovArgs a, ChunkList cl) {
+__global__ __launch_bounds__(kAnisoThreads, 6) void k_covariance(CovArgs a, ChunkList cl) {
     __shared__ float4 queue[kAnisoThreads / 64][128];
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x >> 6;
@@ -559,7 +564,7 @@ __device__ __forceinline__ float aniso_pairs(const float4* __restrict__ q0s,
     return sum;
 }
 
-__global__ __launch_bounds__(kAnisoThreads) void k_field_aniso(AnisoFieldArgs a) {
+__global__ __launch_bounds__(kAnisoThreads, 5) void k_field_aniso(AnisoFieldArgs a) {
     __shared__ float4 queue[kAnisoThreads / 64][3][128];
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x >> 6;
