@@ -194,7 +194,7 @@ __device__ __forceinline__ void accumulate(const float* __restrict__ rows, uint3
     }
 }
 
-__global__ __launch_bounds__(kFieldThreads, 5) void k_field_zb05(FieldArgs a, BandCompactOut bc,
+__global__ __launch_bounds__(kFieldThreads, 6) void k_field_zb05(FieldArgs a, BandCompactOut bc,
                                                               VertexDiv vd) {
     // one LDS block per wave, so every per-lane LDS address is one base
     // register plus an immediate offset
