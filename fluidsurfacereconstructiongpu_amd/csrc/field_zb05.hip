@@ -18,9 +18,9 @@
 //   t = 1 - |x - p|^2/R^2 = c_x + a_x . p' - q_p
 // with p' = p - o and x' = x - o relative to the cell corner o, a = 2 x'/R^2,
 // c_x = 1 - |x'|^2/R^2 and q_p = |p'|^2/R^2, is a K = 4 product (one
-// v_mfma_f32_16x16x4_f32 per tile), w = max(t, 0)^3 takes 3 VALU ops per
+// v_mfma_f32_16x16x4_f32 per tile), w = max(t, 0)^3 takes 2 VALU ops per
 // pair, and the sums of w and w p' are rank-1 updates of 4 x 4 blocks
-// (v_mfma_f32_4x4x1_16b_f32), so a pair costs 3 VALU ops where the scalar
+// (v_mfma_f32_4x4x1_16b_f32), so a pair costs 2 VALU ops where the scalar
 // form needs 11.  Coordinates relative to the cell keep the expansion well
 // conditioned and make |x - xbar| free of the cancellation the
 // absolute-coordinate sum has.  Candidates are staged in the reference's
@@ -144,7 +144,7 @@ constexpr int kQueueFloats = 704;
 //      16 x 16 pairs: A = (x', y', z', 1) of the candidates (rows),
 //      B = (a_x, a_y, a_z, c_x) of the vertices (columns), C = -q_p;
 //      lane l holds t of candidates 4(l/16)+r (r = 0..3) for vertex l%16;
-//   2. w = max(t, 0)^3 (3 VALU ops per value);
+//   2. w = max(t, 0)^3 (2 VALU ops per value: t^3 with a clamp to [0, 1]);
 //   3. four v_mfma_f32_4x4x1_16b_f32 (one per r) accumulate, in each 4 x 4
 //      block b = l/4, the outer product of (x', y', z', 1) of candidate
 //      4(b/4)+r with the weights of vertices 4(b%4)..+3, so lane l collects
@@ -172,9 +172,10 @@ __device__ __forceinline__ void accumulate(const float* __restrict__ rows, uint3
         for (int T = 0; T < NT; ++T) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                // max(t, 0) on the bit pattern: negative floats are negative integers
-                const float u = __int_as_float(max(__float_as_int(w[T][r]), 0));
-                w[T][r] = u * u * u;
+                // max(t, 0)^3 as t^3 clamped to [0, 1] (the clamp bit of the
+                // last multiply): t <= 1 up to rounding, and t^3 < 0 for t < 0
+                const float t = w[T][r];
+                w[T][r] = __builtin_amdgcn_fmed3f(t * t * t, 0.0f, 1.0f);
             }
         }
 #pragma unroll
@@ -185,11 +186,11 @@ __device__ __forceinline__ void accumulate(const float* __restrict__ rows, uint3
         for (int T = 0; T < NT; ++T) acc[T] = __builtin_amdgcn_mfma_f32_4x4x1f32(a2.z, w[T][2], acc[T], 0, 0, 0);
 #pragma unroll
         for (int T = 0; T < NT; ++T) acc[T] = __builtin_amdgcn_mfma_f32_4x4x1f32(a2.w, w[T][3], acc[T], 0, 0, 0);
-        // keep that order in the schedule: NT matrix ops, 12 NT VALU ops, 4 NT
+        // keep that order in the schedule: NT matrix ops, 8 NT VALU ops, 4 NT
         // matrix ops (VALU and matrix ops of one SIMD do not overlap, so
         // fewer switches between them pay)
         __builtin_amdgcn_sched_group_barrier(0x008, NT, 0);
-        __builtin_amdgcn_sched_group_barrier(0x002, 12 * NT, 0);
+        __builtin_amdgcn_sched_group_barrier(0x002, 8 * NT, 0);
         __builtin_amdgcn_sched_group_barrier(0x008, 4 * NT, 0);
     }
 }
