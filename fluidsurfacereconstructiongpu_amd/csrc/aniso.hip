@@ -217,7 +217,10 @@ __global__ __launch_bounds__(kAnisoThreads, 6) void k_smooth(
                     s2 = fmaxf(s2, 0.0f);
                     // hardware square root (within an ulp): the weight is a
                     // value, not a decision
-                    const float w = fmaxf(1.0f - s2 * __builtin_amdgcn_sqrtf(s2), 0.0f);
+                    // max(x, 0) with x <= 1 (s2 >= 0): a clamp to [0, 1],
+                    // folded into the subtraction
+                    const float w =
+                        __builtin_amdgcn_fmed3f(1.0f - s2 * __builtin_amdgcn_sqrtf(s2), 0.0f, 1.0f);
                     sw += w;
                     sx = __builtin_fmaf(w, qq.x, sx);
                     sy = __builtin_fmaf(w, qq.y, sy);
@@ -557,7 +560,9 @@ __device__ __forceinline__ float aniso_pairs(const float4* __restrict__ q0s,
             d2 = __builtin_fmaf(q1.z, v.mx2, d2);
             d2 = __builtin_fmaf(q1.w, v.my2, d2);
             d2 = __builtin_fmaf(q2.x, v.mz2, d2);
-            const float x = fmaxf(1.0f - d2, 0.0f);
+            // max(1 - d2, 0) as a clamp to [0, 1] (the subtraction's clamp
+            // bit): 1 - d2 exceeds 1 only where rounding made d2 < 0
+            const float x = __builtin_amdgcn_fmed3f(1.0f - d2, 0.0f, 1.0f);
             sum = __builtin_fmaf(q2.z, x * x * x, sum);
         }
     }
