@@ -257,7 +257,9 @@ __device__ __forceinline__ void gather_pairs(const float4* __restrict__ qx,
             float tt = __builtin_fmaf(az, xv.z, c0 - xv.w);
             tt = __builtin_fmaf(ay, xv.y, tt);
             tt = __builtin_fmaf(ax, xv.x, tt);
-            const float tp = fmaxf(tt, 0.0f);
+            // max(t, 0) as a clamp to [0, 1], folded into the last fma (as the
+            // forward kernel clamps t^3): t exceeds 1 only by rounding
+            const float tp = __builtin_amdgcn_fmed3f(tt, 0.0f, 1.0f);
             const float t2 = tp * tp;
             const float w = t2 * tp;
             const float kd = __builtin_fmaf(uv.x, px, __builtin_fmaf(uv.y, py, __builtin_fmaf(uv.z, pz, uv.w)));
