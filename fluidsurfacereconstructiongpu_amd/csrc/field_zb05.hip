@@ -432,7 +432,9 @@ __global__ __launch_bounds__(kFieldThreads, 6) void k_field_zb05(FieldArgs a, Ba
 fsr_status field_zb05(fsr_context* ctx, const FieldArgs& a, hipStream_t s) {
     // a.counters was cleared when the frame (or the stage call) began, and no
     // other kernel fetches through xcd_field
-    const int blocks = persistent_blocks(ctx, kFieldThreads, 8, TUNE_FIELD);
+    // one workgroup per resident slot: 6 per CU (6 waves per SIMD); workgroups
+    // beyond the resident ones start late and upset the per-XCD fetch order
+    const int blocks = persistent_blocks(ctx, kFieldThreads, 6, TUNE_FIELD);
     VertexDiv vd;
     vd.rx = make_fast_div(a.sc.rx);
     vd.rxy = make_fast_div(a.sc.rx * a.sc.ry);
