@@ -505,7 +505,7 @@ fsr_status field_zb05_backward(fsr_context* ctx, const float* g_phi, float* g_po
     a.radius = st.bc.radius;
     a.radius_sq = st.radius_sq;
     a.g_pos = g_pos;
-    const int blocks = persistent_blocks(ctx, kAdjThreads, 8, TUNE_FIELD);
+    const int blocks = persistent_blocks(ctx, kAdjThreads, 4, TUNE_FIELD);
     hipLaunchKernelGGL(k_field_zb05_backward, dim3(blocks), dim3(kAdjThreads), 0, s, a);
     FSR_LAUNCH_CHECK("k_field_zb05_backward");
     return FSR_OK;

@@ -793,7 +793,7 @@ static fsr_status pass_scratch(fsr_context* ctx, uint32_t C, uint32_t n, PassScr
 
 fsr_status aniso_smooth(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
     const uint32_t C = a.sp.rx * a.sp.ry * a.sp.rz;
-    const int blocks = persistent_blocks(ctx, kAnisoThreads, 8, TUNE_ANISO);
+    const int blocks = persistent_blocks(ctx, kAnisoThreads, 6, TUNE_ANISO);
     PassScratch ps;
     FSR_TRY(pass_scratch(ctx, C, a.n, &ps));
     hipLaunchKernelGGL(k_cell_counts, dim3((C + 255) / 256), dim3(256), 0, s, a.ranges,
@@ -827,7 +827,7 @@ fsr_status aniso_involved(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
 
 fsr_status aniso_cov(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
     const uint32_t C = a.sp.rx * a.sp.ry * a.sp.rz;
-    const int blocks = persistent_blocks(ctx, kAnisoThreads, 8, TUNE_ANISO);
+    const int blocks = persistent_blocks(ctx, kAnisoThreads, 6, TUNE_ANISO);
     PassScratch ps;
     FSR_TRY(pass_scratch(ctx, C, a.n, &ps));
     hipLaunchKernelGGL(k_cell_counts, dim3((C + 255) / 256), dim3(256), 0, s, a.ranges,
@@ -871,7 +871,7 @@ fsr_status aniso_run(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
 
 fsr_status aniso_field(fsr_context* ctx, const AnisoArgs& a, const BandCompactOut& bc,
                        float* phi, hipStream_t s) {
-    const int blocks = persistent_blocks(ctx, kAnisoThreads, 8, TUNE_FIELD);
+    const int blocks = persistent_blocks(ctx, kAnisoThreads, 5, TUNE_FIELD);
     AnisoFieldArgs fa;
     fa.smooth = a.smooth;
     fa.rec = a.rec;
