@@ -98,7 +98,7 @@ class FrameDesc(ctypes.Structure):
         ("tri_nrm", c_void_p), ("stats", c_void_p),
     ]
 
-STAGES = ["hash", "surface", "band", "fill", "field", "mc"]
+STAGES = ["hash", "surface", "band", "prep", "field", "mc"]
 
 METHOD_ZB05 = 0
 METHOD_AKINCI12 = 1
@@ -108,7 +108,8 @@ BUF = {
     "cell_keys": 0, "permutation": 1, "cell_ranges": 2, "occupancy": 3,
     "surface_flags": 4, "band_bits": 5, "band_index": 6, "cube_nverts": 7,
     "valid_cubes": 8, "involve_counts": 9, "mean_pos": 10, "smooth_pos": 11,
-    "g_matrices": 12, "singular": 13, "involved_index": 14,
+    "g_matrices": 12, "singular": 13, "involved_index": 14, "field_work": 15,
+    "color_work": 16,
 }
 
 STATUS = {0: "FSR_OK", 1: "FSR_ERR_INVALID_ARG", 2: "FSR_ERR_HIP", 3: "FSR_ERR_CAPACITY",
@@ -204,6 +205,9 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.fsr_autotune.argtypes = [c_void_p, c_int, POINTER(AnisoParam), POINTER(FrameDesc),
                                  c_uint32, POINTER(c_uint32), c_void_p]
     lib.fsr_autotune.restype = st
+    lib.fsr_batch_autotune.argtypes = [c_void_p, c_int, POINTER(AnisoParam), POINTER(FrameDesc),
+                                       c_uint32, c_uint32, POINTER(c_uint32), c_void_p]
+    lib.fsr_batch_autotune.restype = st
     lib.fsr_set_launch_shape.argtypes = [c_void_p, POINTER(c_uint32)]
     lib.fsr_set_launch_shape.restype = st
     lib.fsr_set_profiling.argtypes = [c_void_p, c_uint32]

@@ -539,11 +539,18 @@ struct VertexTerms {
     float xx, yy, zz, xy2, xz2, yz2, mx2, my2, mz2;
 };
 
-// n staged candidates (a multiple of 8) against this lane's vertex.
+// n staged candidates (a multiple of 8, at most 64) against this lane's
+// vertex, added to `sum` as one partial sum.  A vertex sums thousands of
+// candidates, most of them particles of non-involved cells whose weights are
+// all close to sigma m / rho (about 2 ulp of a sum near 1): added one by one
+// into the running FP32 sum, their rounding has the same sign every time and
+// drifts the field by ~1e-4 over a frame of the configured size, while a batch
+// partial sums them at its own, much smaller, magnitude.
 __device__ __forceinline__ float aniso_pairs(const float4* __restrict__ q0s,
                                              const float4* __restrict__ q1s,
                                              const float4* __restrict__ q2s, uint32_t n,
                                              const VertexTerms& v, float sum) {
+    float part = 0.0f;
     for (uint32_t t0 = 0; t0 < n; t0 += 8u) {
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
@@ -563,10 +570,10 @@ __device__ __forceinline__ float aniso_pairs(const float4* __restrict__ q0s,
             // max(1 - d2, 0) as a clamp to [0, 1] (the subtraction's clamp
             // bit): 1 - d2 exceeds 1 only where rounding made d2 < 0
             const float x = __builtin_amdgcn_fmed3f(1.0f - d2, 0.0f, 1.0f);
-            sum = __builtin_fmaf(q2.z, x * x * x, sum);
+            part = __builtin_fmaf(q2.z, x * x * x, part);
         }
     }
-    return sum;
+    return sum + part;
 }
 
 __global__ __launch_bounds__(kAnisoThreads, 5) void k_field_aniso(AnisoFieldArgs a) {

@@ -91,6 +91,76 @@ __global__ __launch_bounds__(256) void k_dilate_rows(const uint32_t* __restrict_
     out[w] = o;
 }
 
+// x dilation of one band-mask word from the byte seed mask (the body of
+// k_dilate_x): w = word index in the row, rowbase = first byte of the row.
+__device__ __forceinline__ uint32_t dilate_x_word(const uint8_t* __restrict__ in, uint64_t rowbase,
+                                                  uint32_t xw, uint32_t wx, uint32_t rx) {
+    const uint8_t* src = in + rowbase + ((uint64_t)xw << 5);
+    const uint4* chunk = reinterpret_cast<const uint4*>(src);
+    const uint4 lo = chunk[0], hi = chunk[1];
+    const uint32_t cur = pack4(lo.x) | (pack4(lo.y) << 4) | (pack4(lo.z) << 8) |
+                         (pack4(lo.w) << 12) | (pack4(hi.x) << 16) | (pack4(hi.y) << 20) |
+                         (pack4(hi.z) << 24) | (pack4(hi.w) << 28);
+    const uint32_t prev = xw > 0 ? pack4(reinterpret_cast<const uint32_t*>(src)[-1]) << 28 : 0u;
+    const uint32_t next = xw + 1 < wx ? pack4(reinterpret_cast<const uint32_t*>(src)[8]) : 0u;
+    uint32_t o = cur;
+    o |= (cur >> 1) | (next << 31);
+    o |= (cur >> 2) | (next << 30);
+    o |= (cur >> 3) | (next << 29);
+    o |= (cur << 1) | (prev >> 31);
+    o |= (cur << 2) | (prev >> 30);
+    const uint32_t lim = rx >= 1 ? rx - 1 : 0;
+    const uint32_t x0 = xw * 32u;
+    if (x0 >= lim) o = 0;
+    else if (lim - x0 < 32u) o &= (1u << (lim - x0)) - 1u;
+    return o;
+}
+
+// The three dilation passes in one launch: a workgroup owns a tile of kTW
+// words x kTY rows x kTZ layers of the output; it x-dilates the tile's input
+// with a halo of 3 rows and 3 layers on each side into LDS (from the byte
+// mask), ORs the 7 rows (y), then the 7 layers (z), and writes the tile.
+// Rows and layers outside the grid contribute nothing, as in the separate
+// passes.
+constexpr int kTW = 8, kTY = 16, kTZ = 16, kH = 3;
+__global__ __launch_bounds__(256) void k_dilate_box(const uint8_t* __restrict__ in,
+                                                    uint32_t* __restrict__ out, uint32_t wx,
+                                                    GridDev sc) {
+    __shared__ uint32_t xs[kTZ + 2 * kH][kTY + 2 * kH][kTW];
+    __shared__ uint32_t ys[kTZ + 2 * kH][kTY][kTW];
+    const uint32_t w0 = blockIdx.x * kTW;
+    const int y0 = (int)(blockIdx.y * kTY), z0 = (int)(blockIdx.z * kTZ);
+    constexpr int nxs = (kTZ + 2 * kH) * (kTY + 2 * kH) * kTW;
+    for (int t = threadIdx.x; t < nxs; t += blockDim.x) {
+        const int w = t % kTW, r = t / kTW;
+        const int yy = y0 - kH + r % (kTY + 2 * kH), zz = z0 - kH + r / (kTY + 2 * kH);
+        uint32_t v = 0;
+        if (w0 + w < wx && yy >= 0 && yy < (int)sc.ry && zz >= 0 && zz < (int)sc.rz)
+            v = dilate_x_word(in, (((uint64_t)zz * sc.ry + yy) * wx) << 5, w0 + w, wx, sc.rx);
+        xs[r / (kTY + 2 * kH)][r % (kTY + 2 * kH)][w] = v;
+    }
+    __syncthreads();
+    constexpr int nys = (kTZ + 2 * kH) * kTY * kTW;
+    for (int t = threadIdx.x; t < nys; t += blockDim.x) {
+        const int w = t % kTW, y = (t / kTW) % kTY, zr = t / (kTW * kTY);
+        uint32_t v = 0;
+#pragma unroll
+        for (int d = 0; d <= 2 * kH; ++d) v |= xs[zr][y + d][w];
+        ys[zr][y][w] = v;
+    }
+    __syncthreads();
+    constexpr int nout = kTZ * kTY * kTW;
+    for (int t = threadIdx.x; t < nout; t += blockDim.x) {
+        const int w = t % kTW, y = (t / kTW) % kTY, z = t / (kTW * kTY);
+        const int yy = y0 + y, zz = z0 + z;
+        if (w0 + w >= wx || yy >= (int)sc.ry || zz >= (int)sc.rz) continue;
+        uint32_t v = 0;
+#pragma unroll
+        for (int d = 0; d <= 2 * kH; ++d) v |= ys[z + d][y][w];
+        out[((uint64_t)zz * sc.ry + yy) * wx + w0 + w] = v;
+    }
+}
+
 __device__ __forceinline__ int clampi(int v, int lo, int hi) {
     return v < lo ? lo : (v > hi ? hi : v);
 }
@@ -432,15 +502,6 @@ __global__ __launch_bounds__(256) void k_axis_maps(GridDev sc, GridDev sp, int r
     }
 }
 
-// Band bits of one mask word restricted to the vertex range [xlo, xhi).
-__device__ __forceinline__ uint32_t clip_word(uint32_t word, uint32_t w, uint32_t xlo,
-                                              uint32_t xhi) {
-    const uint32_t lo = w * 32u, hi = lo + 32u;
-    if (xlo > lo) word &= ~((1u << (xlo - lo)) - 1u);
-    if (xhi < hi) word &= (1u << (xhi - lo)) - 1u;
-    return word;
-}
-
 // 32 row occupancy bits starting at row `bit0` (bits past the end read 0).
 __device__ __forceinline__ uint32_t row_bits32(const AxisMaps& m, uint64_t bit0) {
     const uint64_t w = bit0 >> 5;
@@ -604,15 +665,22 @@ fsr_status band_build(fsr_context* ctx, const BandArgs& a, hipStream_t s) {
     // S (bytes) -> tmp (x), tmp -> S (y, its first nwords words), S -> bits
     // (z); `sbits` is consumed
     uint32_t* sb = const_cast<uint32_t*>(a.sbits);
-    hipLaunchKernelGGL(k_dilate_x, dim3(blocks), dim3(256), 0, s, (const uint8_t*)sb, a.tmp, wx,
-                       nwords, a.sc.rx);
-    FSR_LAUNCH_CHECK("k_dilate_x");
-    hipLaunchKernelGGL(k_dilate_rows, dim3(blocks), dim3(256), 0, s, a.tmp, sb, nwords,
-                       (uint64_t)wx, a.sc.ry, (uint64_t)a.sc.ry);
-    FSR_LAUNCH_CHECK("k_dilate_rows(y)");
-    hipLaunchKernelGGL(k_dilate_rows, dim3(blocks), dim3(256), 0, s, sb, a.bits, nwords,
-                       (uint64_t)wx * a.sc.ry, a.sc.rz, (uint64_t)a.sc.rz);
-    FSR_LAUNCH_CHECK("k_dilate_rows(z)");
+    const dim3 tiles((wx + kTW - 1) / kTW, (a.sc.ry + kTY - 1) / kTY, (a.sc.rz + kTZ - 1) / kTZ);
+    if (tiles.y <= 65535u && tiles.z <= 65535u) {
+        hipLaunchKernelGGL(k_dilate_box, tiles, dim3(256), 0, s, (const uint8_t*)sb, a.bits, wx,
+                           a.sc);
+        FSR_LAUNCH_CHECK("k_dilate_box");
+    } else {
+        hipLaunchKernelGGL(k_dilate_x, dim3(blocks), dim3(256), 0, s, (const uint8_t*)sb, a.tmp,
+                           wx, nwords, a.sc.rx);
+        FSR_LAUNCH_CHECK("k_dilate_x");
+        hipLaunchKernelGGL(k_dilate_rows, dim3(blocks), dim3(256), 0, s, a.tmp, sb, nwords,
+                           (uint64_t)wx, a.sc.ry, (uint64_t)a.sc.ry);
+        FSR_LAUNCH_CHECK("k_dilate_rows(y)");
+        hipLaunchKernelGGL(k_dilate_rows, dim3(blocks), dim3(256), 0, s, sb, a.bits, nwords,
+                           (uint64_t)wx * a.sc.ry, a.sc.rz, (uint64_t)a.sc.rz);
+        FSR_LAUNCH_CHECK("k_dilate_rows(z)");
+    }
     const int eb = persistent_blocks(ctx, 256, 4, TUNE_BAND);
     hipLaunchKernelGGL(k_band_exceptions, dim3(eb), dim3(256), 0, s, a);
     FSR_LAUNCH_CHECK("k_band_exceptions");
@@ -621,7 +689,7 @@ fsr_status band_build(fsr_context* ctx, const BandArgs& a, hipStream_t s) {
 
 fsr_status band_compact(fsr_context* ctx, const uint32_t* bits, const GridDev& sc,
                         const GridDev& sp, int radius, Counters* counters, fsr_frame_stats* st,
-                        BandCompactOut* out, hipStream_t s) {
+                        BandCompactOut* out, hipStream_t s, bool materialize) {
     const uint32_t wx = row_words(sc.rx);
     const uint64_t nwords = (uint64_t)wx * sc.ry * sc.rz;
     const uint64_t V = (uint64_t)sc.rx * sc.ry * sc.rz;
@@ -657,7 +725,8 @@ fsr_status band_compact(fsr_context* ctx, const uint32_t* bits, const GridDev& s
     FSR_TRY(arena_typed(ctx, A_CELL_CNT, nvc, &cnt));
     FSR_TRY(arena_typed(ctx, A_CELL_OFF, nvc, &cell_off));
     FSR_TRY(arena_typed(ctx, A_CHUNK_OFF, nvc, &chunk_off));
-    FSR_TRY(arena_typed(ctx, A_CELL_BAND, V, &cell_band));
+    cell_band = nullptr;
+    if (materialize) FSR_TRY(arena_typed(ctx, A_CELL_BAND, V, &cell_band));
     const uint64_t max_chunks = V / 64 + nvc + 1;
     FSR_TRY(arena_typed(ctx, A_CHUNK_CELL, max_chunks, &chunk_cell));
     AxisMaps m;
@@ -688,18 +757,24 @@ fsr_status band_compact(fsr_context* ctx, const uint32_t* bits, const GridDev& s
     hipLaunchKernelGGL(k_slab_rows, dim3(slab_blocks), dim3(256), 0, s, bits, sc, wx, m, vcx, vcy,
                        slab, cnt);
     FSR_LAUNCH_CHECK("k_slab_rows");
-    FSR_TRY(scan_exclusive(ctx, SCAN_U32, slab, slab_off, nslabs, nullptr, nullptr, s));
     FSR_TRY(scan_cells(ctx, cnt, nvc, cell_off, chunk_off, chunk_cell, max_chunks,
                        &counters->num_chunks, s));
-    hipLaunchKernelGGL(k_slab_fill, dim3((uint32_t)((nslabs + 255) / 256)), dim3(256), 0, s, bits,
-                       sc, wx, m, vcx, vcy, (const uint32_t*)slab, (const uint32_t*)slab_off,
-                       cell_band, V, nslabs);
-    FSR_LAUNCH_CHECK("k_slab_fill");
+    if (materialize) {
+        FSR_TRY(scan_exclusive(ctx, SCAN_U32, slab, slab_off, nslabs, nullptr, nullptr, s));
+        hipLaunchKernelGGL(k_slab_fill, dim3((uint32_t)((nslabs + 255) / 256)), dim3(256), 0, s,
+                           bits, sc, wx, m, vcx, vcy, (const uint32_t*)slab,
+                           (const uint32_t*)slab_off, cell_band, V, nslabs);
+        FSR_LAUNCH_CHECK("k_slab_fill");
+    }
     out->cell_off = cell_off;
     out->cell_cnt = cnt;
     out->chunk_off = chunk_off;
-    out->cell_band = cell_band;
+    out->cell_band = materialize ? cell_band : nullptr;
     out->chunk_cell = chunk_cell;
+    out->bits = bits;
+    out->wx = wx;
+    out->axis = m.lo[0];  // lo_x, hi_x, lo_y, hi_y, lo_z, hi_z follow one another
+    out->slab = slab;
     out->vcx = vcx;
     out->vcy = vcy;
     out->vcz = vcz;

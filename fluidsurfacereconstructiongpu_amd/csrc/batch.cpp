@@ -136,10 +136,10 @@ fsr_status fsr_batch_create(fsr_context* ctx, uint32_t lanes, fsr_batch** out) {
         if (st != FSR_OK) return fail(st);
         std::copy(ctx->tune, ctx->tune + TUNE_NUM, c->tune);
         // Lanes share the GPU: unless a launch shape was chosen, a lane's
-        // persistent field kernel takes 2 workgroups per CU (of the 8 a lone
+        // persistent field kernel takes 2 workgroups per CU (of the 6 a lone
         // frame uses), so the field kernels of concurrent lanes run side by
-        // side instead of queueing behind each other's workgroups.  Measured
-        // on 64 random-shape frames: 2 per CU beats 1, 3, 4 and 8.
+        // side instead of queueing behind each other's workgroups
+        // (fsr_batch_autotune picks another value when it is faster).
         if (c->tune[TUNE_FIELD] == 0) c->tune[TUNE_FIELD] = kLaneFieldWorkgroupsPerCu;
         b->lanes.push_back(c);
         hipStream_t s;
@@ -219,6 +219,64 @@ fsr_status fsr_batch_run(fsr_batch* b, fsr_method method, const fsr_aniso_param*
     return FSR_OK;
 }
 
+fsr_status fsr_batch_autotune(fsr_batch* b, fsr_method method, const fsr_aniso_param* aniso,
+                              const fsr_frame_desc* frames, uint32_t n, uint32_t repeats,
+                              uint32_t* choice, void* stream) {
+    if (!b || !frames || n == 0 || repeats == 0 || repeats > 64) {
+        set_error("fsr_batch_autotune: need a batch, frames and 1..64 repeats");
+        return FSR_ERR_INVALID_ARG;
+    }
+    FSR_HIP(hipSetDevice(b->parent->device));
+    hipStream_t caller = (hipStream_t)stream;
+    release_graph(b);
+    FSR_HIP(hipEventRecord(b->in_ready, caller));
+    FSR_HIP(hipStreamWaitEvent(b->origin, b->in_ready, 0));
+    // one eager run sizes the lane arenas
+    FSR_TRY(enqueue(b, method, aniso, frames, n, nullptr));
+    FSR_HIP(hipStreamSynchronize(b->origin));
+    hipEvent_t e0, e1;
+    FSR_HIP(hipEventCreate(&e0));
+    hipError_t he = hipEventCreate(&e1);
+    if (he != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return hip_fail(he, "hipEventCreate", __FILE__, __LINE__);
+    }
+    // workgroups per CU of a lane's field kernel (0: the kernel's resident
+    // count); the whole batch, eager, median of `repeats` passes
+    static const int kCand[] = {2, 1, 3, 4, 0};
+    int best = kLaneFieldWorkgroupsPerCu;
+    float best_t = 0.f;
+    fsr_status st = FSR_OK;
+    for (int cand : kCand) {
+        for (fsr_context* c : b->lanes) c->tune[TUNE_FIELD] = cand;
+        std::vector<float> t;
+        for (uint32_t r = 0; r < repeats && st == FSR_OK; ++r) {
+            he = hipEventRecord(e0, b->origin);
+            if (he == hipSuccess) st = enqueue(b, method, aniso, frames, n, nullptr);
+            if (he == hipSuccess && st == FSR_OK) he = hipEventRecord(e1, b->origin);
+            if (he == hipSuccess && st == FSR_OK) he = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (he == hipSuccess && st == FSR_OK) he = hipEventElapsedTime(&ms, e0, e1);
+            if (he != hipSuccess && st == FSR_OK) st = hip_fail(he, "batch autotune timing", __FILE__, __LINE__);
+            t.push_back(ms);
+        }
+        if (st != FSR_OK) break;
+        std::sort(t.begin(), t.end());
+        const float med = t[t.size() / 2];
+        if (cand == kCand[0] || med < 0.99f * best_t) {
+            best_t = med;
+            best = cand;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    for (fsr_context* c : b->lanes) c->tune[TUNE_FIELD] = best;
+    if (choice) *choice = (uint32_t)best;
+    FSR_HIP(hipEventRecord(b->out_ready, b->origin));
+    FSR_HIP(hipStreamWaitEvent(caller, b->out_ready, 0));
+    return st;
+}
+
 fsr_status fsr_set_launch_shape(fsr_context* ctx, const uint32_t* wg) {
     if (!ctx || !wg) {
         set_error("fsr_set_launch_shape: null argument");
@@ -237,9 +295,10 @@ fsr_status fsr_autotune(fsr_context* ctx, fsr_method method, const fsr_aniso_par
     }
     FSR_HIP(hipSetDevice(ctx->device));
     hipStream_t s = (hipStream_t)stream;
-    static const int kCand[] = {2, 4, 8, 16};
+    // 0 = each kernel's resident count (the default); the others limit it
+    static const int kCand[] = {0, 2, 3, 4};
     // stage whose time decides each class
-    const int stage_of[TUNE_NUM] = {FSR_STAGE_FIELD, FSR_STAGE_MC, FSR_STAGE_FIELD, FSR_STAGE_BAND};
+    const int stage_of[TUNE_NUM] = {FSR_STAGE_FIELD, FSR_STAGE_MC, FSR_STAGE_PREP, FSR_STAGE_BAND};
     const bool used[TUNE_NUM] = {true, true, method == FSR_METHOD_YU13, true};
     std::vector<float> ms((size_t)repeats * FSR_NUM_STAGES);
     auto measure = [&](int stage, float* out) -> fsr_status {
@@ -269,13 +328,15 @@ fsr_status fsr_autotune(fsr_context* ctx, fsr_method method, const fsr_aniso_par
     FSR_TRY(measure(FSR_STAGE_FIELD, &dummy));
     for (int c = 0; c < TUNE_NUM; ++c) {
         if (!used[c]) continue;
-        int best = ctx->tune[c];
-        float best_t = 1e30f;
+        // the default is timed first and kept unless a candidate beats it by
+        // more than 1% (stage medians of `repeats` frames)
+        int best = 0;
+        float best_t = 0.f;
         for (int cand : kCand) {
             ctx->tune[c] = cand;
             float t;
             FSR_TRY(measure(stage_of[c], &t));
-            if (t < best_t) {
+            if (cand == 0 || t < 0.99f * best_t) {
                 best_t = t;
                 best = cand;
             }

@@ -63,8 +63,11 @@ static void prof_release(fsr_context* ctx) {
 
 int persistent_blocks(fsr_context* ctx, int threads, int per_cu, TuneClass cls) {
     (void)threads;
+    // a tuned value limits the workgroups per CU below the kernel's resident
+    // count (per_cu); it never raises it: workgroups beyond the resident ones
+    // would only start late
     const int t = ctx->tune[cls];
-    return std::max(1, ctx->num_cus * (t > 0 ? t : per_cu));
+    return std::max(1, ctx->num_cus * (t > 0 ? std::min(t, per_cu) : per_cu));
 }
 
 static fsr_status check_grid(const fsr_grid_info* g, const char* name) {
@@ -126,6 +129,7 @@ fsr_status fsr_create(fsr_context** out_ctx, int device) {
     FSR_HIP(hipSetDevice(device));
     fsr_context* ctx = new fsr_context();
     ctx->device = device;
+
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
         cus > 0)
@@ -287,9 +291,7 @@ fsr_status frame_impl(const char* who, fsr_context* ctx, fsr_method method,
     if (method == FSR_METHOD_AKINCI12) {
         FSR_TRY(surface_colorfield(ctx, sa, sim->particle_mass, s));
     } else {
-        uint8_t* surf_cell;
-        FSR_TRY(arena_typed(ctx, A_SURF_CELL, C, &surf_cell));
-        FSR_TRY(surface_occupancy(ctx, sa, surf_cell, involve, s));
+        FSR_TRY(surface_particles(ctx, sa, involve, s));
     }
 
     // 3. band
@@ -310,12 +312,14 @@ fsr_status frame_impl(const char* who, fsr_context* ctx, fsr_method method,
     ba.all_exceptions = sa.all_exceptions;
     FSR_TRY(band_build(ctx, ba, s));
     BandCompactOut bc;
-    FSR_TRY(band_compact(ctx, bits, sc, sp, yu13 ? 2 : 1, counters, stats, &bc, s));
+    // the cell-major list itself is needed by the anisotropic field and the
+    // adjoint; the ZB05 field lists its chunks from the mask
+    const bool materialize = yu13 || ctx->save_adjoint;
+    FSR_TRY(band_compact(ctx, bits, sc, sp, yu13 ? 2 : 1, counters, stats, &bc, s, materialize));
 
     // 4. field
     // (the dense grid was filled in the frame's first launch)
-    prof_mark(ctx, FSR_STAGE_FILL, s);
-    prof_mark(ctx, FSR_STAGE_FIELD, s);
+    prof_mark(ctx, FSR_STAGE_PREP, s);
     if (yu13) {
         AnisoArgs aa;
         const uint64_t nn = n ? n : 1;
@@ -344,8 +348,10 @@ fsr_status frame_impl(const char* who, fsr_context* ctx, fsr_method method,
         aa.counters = counters;
         aa.st = stats;
         FSR_TRY(aniso_run(ctx, aa, s));
+        prof_mark(ctx, FSR_STAGE_FIELD, s);
         FSR_TRY(aniso_field(ctx, aa, bc, phi, s));
     } else {
+        prof_mark(ctx, FSR_STAGE_FIELD, s);
         FieldArgs fa;
         fa.sorted = ho.sorted;
         fa.ranges = ho.ranges;
@@ -357,6 +363,7 @@ fsr_status frame_impl(const char* who, fsr_context* ctx, fsr_method method,
         fa.phi = phi;
         fa.counters = counters;
         fa.saved = nullptr;
+        fa.work = nullptr;
         if (ctx->save_adjoint) FSR_TRY(arena_typed(ctx, A_ADJ_VERT, V, &fa.saved));
         FSR_TRY(field_zb05(ctx, fa, s));
     }
@@ -596,6 +603,8 @@ fsr_status fsr_frame_buffer(fsr_context* ctx, fsr_buffer which, void** dev_ptr, 
         case FSR_BUF_G_MATRICES: slot = A_GMAT; break;
         case FSR_BUF_SINGULAR: slot = A_SIGMA; break;
         case FSR_BUF_INVOLVED_INDEX: slot = A_INVOLVED_IDX; break;
+        case FSR_BUF_FIELD_WORK: slot = A_FIELD_WORK; break;
+        case FSR_BUF_COLOR_WORK: slot = A_COLOR_WORK; break;
         default:
             set_error("fsr_frame_buffer: unknown buffer");
             return FSR_ERR_INVALID_ARG;

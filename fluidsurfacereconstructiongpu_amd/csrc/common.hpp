@@ -121,6 +121,16 @@ __device__ __forceinline__ uint32_t wave_or(uint32_t v) {
     return v;
 }
 
+// Band bits of one mask word (vertices 32 w .. 32 w + 31 of a row) restricted
+// to the vertex range [xlo, xhi).
+__device__ __forceinline__ uint32_t clip_word(uint32_t word, uint32_t w, uint32_t xlo,
+                                              uint32_t xhi) {
+    const uint32_t lo = w * 32u, hi = lo + 32u;
+    if (xlo > lo) word &= ~((1u << (xlo - lo)) - 1u);
+    if (xhi < hi) word &= (1u << (xhi - lo)) - 1u;
+    return word;
+}
+
 // XCD-aware dynamic work fetch for persistent kernels.  The work list [0, total)
 // is cut into 8 contiguous segments; workgroups are dispatched round-robin over
 // the 8 XCDs, so blockIdx % 8 names the XCD, and a wave takes items from its
@@ -191,7 +201,7 @@ enum ArenaSlot {
     A_COUNTERS, A_SCAN_PARTIALS, A_STATS, A_INVOLVE, A_INVOLVE_OFF,
     A_INVOLVED_IDX, A_MEAN, A_SMOOTH, A_GMAT, A_SIGMA, A_AUX0, A_AUX1, A_REC,
     A_ADJ_VERT, A_ADJ_COEF, A_ADJ_AUX, A_ADJ_TOUCH, A_EDGE_MASK, A_EDGE_OFF, A_SURF_CHUNKS, A_SLAB, A_BAND_PARTIALS,
-    A_NUM_SLOTS
+    A_FIELD_WORK, A_COLOR_WORK, A_MC_PARTIALS, A_NUM_SLOTS
 };
 
 // Band vertices grouped by hash cell (band.hip).
@@ -200,11 +210,21 @@ struct BandCompactOut {
     uint32_t* cell_off;     // per virtual cell offset into cell_band
     uint32_t* cell_cnt;
     uint32_t* chunk_off;
-    uint32_t* cell_band;    // u32[B] cell-major
+    uint32_t* cell_band;    // u32[B] cell-major, or null when not materialised:
+                            // then a chunk's vertices are read from the mask
     uint32_t* chunk_cell;   // chunk -> virtual cell
     uint32_t vcx, vcy, vcz; // virtual cell grid (hash grid padded by `radius`)
     int radius;             // 1 (3^3 blocks) or 2 (5^3 blocks)
     uint64_t max_chunks;
+    // what a kernel needs to list a chunk's vertices from the band mask: the
+    // mask, its words per row, each virtual cell's vertex interval per axis
+    // (axis: lo_x[vcx] hi_x[vcx] lo_y[vcy] hi_y[vcy] lo_z[vcz] hi_z[vcz]) and
+    // the per-slab band counts (slab = one scalar layer of a cell; a cell's
+    // slabs are consecutive, layer by layer)
+    const uint32_t* bits;
+    const uint32_t* axis;
+    const uint32_t* slab;
+    uint32_t wx;
 };
 
 struct Arena {
@@ -378,6 +398,10 @@ fsr_status surface_occupancy(fsr_context* ctx, const SurfaceArgs& a, uint8_t* su
                              uint32_t* involve, hipStream_t s);
 fsr_status surface_colorfield(fsr_context* ctx, const SurfaceArgs& a,
                               float mass, hipStream_t s);
+// surface_occupancy for a whole frame in one pass over the particles (no
+// per-cell surface array)
+fsr_status surface_particles(fsr_context* ctx, const SurfaceArgs& a, uint32_t* involve,
+                             hipStream_t s);
 
 // band.hip ------------------------------------------------------------------
 struct BandArgs {
@@ -403,9 +427,12 @@ fsr_status band_build(fsr_context* ctx, const BandArgs& a, hipStream_t s);
 // the particle, so the band equals a box dilation of the surface cells.
 int band_dilation_exact(const fsr_grid_info& sc, float h);
 
+// materialize: also write the cell-major list (cell_band); otherwise the
+// field kernel lists each chunk's vertices from the mask.
 fsr_status band_compact(fsr_context* ctx, const uint32_t* bits, const GridDev& sc,
                         const GridDev& sp, int radius, Counters* counters,
-                        fsr_frame_stats* st, BandCompactOut* out, hipStream_t s);
+                        fsr_frame_stats* st, BandCompactOut* out, hipStream_t s,
+                        bool materialize = true);
 
 // field_zb05.hip ------------------------------------------------------------
 struct FieldArgs {
@@ -419,6 +446,8 @@ struct FieldArgs {
     float* phi;
     Counters* counters;
     float4* saved;         // optional, cell-major [B]: (x - xbar, sum w) for the adjoint
+    uint32_t* work;        // set by field_zb05: per chunk, candidates staged, candidates
+                           // kept by the box test, 16 x 16 tile steps evaluated
 };
 fsr_status field_zb05(fsr_context* ctx, const FieldArgs& a, hipStream_t s);
 fsr_status fill_f32(float* p, float value, uint64_t n, hipStream_t s);

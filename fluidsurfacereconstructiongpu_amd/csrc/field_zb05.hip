@@ -126,6 +126,72 @@ struct VertexDiv {
     FastDiv rx, rxy;
 };
 
+// The vertex ids of chunk k of virtual cell v, listed from the band mask in
+// the cell-major order (layer z, row y, then x ascending), when the cell-major
+// list is not materialised: lane l < cnt receives the (64 k + l)-th band vertex
+// of the cell.  The cell's layers (slabs) are consecutive in the slab counts;
+// 64 layers at a time, those that overlap [64 k, 64 k + 64) are expanded word
+// by word, each lane taking one (layer, row, word) of them, and the wanted
+// ranks go to `scratch` (64 LDS words of the wave).
+__device__ uint32_t chunk_vertex(const BandCompactOut& bc, uint32_t rx, uint32_t ry, uint32_t v,
+                                 uint32_t k, uint32_t cnt, uint32_t* scratch) {
+    const uint32_t lane = __lane_id();
+    const uint32_t cxv = v % bc.vcx, cyv = (v / bc.vcx) % bc.vcy, czv = v / (bc.vcx * bc.vcy);
+    const uint32_t* lo_x = bc.axis;
+    const uint32_t* lo_y = lo_x + 2u * bc.vcx;
+    const uint32_t* lo_z = lo_y + 2u * bc.vcy;
+    const uint32_t zlo = lo_z[czv], nz = lo_z[bc.vcz + czv] - zlo;
+    const uint64_t sbase = (uint64_t)bc.vcx * bc.vcy * zlo + ((uint64_t)cyv * bc.vcx + cxv) * nz;
+    const uint32_t ylo = lo_y[cyv], ny = lo_y[bc.vcy + cyv] - ylo;
+    const uint32_t xlo = lo_x[cxv], xhi = lo_x[bc.vcx + cxv];
+    const uint32_t w0 = xlo >> 5, nw = ((xhi + 31u) >> 5) - w0;
+    const uint32_t per = ny * nw;
+    const uint32_t lo = 64u * k, hi = lo + 64u;
+    // every slot starts empty, so a slot the expansion does not fill reads as
+    // "no vertex" rather than as a stale id
+    scratch[lane] = 0xFFFFFFFFu;
+    wave_lds_sync();
+    uint32_t before = 0;  // band vertices of the cell's layers before the group
+    for (uint32_t g0 = 0; g0 < nz && before < hi; g0 += 64u) {
+        const uint32_t sn = g0 + lane < nz ? bc.slab[sbase + g0 + lane] : 0u;
+        const uint32_t sincl = before + wave_inclusive_sum(sn);
+        const uint32_t sexcl = sincl - sn;
+        const uint64_t hit = __ballot(g0 + lane < nz && sn != 0u && sexcl < hi && sincl > lo);
+        if (hit) {
+            // the overlapping layers of the group are consecutive: j0 .. j1
+            const uint32_t j0 = (uint32_t)__builtin_ctzll(hit);
+            const uint32_t j1 = 63u - (uint32_t)__builtin_clzll(hit);
+            const uint32_t items = (j1 - j0 + 1u) * per;
+            uint32_t rank0 = (uint32_t)__builtin_amdgcn_readlane((int)sexcl, (int)j0);
+            for (uint32_t i0 = 0; i0 < items; i0 += 64u) {
+                const uint32_t i = i0 + lane;
+                uint32_t word = 0, base = 0;
+                if (i < items) {
+                    const uint32_t js = g0 + j0 + i / per, rem = i % per;
+                    const uint32_t y = ylo + rem / nw, w = w0 + rem % nw;
+                    const uint64_t row = (uint64_t)(zlo + js) * ry + y;
+                    word = clip_word(bc.bits[row * bc.wx + w], w, xlo, xhi);
+                    base = (uint32_t)(row * rx) + w * 32u;
+                }
+                const uint32_t c = (uint32_t)__popc(word);
+                const uint32_t incl = wave_inclusive_sum(c);
+                uint32_t r = rank0 + incl - c;
+                while (word) {
+                    if (r >= lo && r < hi) scratch[r - lo] = base + (uint32_t)__builtin_ctz(word);
+                    ++r;
+                    word &= word - 1u;
+                }
+                rank0 += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+            }
+        }
+        before = (uint32_t)__builtin_amdgcn_readlane((int)sincl, 63);
+    }
+    wave_lds_sync();
+    const uint32_t id = lane < cnt ? scratch[lane] : 0xFFFFFFFFu;
+    wave_lds_sync();
+    return id;
+}
+
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 // LDS queue of one wave, structure of arrays: rows x', y', z', a row of ones
@@ -214,6 +280,8 @@ __global__ __launch_bounds__(kFieldThreads, 6) void k_field_zb05(FieldArgs a, Ba
     float* q = lds[wave].queue;
     q[kRowOne + lane] = 1.0f;
     q[kRowOne + 64 + lane] = 1.0f;
+    // the work record ends with a marker after the last chunk
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.work[3u * num_chunks] = 0xFFFFFFFFu;
 
     uint32_t step = 0;
     for (;;) {
@@ -228,6 +296,14 @@ __global__ __launch_bounds__(kFieldThreads, 6) void k_field_zb05(FieldArgs a, Ba
         const uint32_t ccount = bc.cell_cnt[v];
         const uint32_t first = bc.cell_off[v] + 64u * k;
         const uint32_t cnt = min(64u, ccount - 64u * k);
+        // the chunk's vertex ids (before the candidate rows, whose registers
+        // live through the staging loop)
+        uint32_t vid = 0;
+        if (bc.cell_band) {
+            if (lane < cnt) vid = bc.cell_band[first + lane];
+        } else {
+            vid = chunk_vertex(bc, a.sc.rx, a.sc.ry, v, k, cnt, reinterpret_cast<uint32_t*>(q));
+        }
 
         // candidate rows of the clamped 3^3 block
         const int x0 = clampi(cx - 1, 0, a.sp.rx - 1), x1 = clampi(cx + 1, 0, a.sp.rx - 1);
@@ -262,10 +338,10 @@ __global__ __launch_bounds__(kFieldThreads, 6) void k_field_zb05(FieldArgs a, Ba
         const float oy = __builtin_fmaf((float)cy, a.sp.cell, a.sp.miny);
         const float oz = __builtin_fmaf((float)cz, a.sp.cell, a.sp.minz);
 
-        uint32_t vid = 0;
         float vx = 0.f, vy = 0.f, vz = 0.f;
-        if (lane < cnt) {
-            vid = bc.cell_band[first + lane];
+        // lanes with a vertex (an id past the grid would mean an empty slot)
+        const bool has_vertex = lane < cnt && vid < a.sc.rx * a.sc.ry * a.sc.rz;
+        if (has_vertex) {
             const uint32_t iz = fast_div(vid, vd.rxy);
             const uint32_t rem = vid - iz * (a.sc.rx * a.sc.ry);
             const uint32_t iy = fast_div(rem, vd.rx);
@@ -283,7 +359,7 @@ __global__ __launch_bounds__(kFieldThreads, 6) void k_field_zb05(FieldArgs a, Ba
         vt[320 + lane] = 2.0f * vz * inv_r2;
         vt[384 + lane] = 1.0f - __builtin_fmaf(vz, vz, __builtin_fmaf(vy, vy, vx * vx)) * inv_r2;
         // bounding box of the chunk's vertices
-        const bool live = lane < cnt;
+        const bool live = has_vertex;
         const float bx0 = wave_min_u(live ? vx : 3.0e38f), bx1 = wave_max_u(live ? vx : -3.0e38f);
         const float by0 = wave_min_u(live ? vy : 3.0e38f), by1 = wave_max_u(live ? vy : -3.0e38f);
         const float bz0 = wave_min_u(live ? vz : 3.0e38f), bz1 = wave_max_u(live ? vz : -3.0e38f);
@@ -306,6 +382,8 @@ __global__ __launch_bounds__(kFieldThreads, 6) void k_field_zb05(FieldArgs a, Ba
             // the queue is a ring of kQueue entries; entries [head, head + fill)
             // wait for evaluation, and head is 0 or 64
             uint32_t head = 0, fill = 0;
+            // work of the chunk: candidates kept by the box test, tile steps
+            uint32_t n_kept = 0, n_steps = 0;
             // this lane's candidate of the next round is loaded one round ahead,
             // so the load overlaps the culling and the accumulation of the
             // current one
@@ -364,7 +442,9 @@ __global__ __launch_bounds__(kFieldThreads, 6) void k_field_zb05(FieldArgs a, Ba
                         -(__builtin_fmaf(pz, pz, __builtin_fmaf(py, py, px * px)) * inv_r2);
                 }
                 fill += (uint32_t)__popcll(m);
+                n_kept += (uint32_t)__popcll(m);
                 if (fill >= 64u) {
+                    n_steps += 4u * NT;
                     wave_lds_sync();
                     accumulate<NT>(q + head, 64u, b1, acc);
                     wave_lds_sync();
@@ -382,6 +462,7 @@ __global__ __launch_bounds__(kFieldThreads, 6) void k_field_zb05(FieldArgs a, Ba
                     q[kRowZ + head + lane] = 0.f;
                     q[kRowQ + head + lane] = -1.0e30f;
                 }
+                n_steps += (n >> 4) * NT;
                 wave_lds_sync();
                 accumulate<NT>(q + head, n, b1, acc);
                 wave_lds_sync();
@@ -401,7 +482,7 @@ __global__ __launch_bounds__(kFieldThreads, 6) void k_field_zb05(FieldArgs a, Ba
                 }
                 if (my == (uint32_t)T) s4 = acc[T];
             }
-            if (lane < cnt) {
+            if (has_vertex) {
                 float4 keep_out = make_float4(0.f, 0.f, 0.f, 0.f);
                 const float sw = s4[3];
                 if (sw > 0.0f) {
@@ -416,6 +497,12 @@ __global__ __launch_bounds__(kFieldThreads, 6) void k_field_zb05(FieldArgs a, Ba
                 }
                 // for the adjoint: x - xbar and sum w, at the vertex's cell-major slot
                 if (a.saved) a.saved[first + lane] = keep_out;
+            }
+            if (lane == 0) {
+                uint32_t* w = a.work + 3u * ch;
+                w[0] = total;
+                w[1] = n_kept;
+                w[2] = n_steps;
             }
         };
         switch (nt) {
@@ -435,10 +522,12 @@ fsr_status field_zb05(fsr_context* ctx, const FieldArgs& a, hipStream_t s) {
     // one workgroup per resident slot: 6 per CU (6 waves per SIMD); workgroups
     // beyond the resident ones start late and upset the per-XCD fetch order
     const int blocks = persistent_blocks(ctx, kFieldThreads, 6, TUNE_FIELD);
+    FieldArgs fa = a;
+    FSR_TRY(arena_typed(ctx, A_FIELD_WORK, 3ull * a.bc->max_chunks + 3, &fa.work));
     VertexDiv vd;
     vd.rx = make_fast_div(a.sc.rx);
     vd.rxy = make_fast_div(a.sc.rx * a.sc.ry);
-    hipLaunchKernelGGL(k_field_zb05, dim3(blocks), dim3(kFieldThreads), 0, s, a, *a.bc, vd);
+    hipLaunchKernelGGL(k_field_zb05, dim3(blocks), dim3(kFieldThreads), 0, s, fa, *a.bc, vd);
     FSR_LAUNCH_CHECK("k_field_zb05");
     return FSR_OK;
 }

@@ -282,7 +282,7 @@ int main(int argc, char** argv) {
         sum_svr += V ? (double)hs.num_band_vertices / (double)V : 0.0;
         if (nf) {
             sum_es += stage[FSR_STAGE_SURFACE] + stage[FSR_STAGE_BAND];
-            sum_sf += stage[FSR_STAGE_FILL] + stage[FSR_STAGE_FIELD];
+            sum_sf += stage[FSR_STAGE_PREP] + stage[FSR_STAGE_FIELD];
             sum_mc += stage[FSR_STAGE_MC];
         }
         if (save_obj && T3) {

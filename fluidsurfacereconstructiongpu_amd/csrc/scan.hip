@@ -393,13 +393,18 @@ __global__ __launch_bounds__(256) void k_fill_multi(FillArgs a) {
     if (t < n - tail) p[tail + t] = v;
 }
 
-template <int F>
-fsr_status scan_impl(fsr_context* ctx, const uint32_t* in, uint32_t* out, uint64_t n_max,
-                     const uint32_t* d_count, uint32_t* d_total, hipStream_t s) {
+// Workgroups of a scan over n_max elements (each owns whole 4096-element tiles).
+uint32_t scan_blocks(uint64_t n_max) {
     const uint64_t want = (n_max + kScanTile - 1) / kScanTile;
-    const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, kMaxScanBlocks));
-    uint32_t* partials = nullptr;
-    FSR_TRY(arena_typed(ctx, A_SCAN_PARTIALS, 2ull * kMaxCellBlocks, &partials));
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, kMaxScanBlocks));
+}
+
+// The two launches of a scan whose workgroup partials go to `partials`
+// (scan_blocks(n_max) words).
+template <int F>
+fsr_status scan_launch(const uint32_t* in, uint32_t* out, uint64_t n_max, const uint32_t* d_count,
+                       uint32_t* d_total, uint32_t* partials, hipStream_t s) {
+    const uint32_t g = scan_blocks(n_max);
     const int vec_ok = ((((uintptr_t)in) | ((uintptr_t)out)) & 15u) == 0 ? 1 : 0;
     hipLaunchKernelGGL(k_scan_reduce<F>, dim3(g), dim3(kScanThreads), 0, s, in, n_max, d_count,
                        partials, vec_ok);
@@ -408,6 +413,14 @@ fsr_status scan_impl(fsr_context* ctx, const uint32_t* in, uint32_t* out, uint64
                        partials, d_total, vec_ok);
     FSR_LAUNCH_CHECK("k_scan_down");
     return FSR_OK;
+}
+
+template <int F>
+fsr_status scan_impl(fsr_context* ctx, const uint32_t* in, uint32_t* out, uint64_t n_max,
+                     const uint32_t* d_count, uint32_t* d_total, hipStream_t s) {
+    uint32_t* partials = nullptr;
+    FSR_TRY(arena_typed(ctx, A_SCAN_PARTIALS, 2ull * kMaxCellBlocks, &partials));
+    return scan_launch<F>(in, out, n_max, d_count, d_total, partials, s);
 }
 
 }  // namespace

@@ -585,6 +585,7 @@ fsr_status fsr_update_scalar_grid_values_compacted(
     fa.phi = scalar_field;
     fa.counters = ct;
     fa.saved = nullptr;
+    fa.work = nullptr;
     return field_zb05(ctx, fa, s);
 }
 

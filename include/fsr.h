@@ -217,7 +217,14 @@ typedef enum {
     FSR_BUF_SMOOTH_POS = 11,    /* float4[N] Yu13 smoothed positions (x, y, z, unused) */
     FSR_BUF_G_MATRICES = 12,    /* fsr_matrix[I] Yu13 G of the involved particles */
     FSR_BUF_SINGULAR = 13,      /* float3[I] Yu13 covariance singular values (before clamping) */
-    FSR_BUF_INVOLVED_INDEX = 14 /* u32[I]   Yu13 involved particle ids (sorted order) */
+    FSR_BUF_INVOLVED_INDEX = 14,/* u32[I]   Yu13 involved particle ids (sorted order) */
+    FSR_BUF_FIELD_WORK = 15,    /* u32[3 * chunks] work of the last ZB05 field launch, per
+                                 * 64-vertex chunk: candidates staged, candidates kept by the
+                                 * chunk-box test, 16 x 16 vertex/candidate tiles evaluated;
+                                 * after the last chunk, one word 0xFFFFFFFF */
+    FSR_BUF_COLOR_WORK = 16     /* u32[3 * waves] work of the last Akinci12 colour-field launch,
+                                 * per wave: candidates staged, candidates kept by the
+                                 * chunk-box test, pairs with 0 < |r| < h */
 } fsr_buffer;
 
 fsr_status fsr_frame_buffer(fsr_context* ctx, fsr_buffer which,
@@ -265,15 +272,26 @@ fsr_status fsr_batch_run(fsr_batch* batch, fsr_method method, const fsr_aniso_pa
                          const fsr_frame_desc* frames, uint32_t num_frames,
                          uint64_t* timestamps, void* stream);
 
-/* Autotuner: runs `frame` (blocking) with each candidate launch shape of the
- * persistent kernels (workgroups per CU in {2, 4, 8, 16} for the field, MC,
- * anisotropic and band kernels), times each stage with HIP events (median of
- * `repeats`), and keeps the fastest per kernel class in ctx.  choice (may be
- * NULL) receives the chosen workgroups per CU, one per class
- * (field, MC, anisotropic, band). */
+/* Launch shapes of the persistent kernels: per kernel class (field, MC,
+ * anisotropic, band) a limit on the workgroups per CU, which never exceeds a
+ * kernel's resident count (0 = the resident count, the default).
+ *
+ * Autotuner: runs `frame` (blocking) with the default and the limits 2, 3 and
+ * 4 for each class, times the class's stage with HIP events (median of
+ * `repeats`), and keeps a limit in ctx only when it beats the default by more
+ * than 1%.  choice (may be NULL) receives the chosen values, one per class. */
 fsr_status fsr_autotune(fsr_context* ctx, fsr_method method, const fsr_aniso_param* aniso,
                         const fsr_frame_desc* frame, uint32_t repeats, uint32_t* choice,
                         void* stream);
+/* Batch autotuner: runs the frames through the batch eagerly (blocking) with
+ * each candidate limit of the lanes' field kernels (2, 1, 3, 4 workgroups per
+ * CU, 0 = resident count), times the whole batch (median of `repeats`), and
+ * keeps the fastest (a candidate must beat the first by more than 1%) in the
+ * lanes.  The next fsr_batch_run captures a new graph.  choice (may be NULL)
+ * receives the chosen value. */
+fsr_status fsr_batch_autotune(fsr_batch* batch, fsr_method method, const fsr_aniso_param* aniso,
+                              const fsr_frame_desc* frames, uint32_t num_frames,
+                              uint32_t repeats, uint32_t* choice, void* stream);
 /* Sets the launch shape directly (0 = default), one value per class. */
 fsr_status fsr_set_launch_shape(fsr_context* ctx, const uint32_t* workgroups_per_cu);
 
@@ -454,8 +472,9 @@ typedef enum {
     FSR_STAGE_HASH = 0,     /* cell ids, radix sort, gather, cell ranges */
     FSR_STAGE_SURFACE = 1,  /* surface particles and band seeds */
     FSR_STAGE_BAND = 2,     /* band dilation, compaction, cell-major work list */
-    FSR_STAGE_FILL = 3,     /* fill of the dense field with the non-band value */
-    FSR_STAGE_FIELD = 4,    /* field kernel(s) on the band */
+    FSR_STAGE_PREP = 3,     /* Yu13: smoothing, involved list, covariance -> G (empty for
+                             * ZB05 / Akinci12; the dense fill runs in the hash stage) */
+    FSR_STAGE_FIELD = 4,    /* field kernel on the band */
     FSR_STAGE_MC = 5,       /* marching cubes classify, scans, compaction, emit */
     FSR_NUM_STAGES = 6
 } fsr_stage;

@@ -62,7 +62,8 @@ def load():
     lib.orc_field_zb05.argtypes = [u32p, ctypes.c_uint32, f32p, u32p, OGrid, OGrid,
                                    ctypes.c_float, ctypes.c_float, f64p, f64p]
     lib.orc_mc.argtypes = [u32p, ctypes.c_uint32, u8p, f64p, OGrid, ctypes.c_float, ctypes.c_int,
-                           u32p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
+                           u32p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                           ctypes.c_void_p]
     lib.orc_mc.restype = ctypes.c_uint64
     lib.orc_frame.argtypes = [ctypes.c_int, f32p, f32p, ctypes.c_uint32, OGrid, OGrid,
                               ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
@@ -252,19 +253,25 @@ def field_zb05(index, pos, ranges, sp, sc, radius_sq, particle_radius, fill=0.0)
     return phi, wsum
 
 
-def mc(index, band_u8, phi, sc, iso, cpu=False, with_mesh=True):
+def mc(index, band_u8, phi, sc, iso, cpu=False, with_mesh=True, with_len=False):
+    """(nverts, soup positions, soup normals[, normal lengths before the
+    normalisation when with_len])."""
     lib = load()
     index = np.ascontiguousarray(index, np.uint32)
     B = index.shape[0]
     nverts = np.empty(B, np.uint32)
     phi = np.ascontiguousarray(phi, np.float64)
-    total = lib.orc_mc(index, B, band_u8, phi, ogrid(sc), iso, int(cpu), nverts, None, None, 0)
-    pos = nrm = None
+    total = lib.orc_mc(index, B, band_u8, phi, ogrid(sc), iso, int(cpu), nverts, None, None, 0,
+                       None)
+    pos = nrm = nlen = None
     if with_mesh:
         pos = np.empty((total, 3), np.float64)
         nrm = np.empty((total, 3), np.float64)
+        nlen = np.empty(total, np.float64)
         lib.orc_mc(index, B, band_u8, phi, ogrid(sc), iso, int(cpu), nverts, pos.ctypes.data,
-                   nrm.ctypes.data, total)
+                   nrm.ctypes.data, total, nlen.ctypes.data)
+    if with_len:
+        return nverts, pos, nrm, nlen
     return nverts, pos, nrm
 
 
@@ -426,3 +433,27 @@ class AdjointWorkload:
 
     def fd(self, j: int, axis: int, eps: float = 1e-6) -> float:
         return float(self.lib.orc_adj_fd(self.ptr, int(j), int(axis), float(eps)))
+
+
+def frame_yu13(fr, aniso):
+    """Whole anisotropic (Yu13) frame through the oracle's stage functions:
+    hashing, surface + involved cells, band, smoothing, involved list,
+    covariance -> G, field, marching cubes.  Returns the frame's counts."""
+    f = np.float32
+    h = f(fr.h)
+    r = f(h * f(2.0))
+    hs = hash_sort(fr.pos, fr.dens, fr.sp)
+    _, flags, involve = surface_occupancy(hs["occ"], hs["ranges"], fr.sp, hs["keys"],
+                                          involve=True)
+    band_u8 = band(hs["pos"], flags, fr.sc, h)
+    index = compact(band_u8)
+    mean, sm = smooth(hs["pos"], hs["ranges"], fr.sp, r, f(aniso["lam"]))
+    inv_idx, inv_off = involved_list(hs["keys"], hs["ranges"], involve)
+    G, _, _, _ = aniso_matrices(hs["pos"], mean, hs["dens"], inv_idx, hs["ranges"], fr.sp, h, r,
+                                aniso["kr"], aniso["kn"], aniso["ne"])
+    phi, _ = field_aniso(index, sm, hs["dens"], hs["ranges"], involve, inv_off, G, fr.sp, fr.sc,
+                         f(fr.mass))
+    nv, _, _ = mc(index, band_u8, phi, fr.sc, f(fr.iso), with_mesh=False)
+    return {"surface_particles": int(flags.sum()), "involved_particles": int(inv_idx.shape[0]),
+            "band_vertices": int(index.shape[0]), "valid_cubes": int((nv > 0).sum()),
+            "mesh_vertices": int(nv.sum()), "out_of_grid": 0}

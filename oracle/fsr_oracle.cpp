@@ -297,7 +297,7 @@ int orc_field_zb05(const uint32_t* band_index, uint32_t B, const float* pos,
 // negation, normals normalised only above length 1e-7, positions min + i*cell).
 uint64_t orc_mc(const uint32_t* band_index, uint32_t B, const uint8_t* band, const double* phi,
                 orc_grid sc, float iso, int cpu, uint32_t* nverts, double* tri_pos,
-                double* tri_nrm, uint64_t capacity) {
+                double* tri_nrm, uint64_t capacity, double* nrm_len) {
     const int rx = sc.rx, ry = sc.ry, rz = sc.rz;
     const uint64_t rxy = (uint64_t)rx * ry;
     static const int off[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {0, 1, 0},
@@ -377,6 +377,9 @@ uint64_t orc_mc(const uint32_t* band_index, uint32_t B, const uint8_t* band, con
                 nn[a] = cn[s0][a] + t * (cn[s1][a] - cn[s0][a]);
             }
             const double len = std::sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+            // length of the interpolated normal before its normalisation: the
+            // normalised direction's sensitivity to rounding is 1 / len
+            if (nrm_len) nrm_len[offs[b] + k] = len;
             if (cpu) {
                 if (len > 1e-7) for (int a = 0; a < 3; ++a) nn[a] /= len;
             } else {
@@ -656,10 +659,10 @@ int orc_frame(int method, float* pos, float* dens, uint32_t n, orc_grid sp, orc_
                    phi.data(), nullptr);
     std::vector<uint32_t> nverts(B);
     const uint64_t total = orc_mc(index.data(), B, band.data(), phi.data(), sc, iso, method == 1,
-                                  nverts.data(), nullptr, nullptr, 0);
+                                  nverts.data(), nullptr, nullptr, 0, nullptr);
     std::vector<double> tp(3 * total), tn(3 * total);
     orc_mc(index.data(), B, band.data(), phi.data(), sc, iso, method == 1, nverts.data(),
-           tp.data(), tn.data(), total);
+           tp.data(), tn.data(), total, nullptr);
     uint64_t q = 0;
     for (uint32_t b = 0; b < B; ++b) q += nverts[b] > 0;
     if (phi_out)
@@ -793,7 +796,7 @@ uint64_t soup_edges(const uint32_t* band_index, uint32_t B, const uint8_t* band,
                     Soup* out) {
     const uint64_t rx = sc.rx, ry = sc.ry, rz = sc.rz, rxy = rx * ry;
     std::vector<uint32_t> nverts(B);
-    orc_mc(band_index, B, band, phi, sc, iso, cpu, nverts.data(), nullptr, nullptr, 0);
+    orc_mc(band_index, B, band, phi, sc, iso, cpu, nverts.data(), nullptr, nullptr, 0, nullptr);
     uint64_t off = 0;
     for (uint32_t bb = 0; bb < B; ++bb) {
         const uint32_t nv = nverts[bb];
@@ -1136,7 +1139,7 @@ uint64_t orc_mc_indexed(const uint32_t* band_index, uint32_t B, const uint8_t* b
     const uint64_t stride[3] = {1, rx, rxy};
     const double isod = iso;
     std::vector<uint32_t> nverts(B);
-    orc_mc(band_index, B, band, phi, sc, iso, cpu, nverts.data(), nullptr, nullptr, 0);
+    orc_mc(band_index, B, band, phi, sc, iso, cpu, nverts.data(), nullptr, nullptr, 0, nullptr);
     auto bpos = [&](uint64_t vid) -> int64_t {
         const uint32_t* it = std::lower_bound(band_index, band_index + B, (uint32_t)vid);
         return (it != band_index + B && *it == vid) ? (int64_t)(it - band_index) : -1;

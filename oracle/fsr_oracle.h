@@ -61,11 +61,12 @@ int orc_field_zb05(const uint32_t* band_index, uint32_t B, const float* pos,
 /* Marching cubes over the band (ReconstructionCUDA.cu:328-463; with cpu = 1 the
  * host MC of GPU/MarchingCubesCPU.cpp).  Cube index from phi (FP64 compare
  * against iso), vertices in FP64.  nverts[B] out; tri_pos/tri_nrm (3 doubles
- * per soup vertex) written when not NULL and the total fits `capacity`.
- * Returns the total soup vertex count (3T). */
+ * per soup vertex) written when not NULL and the total fits `capacity`;
+ * nrm_len (may be NULL) receives each soup normal's length before its
+ * normalisation.  Returns the total soup vertex count (3T). */
 uint64_t orc_mc(const uint32_t* band_index, uint32_t B, const uint8_t* band,
                 const double* phi, orc_grid sc, float iso, int cpu, uint32_t* nverts,
-                double* tri_pos, double* tri_nrm, uint64_t capacity);
+                double* tri_pos, double* tri_nrm, uint64_t capacity, double* nrm_len);
 
 /* Yu13: smoothing (ReconstructionCUDA.cu:598-658); mean/smooth: 3 doubles each. */
 int orc_smooth(const float* pos, uint32_t n, const uint32_t* ranges, orc_grid sp, float radius,

@@ -28,10 +28,14 @@ def _frames():
                                               sphere_center=(0.125, 0.25, 0.0625),
                                               sphere_radius=0.035, grid_res=112, scale=1.3,
                                               iso=0.0, method="yu13"),
+        # the configured anisotropic workload: 1,000,000-particle dam-break, 384^3
+        "dambreak1M_384": lambda: ds.dam_break(n_target=1_000_000, grid_res=384, method="yu13",
+                                               scale=1.3, iso=0.0),
     }
 
 
-CASES = [("lattice16", True), ("lattice16", False), ("lattice36", True), ("dambreak_mini", True)]
+CASES = [("lattice16", True), ("lattice16", False), ("lattice36", True), ("dambreak_mini", True),
+         ("dambreak1M_384", True), ("dambreak1M_384", False)]
 
 
 @pytest.fixture(scope="module", params=CASES,
@@ -109,9 +113,9 @@ def test_field(case):
 
 def test_mc_given_gpu_field(case):
     fr, buf = case["fr"], case["buf"]
-    nv, pos, nrm = oracle.mc(buf["index"], buf["band"], case["phi"].astype(np.float64), fr.sc,
-                             np.float32(fr.iso))
+    nv, pos, nrm, nlen = oracle.mc(buf["index"], buf["band"], case["phi"].astype(np.float64),
+                                   fr.sc, np.float32(fr.iso), with_len=True)
     assert np.array_equal(nv, buf["nverts"])
     assert case["st"]["num_mesh_vertices"] == pos.shape[0] > 0
-    vbad, nbad, worst = parity.compare_mesh(fr, case["pos"], case["nrm"], pos, nrm)
+    vbad, nbad, worst = parity.compare_mesh(fr, case["pos"], case["nrm"], pos, nrm, nlen)
     assert vbad == 0 and nbad == 0

@@ -286,6 +286,52 @@ def test_mc_planar_field_exact():
         assert np.allclose(nrm[interior], [0, 0, -sign], atol=1e-12)
 
 
+def test_mc_cpu_flavour_normals_are_not_negated():
+    """The host marching cubes of the Akinci12 path (GPU/MarchingCubesCPU.cpp:
+    53-54) has no normal negation for iso < 0: the normal is the normalised
+    central difference phi(i-1) - phi(i+1) whatever the iso value, while the
+    GPU flavour multiplies it by -1 when iso < 0 (ReconstructionCUDA.cu:430)."""
+    sc = grid((0, 0, 0), 0.1, (6, 6, 8))
+    c = 0.3137
+    zz, _, _ = np.meshgrid(np.arange(8), np.arange(6), np.arange(6), indexing="ij")
+    phi = ((zz.astype(np.float32) * np.float32(0.1)).astype(np.float64) - c).ravel()
+    band = np.ones(sc.count, np.uint8)
+    idx = np.arange(sc.count, dtype=np.uint32)
+    for iso in (0.0, -1e-4):
+        _, pos, nrm = oracle.mc(idx, band, phi, sc, iso, cpu=True)
+        interior = (pos[:, 0] > 0.05) & (pos[:, 0] < 0.45) & (pos[:, 1] > 0.05) & (pos[:, 1] < 0.45)
+        assert interior.sum() > 50
+        assert np.allclose(nrm[interior], [0, 0, -1], atol=1e-12), iso
+        _, _, ng = oracle.mc(idx, band, phi, sc, iso, cpu=False)
+        assert np.allclose(ng[interior], [0, 0, -1 if iso == 0.0 else 1], atol=1e-12), iso
+
+
+@pytest.mark.parametrize("slope", [0.0, 2e-8])
+def test_mc_cpu_flavour_normalises_only_above_1e7(slope):
+    """fVector3::normalize (GPU/MathVector.cpp:5-13) leaves vectors of length
+    <= 1e-7 as they are, at the corners (MarchingCubesCPU.cpp:78-81) and after
+    the lerp (:53-54).  Field: -1e-3 on even z layers, +1e-3 on odd ones, plus
+    slope * x; at an interior cube the z and y differences vanish and the x
+    difference is -2 slope (length 4e-8 < 1e-7, or 0).  The CPU flavour keeps
+    that vector; the GPU flavour's v * rsqrt(dot(v, v)) gives a unit vector, or
+    NaN for the zero vector."""
+    sc = grid((0, 0, 0), 1.0, (6, 6, 6))
+    zz, yy, xx = np.meshgrid(np.arange(6), np.arange(6), np.arange(6), indexing="ij")
+    phi = (np.where(zz % 2 == 0, -1e-3, 1e-3) + slope * xx).ravel()
+    band = np.ones(sc.count, np.uint8)
+    cube = np.array([(2 * 6 + 2) * 6 + 2], np.uint32)
+    nv, pos, nrm = oracle.mc(cube, band, phi, sc, 0.0, cpu=True)
+    assert nv[0] == 6     # case 153: the four z-edges cross, two triangles
+    assert np.all(np.abs(pos[:, 2] - 2.5) < 1e-3)
+    expect = np.array([-2.0 * slope, 0.0, 0.0])
+    assert np.allclose(nrm, expect, rtol=1e-6, atol=0.0)
+    _, _, ng = oracle.mc(cube, band, phi, sc, 0.0, cpu=False)
+    if slope == 0.0:
+        assert np.all(np.isnan(ng))
+    else:
+        assert np.allclose(ng, [-1.0, 0.0, 0.0], atol=1e-9)
+
+
 def test_mc_sphere_vertices_on_sphere():
     sc = grid((-0.6, -0.6, -0.6), 0.05, (25, 25, 25))
     g = np.arange(25, dtype=np.float32) * np.float32(0.05) + np.float32(-0.6)
