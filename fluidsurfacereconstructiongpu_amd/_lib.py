@@ -195,6 +195,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         f = getattr(lib, name)
         f.argtypes = sig
         f.restype = st
+    lib.fsr_exclusive_scan_noalloc.argtypes = [P, P, P, c_uint32, P, P]
+    lib.fsr_exclusive_scan_noalloc.restype = st
     lib.fsr_batch_create.argtypes = [c_void_p, c_uint32, POINTER(c_void_p)]
     lib.fsr_batch_create.restype = st
     lib.fsr_batch_destroy.argtypes = [c_void_p]

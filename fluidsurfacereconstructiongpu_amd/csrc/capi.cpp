@@ -127,6 +127,7 @@ fsr_status fsr_create(fsr_context** out_ctx, int device) {
         return FSR_ERR_INVALID_ARG;
     }
     FSR_HIP(hipSetDevice(device));
+    FSR_TRY(noalloc_scratch_init(device));
     fsr_context* ctx = new fsr_context();
     ctx->device = device;
 

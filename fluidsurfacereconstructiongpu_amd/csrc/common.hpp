@@ -326,6 +326,12 @@ fsr_status scan_exclusive(fsr_context* ctx, ScanInput f, const uint32_t* in,
                           uint32_t* out, uint64_t n_max, const uint32_t* d_count,
                           uint32_t* d_total, hipStream_t s);
 fsr_status fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
+// Sets up the per-device scratch of the allocation-free scan (device address of
+// the scratch array, looked up once per device).
+fsr_status noalloc_scratch_init(int device);
+// Exclusive scan of in[0, n) with its workgroup partials in that scratch.
+fsr_status scan_exclusive_noalloc(int device, const uint32_t* in, uint32_t* out, uint64_t n,
+                                  uint32_t* d_total, hipStream_t s);
 
 // Cell-major work list offsets from the per-cell band counts cnt[n]: cell_off
 // (exclusive scan), chunk_off (exclusive scan of the 64-vertex chunk counts),

@@ -14,8 +14,17 @@
 // look-back form measured no faster here: its per-tile ticket is a
 // same-address atomic, and those serialise at roughly 30 ns each.)
 #include <algorithm>
+#include <mutex>
 
 #include "common.hpp"
+
+// Scratch of the allocation-free scan (fsr_exclusive_scan_noalloc): the
+// partial sums of its at most 1024 workgroups (4 KiB).  One copy per device,
+// held for the life of the process.  Host code never passes this name to a
+// kernel (on the host it is only a stand-in): it uses the device address that
+// hipGetSymbolAddress returns, kept per device in g_scratch_ptr.
+constexpr uint32_t kNoAllocScratchWords = 1024;
+static __device__ __align__(256) uint32_t scratch[kNoAllocScratchWords];
 
 namespace fsr {
 
@@ -423,7 +432,59 @@ fsr_status scan_impl(fsr_context* ctx, const uint32_t* in, uint32_t* out, uint64
     return scan_launch<F>(in, out, n_max, d_count, d_total, partials, s);
 }
 
+static_assert(kNoAllocScratchWords >= kMaxScanBlocks, "scratch holds every workgroup partial");
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[kMaxDevices] = {};
+
 }  // namespace
+
+fsr_status noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) {
+        set_error("noalloc_scratch_init: device index beyond the scratch table");
+        return FSR_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return FSR_OK;
+    int cur = -1;
+    FSR_HIP(hipGetDevice(&cur));
+    if (cur != device) FSR_HIP(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+    if (!p) {
+        set_error("hipGetSymbolAddress(scratch) returned a null address");
+        return FSR_ERR_HIP;
+    }
+    g_scratch_ptr[device] = static_cast<uint32_t*>(p);
+    return FSR_OK;
+}
+
+fsr_status scan_exclusive_noalloc(int device, const uint32_t* in, uint32_t* out, uint64_t n,
+                                  uint32_t* d_total, hipStream_t s) {
+    if (n > FSR_NOALLOC_SCAN_MAX_ELEMENTS) {
+        set_error("fsr_exclusive_scan_noalloc: more than FSR_NOALLOC_SCAN_MAX_ELEMENTS (2^30) "
+                  "elements");
+        return FSR_ERR_CAPACITY;
+    }
+    uint32_t* partials = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (device >= 0 && device < kMaxDevices) partials = g_scratch_ptr[device];
+    }
+    if (!partials) {
+        set_error("fsr_exclusive_scan_noalloc: the scratch of this device was not set up "
+                  "(fsr_create on the device does it)");
+        return FSR_ERR_INVALID_ARG;
+    }
+    if (scan_blocks(n) > kNoAllocScratchWords) {
+        set_error("fsr_exclusive_scan_noalloc: scan needs more partials than the scratch holds");
+        return FSR_ERR_CAPACITY;
+    }
+    return scan_launch<SCAN_U32>(in, out, n, nullptr, d_total, partials, s);
+}
 
 fsr_status scan_exclusive(fsr_context* ctx, ScanInput f, const uint32_t* in, uint32_t* out,
                           uint64_t n_max, const uint32_t* d_count, uint32_t* d_total,
@@ -519,3 +580,15 @@ fsr_status fill_f32(float* p, float value, uint64_t n, hipStream_t s) {
 }
 
 }  // namespace fsr
+
+extern "C" fsr_status fsr_exclusive_scan_noalloc(fsr_context* ctx, uint32_t* output,
+                                                 const uint32_t* input, uint32_t num_elements,
+                                                 uint32_t* d_total, void* stream) {
+    if (!ctx || (num_elements && (!output || !input))) {
+        fsr::set_error("fsr_exclusive_scan_noalloc: null context or buffer");
+        return FSR_ERR_INVALID_ARG;
+    }
+    FSR_HIP(hipSetDevice(ctx->device));
+    return fsr::scan_exclusive_noalloc(ctx->device, input, output, num_elements, d_total,
+                                       (hipStream_t)stream);
+}

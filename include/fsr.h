@@ -360,6 +360,21 @@ fsr_status fsr_thrust_exclusive_prefix_sum_scan(fsr_context* ctx, uint32_t* outp
                                                 const uint32_t* input, uint32_t num_elements,
                                                 uint32_t* total, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the exclusive scan of
+ * fsr_thrust_exclusive_prefix_sum_scan with its workgroup partials in a fixed
+ * scratch array of the library (4 KiB per device, set up by fsr_create and
+ * held until the process ends) instead of the context arena.  Allocates
+ * nothing.  d_total (device u32, may be NULL) receives the sum; nothing is
+ * copied to the host and nothing blocks.  num_elements above
+ * FSR_NOALLOC_SCAN_MAX_ELEMENTS returns FSR_ERR_CAPACITY.  All calls on one
+ * device share its scratch: calls that overlap in time on one device (two
+ * streams, two threads) race and must be ordered by the caller. */
+#define FSR_NOALLOC_SCAN_MAX_ELEMENTS (1u << 30)
+fsr_status fsr_exclusive_scan_noalloc(fsr_context* ctx, uint32_t* output, const uint32_t* input,
+                                      uint32_t num_elements, uint32_t* d_total, void* stream);
+
 /* launchCompactSurfaceVertex (ReconstructionCUDA.cuh:58-65):
  * sv_index[scan[v]] = v for is_surface[v] == 1, v < num_vertices. */
 fsr_status fsr_compact_surface_vertex(fsr_context* ctx, uint32_t* sv_index,
