@@ -13,6 +13,8 @@ Beside the headline the same JSON line carries
   akinci12   the same frame with the Akinci12 colour-field band;
   yu13       the anisotropic (TC01) frame: same input, 384^3 grid;
   batched    64 random-shape frames, ZB05 and Yu13, batched vs per-frame;
+  frame_sequence  the frame-sequence driver on .xyz copies of the headline
+             frame, pipelined against strictly serial (read, reconstruct, write);
   adjoint    the 256,000-particle ball, backward vs forward time;
   each with its kernels' roofline figures, and the CPU oracle beside them.
 
@@ -230,14 +232,17 @@ def zb05_roofline(rec, fr, st, stage):
     ms = float(stage["stage_ms"]["field"])
     pairs_exec = 256 * steps
     alg_bytes = 8.0 * B + 8.0 * fr.sp.count + 12.0 * n_touched
+    ref_rate = 20.0 * p27 / (ms * 1e-3) / 1e12
     return flop_roofline(
-        "k_field_zb05", 20.0 * p27, ms,
-        pairs_p27=p27,
-        frac_executed=20.0 * pairs_exec / (ms * 1e-3) / FP32_PEAK,
+        "k_field_zb05", 20.0 * pairs_exec, ms,
         executed_pairs=pairs_exec, candidates_staged=staged, candidates_kept=kept,
-        note=("achieved = 20 FLOP per candidate pair of the reference (P27: the clamped 3^3 "
-              "cell block of each band vertex) / kernel time; frac_executed counts the "
-              "16x16 tiles the kernel evaluates after its box cull"),
+        pairs_p27=p27, reference_pair_rate=ref_rate,
+        reference_pair_frac=ref_rate / (FP32_PEAK / 1e12),
+        note=("achieved = 20 FLOP per pair the kernel evaluates (256 per 16x16 "
+              "vertex/candidate tile left after its box cull, counted on the device) / "
+              "kernel time; reference_pair_rate counts 20 FLOP per candidate pair of the "
+              "reference (P27: the clamped 3^3 cell block of each band vertex), most of "
+              "which the cull never evaluates"),
         algorithmic_bytes=alg_bytes, algorithmic_gbps=alg_bytes / (ms * 1e-3) / 1e9,
         traffic=None)
 
@@ -263,6 +268,28 @@ def akinci_line(fr, K, W):
     return out, rec, st
 
 
+def aniso_work(rec):
+    """(staged, kept) candidate totals of the last smoothing, covariance and
+    field launches, from their per-wave records."""
+    _, cap = rec.ctx.buffer_ptr("aniso_work")
+    w = rec.buffer("aniso_work", np.uint32, cap).astype(np.int64)
+    per = cap // 3
+    out = []
+    for k in range(3):
+        sec = w[k * per:(k + 1) * per]
+        waves = int(sec[0])
+        rec2 = sec[1:1 + 2 * waves].reshape(-1, 2)
+        out.append((int(rec2[:, 0].sum()), int(rec2[:, 1].sum())))
+    return out
+
+
+# FP32 operations per evaluated pair (fused multiply-add = 2), from the pair
+# loops: smoothing (distance by expansion, square root, weight, 4 sums),
+# covariance (difference, distance, count, weight, 7 sums), field (9 FMA
+# quadratic form, clamp, cube, weighted sum).
+ANISO_PAIR_FLOP = {"smooth": 19.0, "cov": 33.0, "field": 24.0}
+
+
 def yu13_line(fr, aniso, K, W):
     rec, total_ms, stages, st = time_frames("yu13", fr, K, W, aniso=aniso)
     stage = stage_summary(stages)
@@ -271,15 +298,32 @@ def yu13_line(fr, aniso, K, W):
     ranges = rec.buffer("cell_ranges", np.uint32, 2 * fr.sp.count).reshape(-1, 2)
     involve = rec.buffer("involve_counts", np.uint32, fr.sp.count)
     c = yu13_counts(fr, band_index, ranges, involve)
-    f_field = 30.0 * c["p125_field_inv"] + 15.0 * c["p125_field_non"]
-    f_prep = 20.0 * c["p125_smooth"] + 32.0 * c["p125_cov"] + 600.0 * c["involved"]
-    field = flop_roofline("k_field_aniso", f_field, float(stage["stage_ms"]["field"]),
-                          pairs_involved=c["p125_field_inv"], pairs_other=c["p125_field_non"],
-                          note="F = 30 P125(involved) + 15 P125(other) over the band")
-    prep = flop_roofline("k_smooth + k_covariance", f_prep, float(stage["stage_ms"]["prep"]),
-                         pairs_smooth=c["p125_smooth"], pairs_cov=c["p125_cov"],
-                         involved=c["involved"],
-                         note="F = 20 P125(N) + 32 P125(I) + 600 I; time = the prep stage")
+    (sm_st, sm_kept), (cv_st, cv_kept), (fd_st, fd_kept) = aniso_work(rec)
+    ms_field = float(stage["stage_ms"]["field"])
+    ms_prep = float(stage["stage_ms"]["prep"])
+    f_field = ANISO_PAIR_FLOP["field"] * 64.0 * fd_kept
+    f_prep = 64.0 * (ANISO_PAIR_FLOP["smooth"] * sm_kept + ANISO_PAIR_FLOP["cov"] * cv_kept)
+    ref_field = 30.0 * c["p125_field_inv"] + 15.0 * c["p125_field_non"]
+    ref_prep = 20.0 * c["p125_smooth"] + 32.0 * c["p125_cov"] + 600.0 * c["involved"]
+    field = flop_roofline(
+        "k_field_aniso", f_field, ms_field, candidates_staged=fd_st, candidates_kept=fd_kept,
+        executed_pairs=64 * fd_kept, pairs_involved=c["p125_field_inv"],
+        pairs_other=c["p125_field_non"],
+        reference_pair_rate=ref_field / (ms_field * 1e-3) / 1e12,
+        note=("achieved = 24 FLOP per pair evaluated (64 lanes x each candidate kept by the "
+              "chunk-box cull, counted on the device) / field stage time; particles of "
+              "non-involved cells enter as 35 per-block moments (FP64 polynomial per vertex), "
+              "not as pairs; reference_pair_rate counts 30 P125(involved) + 15 P125(other) "
+              "FLOP of the reference's 5^3-block loop"))
+    prep = flop_roofline(
+        "k_smooth + k_covariance", f_prep, ms_prep,
+        smooth_staged=sm_st, smooth_kept=sm_kept, cov_staged=cv_st, cov_kept=cv_kept,
+        pairs_smooth=c["p125_smooth"], pairs_cov=c["p125_cov"], involved=c["involved"],
+        reference_pair_rate=ref_prep / (ms_prep * 1e-3) / 1e12,
+        note=("achieved = 64 lanes x (19 FLOP per kept smoothing candidate + 33 per kept "
+              "covariance candidate) / prep stage time (the two kernels, their chunk lists "
+              "and the involved list); reference_pair_rate counts 20 P125(N) + 32 P125(I) + "
+              "600 I"))
     out = {"value": K / (total_ms * 1e-3), "unit": "frames/s",
            "workload": "dam-break frame, %d particles, 384^3 grid, Yu13 anisotropic (TC01) "
                        "field, lambda %.2f kn %.2f kr %.1f Ne %d, full marching cubes"
@@ -401,6 +445,46 @@ def batched_vs_per_frame(method: str, frames: int, lanes: int, reps: int, autotu
         "per_frame_ms_p50": float(np.percentile(per_frame_ms, 50)),
         "per_frame_ms_p99": float(np.percentile(per_frame_ms, 99)),
     }
+
+
+def frame_sequence(fr, frames: int = 4):
+    """Frame-sequence driver (the reference's ReconstructorGPU::reconstruct flow:
+    read .xyz, reconstruct, write .obj, frame after frame) on `frames` copies of
+    the headline frame written as .xyz files: end-to-end frames/s of the
+    pipelined order (frame k+1 parsed and uploaded while frame k runs and its
+    mesh is written) against --serial (strictly one after another)."""
+    import re
+
+    from fluidsurfacereconstructiongpu_amd import frame_io
+    res = {"frames": frames}
+    with tempfile.TemporaryDirectory(prefix="fsr_seq_") as d:
+        lo, hi = fr.pos.min(axis=0), fr.pos.max(axis=0)
+        first = os.path.join(d, "frame_000000.xyz")
+        frame_io.write_xyz(first, lo, hi, fr.h, fr.particle_radius, fr.mass, fr.pos, fr.dens)
+        for k in range(1, frames):
+            os.link(first, os.path.join(d, "frame_%06d.xyz" % k))
+        for mode in ("serial", "pipelined"):
+            cmd = [frame_io.DRIVER, d + "/", "frame_%06d", "0", str(frames)]
+            if mode == "serial":
+                cmd.append("--serial")
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+            if r.returncode != 0:
+                raise RuntimeError("fsr_reconstruct failed: %s" % r.stderr[-500:])
+            m = re.search(r"(\d+) frame\(s\) in ([0-9.eE+-]+) ms end to end", r.stdout)
+            rec = [float(x) for x in re.findall(r"took ([0-9.eE+-]+) milliseconds", r.stdout)]
+            res[mode + "_frames_per_s"] = int(m.group(1)) * 1e3 / float(m.group(2))
+            res[mode + "_reconstruct_ms_median"] = float(np.median(rec))
+            g = re.search(r"Scalar Field Resolution: (\d+) \* (\d+) \* (\d+)", r.stdout)
+            res["grid"] = [int(g.group(i)) for i in (1, 2, 3)]
+            for k in range(frames):
+                obj = os.path.join(d, "frame_%06d.obj" % k)
+                if os.path.exists(obj):
+                    res.setdefault("obj_bytes", os.path.getsize(obj))
+                    os.remove(obj)
+    res["speedup"] = res["pipelined_frames_per_s"] / res["serial_frames_per_s"]
+    res["workload"] = ("%d .xyz frames of %d particles (grid from the reference's rule), ZB05, "
+                       "soup .obj written per frame" % (frames, fr.n))
+    return res
 
 
 def adjoint_cost(reps: int = 20):
@@ -632,6 +716,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default lane launch shape (no fsr_batch_autotune)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost")
+    ap.add_argument("--no-sequence", action="store_true",
+                    help="skip the frame-sequence driver (pipelined vs serial)")
     ap.add_argument("--no-counters", action="store_true",
                     help="skip the rocprofv3 counter runs (traffic, L2 hit rate)")
     ap.add_argument("--counter-probe", action="store_true", help=argparse.SUPPRESS)
@@ -705,6 +791,13 @@ def main():
                                                    args.batch_reps, not args.no_autotune)
         log(f"batched yu13: {out['batched_yu13']}")
         torch.cuda.empty_cache()
+    if not args.no_sequence:
+        try:
+            out["frame_sequence"] = frame_sequence(fr)
+            log(f"frame sequence: {out['frame_sequence']}")
+        except Exception as e:  # the headline line stands without it
+            out["frame_sequence"] = {"error": str(e)}
+            log(f"frame sequence failed: {e}")
     if not args.no_adjoint:
         out["adjoint"] = adjoint_cost()
         log(f"adjoint: {out['adjoint']}")

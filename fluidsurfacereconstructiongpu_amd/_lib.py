@@ -109,7 +109,7 @@ BUF = {
     "surface_flags": 4, "band_bits": 5, "band_index": 6, "cube_nverts": 7,
     "valid_cubes": 8, "involve_counts": 9, "mean_pos": 10, "smooth_pos": 11,
     "g_matrices": 12, "singular": 13, "involved_index": 14, "field_work": 15,
-    "color_work": 16,
+    "color_work": 16, "aniso_work": 17,
 }
 
 STATUS = {0: "FSR_OK", 1: "FSR_ERR_INVALID_ARG", 2: "FSR_ERR_HIP", 3: "FSR_ERR_CAPACITY",

@@ -94,9 +94,32 @@ struct BlockRows {
     }
 };
 
+// Cell-level culling (cell_lo / cell_hi may be null): a cell counts only when
+// the bounding box of its particles (cell_lo.xyz, cell_hi.xyz), grown by their
+// largest support radius (cell_lo.w = its square, negative for a cell with
+// nothing to stage), reaches the box [b0, b1]; a row's run then spans its first
+// to its last such cell (cells between them stay in it, their particles are
+// culled one by one).
+struct CellCull {
+    const float4* lo;
+    const float4* hi;
+    float b0[3], b1[3];
+};
+
+__device__ __forceinline__ bool cell_reaches(const CellCull& cc, uint32_t c) {
+    const float4 lo = cc.lo[c];
+    if (!(lo.w > 0.0f)) return false;
+    const float4 hi = cc.hi[c];
+    const float dx = fmaxf(fmaxf(lo.x - cc.b1[0], cc.b0[0] - hi.x), 0.0f);
+    const float dy = fmaxf(fmaxf(lo.y - cc.b1[1], cc.b0[1] - hi.y), 0.0f);
+    const float dz = fmaxf(fmaxf(lo.z - cc.b1[2], cc.b0[2] - hi.z), 0.0f);
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < lo.w;
+}
+
 template <int R>
 __device__ __forceinline__ BlockRows block_rows(const fsr_index_range* __restrict__ ranges,
-                                                const GridDev& sp, int cx, int cy, int cz) {
+                                                const GridDev& sp, int cx, int cy, int cz,
+                                                const CellCull* cull = nullptr) {
     const uint32_t lane = __lane_id();
     const int x0 = clampi(cx - R, 0, sp.rx - 1), x1 = clampi(cx + R, 0, sp.rx - 1);
     const int y0 = clampi(cy - R, 0, sp.ry - 1), y1 = clampi(cy + R, 0, sp.ry - 1);
@@ -110,7 +133,7 @@ __device__ __forceinline__ BlockRows block_rows(const fsr_index_range* __restric
         uint32_t s = 0xFFFFFFFFu, e = 0;
         for (int xx = x0; xx <= x1; ++xx) {
             const fsr_index_range rg = ranges[rowc + xx];
-            if (rg.start != 0xFFFFFFFFu) {
+            if (rg.start != 0xFFFFFFFFu && (!cull || cell_reaches(*cull, rowc + xx))) {
                 if (s == 0xFFFFFFFFu) s = rg.start;
                 e = rg.end;
             }
@@ -133,6 +156,23 @@ struct ChunkList {
     const uint32_t* num_chunks;  // device count
     uint32_t* next;              // per-XCD fetch counters (next_item_xcd)
 };
+
+// Work records of the neighbour kernels: one section per kernel (smoothing,
+// covariance, field) of 1 + 2 W words, W = kWorkWavesPerCu waves per CU: the
+// launch's wave count, then per wave (candidates staged, candidates kept by
+// the chunk-box test).  The counts are wave-uniform.
+constexpr uint32_t kWorkWavesPerCu = 32;
+enum WorkSection { WORK_SMOOTH = 0, WORK_COV = 1, WORK_FIELD = 2, WORK_SECTIONS = 3 };
+
+__device__ __forceinline__ void write_work(uint32_t* __restrict__ sec, uint32_t staged,
+                                           uint32_t kept) {
+    const uint32_t gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (__lane_id() == 0) {
+        sec[1 + 2 * gw] = staged;
+        sec[2 + 2 * gw] = kept;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) sec[0] = gridDim.x * (blockDim.x >> 6);
+}
 
 // ---------------------------------------------------------------- K10 -------
 __device__ __forceinline__ float wave_minf(float v) {
@@ -174,7 +214,7 @@ constexpr float kRadiusCull = 1.001f;
 __global__ __launch_bounds__(kAnisoThreads, 6) void k_smooth(
     const float4* __restrict__ sorted, const fsr_index_range* __restrict__ ranges, GridDev sp,
     float radius, float lambda, ChunkList cl, float4* __restrict__ mean_out,
-    float4* __restrict__ smooth_out) {
+    float4* __restrict__ smooth_out, uint32_t* __restrict__ work) {
     __shared__ float4 queue[kAnisoThreads / 64][128];
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x >> 6;
@@ -183,6 +223,7 @@ __global__ __launch_bounds__(kAnisoThreads, 6) void k_smooth(
     const float inv_r2 = 1.0f / (radius * radius);
     const float cull2 = radius * radius * kRadiusCull;
     uint32_t step = 0;
+    uint32_t n_staged = 0, n_kept = 0;
     for (;;) {
         const uint32_t ch = next_item_xcd(cl.next, nch, step);
         if (ch >= nch) break;
@@ -194,6 +235,7 @@ __global__ __launch_bounds__(kAnisoThreads, 6) void k_smooth(
         const int cx = (int)(c % sp.rx), cy = (int)((c / sp.rx) % sp.ry), cz = (int)(c / (sp.rx * sp.ry));
         BlockRows br = block_rows<2>(ranges, sp, cx, cy, cz);
         const uint32_t total = br.total;
+        n_staged += total;
         const float ox = __builtin_fmaf((float)cx, sp.cell, sp.minx);
         const float oy = __builtin_fmaf((float)cy, sp.cell, sp.miny);
         const float oz = __builtin_fmaf((float)cz, sp.cell, sp.minz);
@@ -253,6 +295,7 @@ __global__ __launch_bounds__(kAnisoThreads, 6) void k_smooth(
             const uint64_t m = __ballot(keep);
             if (keep) q[fill + mask_rank(m)] = st;
             fill += (uint32_t)__popcll(m);
+            n_kept += (uint32_t)__popcll(m);
             if (fill >= 64u) {
                 wave_lds_sync();
                 run(64u);
@@ -279,6 +322,7 @@ __global__ __launch_bounds__(kAnisoThreads, 6) void k_smooth(
                                         a * self.z + lambda * mz, 0.0f);
         }
     }
+    if (work) write_work(work, n_staged, n_kept);
 }
 
 // ---------------------------------------------------------------- K8 --------
@@ -359,7 +403,63 @@ struct CovArgs {
     fsr_matrix* gmat;       // [I]
     float4* sigma;          // [I] eigenvalues, descending
     float4* rec;            // [2N] field records
+    uint32_t* work;         // work records (may be null)
 };
+
+// G and the field record of involved particle i (its involved index gi) from
+// its neighbour count and weighted sums: C = (h^2 I + sum w v v^T) / sum w.
+__device__ __forceinline__ void cov_finish(const CovArgs& a, uint32_t gi, uint32_t i,
+                                           uint32_t cnt, float sw, float c00, float c01,
+                                           float c02, float c11, float c12, float c22) {
+    float G[3][3];
+    float4 sig;
+    float gmin;  // smallest singular value of G: the support is |r| < 1 / gmin
+    if (cnt < a.min_neighbors) {
+        const float gv = a.inv_h / a.kn;
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q) G[p][q] = p == q ? gv : 0.0f;
+        sig = make_float4(0.f, 0.f, 0.f, 0.f);
+        gmin = gv;
+    } else {
+        const float iw = 1.0f / sw;
+        const float a00 = (a.h2 + c00) * iw, a11 = (a.h2 + c11) * iw, a22 = (a.h2 + c22) * iw;
+        const float a01 = c01 * iw, a02 = c02 * iw, a12 = c12 * iw;
+        float d[3], V[3][3];
+        sym_eigen3(a00, a01, a02, a11, a12, a22, d, V);
+        float s0 = fabsf(d[0]), s1 = fabsf(d[1]), s2 = fabsf(d[2]);
+        // report sorted descending
+        float e0 = fmaxf(s0, fmaxf(s1, s2)), e2 = fminf(s0, fminf(s1, s2));
+        sig = make_float4(e0, s0 + s1 + s2 - e0 - e2, e2, 0.f);
+        const float floor_v = e0 / a.kr;
+        s0 = fmaxf(s0, floor_v);
+        s1 = fmaxf(s1, floor_v);
+        s2 = fmaxf(s2, floor_v);
+        const float scale = cbrtf(s0 * s1 * s2) * a.inv_h;
+        const float g0 = scale / s0, g1 = scale / s1, g2 = scale / s2;
+        gmin = fminf(g0, fminf(g1, g2));
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q)
+                G[p][q] = V[0][p] * g0 * V[0][q] + V[1][p] * g1 * V[1][q] + V[2][p] * g2 * V[2][q];
+    }
+    fsr_matrix out;
+    out.a11 = G[0][0]; out.a12 = G[0][1]; out.a13 = G[0][2];
+    out.a21 = G[1][0]; out.a22 = G[1][1]; out.a23 = G[1][2];
+    out.a31 = G[2][0]; out.a32 = G[2][1]; out.a33 = G[2][2];
+    a.gmat[gi] = out;
+    a.sigma[gi] = sig;
+    // determinant (ReconstructionCUDA.cuh:690-700) and M = G^T G
+    const float det = G[0][0] * G[1][1] * G[2][2] + G[0][1] * G[1][2] * G[2][0] +
+                      G[0][2] * G[1][0] * G[2][1] - G[0][0] * G[1][2] * G[2][1] -
+                      G[0][1] * G[1][0] * G[2][2] - G[0][2] * G[1][1] * G[2][0];
+    float M[3][3];
+    for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q)
+            M[p][q] = G[0][p] * G[0][q] + G[1][p] * G[1][q] + G[2][p] * G[2][q];
+    const float coef = kSigma * det * (a.mass / a.sorted[i].w);
+    a.rec[2ull * i + 0] = make_float4(M[0][0], M[1][1], M[2][2], M[0][1]);
+    // squared culling radius of the field kernel, with a 1% margin
+    a.rec[2ull * i + 1] = make_float4(M[0][2], M[1][2], coef, kCullMargin / (gmin * gmin));
+}
 
 __global__ __launch_bounds__(kAnisoThreads, 6) void k_covariance(CovArgs a, ChunkList cl) {
     __shared__ float4 queue[kAnisoThreads / 64][128];
@@ -371,6 +471,7 @@ __global__ __launch_bounds__(kAnisoThreads, 6) void k_covariance(CovArgs a, Chun
     const float inv_r = 1.0f / r;
     const float cull2 = r * r * kRadiusCull;
     uint32_t step = 0;
+    uint32_t n_staged = 0, n_kept = 0;
     for (;;) {
         const uint32_t ch = next_item_xcd(cl.next, nch, step);
         if (ch >= nch) break;
@@ -383,6 +484,7 @@ __global__ __launch_bounds__(kAnisoThreads, 6) void k_covariance(CovArgs a, Chun
                   cz = (int)(c / (a.sp.rx * a.sp.ry));
         BlockRows br = block_rows<2>(a.ranges, a.sp, cx, cy, cz);
         const uint32_t total = br.total;
+        n_staged += total;
         const float ox = __builtin_fmaf((float)cx, a.sp.cell, a.sp.minx);
         const float oy = __builtin_fmaf((float)cy, a.sp.cell, a.sp.miny);
         const float oz = __builtin_fmaf((float)cz, a.sp.cell, a.sp.minz);
@@ -440,6 +542,7 @@ __global__ __launch_bounds__(kAnisoThreads, 6) void k_covariance(CovArgs a, Chun
             const uint64_t bm = __ballot(keep);
             if (keep) q[fill + mask_rank(bm)] = st;
             fill += (uint32_t)__popcll(bm);
+            n_kept += (uint32_t)__popcll(bm);
             if (fill >= 64u) {
                 wave_lds_sync();
                 run(64u);
@@ -457,58 +560,10 @@ __global__ __launch_bounds__(kAnisoThreads, 6) void k_covariance(CovArgs a, Chun
             run(n);
             wave_lds_sync();
         }
-        if (live) {
-        const uint32_t gi = a.involve_off[c] + (i - own.start);
-        float G[3][3];
-        float4 sig;
-        float gmin;  // smallest singular value of G: the support is |r| < 1 / gmin
-        if (cnt < a.min_neighbors) {
-            const float gv = a.inv_h / a.kn;
-            for (int p = 0; p < 3; ++p)
-                for (int q = 0; q < 3; ++q) G[p][q] = p == q ? gv : 0.0f;
-            sig = make_float4(0.f, 0.f, 0.f, 0.f);
-            gmin = gv;
-        } else {
-            const float iw = 1.0f / sw;
-            const float a00 = (a.h2 + c00) * iw, a11 = (a.h2 + c11) * iw, a22 = (a.h2 + c22) * iw;
-            const float a01 = c01 * iw, a02 = c02 * iw, a12 = c12 * iw;
-            float d[3], V[3][3];
-            sym_eigen3(a00, a01, a02, a11, a12, a22, d, V);
-            float s0 = fabsf(d[0]), s1 = fabsf(d[1]), s2 = fabsf(d[2]);
-            // report sorted descending
-            float e0 = fmaxf(s0, fmaxf(s1, s2)), e2 = fminf(s0, fminf(s1, s2));
-            sig = make_float4(e0, s0 + s1 + s2 - e0 - e2, e2, 0.f);
-            const float floor_v = e0 / a.kr;
-            s0 = fmaxf(s0, floor_v);
-            s1 = fmaxf(s1, floor_v);
-            s2 = fmaxf(s2, floor_v);
-            const float scale = cbrtf(s0 * s1 * s2) * a.inv_h;
-            const float g0 = scale / s0, g1 = scale / s1, g2 = scale / s2;
-            gmin = fminf(g0, fminf(g1, g2));
-            for (int p = 0; p < 3; ++p)
-                for (int q = 0; q < 3; ++q)
-                    G[p][q] = V[0][p] * g0 * V[0][q] + V[1][p] * g1 * V[1][q] + V[2][p] * g2 * V[2][q];
-        }
-        fsr_matrix out;
-        out.a11 = G[0][0]; out.a12 = G[0][1]; out.a13 = G[0][2];
-        out.a21 = G[1][0]; out.a22 = G[1][1]; out.a23 = G[1][2];
-        out.a31 = G[2][0]; out.a32 = G[2][1]; out.a33 = G[2][2];
-        a.gmat[gi] = out;
-        a.sigma[gi] = sig;
-        // determinant (ReconstructionCUDA.cuh:690-700) and M = G^T G
-        const float det = G[0][0] * G[1][1] * G[2][2] + G[0][1] * G[1][2] * G[2][0] +
-                          G[0][2] * G[1][0] * G[2][1] - G[0][0] * G[1][2] * G[2][1] -
-                          G[0][1] * G[1][0] * G[2][2] - G[0][2] * G[1][1] * G[2][0];
-        float M[3][3];
-        for (int p = 0; p < 3; ++p)
-            for (int q = 0; q < 3; ++q)
-                M[p][q] = G[0][p] * G[0][q] + G[1][p] * G[1][q] + G[2][p] * G[2][q];
-        const float coef = kSigma * det * (a.mass / a.sorted[i].w);
-        a.rec[2ull * i + 0] = make_float4(M[0][0], M[1][1], M[2][2], M[0][1]);
-        // squared culling radius of the field kernel, with a 1% margin
-        a.rec[2ull * i + 1] = make_float4(M[0][2], M[1][2], coef, kCullMargin / (gmin * gmin));
-        }
+        if (live) cov_finish(a, a.involve_off[c] + (i - own.start), i, cnt, sw, c00, c01, c02,
+                             c11, c12, c22);
     }
+    if (a.work) write_work(a.work, n_staged, n_kept);
 }
 
 // ---------------------------------------------------------------- K12 -------
@@ -521,7 +576,91 @@ struct AnisoFieldArgs {
     float* phi;
     uint32_t* next;
     const uint32_t* num_chunks;
+    // when set: kNonInvMoments moments per virtual cell of the non-involved
+    // particles of its clamped 5^3 block (the field kernel then skips those
+    // particles and evaluates their sum as a polynomial), about the origin
+    // (ox, oy, oz)
+    const double* block_mom;
+    double ox, oy, oz;
+    // with block_mom: per cell, the box of its involved particles' smoothed
+    // positions and their largest squared support radius (see CellCull)
+    const float4* cell_lo;
+    const float4* cell_hi;
+    uint32_t* work;              // work records (may be null)
 };
+
+// The non-involved part of the field as a polynomial.  For the particles p of
+// non-involved cells the reference adds c (1 - |x - p|^2)^3 with c = sigma m /
+// rho in world units; over a block whose every |x - p| < 1 (checked on the
+// host) the sum is, with A = 1 - |x|^2, B = 2 x.p - |p|^2 and P = |p|^2,
+//   sum c (A + B)^3 = A^3 S0 + 3 A^2 sum cB + 3 A sum cB^2 + sum cB^3,
+//   sum cB   = 2 x.M1 - S_P
+//   sum cB^2 = 4 x^T M2 x - 4 x.M1P + S_P2
+//   sum cB^3 = 8 sum c (x.p)^3 - 12 x^T M2P x + 6 x.M1P2 - S_P3,
+// exactly, in terms of 35 weighted moments of the particles (about one origin
+// for the whole grid, in FP64):
+//   0 S0  1-3 M1  4 S_P  5-10 M2 (xx xy xz yy yz zz)  11-13 M1P  14 S_P2
+//   15-24 M3 (xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz)  25-30 M2P  31-33 M1P2
+//   34 S_P3
+// A block's moments are the sums of its cells' moments.
+constexpr int kNonInvMoments = 35;
+
+__device__ __forceinline__ double moment_term(int k, double x, double y, double z) {
+    const double P = x * x + y * y + z * z;
+    switch (k) {
+        case 0: return 1.0;
+        case 1: return x;
+        case 2: return y;
+        case 3: return z;
+        case 4: return P;
+        case 5: return x * x;
+        case 6: return x * y;
+        case 7: return x * z;
+        case 8: return y * y;
+        case 9: return y * z;
+        case 10: return z * z;
+        case 11: return x * P;
+        case 12: return y * P;
+        case 13: return z * P;
+        case 14: return P * P;
+        case 15: return x * x * x;
+        case 16: return x * x * y;
+        case 17: return x * x * z;
+        case 18: return x * y * y;
+        case 19: return x * y * z;
+        case 20: return x * z * z;
+        case 21: return y * y * y;
+        case 22: return y * y * z;
+        case 23: return y * z * z;
+        case 24: return z * z * z;
+        case 25: return x * x * P;
+        case 26: return x * y * P;
+        case 27: return x * z * P;
+        case 28: return y * y * P;
+        case 29: return y * z * P;
+        case 30: return z * z * P;
+        case 31: return x * P * P;
+        case 32: return y * P * P;
+        case 33: return z * P * P;
+        default: return P * P * P;
+    }
+}
+
+__device__ __forceinline__ double noninv_poly(const double* __restrict__ m, double x, double y,
+                                              double z) {
+    const double A = 1.0 - (x * x + y * y + z * z);
+    const double xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
+    const double sB = 2.0 * (x * m[1] + y * m[2] + z * m[3]) - m[4];
+    const double xM2x = xx * m[5] + yy * m[8] + zz * m[10] + 2.0 * (xy * m[6] + xz * m[7] + yz * m[9]);
+    const double sB2 = 4.0 * xM2x - 4.0 * (x * m[11] + y * m[12] + z * m[13]) + m[14];
+    const double T3 = x * (xx * m[15] + 3.0 * (xy * m[16] + xz * m[17] + yy * m[18]) +
+                           6.0 * yz * m[19] + 3.0 * zz * m[20]) +
+                      y * (yy * m[21] + 3.0 * (yz * m[22] + zz * m[23])) + z * zz * m[24];
+    const double xM2Px =
+        xx * m[25] + yy * m[28] + zz * m[30] + 2.0 * (xy * m[26] + xz * m[27] + yz * m[29]);
+    const double sB3 = 8.0 * T3 - 12.0 * xM2Px + 6.0 * (x * m[31] + y * m[32] + z * m[33]) - m[34];
+    return A * (A * (A * m[0] + 3.0 * sB) + 3.0 * sB2) + sB3;
+}
 
 __device__ __forceinline__ float wave_min(float v) {
 #pragma unroll
@@ -576,7 +715,10 @@ __device__ __forceinline__ float aniso_pairs(const float4* __restrict__ q0s,
     return sum + part;
 }
 
-__global__ __launch_bounds__(kAnisoThreads, 5) void k_field_aniso(AnisoFieldArgs a) {
+// kWaves: the resident waves per SIMD the register allocation is bounded for
+// (5: 96 VGPRs, a few spilled; 4: 128 VGPRs, none spilled).
+template <int kWaves>
+__global__ __launch_bounds__(kAnisoThreads, kWaves) void k_field_aniso(AnisoFieldArgs a) {
     __shared__ float4 queue[kAnisoThreads / 64][3][128];
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x >> 6;
@@ -586,6 +728,7 @@ __global__ __launch_bounds__(kAnisoThreads, 5) void k_field_aniso(AnisoFieldArgs
     float4* q1s = queue[wave][1];
     float4* q2s = queue[wave][2];
     uint32_t step = 0;
+    uint32_t n_staged = 0, n_kept = 0;
     for (;;) {
         const uint32_t ch = next_item_xcd(a.next, nch, step);
         if (ch >= nch) break;
@@ -596,8 +739,6 @@ __global__ __launch_bounds__(kAnisoThreads, 5) void k_field_aniso(AnisoFieldArgs
         const uint32_t k = ch - a.bc.chunk_off[v];
         const uint32_t first = a.bc.cell_off[v] + 64u * k;
         const uint32_t cnt = min(64u, a.bc.cell_cnt[v] - 64u * k);
-        BlockRows br = block_rows<2>(a.ranges, a.sp, cx, cy, cz);
-        const uint32_t total = br.total;
         const float ox = __builtin_fmaf((float)cx, a.sp.cell, a.sp.minx);
         const float oy = __builtin_fmaf((float)cy, a.sp.cell, a.sp.miny);
         const float oz = __builtin_fmaf((float)cz, a.sp.cell, a.sp.minz);
@@ -620,6 +761,21 @@ __global__ __launch_bounds__(kAnisoThreads, 5) void k_field_aniso(AnisoFieldArgs
         const float bx0 = wave_min(live ? vx : 3.0e38f), bx1 = wave_max(live ? vx : -3.0e38f);
         const float by0 = wave_min(live ? vy : 3.0e38f), by1 = wave_max(live ? vy : -3.0e38f);
         const float bz0 = wave_min(live ? vz : 3.0e38f), bz1 = wave_max(live ? vz : -3.0e38f);
+        // the block's candidate rows; with the non-involved moments only the
+        // cells whose involved particles can reach the chunk's vertices
+        BlockRows br;
+        if (a.block_mom) {
+            CellCull cc;
+            cc.lo = a.cell_lo;
+            cc.hi = a.cell_hi;
+            cc.b0[0] = bx0 + ox; cc.b0[1] = by0 + oy; cc.b0[2] = bz0 + oz;
+            cc.b1[0] = bx1 + ox; cc.b1[1] = by1 + oy; cc.b1[2] = bz1 + oz;
+            br = block_rows<2>(a.ranges, a.sp, cx, cy, cz, &cc);
+        } else {
+            br = block_rows<2>(a.ranges, a.sp, cx, cy, cz);
+        }
+        const uint32_t total = br.total;
+        n_staged += total;
         float sum = 0.f;
         uint32_t fill = 0;
         // Staging in two steps.  Each lane's candidate position (with its
@@ -692,6 +848,7 @@ __global__ __launch_bounds__(kAnisoThreads, 5) void k_field_aniso(AnisoFieldArgs
             px_p = px; py_p = py; pz_p = pz;
             r0_p = r0; r1_p = r1;
             fill += (uint32_t)__popcll(m);
+            n_kept += (uint32_t)__popcll(m);
             if (fill >= 64u) {
                 wave_lds_sync();
                 sum = aniso_pairs(q0s, q1s, q2s, 64u, vt, sum);
@@ -718,18 +875,117 @@ __global__ __launch_bounds__(kAnisoThreads, 5) void k_field_aniso(AnisoFieldArgs
             sum = aniso_pairs(q0s, q1s, q2s, n, vt, sum);
             wave_lds_sync();
         }
-        if (live) a.phi[vid] = sum - 0.5f;
+        if (live) {
+            if (a.block_mom) {
+                const double px = (double)(vx + ox) - a.ox, py = (double)(vy + oy) - a.oy,
+                             pz = (double)(vz + oz) - a.oz;
+                const double pn = noninv_poly(a.block_mom + (uint64_t)kNonInvMoments * v, px, py, pz);
+                a.phi[vid] = (float)((double)sum + pn - 0.5);
+            } else {
+                a.phi[vid] = sum - 0.5f;
+            }
+        }
     }
+    if (a.work) write_work(a.work, n_staged, n_kept);
+}
+
+// Per cell: the box of its particles' smoothed positions and their largest
+// squared support radius (the w of the smoothed positions, negative for
+// particles the field never stages), widened by 0.1% in radius and a few ulps
+// in position (the cell test runs in absolute coordinates, the particle test
+// relative to the chunk's cell); a cell whose particles all have a negative w
+// gets w = -1.
+__global__ __launch_bounds__(256) void k_cell_boxes(const fsr_index_range* __restrict__ ranges,
+                                                    const float4* __restrict__ smooth, uint32_t C,
+                                                    float4* __restrict__ lo,
+                                                    float4* __restrict__ hi) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    float4 l = make_float4(3.0e38f, 3.0e38f, 3.0e38f, -1.0f);
+    float4 u = make_float4(-3.0e38f, -3.0e38f, -3.0e38f, 0.0f);
+    const fsr_index_range r = ranges[c];
+    if (r.start != 0xFFFFFFFFu) {
+        for (uint32_t j = r.start; j < r.end; ++j) {
+            const float4 p = smooth[j];
+            if (!(p.w > 0.0f)) continue;
+            l.x = fminf(l.x, p.x); l.y = fminf(l.y, p.y); l.z = fminf(l.z, p.z);
+            u.x = fmaxf(u.x, p.x); u.y = fmaxf(u.y, p.y); u.z = fmaxf(u.z, p.z);
+            l.w = fmaxf(l.w, p.w);
+        }
+    }
+    if (l.w > 0.0f) {
+        l.w = l.w * 1.002f + 1.0e-12f;
+        // the chunk box and this one meet in absolute coordinates, rounded at
+        // their magnitude: grow the box by a few of its ulps
+        const float e = 1.0e-6f * fmaxf(fmaxf(fmaxf(fabsf(l.x), fabsf(u.x)),
+                                              fmaxf(fabsf(l.y), fabsf(u.y))),
+                                        fmaxf(fabsf(l.z), fabsf(u.z)));
+        l.x -= e; l.y -= e; l.z -= e;
+        u.x += e; u.y += e; u.z += e;
+    }
+    lo[c] = l;
+    hi[c] = u;
+}
+
+// Moment k of the non-involved particles of every cell (0 for involved and
+// empty cells): thread (cell c, moment blockIdx.y) over the cell's particles,
+// smoothed positions about the origin, weights sigma m / rho.
+__global__ __launch_bounds__(256) void k_noninv_cell_moments(
+    const fsr_index_range* __restrict__ ranges, const uint32_t* __restrict__ involve,
+    const float4* __restrict__ smooth, const float4* __restrict__ sorted, uint32_t C, float mass,
+    double ox, double oy, double oz, double* __restrict__ mom) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = (int)blockIdx.y;
+    if (c >= C) return;
+    double acc = 0.0;
+    const fsr_index_range r = ranges[c];
+    if (r.start != 0xFFFFFFFFu && involve[c] == 0) {
+        for (uint32_t j = r.start; j < r.end; ++j) {
+            const float4 p = smooth[j];
+            const float cj = kSigma * (mass / sorted[j].w);
+            acc += (double)cj * moment_term(k, (double)p.x - ox, (double)p.y - oy,
+                                            (double)p.z - oz);
+        }
+    }
+    mom[(uint64_t)c * kNonInvMoments + k] = acc;
+}
+
+// One pass of the 5^3 block sums: along one axis, out(e) = sum of in over the
+// clamped window [c - 2, c + 2] of the real cells, where the output index along
+// the axis is a virtual cell (c + 2, c in [-2, res + 1]).  Elements are
+// (a, i, b) with i along the axis; in has nin cells along it, out nout.
+__global__ __launch_bounds__(256) void k_window5(const double* __restrict__ in,
+                                                 double* __restrict__ out, uint32_t na,
+                                                 uint32_t nin, uint32_t nout, uint32_t nb) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t total = (uint64_t)na * nout * nb * kNonInvMoments;
+    if (t >= total) return;
+    const uint32_t k = (uint32_t)(t % kNonInvMoments);
+    const uint64_t e = t / kNonInvMoments;
+    const uint32_t b = (uint32_t)(e % nb);
+    const uint32_t i = (uint32_t)((e / nb) % nout);
+    const uint32_t a = (uint32_t)(e / ((uint64_t)nb * nout));
+    const int c = (int)i - 2;
+    const int lo = clampi(c - 2, 0, (int)nin - 1), hi = clampi(c + 2, 0, (int)nin - 1);
+    double acc = 0.0;
+    for (int j = lo; j <= hi; ++j)
+        acc += in[(((uint64_t)a * nin + (uint32_t)j) * nb + b) * kNonInvMoments + k];
+    out[t] = acc;
 }
 
 // The field kernel tests a candidate's box distance against the candidate's
 // squared culling radius (w of its second record word) before it reads the
 // records: a copy of that radius goes into the w of the smoothed position.
+// With the non-involved moments, those particles (never culled otherwise)
+// get a negative radius: the box test then drops them.
 __global__ __launch_bounds__(256) void k_cull_radius_to_w(const float4* __restrict__ rec,
-                                                          uint32_t n, float4* __restrict__ smooth) {
+                                                          uint32_t n, float4* __restrict__ smooth,
+                                                          int drop_noninv) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    reinterpret_cast<float*>(smooth)[4ull * i + 3] = rec[2ull * i + 1].w;
+    float w = rec[2ull * i + 1].w;
+    if (drop_noninv && w == kNoCull) w = -1.0f;
+    reinterpret_cast<float*>(smooth)[4ull * i + 3] = w;
 }
 
 __global__ __launch_bounds__(256) void k_cell_counts(const fsr_index_range* __restrict__ ranges,
@@ -783,6 +1039,16 @@ fsr_status cell_chunk_list(fsr_context* ctx, const fsr_index_range* ranges,
 
 // Scratch of the two particle passes: per-cell counts and chunk offsets, and
 // the chunk -> cell lists.
+// The work-record sections (see write_work), one arena slot for the three
+// kernels.
+static fsr_status work_section(fsr_context* ctx, WorkSection w, uint32_t** out) {
+    const uint64_t per = 1ull + 2ull * kWorkWavesPerCu * (uint64_t)ctx->num_cus;
+    uint32_t* base;
+    FSR_TRY(arena_typed(ctx, A_ANISO_WORK, WORK_SECTIONS * per, &base));
+    *out = base + (uint64_t)w * per;
+    return FSR_OK;
+}
+
 struct PassScratch {
     uint32_t *cnt, *chunk_off, *cnt2, *chunk_off2, *chunk_cell, *chunk_cell2;
 };
@@ -813,8 +1079,10 @@ fsr_status aniso_smooth(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
     FSR_LAUNCH_CHECK("k_chunks_of_cells");
     ChunkList cl{ps.chunk_cell, ps.chunk_off, &a.counters->aux[0], a.counters->xcd_next};
     FSR_TRY(reset_xcd(a.counters, s));
+    uint32_t* work;
+    FSR_TRY(work_section(ctx, WORK_SMOOTH, &work));
     hipLaunchKernelGGL(k_smooth, dim3(blocks), dim3(kAnisoThreads), 0, s, a.sorted, a.ranges,
-                       a.sp, a.aniso_radius, a.lambda, cl, a.mean, a.smooth);
+                       a.sp, a.aniso_radius, a.lambda, cl, a.mean, a.smooth, work);
     FSR_LAUNCH_CHECK("k_smooth");
     return FSR_OK;
 }
@@ -864,6 +1132,7 @@ fsr_status aniso_cov(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
     ca.gmat = a.gmat;
     ca.sigma = a.sigma;
     ca.rec = a.rec;
+    FSR_TRY(work_section(ctx, WORK_COV, &ca.work));
     FSR_TRY(reset_xcd(a.counters, s));
     hipLaunchKernelGGL(k_covariance, dim3(blocks), dim3(kAnisoThreads), 0, s, ca, cl2);
     FSR_LAUNCH_CHECK("k_covariance");
@@ -876,10 +1145,61 @@ fsr_status aniso_run(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
     return aniso_cov(ctx, a, s);
 }
 
+// Whether every particle of a clamped 5^3 block lies within distance 1 of
+// every vertex whose cell owns the block, smoothing displacement included: the
+// reference's non-involved kernel then never reaches its cutoff |r| >= 1 and
+// the block sum is the polynomial of noninv_poly.
+static bool noninv_polynomial_exact(const GridDev& sp, float smoothing_radius) {
+    const double reach = std::sqrt(3.0) * 6.0 * (double)sp.cell + 2.0 * (double)smoothing_radius;
+    return reach < 0.9;
+}
+
 fsr_status aniso_field(fsr_context* ctx, const AnisoArgs& a, const BandCompactOut& bc,
                        float* phi, hipStream_t s) {
-    const int blocks = persistent_blocks(ctx, kAnisoThreads, 5, TUNE_FIELD);
+    // a field limit of at most 4 workgroups per CU runs the build bounded for
+    // 4 waves per SIMD (no spills); the default the one for 5
+    const int tuned = ctx->tune[TUNE_FIELD];
+    const int field_waves = (tuned > 0 && tuned <= 4) ? 4 : 5;
+    const int blocks = persistent_blocks(ctx, kAnisoThreads, field_waves, TUNE_FIELD);
     AnisoFieldArgs fa;
+    fa.block_mom = nullptr;
+    fa.cell_lo = nullptr;
+    fa.cell_hi = nullptr;
+    FSR_TRY(work_section(ctx, WORK_FIELD, &fa.work));
+    fa.ox = a.sp.minx;
+    fa.oy = a.sp.miny;
+    fa.oz = a.sp.minz;
+    const bool moments = a.noninv_moments && noninv_polynomial_exact(a.sp, a.aniso_radius);
+    if (moments) {
+        const uint32_t C = a.sp.rx * a.sp.ry * a.sp.rz;
+        const uint32_t rx = a.sp.rx, ry = a.sp.ry, rz = a.sp.rz;
+        const uint32_t vx = bc.vcx, vy = bc.vcy, vz = bc.vcz;
+        double *cm, *t1, *t2, *bm;
+        FSR_TRY(arena_typed(ctx, A_MOM_CELL, (uint64_t)C * kNonInvMoments, &cm));
+        FSR_TRY(arena_typed(ctx, A_MOM_T1, (uint64_t)rz * ry * vx * kNonInvMoments, &t1));
+        FSR_TRY(arena_typed(ctx, A_MOM_T2, (uint64_t)rz * vy * vx * kNonInvMoments, &t2));
+        FSR_TRY(arena_typed(ctx, A_MOM_BLOCK, (uint64_t)vz * vy * vx * kNonInvMoments, &bm));
+        hipLaunchKernelGGL(k_noninv_cell_moments, dim3((C + 255) / 256, kNonInvMoments), dim3(256),
+                           0, s, a.ranges, a.involve, (const float4*)a.smooth, a.sorted, C,
+                           a.mass, fa.ox, fa.oy, fa.oz, cm);
+        FSR_LAUNCH_CHECK("k_noninv_cell_moments");
+        auto window = [&](const double* in, double* out, uint32_t na, uint32_t nin, uint32_t nout,
+                          uint32_t nb) -> fsr_status {
+            const uint64_t total = (uint64_t)na * nout * nb * kNonInvMoments;
+            hipLaunchKernelGGL(k_window5, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, s,
+                               in, out, na, nin, nout, nb);
+            FSR_LAUNCH_CHECK("k_window5");
+            return FSR_OK;
+        };
+        FSR_TRY(window(cm, t1, rz * ry, rx, vx, 1));   // x: (z, y) x vx
+        FSR_TRY(window(t1, t2, rz, ry, vy, vx));        // y: z x vy x vx
+        FSR_TRY(window(t2, bm, 1, rz, vz, vy * vx));    // z: vz x (vy, vx)
+        fa.block_mom = bm;
+        float4* boxes;
+        FSR_TRY(arena_typed(ctx, A_CELL_BOX, 2ull * C, &boxes));
+        fa.cell_lo = boxes;
+        fa.cell_hi = boxes + C;
+    }
     fa.smooth = a.smooth;
     fa.rec = a.rec;
     fa.ranges = a.ranges;
@@ -891,11 +1211,21 @@ fsr_status aniso_field(fsr_context* ctx, const AnisoArgs& a, const BandCompactOu
     fa.num_chunks = &a.counters->num_chunks;
     if (a.n) {
         hipLaunchKernelGGL(k_cull_radius_to_w, dim3((a.n + 255) / 256), dim3(256), 0, s, a.rec,
-                           a.n, a.smooth);
+                           a.n, a.smooth, moments ? 1 : 0);
         FSR_LAUNCH_CHECK("k_cull_radius_to_w");
     }
+    if (moments) {
+        const uint32_t C = a.sp.rx * a.sp.ry * a.sp.rz;
+        hipLaunchKernelGGL(k_cell_boxes, dim3((C + 255) / 256), dim3(256), 0, s, a.ranges,
+                           (const float4*)a.smooth, C, const_cast<float4*>(fa.cell_lo),
+                           const_cast<float4*>(fa.cell_hi));
+        FSR_LAUNCH_CHECK("k_cell_boxes");
+    }
     FSR_TRY(reset_xcd(a.counters, s));
-    hipLaunchKernelGGL(k_field_aniso, dim3(blocks), dim3(kAnisoThreads), 0, s, fa);
+    if (field_waves == 4)
+        hipLaunchKernelGGL(k_field_aniso<4>, dim3(blocks), dim3(kAnisoThreads), 0, s, fa);
+    else
+        hipLaunchKernelGGL(k_field_aniso<5>, dim3(blocks), dim3(kAnisoThreads), 0, s, fa);
     FSR_LAUNCH_CHECK("k_field_aniso");
     return FSR_OK;
 }

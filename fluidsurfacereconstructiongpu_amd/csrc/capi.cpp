@@ -348,6 +348,7 @@ fsr_status frame_impl(const char* who, fsr_context* ctx, fsr_method method,
         FSR_TRY(arena_typed(ctx, A_REC, 2 * nn, &aa.rec));
         aa.counters = counters;
         aa.st = stats;
+        aa.noninv_moments = 1;
         FSR_TRY(aniso_run(ctx, aa, s));
         prof_mark(ctx, FSR_STAGE_FIELD, s);
         FSR_TRY(aniso_field(ctx, aa, bc, phi, s));
@@ -606,6 +607,7 @@ fsr_status fsr_frame_buffer(fsr_context* ctx, fsr_buffer which, void** dev_ptr, 
         case FSR_BUF_INVOLVED_INDEX: slot = A_INVOLVED_IDX; break;
         case FSR_BUF_FIELD_WORK: slot = A_FIELD_WORK; break;
         case FSR_BUF_COLOR_WORK: slot = A_COLOR_WORK; break;
+        case FSR_BUF_ANISO_WORK: slot = A_ANISO_WORK; break;
         default:
             set_error("fsr_frame_buffer: unknown buffer");
             return FSR_ERR_INVALID_ARG;

@@ -13,12 +13,14 @@
 // Frames are pipelined (the reference runs them strictly one after another):
 // while frame k is reconstructed, a host thread parses frame k+1's .xyz into a
 // pinned buffer, and frame k+1 is uploaded on a copy stream while frame k's
-// mesh is written.  Per-frame times cover the reconstruction only, as the
-// reference's timer does; the end-to-end rate is printed at the end.
+// mesh is written; --serial runs them strictly one after another instead
+// (read, upload, reconstruct, write), as the reference does.  Per-frame times
+// cover the reconstruction only, as the reference's timer does; the end-to-end
+// rate, from the first read to the last write, is printed at the end.
 //
 //   fsr_reconstruct <directory/> <pattern> <from> <to>
 //                   [--method zb05|akinci12|yu13] [--no-obj] [--indexed]
-//                   [--vis] [--config] [--device N]
+//                   [--vis] [--config] [--serial] [--device N]
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -124,7 +126,7 @@ void usage() {
     std::fprintf(stderr,
                  "usage: fsr_reconstruct <directory/> <pattern> <from> <to> "
                  "[--method zb05|akinci12|yu13] [--no-obj] [--indexed] [--vis] [--config] "
-                 "[--device N]\n");
+                 "[--serial] [--device N]\n");
     std::exit(2);
 }
 
@@ -136,7 +138,7 @@ int main(int argc, char** argv) {
     const std::string pattern = argv[2];
     const long from = std::strtol(argv[3], nullptr, 10), to = std::strtol(argv[4], nullptr, 10);
     fsr_method method = FSR_METHOD_ZB05;
-    bool save_obj = true, indexed = false, save_cfg = false, save_vis = false;
+    bool save_obj = true, indexed = false, save_cfg = false, save_vis = false, serial = false;
     int device = 0;
     for (int i = 5; i < argc; ++i) {
         const std::string a = argv[i];
@@ -154,6 +156,8 @@ int main(int argc, char** argv) {
             save_cfg = true;
         } else if (a == "--vis") {
             save_vis = true;
+        } else if (a == "--serial") {
+            serial = true;
         } else if (a == "--device" && i + 1 < argc) {
             device = std::atoi(argv[++i]);
         } else {
@@ -195,16 +199,21 @@ int main(int argc, char** argv) {
             std::exit(1);
         }
     };
+    const auto wall0 = std::chrono::steady_clock::now();
     load_frame(host[0], dir, pattern, from);
     check_loaded(host[0]);
     upload(0);
-    const auto wall0 = std::chrono::steady_clock::now();
     for (long frame = from; frame < to; ++frame) {
-        const int slot = (int)((frame - from) & 1);
+        const int slot = serial ? 0 : (int)((frame - from) & 1);
         HostFrame& hf = host[slot];
-        // the next frame is parsed while this one runs
+        if (serial && frame != from) {
+            load_frame(hf, dir, pattern, frame);
+            check_loaded(hf);
+            upload(slot);
+        }
+        // pipelined: the next frame is parsed while this one runs
         std::future<void> next;
-        if (frame + 1 < to)
+        if (!serial && frame + 1 < to)
             next = std::async(std::launch::async, load_frame, std::ref(host[slot ^ 1]), dir,
                               pattern, frame + 1);
         const std::string& stem = hf.stem;

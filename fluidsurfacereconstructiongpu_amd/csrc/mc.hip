@@ -231,116 +231,6 @@ __global__ __launch_bounds__(256) void k_mc_emit(McArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// The frame's marching cubes in two launches.  Workgroup g of G owns the band
-// entries [g seg, (g + 1) seg) (seg a multiple of 256, from the device-side
-// band size).
-//   k_mc_classify_partials: classify every entry of the segment (nverts) and
-//     write the segment's vertex and valid-cube totals to partials[g],
-//     partials[G + g].
-//   k_mc_offsets_emit: the offsets of the segment are the totals of the
-//     segments before it; per 256-entry tile a block scan gives every entry's
-//     soup offset (vert_off) and the valid cubes' list (valid_idx), and the
-//     tile's valid cubes are emitted at once, 32 at a time (8 lanes each).  The
-//     last workgroup writes the totals and the overflow flag.
-// Outputs and arithmetic are those of k_mc_classify + scan_mc + k_mc_emit.
-// ---------------------------------------------------------------------------
-constexpr uint32_t kMcTile = 256;
-constexpr uint32_t kMcMaxGroups = 1024;
-
-__device__ __forceinline__ uint32_t mc_segment(uint32_t B, uint32_t G) {
-    const uint32_t seg = (B + G - 1) / G;
-    return (seg + kMcTile - 1) / kMcTile * kMcTile;
-}
-
-// Block-wide (256 threads) exclusive scan of one value per thread; returns the
-// block total.  lds: 4 words.
-__device__ __forceinline__ uint32_t block_scan256(uint32_t v, uint32_t* lds, uint32_t* excl) {
-    const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t inc = wave_inclusive_sum(v);
-    if (__lane_id() == 63) lds[wave] = inc;
-    __syncthreads();
-    uint32_t before = 0, total = 0;
-#pragma unroll
-    for (int w = 0; w < 4; ++w) {
-        const uint32_t t = lds[w];
-        if ((uint32_t)w < wave) before += t;
-        total += t;
-    }
-    __syncthreads();
-    *excl = before + inc - v;
-    return total;
-}
-
-__global__ __launch_bounds__(256) void k_mc_classify_partials(McArgs a, uint32_t* partials) {
-    __shared__ uint32_t lds[4];
-    const uint32_t B = a.st->num_band_vertices;
-    const uint32_t seg = mc_segment(B, gridDim.x);
-    const uint32_t lo = blockIdx.x * seg, hi = min(lo + seg, B);
-    uint32_t sum = 0, cnt = 0;
-    for (uint32_t b = lo + threadIdx.x; b < hi; b += kMcTile) {
-        const uint32_t nv = classify_cube(a, a.band_index[b]);
-        a.nverts[b] = nv;
-        sum += nv;
-        cnt += nv ? 1u : 0u;
-    }
-    uint32_t excl;
-    const uint32_t tsum = block_scan256(sum, lds, &excl);
-    const uint32_t tcnt = block_scan256(cnt, lds, &excl);
-    if (threadIdx.x == 0) {
-        partials[blockIdx.x] = tsum;
-        partials[gridDim.x + blockIdx.x] = tcnt;
-    }
-}
-
-template <bool kCpu>
-__global__ __launch_bounds__(256) void k_mc_offsets_emit(McArgs a, const uint32_t* partials) {
-    __shared__ float4 corner[256][2];
-    __shared__ uint32_t lds[4];
-    __shared__ uint32_t tile_vid[kMcTile], tile_off[kMcTile];
-    const uint32_t B = a.st->num_band_vertices;
-    const uint32_t G = gridDim.x;
-    const uint32_t seg = mc_segment(B, G);
-    const uint32_t lo = blockIdx.x * seg, hi = min(lo + seg, B);
-    uint32_t s0 = 0, s1 = 0;
-    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kMcTile) {
-        s0 += partials[i];
-        s1 += partials[G + i];
-    }
-    uint32_t excl;
-    uint32_t run_v = block_scan256(s0, lds, &excl);
-    uint32_t run_c = block_scan256(s1, lds, &excl);
-    for (uint32_t base = lo; base < hi; base += kMcTile) {
-        const uint32_t b = base + threadIdx.x;
-        const uint32_t nv = b < hi ? a.nverts[b] : 0u;
-        uint32_t ev, ec;
-        const uint32_t totv = block_scan256(nv, lds, &ev);
-        const uint32_t totc = block_scan256(nv ? 1u : 0u, lds, &ec);
-        if (b < hi) {
-            a.vert_off[b] = run_v + ev;
-            if (nv) {
-                a.valid_idx[run_c + ec] = b;
-                tile_vid[ec] = a.band_index[b];
-                tile_off[ec] = run_v + ev;
-            }
-        }
-        __syncthreads();
-        // the tile's valid cubes, 32 groups of 8 lanes at a time
-        for (uint32_t q0 = 0; q0 < totc; q0 += kMcTile / 8u) {
-            const uint32_t q = q0 + threadIdx.x / 8u;
-            if (q < totc) emit_cube<kCpu>(a, tile_vid[q], tile_off[q], corner);
-        }
-        __syncthreads();
-        run_v += totv;
-        run_c += totc;
-    }
-    if (blockIdx.x == G - 1 && threadIdx.x == 0) {
-        a.st->num_mesh_vertices = run_v;
-        a.st->num_valid_cubes = run_c;
-        if (run_v > a.capacity) a.st->overflow = 1u;
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Deduplicated mesh: one vertex per grid edge (v, v + e_axis) that is crossed by
 // the iso-surface and belongs to an emitting cube (the case tables place a
 // vertex on exactly the crossed edges of a cube, so these are the soup's
@@ -566,20 +456,10 @@ fsr_status mc_emit(fsr_context* ctx, const McArgs& a, hipStream_t s) {
 }
 
 fsr_status mc_run(fsr_context* ctx, const McArgs& a, hipStream_t s) {
-    const uint32_t G = (uint32_t)std::max<uint64_t>(
-        1, std::min<uint64_t>((a.max_band + kMcTile - 1) / kMcTile, kMcMaxGroups));
-    uint32_t* partials;
-    FSR_TRY(arena_typed(ctx, A_MC_PARTIALS, 2ull * kMcMaxGroups, &partials));
-    hipLaunchKernelGGL(k_mc_classify_partials, dim3(G), dim3(256), 0, s, a, partials);
-    FSR_LAUNCH_CHECK("k_mc_classify_partials");
-    if (a.cpu_flavor)
-        hipLaunchKernelGGL(k_mc_offsets_emit<true>, dim3(G), dim3(256), 0, s, a,
-                           (const uint32_t*)partials);
-    else
-        hipLaunchKernelGGL(k_mc_offsets_emit<false>, dim3(G), dim3(256), 0, s, a,
-                           (const uint32_t*)partials);
-    FSR_LAUNCH_CHECK("k_mc_offsets_emit");
-    return FSR_OK;
+    FSR_TRY(mc_classify(ctx, a, s));
+    FSR_TRY(scan_mc(ctx, a.nverts, a.max_band, &a.st->num_band_vertices, a.vert_off, a.valid_idx,
+                    &a.st->num_mesh_vertices, &a.st->num_valid_cubes, s));
+    return mc_emit(ctx, a, s);
 }
 
 }  // namespace fsr

@@ -222,9 +222,14 @@ typedef enum {
                                  * 64-vertex chunk: candidates staged, candidates kept by the
                                  * chunk-box test, 16 x 16 vertex/candidate tiles evaluated;
                                  * after the last chunk, one word 0xFFFFFFFF */
-    FSR_BUF_COLOR_WORK = 16     /* u32[3 * waves] work of the last Akinci12 colour-field launch,
+    FSR_BUF_COLOR_WORK = 16,    /* u32[3 * waves] work of the last Akinci12 colour-field launch,
                                  * per wave: candidates staged, candidates kept by the
                                  * chunk-box test, pairs with 0 < |r| < h */
+    FSR_BUF_ANISO_WORK = 17     /* u32[3 * (1 + 2 W)] work of the last Yu13 smoothing,
+                                 * covariance and field launches (in that order), W = 32 x
+                                 * the CU count: per launch its wave count w, then per wave
+                                 * (w of them) candidates staged, candidates kept by the
+                                 * chunk-box test */
 } fsr_buffer;
 
 fsr_status fsr_frame_buffer(fsr_context* ctx, fsr_buffer which,

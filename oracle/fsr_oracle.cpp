@@ -11,6 +11,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -291,6 +292,28 @@ int orc_field_zb05(const uint32_t* band_index, uint32_t B, const float* pos,
     return 0;
 }
 
+// Corner normal of the reference's marching cubes: central differences
+// normalised as helper_math's normalize, v * rsqrtf(dot(v, v))
+// (getVertexNorm, ReconstructionCUDA.cuh:419-430; NaN for a zero vector), or,
+// for the host flavour, as fVector3::normalize (GPU/MathVector.cpp:5-13: only
+// above length 1e-7).  Both square in FP32: a squared length above FLT_MAX is
+// +inf there (a central difference that reads the non-band fill 0x64646464,
+// about 1.7e22, makes one), and v * rsqrt(inf) = v / inf is the zero vector.
+static void normalise_corner(double n[3], int cpu) {
+    const double d = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+    if (d > (double)FLT_MAX) {
+        n[0] = n[1] = n[2] = 0.0;
+        return;
+    }
+    const double len = std::sqrt(d);
+    if (cpu) {
+        if (len > 1e-7) for (int a = 0; a < 3; ++a) n[a] /= len;
+    } else {
+        const double r = 1.0 / len;
+        for (int a = 0; a < 3; ++a) n[a] *= r;
+    }
+}
+
 // detectValidSurfaceCubes + compactValidSurfaceCubes + generateTriangles
 // (ReconstructionCUDA.cu:328-463, .cuh:366-430, 534-609); cpu = 1 follows the
 // host marching cubes of GPU/MarchingCubesCPU.cpp:3-89 instead (no normal
@@ -352,18 +375,10 @@ uint64_t orc_mc(const uint32_t* band_index, uint32_t B, const uint8_t* band, con
                 cp[c][1] = vpos(Y, sc.miny, sc.cell);
                 cp[c][2] = vpos(Z, sc.minz, sc.cell);
             }
-            double n0 = sample(X - 1, Y, Z) - sample(X + 1, Y, Z);
-            double n1 = sample(X, Y - 1, Z) - sample(X, Y + 1, Z);
-            double n2 = sample(X, Y, Z - 1) - sample(X, Y, Z + 1);
-            const double len = std::sqrt(n0 * n0 + n1 * n1 + n2 * n2);
-            if (cpu) {
-                if (len > 1e-7) { n0 /= len; n1 /= len; n2 /= len; }
-            } else {
-                // v * rsqrt(dot(v, v)): NaN for a zero vector, as on the GPU
-                const double r = 1.0 / len;
-                n0 *= r; n1 *= r; n2 *= r;
-            }
-            cn[c][0] = n0; cn[c][1] = n1; cn[c][2] = n2;
+            cn[c][0] = sample(X - 1, Y, Z) - sample(X + 1, Y, Z);
+            cn[c][1] = sample(X, Y - 1, Z) - sample(X, Y + 1, Z);
+            cn[c][2] = sample(X, Y, Z - 1) - sample(X, Y, Z + 1);
+            normalise_corner(cn[c], cpu);
         }
         const uint32_t nv = nverts[b];
         for (uint32_t k = 0; k < nv; ++k) {
@@ -1181,12 +1196,7 @@ uint64_t orc_mc_indexed(const uint32_t* band_index, uint32_t B, const uint8_t* b
         n[0] = sample(X - 1, Y, Z) - sample(X + 1, Y, Z);
         n[1] = sample(X, Y - 1, Z) - sample(X, Y + 1, Z);
         n[2] = sample(X, Y, Z - 1) - sample(X, Y, Z + 1);
-        const double len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-        if (cpu) {
-            if (len > 1e-7) for (int a = 0; a < 3; ++a) n[a] /= len;
-        } else {
-            for (int a = 0; a < 3; ++a) n[a] *= 1.0 / len;
-        }
+        normalise_corner(n, cpu);
     };
     const float mins[3] = {sc.minx, sc.miny, sc.minz};
     const double sign = (!cpu && iso < 0.0f) ? -1.0 : 1.0;

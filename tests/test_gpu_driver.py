@@ -97,3 +97,28 @@ def test_driver_indexed(torch_cuda, tmp_path):
     _, cnt = np.unique(e, axis=0, return_counts=True)
     assert f.min() == 1 and f.max() == v.shape[0]
     assert np.mean(cnt == 2) > 0.99
+
+
+def test_driver_serial_equals_pipelined(torch_cuda, tmp_path):
+    """--serial (read, upload, reconstruct, write, one frame after another, as
+    the reference runs) writes the same .obj files as the pipelined order."""
+    print("\n[gpu case] driver serial vs pipelined", flush=True)
+    for k in range(3):
+        base = ds.jittered_lattice(n_side=12 + 2 * k, seed=ds.BASE_SEED + 50 + k)
+        lo, hi = base.pos.min(axis=0), base.pos.max(axis=0)
+        frame_io.write_xyz(str(tmp_path / ("frame_%06d.xyz" % k)), lo, hi, base.h,
+                           base.particle_radius, base.mass, base.pos, base.dens)
+    out = {}
+    for mode in ("pipelined", "serial"):
+        d = tmp_path / mode
+        d.mkdir()
+        for k in range(3):
+            os.link(str(tmp_path / ("frame_%06d.xyz" % k)), str(d / ("frame_%06d.xyz" % k)))
+        args = [frame_io.DRIVER, str(d) + "/", "frame_%06d", "0", "3"]
+        r = subprocess.run(args + (["--serial"] if mode == "serial" else []), capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        assert "3 frame(s) in" in r.stdout
+        out[mode] = [open(str(d / ("frame_%06d.obj" % k)), "rb").read() for k in range(3)]
+    assert all(len(o) > 0 for o in out["serial"])
+    assert out["serial"] == out["pipelined"]

@@ -332,6 +332,39 @@ def test_mc_cpu_flavour_normalises_only_above_1e7(slope):
         assert np.allclose(ng, [-1.0, 0.0, 0.0], atol=1e-9)
 
 
+@pytest.mark.parametrize("cpu", [False, True])
+def test_mc_corner_normal_overflows_to_zero(cpu):
+    """A corner whose central difference reads the non-band fill 0x64646464
+    (about 1.7e22) has a squared length beyond FLT_MAX, +inf in the reference's
+    FP32 normalize, so its normal is v * rsqrt(inf) = 0 (GPU flavour) or
+    v / inf = 0 (fVector3::normalize).  Grid 4 x 3 x 3, phi = x - 1.5 on the
+    band (x >= 1), fill on the x = 0 layer (out of the band): the surface
+    crosses every x-edge from 1 to 2 at t = 0.5, the x = 1 corners' normals are
+    zero, so each vertex's normal is its x = 2 corner's normal, normalised."""
+    fill = float(np.array([0x64646464], np.uint32).view(np.float32)[0])
+    sc = grid((0, 0, 0), 1.0, (4, 3, 3))
+    zz, yy, xx = np.meshgrid(np.arange(3), np.arange(3), np.arange(4), indexing="ij")
+    phi = np.where(xx == 0, fill, xx - 1.5).astype(np.float64)
+    band = (xx > 0).astype(np.uint8).ravel()
+    idx = np.flatnonzero(band).astype(np.uint32)
+    nv, pos, nrm = oracle.mc(idx, band, phi.ravel(), sc, 0.0, cpu=cpu)
+    assert pos.shape[0] == 4 * 6     # the 4 cubes spanning x 1..2, two triangles each
+    assert np.allclose(pos[:, 0], 1.5, atol=1e-12)
+
+    def s(x, y, z):
+        return 0.0 if min(x, y, z) < 0 or x > 3 or y > 2 or z > 2 else phi[z, y, x]
+
+    for k in range(pos.shape[0]):
+        y, z = pos[k, 1], pos[k, 2]
+        assert y == int(y) and z == int(z)
+        y, z = int(y), int(z)
+        n = np.array([s(1, y, z) - s(3, y, z), s(2, y - 1, z) - s(2, y + 1, z),
+                      s(2, y, z - 1) - s(2, y, z + 1)])
+        n = n / np.linalg.norm(n)
+        # the lerp halves it and the final normalisation restores it
+        assert np.allclose(nrm[k], n, atol=1e-12), (k, nrm[k], n)
+
+
 def test_mc_sphere_vertices_on_sphere():
     sc = grid((-0.6, -0.6, -0.6), 0.05, (25, 25, 25))
     g = np.arange(25, dtype=np.float32) * np.float32(0.05) + np.float32(-0.6)
