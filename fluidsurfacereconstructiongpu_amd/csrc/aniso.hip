@@ -1,13 +1,13 @@
 // Anisotropic kernel of Yu & Turk 2013 (the reference's "Yu13" method, TC01).
 //
-//   k_smooth        calculationOfMeanAndSmoothParticles (ReconstructionCUDA.cu:598-658):
+//   k_smooth_mc     calculationOfMeanAndSmoothParticles (ReconstructionCUDA.cu:598-658):
 //                   over the clamped 5^3 cell block, w = 1 - (d/r)^3 for d < r
 //                   (wij, ReconstructionCUDA.cuh:282-292), r = anisotropic radius;
 //                   mean = sum w p / sum w; smoothed = (1 - lambda) x + lambda mean.
 //   k_involved      compactationOfSurfaceVerticesAndParticles, particle half
 //                   (ReconstructionCUDA.cu:661-686): particles of cells with a
 //                   nonzero involve count, in cell order.
-//   k_covariance    calculationOfTransformMatricesForParticles (ReconstructionCUDA.cu:689-827):
+//   k_covariance_mc calculationOfTransformMatricesForParticles (ReconstructionCUDA.cu:689-827):
 //                   C = (h^2 I + sum w v v^T) / sum w with v = p_j - mean_i over the
 //                   clamped 5^3 block; #{|v| <= r} < Ne gives G = I / (kn h); otherwise
 //                   the singular values of C (|sigma|) are clamped to sigma_max / kr and
@@ -26,13 +26,15 @@
 //
 // MI355X mapping: as for the ZB05 field, a wave owns 64 particles (or band
 // vertices) of one hash cell, all lanes share the candidate block, and the 25
-// x-runs of the 5^3 block are staged through LDS 64 candidates at a time and
-// read as broadcasts.  The field kernel evaluates |G r|^2 as the quadratic form
+// x-runs of the 5^3 block are staged through LDS 64 candidates at a time.
+// Smoothing and covariance evaluate their pairs in 16 x 16 tiles on the matrix
+// cores (see k_smooth_mc).  The field kernel evaluates |G r|^2 as the quadratic form
 // r^T M r with M = G^T G stored per particle (6 values): with v the vertex and
 // p the particle relative to the cell corner, |G(p - v)|^2 =
 // v^T M v - 2 v.(M p) + p^T M p, where M p and p^T M p are formed once per staged
 // candidate; the per-pair cost is 9 fused multiply-adds.  Particles of
-// non-involved cells carry M = I and the unscaled kernel, as the reference has it.
+// non-involved cells carry M = I and the unscaled kernel, as the reference has
+// it; the field sums them through per-block moments (noninv_poly).
 #include <cmath>
 #include <limits>
 
@@ -42,11 +44,11 @@ namespace fsr {
 
 namespace {
 
-// Workgroups of 4 waves.  The neighbour kernels ask for 6 (smoothing,
-// covariance) and 5 (field) resident workgroups per CU, i.e. waves per SIMD:
-// the register limit that implies (80 and 96 VGPRs) leaves the first two
-// without spills and the field with 4 spilled registers, and each ran faster
-// than at the 4 waves per SIMD its unbounded register use allowed.
+// Workgroups of 4 waves.  Resident workgroups per CU, i.e. waves per SIMD:
+// smoothing 5 (96 VGPRs, no spills), covariance 4 (128 VGPRs and 34 KB of LDS
+// per workgroup), field 5 (96 VGPRs, a few spilled; the 4-wave build of the
+// field, chosen by a field launch limit of at most 4, has none and measured
+// the same).
 constexpr int kAnisoThreads = 256;
 // rows of a 5^3 cell block, one per lane
 constexpr int kMaxRows = 25;
@@ -206,124 +208,10 @@ __device__ __forceinline__ void wave_box(bool live, float x, float y, float z, f
 }
 
 // Candidates farther than r * (1 + 5e-4) from the bounding box of a chunk's
-// points have weight exactly 0 (and lie outside the neighbour radius) at every
-// lane: they are dropped at staging, the rest compacted in order into a
-// 128-entry queue (as in the field kernels), so the sums are bit-identical.
+// points have weight 0 (and lie outside the neighbour radius) at every lane:
+// they are dropped at staging, the rest compacted in order into the LDS queue.
 constexpr float kRadiusCull = 1.001f;
 
-__global__ __launch_bounds__(kAnisoThreads, 6) void k_smooth(
-    const float4* __restrict__ sorted, const fsr_index_range* __restrict__ ranges, GridDev sp,
-    float radius, float lambda, ChunkList cl, float4* __restrict__ mean_out,
-    float4* __restrict__ smooth_out, uint32_t* __restrict__ work) {
-    __shared__ float4 queue[kAnisoThreads / 64][128];
-    const uint32_t lane = __lane_id();
-    const uint32_t wave = threadIdx.x >> 6;
-    float4* q = queue[wave];
-    const uint32_t nch = *cl.num_chunks;
-    const float inv_r2 = 1.0f / (radius * radius);
-    const float cull2 = radius * radius * kRadiusCull;
-    uint32_t step = 0;
-    uint32_t n_staged = 0, n_kept = 0;
-    for (;;) {
-        const uint32_t ch = next_item_xcd(cl.next, nch, step);
-        if (ch >= nch) break;
-        const uint32_t c = cl.chunk_cell[ch];
-        const uint32_t k = ch - cl.chunk_off[c];
-        const fsr_index_range own = ranges[c];
-        const uint32_t i = own.start + 64u * k + lane;
-        const bool live = i < own.end;
-        const int cx = (int)(c % sp.rx), cy = (int)((c / sp.rx) % sp.ry), cz = (int)(c / (sp.rx * sp.ry));
-        BlockRows br = block_rows<2>(ranges, sp, cx, cy, cz);
-        const uint32_t total = br.total;
-        n_staged += total;
-        const float ox = __builtin_fmaf((float)cx, sp.cell, sp.minx);
-        const float oy = __builtin_fmaf((float)cy, sp.cell, sp.miny);
-        const float oz = __builtin_fmaf((float)cz, sp.cell, sp.minz);
-        float4 self = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (live) self = sorted[i];
-        const float px = self.x - ox, py = self.y - oy, pz = self.z - oz;
-        const float ax = 2.0f * px * inv_r2, ay = 2.0f * py * inv_r2, az = 2.0f * pz * inv_r2;
-        const float c0 = __builtin_fmaf(pz, pz, __builtin_fmaf(py, py, px * px)) * inv_r2;
-        float b0[3], b1[3];
-        wave_box(live, px, py, pz, b0, b1);
-        float sw = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
-        auto run = [&](uint32_t n) {
-            for (uint32_t t0 = 0; t0 < n; t0 += 8u) {
-#pragma unroll
-                for (int t = 0; t < 8; ++t) {
-                    const float4 qq = q[t0 + t];
-                    // s2 = |x - q|^2 / r^2
-                    float s2 = __builtin_fmaf(-az, qq.z, c0 + qq.w);
-                    s2 = __builtin_fmaf(-ay, qq.y, s2);
-                    s2 = __builtin_fmaf(-ax, qq.x, s2);
-                    s2 = fmaxf(s2, 0.0f);
-                    // hardware square root (within an ulp): the weight is a
-                    // value, not a decision
-                    // max(x, 0) with x <= 1 (s2 >= 0): a clamp to [0, 1],
-                    // folded into the subtraction
-                    const float w =
-                        __builtin_amdgcn_fmed3f(1.0f - s2 * __builtin_amdgcn_sqrtf(s2), 0.0f, 1.0f);
-                    sw += w;
-                    sx = __builtin_fmaf(w, qq.x, sx);
-                    sy = __builtin_fmaf(w, qq.y, sy);
-                    sz = __builtin_fmaf(w, qq.z, sz);
-                }
-            }
-        };
-        uint32_t fill = 0;
-        // each lane's candidate is loaded one staging round ahead
-        float4 next = make_float4(0.f, 0.f, 0.f, 0.f);
-        {
-            const uint32_t off = br.offset(0u, lane);
-            if (lane < total) next = sorted[off + lane];
-        }
-        for (uint32_t base = 0; base < total; base += 64u) {
-            const uint32_t g = base + lane;
-            const float4 cand = next;
-            if (base + 64u < total) {
-                const uint32_t off = br.offset(base + 64u, g + 64u);
-                if (g + 64u < total) next = sorted[off + g + 64u];
-            }
-            bool keep = false;
-            float4 st;
-            if (g < total) {
-                const float qx = cand.x - ox, qy = cand.y - oy, qz = cand.z - oz;
-                keep = box_dist2(qx, qy, qz, b0, b1) < cull2;
-                st = make_float4(qx, qy, qz,
-                                 __builtin_fmaf(qz, qz, __builtin_fmaf(qy, qy, qx * qx)) * inv_r2);
-            }
-            const uint64_t m = __ballot(keep);
-            if (keep) q[fill + mask_rank(m)] = st;
-            fill += (uint32_t)__popcll(m);
-            n_kept += (uint32_t)__popcll(m);
-            if (fill >= 64u) {
-                wave_lds_sync();
-                run(64u);
-                wave_lds_sync();
-                const uint32_t rem = fill - 64u;
-                if (lane < rem) q[lane] = q[64u + lane];
-                wave_lds_sync();
-                fill = rem;
-            }
-        }
-        if (fill) {
-            const uint32_t n = (fill + 7u) & ~7u;
-            if (lane >= fill && lane < n) q[lane] = make_float4(0.f, 0.f, 0.f, 1.0e30f);
-            wave_lds_sync();
-            run(n);
-            wave_lds_sync();
-        }
-        if (live && sw > 0.0f) {
-            const float inv = 1.0f / sw;
-            const float mx = ox + sx * inv, my = oy + sy * inv, mz = oz + sz * inv;
-            mean_out[i] = make_float4(mx, my, mz, 0.0f);
-            const float a = 1.0f - lambda;
-            smooth_out[i] = make_float4(a * self.x + lambda * mx, a * self.y + lambda * my,
-                                        a * self.z + lambda * mz, 0.0f);
-        }
-    }
-    if (work) write_work(work, n_staged, n_kept);
-}
 
 // ---------------------------------------------------------------- K8 --------
 // Involved list and the per-particle field records of non-involved particles.
@@ -461,15 +349,349 @@ __device__ __forceinline__ void cov_finish(const CovArgs& a, uint32_t gi, uint32
     a.rec[2ull * i + 1] = make_float4(M[0][2], M[1][2], coef, kCullMargin / (gmin * gmin));
 }
 
-__global__ __launch_bounds__(kAnisoThreads, 6) void k_covariance(CovArgs a, ChunkList cl) {
-    __shared__ float4 queue[kAnisoThreads / 64][128];
+
+// ------------------------------------------- K10 / K11 on the matrix cores ---
+// Smoothing and covariance as pair tiles on the matrix cores, as the ZB05
+// field does: a wave owns a chunk of up to 64 particles of one hash cell (the
+// "lanes"), candidates of the clamped 5^3 block that pass the chunk-box test
+// are staged in LDS, and per 16 candidates and 16 lanes
+//   1. one v_mfma_f32_16x16x4_f32 gives s2 = |q - x|^2 / r^2 = q''.(-2 x''/r^2)
+//      + |x''|^2/r^2 + |q''|^2/r^2 for the 16 x 16 pairs, with q'' and x''
+//      relative to the chunk's origin (the centre of its box): A = (x'', y'',
+//      z'', 1) of the candidates, B = (-2x''/r^2, -2y''/r^2, -2z''/r^2,
+//      |x''|^2/r^2) of the lanes, C = |q''|^2/r^2;
+//   2. w = max(1 - s2^(3/2), 0) (the reference's 1 - (d/r)^3 for d < r, 0
+//      beyond), and for the covariance the neighbour count s2 <= le;
+//   3. rank-1 updates of 4 x 4 blocks (v_mfma_f32_4x4x1_16b_f32) accumulate
+//      sum w (q'', 1) and, for the covariance, sum w (q''q''^T) in two more
+//      groups (xx, xy, xz, yy) and (yz, zz, 0, 0).
+// x is the particle (smoothing) or its weighted mean (covariance).  The
+// covariance is then sum w (q - m)(q - m)^T = S2 - S1 m''^T - m'' S1^T +
+// S0 m'' m''^T with m'' the mean relative to the chunk's origin (small: the
+// box centre lies among the chunk's means).
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int kNbQueue = 128;
+constexpr int kNbStride = 144;
+// queue rows: x'', y'', z'', 1, then (covariance) xx, xy, xz, yy, yz, zz, 0, 0,
+// then |q''|^2 / r^2
+constexpr int kNbOne = 3, kNbXX = 4, kNbYZ = 8, kNbZero = 10;
+template <bool kCov>
+struct NbRows {
+    static constexpr int kQ = kCov ? 12 : 4;
+    static constexpr int kRows = kQ + 1;
+};
+
+template <int M>
+__device__ __forceinline__ float nb_swz_xor(float v) {
+    return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x1F | (M << 10)));
+}
+
+template <int M>
+__device__ __forceinline__ uint32_t nb_swz_xor_u(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (M << 10));
+}
+
+// Sum over the four partial sums of a lane-particle (lanes l%16 + 16m).
+__device__ __forceinline__ float nb_reduce4(float x) {
+    x += nb_swz_xor<16>(x);
+    return x + __shfl_xor(x, 32, 64);
+}
+
+__device__ __forceinline__ uint32_t nb_reduce4_u(uint32_t x) {
+    x += nb_swz_xor_u<16>(x);
+    return x + (uint32_t)__shfl_xor((int)x, 32, 64);
+}
+
+// The tiles of n staged candidates (a multiple of 16) against the NT lane
+// tiles.  rows: the queue at the first staged entry.
+template <bool kCov, int NT>
+__device__ __forceinline__ void nb_tiles(const float* __restrict__ rows, uint32_t n,
+                                         const float (&b1)[NT], v4f (&acc)[kCov ? 3 : 1][NT],
+                                         uint32_t (&cnt)[NT], float le_s2) {
+    const uint32_t lane = __lane_id();
+    const float* a1p = rows + (lane >> 4) * kNbStride + (lane & 15u);
+    const float* a2p = rows + (lane & 3u) * kNbStride + 4u * (lane >> 4);
+    const float* cqp = rows + NbRows<kCov>::kQ * kNbStride + 4u * (lane >> 4);
+    for (uint32_t tb = 0; tb < n; tb += 16u) {
+        const float a1 = a1p[tb];
+        const float4 cq = *reinterpret_cast<const float4*>(cqp + tb);
+        const v4f c1 = v4f{cq.x, cq.y, cq.z, cq.w};
+        v4f w[NT];
+#pragma unroll
+        for (int T = 0; T < NT; ++T) w[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1[T], c1, 0, 0, 0);
+#pragma unroll
+        for (int T = 0; T < NT; ++T) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float s2 = fmaxf(w[T][r], 0.0f);
+                if (kCov) cnt[T] += s2 <= le_s2 ? 1u : 0u;
+                // 1 - s2 sqrt(s2) <= 1: the clamp to [0, 1] is max(., 0)
+                w[T][r] = __builtin_amdgcn_fmed3f(
+                    __builtin_fmaf(-s2, __builtin_amdgcn_sqrtf(s2), 1.0f), 0.0f, 1.0f);
+            }
+        }
+        const float4 a2 = *reinterpret_cast<const float4*>(a2p + tb);
+#pragma unroll
+        for (int T = 0; T < NT; ++T) acc[0][T] = __builtin_amdgcn_mfma_f32_4x4x1f32(a2.x, w[T][0], acc[0][T], 0, 0, 0);
+#pragma unroll
+        for (int T = 0; T < NT; ++T) acc[0][T] = __builtin_amdgcn_mfma_f32_4x4x1f32(a2.y, w[T][1], acc[0][T], 0, 0, 0);
+#pragma unroll
+        for (int T = 0; T < NT; ++T) acc[0][T] = __builtin_amdgcn_mfma_f32_4x4x1f32(a2.z, w[T][2], acc[0][T], 0, 0, 0);
+#pragma unroll
+        for (int T = 0; T < NT; ++T) acc[0][T] = __builtin_amdgcn_mfma_f32_4x4x1f32(a2.w, w[T][3], acc[0][T], 0, 0, 0);
+        if (kCov) {
+#pragma unroll
+            for (int g = 1; g < 3; ++g) {
+                const float4 ag = *reinterpret_cast<const float4*>(a2p + (4 * g) * kNbStride + tb);
+#pragma unroll
+                for (int T = 0; T < NT; ++T) acc[g][T] = __builtin_amdgcn_mfma_f32_4x4x1f32(ag.x, w[T][0], acc[g][T], 0, 0, 0);
+#pragma unroll
+                for (int T = 0; T < NT; ++T) acc[g][T] = __builtin_amdgcn_mfma_f32_4x4x1f32(ag.y, w[T][1], acc[g][T], 0, 0, 0);
+#pragma unroll
+                for (int T = 0; T < NT; ++T) acc[g][T] = __builtin_amdgcn_mfma_f32_4x4x1f32(ag.z, w[T][2], acc[g][T], 0, 0, 0);
+#pragma unroll
+                for (int T = 0; T < NT; ++T) acc[g][T] = __builtin_amdgcn_mfma_f32_4x4x1f32(ag.w, w[T][3], acc[g][T], 0, 0, 0);
+            }
+        }
+    }
+}
+
+// Sums of one chunk over the candidates of its 5^3 block: staging (box test,
+// ordered compaction into the LDS ring), the tiles, the reduction.  vt: the
+// lanes' B operands (4 rows of 64).  On return lane l holds, for its own
+// particle, s1 = (sum w q''x, y, z, sum w), s2a = (xx, xy, xz, yy), s2b = (yz,
+// zz, -, -) and the count (covariance).
+template <bool kCov, int NT>
+__device__ __forceinline__ void nb_chunk(float* __restrict__ q, const float* __restrict__ vt,
+                                         const float4* __restrict__ sorted, BlockRows& br,
+                                         uint32_t total, float ox, float oy, float oz,
+                                         const float* bb0, const float* bb1, float inv_r2,
+                                         float cull2, float le_s2, uint32_t& n_kept, v4f& s1,
+                                         v4f& s2a, v4f& s2b, uint32_t& count) {
+    const uint32_t lane = __lane_id();
+    constexpr int G = kCov ? 3 : 1;
+    float b1[NT];
+    v4f acc[G][NT];
+    uint32_t cnt[NT];
+    const float* bsrc = vt + 64 * (lane >> 4) + (lane & 15u);
+#pragma unroll
+    for (int T = 0; T < NT; ++T) {
+        b1[T] = bsrc[16 * T];
+        cnt[T] = 0;
+#pragma unroll
+        for (int g = 0; g < G; ++g) acc[g][T] = v4f{0.f, 0.f, 0.f, 0.f};
+    }
+    uint32_t head = 0, fill = 0;
+    float4 next = make_float4(0.f, 0.f, 0.f, 0.f);
+    {
+        const uint32_t off = br.offset(0u, lane);
+        if (lane < total) next = sorted[off + lane];
+    }
+    for (uint32_t base = 0; base < total; base += 64u) {
+        const uint32_t g = base + lane;
+        const float4 cand = next;
+        if (base + 64u < total) {
+            const uint32_t off = br.offset(base + 64u, g + 64u);
+            if (g + 64u < total) next = sorted[off + g + 64u];
+        }
+        bool keep = false;
+        float px = 0.f, py = 0.f, pz = 0.f;
+        if (g < total) {
+            px = cand.x - ox;
+            py = cand.y - oy;
+            pz = cand.z - oz;
+            keep = box_dist2(px, py, pz, bb0, bb1) < cull2;
+        }
+        const uint64_t m = __ballot(keep);
+        if (keep) {
+            const uint32_t slot = (head + fill + mask_rank(m)) & (kNbQueue - 1);
+            q[slot] = px;
+            q[kNbStride + slot] = py;
+            q[2 * kNbStride + slot] = pz;
+            q[NbRows<kCov>::kQ * kNbStride + slot] =
+                __builtin_fmaf(pz, pz, __builtin_fmaf(py, py, px * px)) * inv_r2;
+            if (kCov) {
+                q[(kNbXX + 0) * kNbStride + slot] = px * px;
+                q[(kNbXX + 1) * kNbStride + slot] = px * py;
+                q[(kNbXX + 2) * kNbStride + slot] = px * pz;
+                q[(kNbXX + 3) * kNbStride + slot] = py * py;
+                q[(kNbYZ + 0) * kNbStride + slot] = py * pz;
+                q[(kNbYZ + 1) * kNbStride + slot] = pz * pz;
+            }
+        }
+        fill += (uint32_t)__popcll(m);
+        n_kept += (uint32_t)__popcll(m);
+        if (fill >= 64u) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            nb_tiles<kCov, NT>(q + head, 64u, b1, acc, cnt, le_s2);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            head ^= 64u;
+            fill -= 64u;
+        }
+    }
+    if (fill) {
+        // pad to a multiple of 16 with entries of weight 0 and no count
+        // (s2 = 1e30); head + 64 <= kNbQueue, so the pad does not wrap
+        const uint32_t n = (fill + 15u) & ~15u;
+        if (lane >= fill && lane < n) {
+            q[head + lane] = 0.f;
+            q[kNbStride + head + lane] = 0.f;
+            q[2 * kNbStride + head + lane] = 0.f;
+            q[NbRows<kCov>::kQ * kNbStride + head + lane] = 1.0e30f;
+            if (kCov) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) q[(kNbXX + k) * kNbStride + head + lane] = 0.f;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        nb_tiles<kCov, NT>(q + head, n, b1, acc, cnt, le_s2);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    // the four partial sums of each lane-particle; lane l keeps those of
+    // particle 16 T + l % 16 with T = l / 16, i.e. particle l
+    const uint32_t my = lane >> 4;
+    s1 = v4f{0.f, 0.f, 0.f, 0.f};
+    s2a = s1;
+    s2b = s1;
+    count = 0;
+#pragma unroll
+    for (int T = 0; T < NT; ++T) {
+        v4f r0 = acc[0][T];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) r0[c] = nb_reduce4(r0[c]);
+        if (my == (uint32_t)T) s1 = r0;
+        if (kCov) {
+            v4f r1 = acc[1][T], r2 = acc[2][T];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) r1[c] = nb_reduce4(r1[c]);
+            r2[0] = nb_reduce4(r2[0]);
+            r2[1] = nb_reduce4(r2[1]);
+            const uint32_t ct = nb_reduce4_u(cnt[T]);
+            if (my == (uint32_t)T) {
+                s2a = r1;
+                s2b = r2;
+                count = ct;
+            }
+        }
+    }
+}
+
+// The chunk's origin (centre of the box of the lanes' points, absolute) and
+// the box relative to it; the lanes' B operands into vt.
+__device__ __forceinline__ void nb_setup(bool live, float x, float y, float z, float inv_r2,
+                                         float* vt, float* o, float* bb0, float* bb1,
+                                         float* rel) {
+    float b0[3], b1[3];
+    wave_box(live, x, y, z, b0, b1);
+    // a chunk without live lanes does not occur (chunks list live particles)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) o[d] = 0.5f * (b0[d] + b1[d]);
+    const float px = x - o[0], py = y - o[1], pz = z - o[2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        bb0[d] = b0[d] - o[d];
+        bb1[d] = b1[d] - o[d];
+    }
+    const uint32_t lane = __lane_id();
+    vt[lane] = -2.0f * px * inv_r2;
+    vt[64 + lane] = -2.0f * py * inv_r2;
+    vt[128 + lane] = -2.0f * pz * inv_r2;
+    vt[192 + lane] = __builtin_fmaf(pz, pz, __builtin_fmaf(py, py, px * px)) * inv_r2;
+    rel[0] = px;
+    rel[1] = py;
+    rel[2] = pz;
+}
+
+template <bool kCov>
+struct NbLds {
+    float queue[NbRows<kCov>::kRows * kNbStride];
+    float vtx[4][64];
+};
+
+__global__ __launch_bounds__(kAnisoThreads, 5) void k_smooth_mc(
+    const float4* __restrict__ sorted, const fsr_index_range* __restrict__ ranges, GridDev sp,
+    float radius, float lambda, ChunkList cl, float4* __restrict__ mean_out,
+    float4* __restrict__ smooth_out, uint32_t* __restrict__ work) {
+    __shared__ __align__(16) NbLds<false> lds[kAnisoThreads / 64];
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x >> 6;
-    float4* q = queue[wave];
+    float* q = lds[wave].queue;
+    float* vt = &lds[wave].vtx[0][0];
+    q[kNbOne * kNbStride + lane] = 1.0f;
+    q[kNbOne * kNbStride + 64 + lane] = 1.0f;
+    const uint32_t nch = *cl.num_chunks;
+    const float inv_r2 = 1.0f / (radius * radius);
+    uint32_t step = 0;
+    uint32_t n_staged = 0, n_kept = 0;
+    for (;;) {
+        const uint32_t ch = next_item_xcd(cl.next, nch, step);
+        if (ch >= nch) break;
+        const uint32_t c = cl.chunk_cell[ch];
+        const uint32_t k = ch - cl.chunk_off[c];
+        const fsr_index_range own = ranges[c];
+        const uint32_t i = own.start + 64u * k + lane;
+        const bool live = i < own.end;
+        const int cx = (int)(c % sp.rx), cy = (int)((c / sp.rx) % sp.ry), cz = (int)(c / (sp.rx * sp.ry));
+        BlockRows br = block_rows<2>(ranges, sp, cx, cy, cz);
+        const uint32_t total = br.total;
+        n_staged += total;
+        float4 self = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (live) self = sorted[i];
+        float o[3], bb0[3], bb1[3], rel[3];
+        nb_setup(live, self.x, self.y, self.z, inv_r2, vt, o, bb0, bb1, rel);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        v4f s1, s2a, s2b;
+        uint32_t count;
+        const uint32_t cnt = min(64u, own.end - (own.start + 64u * k));
+        const float c2 = radius * radius * kRadiusCull;
+        switch ((cnt + 15u) >> 4) {
+            case 1: nb_chunk<false, 1>(q, vt, sorted, br, total, o[0], o[1], o[2], bb0, bb1, inv_r2, c2, 0.f, n_kept, s1, s2a, s2b, count); break;
+            case 2: nb_chunk<false, 2>(q, vt, sorted, br, total, o[0], o[1], o[2], bb0, bb1, inv_r2, c2, 0.f, n_kept, s1, s2a, s2b, count); break;
+            case 3: nb_chunk<false, 3>(q, vt, sorted, br, total, o[0], o[1], o[2], bb0, bb1, inv_r2, c2, 0.f, n_kept, s1, s2a, s2b, count); break;
+            default: nb_chunk<false, 4>(q, vt, sorted, br, total, o[0], o[1], o[2], bb0, bb1, inv_r2, c2, 0.f, n_kept, s1, s2a, s2b, count); break;
+        }
+        if (live && s1[3] > 0.0f) {
+            const float inv = 1.0f / s1[3];
+            const float mx = o[0] + s1[0] * inv, my = o[1] + s1[1] * inv, mz = o[2] + s1[2] * inv;
+            mean_out[i] = make_float4(mx, my, mz, 0.0f);
+            const float a = 1.0f - lambda;
+            smooth_out[i] = make_float4(a * self.x + lambda * mx, a * self.y + lambda * my,
+                                        a * self.z + lambda * mz, 0.0f);
+        }
+        (void)rel;
+    }
+    if (work) write_work(work, n_staged, n_kept);
+}
+
+__global__ __launch_bounds__(kAnisoThreads, 4) void k_covariance_mc(CovArgs a, ChunkList cl) {
+    __shared__ __align__(16) NbLds<true> lds[kAnisoThreads / 64];
+    const uint32_t lane = __lane_id();
+    const uint32_t wave = threadIdx.x >> 6;
+    float* q = lds[wave].queue;
+    float* vt = &lds[wave].vtx[0][0];
+    q[kNbOne * kNbStride + lane] = 1.0f;
+    q[kNbOne * kNbStride + 64 + lane] = 1.0f;
+    q[kNbZero * kNbStride + lane] = 0.0f;
+    q[kNbZero * kNbStride + 64 + lane] = 0.0f;
+    q[(kNbZero + 1) * kNbStride + lane] = 0.0f;
+    q[(kNbZero + 1) * kNbStride + 64 + lane] = 0.0f;
     const uint32_t nch = *cl.num_chunks;
     const float r = a.radius;
-    const float inv_r = 1.0f / r;
-    const float cull2 = r * r * kRadiusCull;
+    const float inv_r2 = 1.0f / (r * r);
+    const float c2 = r * r * kRadiusCull;
+    // the neighbour test |q - m| <= r on s2 = |q - m|^2 / r^2
+    const float le_s2 = a.d2_le_r * inv_r2;
     uint32_t step = 0;
     uint32_t n_staged = 0, n_kept = 0;
     for (;;) {
@@ -485,83 +707,36 @@ __global__ __launch_bounds__(kAnisoThreads, 6) void k_covariance(CovArgs a, Chun
         BlockRows br = block_rows<2>(a.ranges, a.sp, cx, cy, cz);
         const uint32_t total = br.total;
         n_staged += total;
-        const float ox = __builtin_fmaf((float)cx, a.sp.cell, a.sp.minx);
-        const float oy = __builtin_fmaf((float)cy, a.sp.cell, a.sp.miny);
-        const float oz = __builtin_fmaf((float)cz, a.sp.cell, a.sp.minz);
         float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
         if (live) m = a.mean[i];
-        const float mx = m.x - ox, my = m.y - oy, mz = m.z - oz;
-        float b0[3], b1[3];
-        wave_box(live, mx, my, mz, b0, b1);
-        float sw = 0.f, c00 = 0.f, c01 = 0.f, c02 = 0.f, c11 = 0.f, c12 = 0.f, c22 = 0.f;
-        uint32_t cnt = 0;
-        auto run = [&](uint32_t n) {
-            for (uint32_t t0 = 0; t0 < n; t0 += 4u) {
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const float4 qq = q[t0 + t];
-                    const float vx = qq.x - mx, vy = qq.y - my, vz = qq.z - mz;
-                    const float d2 = __builtin_fmaf(vz, vz, __builtin_fmaf(vy, vy, vx * vx));
-                    // the count and the support test decide exactly (on d2);
-                    // the weight uses the hardware square root
-                    const float dist = __builtin_amdgcn_sqrtf(d2);
-                    cnt += d2 <= a.d2_le_r ? 1u : 0u;
-                    const float s = dist * inv_r;
-                    const float w = d2 <= a.d2_lt_r ? 1.0f - s * s * s : 0.0f;
-                    sw += w;
-                    const float wx = w * vx, wy = w * vy;
-                    c00 = __builtin_fmaf(wx, vx, c00);
-                    c01 = __builtin_fmaf(wx, vy, c01);
-                    c02 = __builtin_fmaf(wx, vz, c02);
-                    c11 = __builtin_fmaf(wy, vy, c11);
-                    c12 = __builtin_fmaf(wy, vz, c12);
-                    c22 = __builtin_fmaf(w * vz, vz, c22);
-                }
-            }
-        };
-        uint32_t fill = 0;
-        // each lane's candidate is loaded one staging round ahead
-        float4 next = make_float4(0.f, 0.f, 0.f, 0.f);
-        {
-            const uint32_t off = br.offset(0u, lane);
-            if (lane < total) next = a.sorted[off + lane];
+        float o[3], bb0[3], bb1[3], rel[3];
+        nb_setup(live, m.x, m.y, m.z, inv_r2, vt, o, bb0, bb1, rel);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        v4f s1, s2a, s2b;
+        uint32_t count;
+        const uint32_t cnt = min(64u, own.end - (own.start + 64u * k));
+        switch ((cnt + 15u) >> 4) {
+            case 1: nb_chunk<true, 1>(q, vt, a.sorted, br, total, o[0], o[1], o[2], bb0, bb1, inv_r2, c2, le_s2, n_kept, s1, s2a, s2b, count); break;
+            case 2: nb_chunk<true, 2>(q, vt, a.sorted, br, total, o[0], o[1], o[2], bb0, bb1, inv_r2, c2, le_s2, n_kept, s1, s2a, s2b, count); break;
+            case 3: nb_chunk<true, 3>(q, vt, a.sorted, br, total, o[0], o[1], o[2], bb0, bb1, inv_r2, c2, le_s2, n_kept, s1, s2a, s2b, count); break;
+            default: nb_chunk<true, 4>(q, vt, a.sorted, br, total, o[0], o[1], o[2], bb0, bb1, inv_r2, c2, le_s2, n_kept, s1, s2a, s2b, count); break;
         }
-        for (uint32_t base = 0; base < total; base += 64u) {
-            const uint32_t g = base + lane;
-            const float4 cand = next;
-            if (base + 64u < total) {
-                const uint32_t off = br.offset(base + 64u, g + 64u);
-                if (g + 64u < total) next = a.sorted[off + g + 64u];
-            }
-            bool keep = false;
-            float4 st;
-            if (g < total) {
-                st = make_float4(cand.x - ox, cand.y - oy, cand.z - oz, 0.f);
-                keep = box_dist2(st.x, st.y, st.z, b0, b1) < cull2;
-            }
-            const uint64_t bm = __ballot(keep);
-            if (keep) q[fill + mask_rank(bm)] = st;
-            fill += (uint32_t)__popcll(bm);
-            n_kept += (uint32_t)__popcll(bm);
-            if (fill >= 64u) {
-                wave_lds_sync();
-                run(64u);
-                wave_lds_sync();
-                const uint32_t rem = fill - 64u;
-                if (lane < rem) q[lane] = q[64u + lane];
-                wave_lds_sync();
-                fill = rem;
-            }
+        if (live) {
+            // sum w v v^T with v = q'' - m'' from the moments about the origin
+            const float mx = rel[0], my = rel[1], mz = rel[2];
+            const float sw = s1[3];
+            const float sx = s1[0], sy = s1[1], sz = s1[2];
+            const float c00 = s2a[0] - 2.0f * sx * mx + sw * mx * mx;
+            const float c01 = s2a[1] - sx * my - sy * mx + sw * mx * my;
+            const float c02 = s2a[2] - sx * mz - sz * mx + sw * mx * mz;
+            const float c11 = s2a[3] - 2.0f * sy * my + sw * my * my;
+            const float c12 = s2b[0] - sy * mz - sz * my + sw * my * mz;
+            const float c22 = s2b[1] - 2.0f * sz * mz + sw * mz * mz;
+            cov_finish(a, a.involve_off[c] + (i - own.start), i, count, sw, c00, c01, c02, c11,
+                       c12, c22);
         }
-        if (fill) {
-            const uint32_t n = (fill + 3u) & ~3u;
-            if (lane >= fill && lane < n) q[lane] = make_float4(1.0e18f, 1.0e18f, 1.0e18f, 0.f);
-            wave_lds_sync();
-            run(n);
-            wave_lds_sync();
-        }
-        if (live) cov_finish(a, a.involve_off[c] + (i - own.start), i, cnt, sw, c00, c01, c02,
-                             c11, c12, c22);
     }
     if (a.work) write_work(a.work, n_staged, n_kept);
 }
@@ -1066,7 +1241,7 @@ static fsr_status pass_scratch(fsr_context* ctx, uint32_t C, uint32_t n, PassScr
 
 fsr_status aniso_smooth(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
     const uint32_t C = a.sp.rx * a.sp.ry * a.sp.rz;
-    const int blocks = persistent_blocks(ctx, kAnisoThreads, 6, TUNE_ANISO);
+    const int blocks = persistent_blocks(ctx, kAnisoThreads, 5, TUNE_ANISO);
     PassScratch ps;
     FSR_TRY(pass_scratch(ctx, C, a.n, &ps));
     hipLaunchKernelGGL(k_cell_counts, dim3((C + 255) / 256), dim3(256), 0, s, a.ranges,
@@ -1081,9 +1256,9 @@ fsr_status aniso_smooth(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
     FSR_TRY(reset_xcd(a.counters, s));
     uint32_t* work;
     FSR_TRY(work_section(ctx, WORK_SMOOTH, &work));
-    hipLaunchKernelGGL(k_smooth, dim3(blocks), dim3(kAnisoThreads), 0, s, a.sorted, a.ranges,
-                       a.sp, a.aniso_radius, a.lambda, cl, a.mean, a.smooth, work);
-    FSR_LAUNCH_CHECK("k_smooth");
+    hipLaunchKernelGGL(k_smooth_mc, dim3(blocks), dim3(kAnisoThreads), 0, s, a.sorted,
+                       a.ranges, a.sp, a.aniso_radius, a.lambda, cl, a.mean, a.smooth, work);
+    FSR_LAUNCH_CHECK("k_smooth_mc");
     return FSR_OK;
 }
 
@@ -1102,7 +1277,8 @@ fsr_status aniso_involved(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
 
 fsr_status aniso_cov(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
     const uint32_t C = a.sp.rx * a.sp.ry * a.sp.rz;
-    const int blocks = persistent_blocks(ctx, kAnisoThreads, 6, TUNE_ANISO);
+    // 4 resident workgroups per CU (LDS: 34 KB each)
+    const int blocks_cov = persistent_blocks(ctx, kAnisoThreads, 4, TUNE_ANISO);
     PassScratch ps;
     FSR_TRY(pass_scratch(ctx, C, a.n, &ps));
     hipLaunchKernelGGL(k_cell_counts, dim3((C + 255) / 256), dim3(256), 0, s, a.ranges,
@@ -1134,8 +1310,8 @@ fsr_status aniso_cov(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
     ca.rec = a.rec;
     FSR_TRY(work_section(ctx, WORK_COV, &ca.work));
     FSR_TRY(reset_xcd(a.counters, s));
-    hipLaunchKernelGGL(k_covariance, dim3(blocks), dim3(kAnisoThreads), 0, s, ca, cl2);
-    FSR_LAUNCH_CHECK("k_covariance");
+    hipLaunchKernelGGL(k_covariance_mc, dim3(blocks_cov), dim3(kAnisoThreads), 0, s, ca, cl2);
+    FSR_LAUNCH_CHECK("k_covariance_mc");
     return FSR_OK;
 }
 
