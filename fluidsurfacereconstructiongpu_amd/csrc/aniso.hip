@@ -780,45 +780,22 @@ struct AnisoFieldArgs {
 // A block's moments are the sums of its cells' moments.
 constexpr int kNonInvMoments = 35;
 
-__device__ __forceinline__ double moment_term(int k, double x, double y, double z) {
+__device__ __forceinline__ void moment_terms(double x, double y, double z,
+                                             double (&t)[kNonInvMoments]) {
     const double P = x * x + y * y + z * z;
-    switch (k) {
-        case 0: return 1.0;
-        case 1: return x;
-        case 2: return y;
-        case 3: return z;
-        case 4: return P;
-        case 5: return x * x;
-        case 6: return x * y;
-        case 7: return x * z;
-        case 8: return y * y;
-        case 9: return y * z;
-        case 10: return z * z;
-        case 11: return x * P;
-        case 12: return y * P;
-        case 13: return z * P;
-        case 14: return P * P;
-        case 15: return x * x * x;
-        case 16: return x * x * y;
-        case 17: return x * x * z;
-        case 18: return x * y * y;
-        case 19: return x * y * z;
-        case 20: return x * z * z;
-        case 21: return y * y * y;
-        case 22: return y * y * z;
-        case 23: return y * z * z;
-        case 24: return z * z * z;
-        case 25: return x * x * P;
-        case 26: return x * y * P;
-        case 27: return x * z * P;
-        case 28: return y * y * P;
-        case 29: return y * z * P;
-        case 30: return z * z * P;
-        case 31: return x * P * P;
-        case 32: return y * P * P;
-        case 33: return z * P * P;
-        default: return P * P * P;
-    }
+    const double xx = x * x, xy = x * y, xz = x * z, yy = y * y, yz = y * z, zz = z * z;
+    t[0] = 1.0;
+    t[1] = x; t[2] = y; t[3] = z;
+    t[4] = P;
+    t[5] = xx; t[6] = xy; t[7] = xz; t[8] = yy; t[9] = yz; t[10] = zz;
+    t[11] = x * P; t[12] = y * P; t[13] = z * P;
+    t[14] = P * P;
+    t[15] = xx * x; t[16] = xx * y; t[17] = xx * z; t[18] = x * yy; t[19] = xy * z;
+    t[20] = x * zz; t[21] = yy * y; t[22] = yy * z; t[23] = y * zz; t[24] = zz * z;
+    t[25] = xx * P; t[26] = xy * P; t[27] = xz * P; t[28] = yy * P; t[29] = yz * P;
+    t[30] = zz * P;
+    t[31] = x * P * P; t[32] = y * P * P; t[33] = z * P * P;
+    t[34] = P * P * P;
 }
 
 __device__ __forceinline__ double noninv_poly(const double* __restrict__ m, double x, double y,
@@ -1064,65 +1041,77 @@ __global__ __launch_bounds__(kAnisoThreads, kWaves) void k_field_aniso(AnisoFiel
     if (a.work) write_work(a.work, n_staged, n_kept);
 }
 
-// Per cell: the box of its particles' smoothed positions and their largest
-// squared support radius (the w of the smoothed positions, negative for
-// particles the field never stages), widened by 0.1% in radius and a few ulps
-// in position (the cell test runs in absolute coordinates, the particle test
-// relative to the chunk's cell); a cell whose particles all have a negative w
-// gets w = -1.
-__global__ __launch_bounds__(256) void k_cell_boxes(const fsr_index_range* __restrict__ ranges,
-                                                    const float4* __restrict__ smooth, uint32_t C,
-                                                    float4* __restrict__ lo,
-                                                    float4* __restrict__ hi) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= C) return;
+
+// One wave per hash cell.
+//  * Moments: for a non-involved cell the kNonInvMoments moments of its
+//    particles (smoothed positions about the origin (ox, oy, oz), weights
+//    sigma m / rho), zeros otherwise.  The terms of 64 particles at a time go
+//    through LDS and lane k sums term k over them, so the order is fixed.
+//  * Box: for the cell test of the field kernel, the box of the smoothed
+//    positions of the particles the field stages (w > 0: their squared culling
+//    radius) with the largest w, widened by 0.1% in radius and a few ulps in
+//    position (the cell test runs in absolute coordinates, the particle test
+//    relative to the chunk's cell); w = -1 when the cell stages none.
+__global__ __launch_bounds__(64) void k_cell_moments_boxes(
+    const fsr_index_range* __restrict__ ranges, const uint32_t* __restrict__ involve,
+    const float4* __restrict__ smooth, const float4* __restrict__ sorted, float mass, double ox,
+    double oy, double oz, double* __restrict__ mom, float4* __restrict__ lo,
+    float4* __restrict__ hi) {
+    // stride kNonInvMoments + 1 doubles between particles
+    __shared__ double terms[64 * (kNonInvMoments + 1)];
+    const uint32_t c = blockIdx.x;
+    const uint32_t lane = threadIdx.x;
+    const fsr_index_range r = ranges[c];
+    const bool occupied = r.start != 0xFFFFFFFFu;
+    const bool inv = occupied && involve[c] > 0;
+    double acc = 0.0;
     float4 l = make_float4(3.0e38f, 3.0e38f, 3.0e38f, -1.0f);
     float4 u = make_float4(-3.0e38f, -3.0e38f, -3.0e38f, 0.0f);
-    const fsr_index_range r = ranges[c];
-    if (r.start != 0xFFFFFFFFu) {
-        for (uint32_t j = r.start; j < r.end; ++j) {
-            const float4 p = smooth[j];
-            if (!(p.w > 0.0f)) continue;
-            l.x = fminf(l.x, p.x); l.y = fminf(l.y, p.y); l.z = fminf(l.z, p.z);
-            u.x = fmaxf(u.x, p.x); u.y = fmaxf(u.y, p.y); u.z = fmaxf(u.z, p.z);
-            l.w = fmaxf(l.w, p.w);
+    if (occupied) {
+        for (uint32_t base = r.start; base < r.end; base += 64u) {
+            const uint32_t j = base + lane;
+            const bool live = j < r.end;
+            float4 p = make_float4(0.f, 0.f, 0.f, -1.0f);
+            if (live) p = smooth[j];
+            if (inv) {
+                if (p.w > 0.0f) {
+                    l.x = fminf(l.x, p.x); l.y = fminf(l.y, p.y); l.z = fminf(l.z, p.z);
+                    u.x = fmaxf(u.x, p.x); u.y = fmaxf(u.y, p.y); u.z = fmaxf(u.z, p.z);
+                    l.w = fmaxf(l.w, p.w);
+                }
+            } else {
+                const float cj = live ? kSigma * (mass / sorted[j].w) : 0.0f;
+                double t[kNonInvMoments];
+                moment_terms((double)p.x - ox, (double)p.y - oy, (double)p.z - oz, t);
+#pragma unroll
+                for (int k = 0; k < kNonInvMoments; ++k)
+                    terms[lane * (kNonInvMoments + 1) + k] = (double)cj * t[k];
+                wave_lds_sync();
+                if (lane < (uint32_t)kNonInvMoments) {
+                    for (uint32_t q = 0; q < 64u; ++q) acc += terms[q * (kNonInvMoments + 1) + lane];
+                }
+                wave_lds_sync();
+            }
         }
     }
-    if (l.w > 0.0f) {
-        l.w = l.w * 1.002f + 1.0e-12f;
-        // the chunk box and this one meet in absolute coordinates, rounded at
-        // their magnitude: grow the box by a few of its ulps
-        const float e = 1.0e-6f * fmaxf(fmaxf(fmaxf(fabsf(l.x), fabsf(u.x)),
-                                              fmaxf(fabsf(l.y), fabsf(u.y))),
-                                        fmaxf(fabsf(l.z), fabsf(u.z)));
-        l.x -= e; l.y -= e; l.z -= e;
-        u.x += e; u.y += e; u.z += e;
-    }
-    lo[c] = l;
-    hi[c] = u;
-}
-
-// Moment k of the non-involved particles of every cell (0 for involved and
-// empty cells): thread (cell c, moment blockIdx.y) over the cell's particles,
-// smoothed positions about the origin, weights sigma m / rho.
-__global__ __launch_bounds__(256) void k_noninv_cell_moments(
-    const fsr_index_range* __restrict__ ranges, const uint32_t* __restrict__ involve,
-    const float4* __restrict__ smooth, const float4* __restrict__ sorted, uint32_t C, float mass,
-    double ox, double oy, double oz, double* __restrict__ mom) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = (int)blockIdx.y;
-    if (c >= C) return;
-    double acc = 0.0;
-    const fsr_index_range r = ranges[c];
-    if (r.start != 0xFFFFFFFFu && involve[c] == 0) {
-        for (uint32_t j = r.start; j < r.end; ++j) {
-            const float4 p = smooth[j];
-            const float cj = kSigma * (mass / sorted[j].w);
-            acc += (double)cj * moment_term(k, (double)p.x - ox, (double)p.y - oy,
-                                            (double)p.z - oz);
+    if (lane < (uint32_t)kNonInvMoments) mom[(uint64_t)c * kNonInvMoments + lane] = acc;
+    l.x = wave_minf(l.x); l.y = wave_minf(l.y); l.z = wave_minf(l.z);
+    u.x = wave_maxf(u.x); u.y = wave_maxf(u.y); u.z = wave_maxf(u.z);
+    l.w = wave_maxf(l.w);
+    if (lane == 0) {
+        if (l.w > 0.0f) {
+            l.w = l.w * 1.002f + 1.0e-12f;
+            const float e = 1.0e-6f * fmaxf(fmaxf(fmaxf(fabsf(l.x), fabsf(u.x)),
+                                                  fmaxf(fabsf(l.y), fabsf(u.y))),
+                                            fmaxf(fabsf(l.z), fabsf(u.z)));
+            l.x -= e; l.y -= e; l.z -= e;
+            u.x += e; u.y += e; u.z += e;
+        } else {
+            l.w = -1.0f;
         }
+        lo[c] = l;
+        hi[c] = u;
     }
-    mom[(uint64_t)c * kNonInvMoments + k] = acc;
 }
 
 // One pass of the 5^3 block sums: along one axis, out(e) = sum of in over the
@@ -1355,10 +1344,20 @@ fsr_status aniso_field(fsr_context* ctx, const AnisoArgs& a, const BandCompactOu
         FSR_TRY(arena_typed(ctx, A_MOM_T1, (uint64_t)rz * ry * vx * kNonInvMoments, &t1));
         FSR_TRY(arena_typed(ctx, A_MOM_T2, (uint64_t)rz * vy * vx * kNonInvMoments, &t2));
         FSR_TRY(arena_typed(ctx, A_MOM_BLOCK, (uint64_t)vz * vy * vx * kNonInvMoments, &bm));
-        hipLaunchKernelGGL(k_noninv_cell_moments, dim3((C + 255) / 256, kNonInvMoments), dim3(256),
-                           0, s, a.ranges, a.involve, (const float4*)a.smooth, a.sorted, C,
-                           a.mass, fa.ox, fa.oy, fa.oz, cm);
-        FSR_LAUNCH_CHECK("k_noninv_cell_moments");
+        float4* boxes;
+        FSR_TRY(arena_typed(ctx, A_CELL_BOX, 2ull * C, &boxes));
+        fa.cell_lo = boxes;
+        fa.cell_hi = boxes + C;
+        // the boxes need the culling radii in the w of the smoothed positions
+        if (a.n) {
+            hipLaunchKernelGGL(k_cull_radius_to_w, dim3((a.n + 255) / 256), dim3(256), 0, s,
+                               a.rec, a.n, a.smooth, 1);
+            FSR_LAUNCH_CHECK("k_cull_radius_to_w");
+        }
+        hipLaunchKernelGGL(k_cell_moments_boxes, dim3(C), dim3(64), 0, s, a.ranges, a.involve,
+                           (const float4*)a.smooth, a.sorted, a.mass, fa.ox, fa.oy, fa.oz, cm,
+                           boxes, boxes + C);
+        FSR_LAUNCH_CHECK("k_cell_moments_boxes");
         auto window = [&](const double* in, double* out, uint32_t na, uint32_t nin, uint32_t nout,
                           uint32_t nb) -> fsr_status {
             const uint64_t total = (uint64_t)na * nout * nb * kNonInvMoments;
@@ -1371,10 +1370,6 @@ fsr_status aniso_field(fsr_context* ctx, const AnisoArgs& a, const BandCompactOu
         FSR_TRY(window(t1, t2, rz, ry, vy, vx));        // y: z x vy x vx
         FSR_TRY(window(t2, bm, 1, rz, vz, vy * vx));    // z: vz x (vy, vx)
         fa.block_mom = bm;
-        float4* boxes;
-        FSR_TRY(arena_typed(ctx, A_CELL_BOX, 2ull * C, &boxes));
-        fa.cell_lo = boxes;
-        fa.cell_hi = boxes + C;
     }
     fa.smooth = a.smooth;
     fa.rec = a.rec;
@@ -1385,17 +1380,10 @@ fsr_status aniso_field(fsr_context* ctx, const AnisoArgs& a, const BandCompactOu
     fa.phi = phi;
     fa.next = a.counters->xcd_next;
     fa.num_chunks = &a.counters->num_chunks;
-    if (a.n) {
+    if (a.n && !moments) {
         hipLaunchKernelGGL(k_cull_radius_to_w, dim3((a.n + 255) / 256), dim3(256), 0, s, a.rec,
-                           a.n, a.smooth, moments ? 1 : 0);
+                           a.n, a.smooth, 0);
         FSR_LAUNCH_CHECK("k_cull_radius_to_w");
-    }
-    if (moments) {
-        const uint32_t C = a.sp.rx * a.sp.ry * a.sp.rz;
-        hipLaunchKernelGGL(k_cell_boxes, dim3((C + 255) / 256), dim3(256), 0, s, a.ranges,
-                           (const float4*)a.smooth, C, const_cast<float4*>(fa.cell_lo),
-                           const_cast<float4*>(fa.cell_hi));
-        FSR_LAUNCH_CHECK("k_cell_boxes");
     }
     FSR_TRY(reset_xcd(a.counters, s));
     if (field_waves == 4)
