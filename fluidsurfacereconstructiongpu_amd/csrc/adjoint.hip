@@ -27,12 +27,12 @@
 // virtual cell v reads the hash cells clamp(v - 1 .. v + 1), so particle cell q
 // is read exactly by the vertices of virtual cells q - 1 .. q + 1.  Neighbour
 // lists are not stored: the gather walks the same cell ranges the forward used.
-// Per pair: t by the cell-local expansion (4 ops), t^2, t^3, the dot product
-// (3), and three accumulations (7): 18 VALU ops.  Vertices that contribute
-// exactly zero to every particle of the chunk are culled at staging.  Results
-// are written in the caller's particle order through the sort permutation;
-// particles outside the hash grid get 0.
+// The pairs are evaluated on the matrix cores (k_field_zb05_backward_mc).
+// Results are written in the caller's particle order through the sort
+// permutation; particles outside the hash grid get 0.
 #include "common.hpp"
+#include <type_traits>
+
 #include "mc_tables.h"
 
 namespace fsr {
@@ -240,54 +240,97 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
-struct GradSums {
-    float g1x, g1y, g1z, g2x, g2y, g2z, sk;
-};
+// The gather evaluates its pairs in 16 x 16 particle/vertex tiles on the
+// matrix cores (as the forward field): per 16 staged vertices and 16
+// particles of the chunk
+//   1. v_mfma_f32_16x16x4_f32: t = c0_p + (2 p'/R^2).x' - |x'|^2/R^2 with A =
+//      (x', y', z', 1) of the vertices, B = (2p'/R^2, c0_p) of the particles,
+//      C = -|x'|^2/R^2; a second one gives kd = u.p' + (c - u.x') with A =
+//      (u, c - u.x') of the vertices and B = (p', 1);
+//   2. t clamped to [0, 1], w = t^3 and kk = kd t^2 (4 VALU ops per pair);
+//   3. rank-1 v_mfma_f32_4x4x1_16b_f32 updates: sum w (u, c - u.x') (the
+//      last component unused) and sum kk (x', 1).
+// Vertices whose coefficient is zero, or that lie at least 1.00005 R from the
+// bounding box of the chunk's particles, add exactly 0 to every particle and
+// are dropped at staging (order-preserving compaction, as in the forward).
+constexpr int kAdjQueue = 128;
+constexpr int kAdjStride = 144;
+// rows: x', y', z', 1, ux, uy, uz, c - u.x', -|x'|^2/R^2
+constexpr int kAdjOne = 3, kAdjU = 4, kAdjQ = 8, kAdjRows = 9;
+typedef float v4f __attribute__((ext_vector_type(4)));
 
-// n staged vertices (a multiple of 8) against this lane's particle.
-__device__ __forceinline__ void gather_pairs(const float4* __restrict__ qx,
-                                             const float4* __restrict__ qu, uint32_t n,
-                                             float px, float py, float pz, float ax, float ay,
-                                             float az, float c0, GradSums& g) {
-    for (uint32_t t0 = 0; t0 < n; t0 += 8u) {
+template <int M>
+__device__ __forceinline__ float adj_swz_xor(float v) {
+    return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x1F | (M << 10)));
+}
+
+template <int NT>
+__device__ __forceinline__ void adj_tiles(const float* __restrict__ rows, uint32_t n,
+                                          const float (&bt)[NT], const float (&bk)[NT],
+                                          v4f (&gu)[NT], v4f (&gx)[NT]) {
+    const uint32_t lane = __lane_id();
+    const float* a1p = rows + (lane >> 4) * kAdjStride + (lane & 15u);
+    const float* k1p = rows + (kAdjU + (lane >> 4)) * kAdjStride + (lane & 15u);
+    const float* a2p = rows + (lane & 3u) * kAdjStride + 4u * (lane >> 4);
+    const float* u2p = rows + (kAdjU + (lane & 3u)) * kAdjStride + 4u * (lane >> 4);
+    const float* cqp = rows + kAdjQ * kAdjStride + 4u * (lane >> 4);
+    for (uint32_t tb = 0; tb < n; tb += 16u) {
+        const float a1 = a1p[tb];
+        const float k1 = k1p[tb];
+        const float4 cq = *reinterpret_cast<const float4*>(cqp + tb);
+        const v4f c1 = v4f{cq.x, cq.y, cq.z, cq.w};
+        const v4f zero = v4f{0.f, 0.f, 0.f, 0.f};
+        v4f t[NT], kd[NT];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const float4 xv = qx[t0 + t];
-            const float4 uv = qu[t0 + t];
-            float tt = __builtin_fmaf(az, xv.z, c0 - xv.w);
-            tt = __builtin_fmaf(ay, xv.y, tt);
-            tt = __builtin_fmaf(ax, xv.x, tt);
-            // max(t, 0) as a clamp to [0, 1], folded into the last fma (as the
-            // forward kernel clamps t^3): t exceeds 1 only by rounding
-            const float tp = __builtin_amdgcn_fmed3f(tt, 0.0f, 1.0f);
-            const float t2 = tp * tp;
-            const float w = t2 * tp;
-            const float kd = __builtin_fmaf(uv.x, px, __builtin_fmaf(uv.y, py, __builtin_fmaf(uv.z, pz, uv.w)));
-            const float kk = kd * t2;
-            g.g1x = __builtin_fmaf(w, uv.x, g.g1x);
-            g.g1y = __builtin_fmaf(w, uv.y, g.g1y);
-            g.g1z = __builtin_fmaf(w, uv.z, g.g1z);
-            g.g2x = __builtin_fmaf(kk, xv.x, g.g2x);
-            g.g2y = __builtin_fmaf(kk, xv.y, g.g2y);
-            g.g2z = __builtin_fmaf(kk, xv.z, g.g2z);
-            g.sk += kk;
+        for (int T = 0; T < NT; ++T) t[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, bt[T], c1, 0, 0, 0);
+#pragma unroll
+        for (int T = 0; T < NT; ++T) kd[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(k1, bk[T], zero, 0, 0, 0);
+#pragma unroll
+        for (int T = 0; T < NT; ++T) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float tp = __builtin_amdgcn_fmed3f(t[T][r], 0.0f, 1.0f);
+                const float t2 = tp * tp;
+                t[T][r] = t2 * tp;
+                kd[T][r] = kd[T][r] * t2;
+            }
         }
+        const float4 au = *reinterpret_cast<const float4*>(u2p + tb);
+        const float4 ax = *reinterpret_cast<const float4*>(a2p + tb);
+#pragma unroll
+        for (int T = 0; T < NT; ++T) gu[T] = __builtin_amdgcn_mfma_f32_4x4x1f32(au.x, t[T][0], gu[T], 0, 0, 0);
+#pragma unroll
+        for (int T = 0; T < NT; ++T) gu[T] = __builtin_amdgcn_mfma_f32_4x4x1f32(au.y, t[T][1], gu[T], 0, 0, 0);
+#pragma unroll
+        for (int T = 0; T < NT; ++T) gu[T] = __builtin_amdgcn_mfma_f32_4x4x1f32(au.z, t[T][2], gu[T], 0, 0, 0);
+#pragma unroll
+        for (int T = 0; T < NT; ++T) gu[T] = __builtin_amdgcn_mfma_f32_4x4x1f32(au.w, t[T][3], gu[T], 0, 0, 0);
+#pragma unroll
+        for (int T = 0; T < NT; ++T) gx[T] = __builtin_amdgcn_mfma_f32_4x4x1f32(ax.x, kd[T][0], gx[T], 0, 0, 0);
+#pragma unroll
+        for (int T = 0; T < NT; ++T) gx[T] = __builtin_amdgcn_mfma_f32_4x4x1f32(ax.y, kd[T][1], gx[T], 0, 0, 0);
+#pragma unroll
+        for (int T = 0; T < NT; ++T) gx[T] = __builtin_amdgcn_mfma_f32_4x4x1f32(ax.z, kd[T][2], gx[T], 0, 0, 0);
+#pragma unroll
+        for (int T = 0; T < NT; ++T) gx[T] = __builtin_amdgcn_mfma_f32_4x4x1f32(ax.w, kd[T][3], gx[T], 0, 0, 0);
     }
 }
 
-// Vertices whose coefficient is zero, or that lie at least 1.00005 R from the
-// bounding box of the chunk's particles, add exactly 0 to every lane and are
-// dropped at staging (order-preserving compaction, as in the forward kernel).
-__global__ __launch_bounds__(kAdjThreads) void k_field_zb05_backward(FieldBackArgs a) {
-    __shared__ float4 queue_x[kAdjThreads / 64][128];
-    __shared__ float4 queue_u[kAdjThreads / 64][128];
+__global__ __launch_bounds__(kAdjThreads, 4) void k_field_zb05_backward_mc(FieldBackArgs a) {
+    struct WaveLds {
+        float queue[kAdjRows * kAdjStride];
+        float bt[4][64];   // per lane: (2p'/R^2, c0)
+    };
+    __shared__ __align__(16) WaveLds lds[kAdjThreads / 64];
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t nch = *a.num_chunks;
     const float inv_r2 = 1.0f / a.radius_sq;
     const float cull_r2 = a.radius_sq * 1.0001f;
-    float4* qx = queue_x[wave];
-    float4* qu = queue_u[wave];
+    float* q = lds[wave].queue;
+    float* vb = &lds[wave].bt[0][0];
+    q[kAdjOne * kAdjStride + lane] = 1.0f;
+    q[kAdjOne * kAdjStride + 64 + lane] = 1.0f;
     uint32_t step = 0;
     for (;;) {
         const uint32_t ch = next_item_xcd(a.next, nch, step);
@@ -297,6 +340,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_field_zb05_backward(FieldBackAr
         const fsr_index_range own = a.ranges[c];
         const uint32_t i = own.start + 64u * k + lane;
         const bool live = i < own.end;
+        const uint32_t cnt = min(64u, own.end - (own.start + 64u * k));
         const uint32_t qcx = c % a.sp.rx, qcy = (c / a.sp.rx) % a.sp.ry, qcz = c / (a.sp.rx * a.sp.ry);
 
         // 9 runs of band slots: virtual cells (q + radius) + {-1, 0, 1}^3
@@ -308,17 +352,10 @@ __global__ __launch_bounds__(kAdjThreads) void k_field_zb05_backward(FieldBackAr
             rs = a.cell_off[v0];
             len = a.cell_off[v0 + 2] + a.cell_cnt[v0 + 2] - rs;
         }
-        // lane j < 9 holds run j's first index (rp) and the offset from index
-        // to band slot in that run (ro)
         const uint32_t incl = wave_inclusive_sum(len);
         const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
         const uint32_t rp = incl - len;
         const uint32_t ro = rs - rp;
-        // Run of index gg of the staging round that starts at nb: jc, a run
-        // whose first index is at most nb, only grows; the runs that start
-        // inside the round are visited with wave-uniform reads, and a lane
-        // takes the last one that starts at or before gg (an empty run shares
-        // its first index with the next run, visited after it).
         uint32_t jc = 0;
         auto run_offset = [&](uint32_t nb, uint32_t gg) {
             uint32_t off = (uint32_t)__builtin_amdgcn_readlane((int)ro, (int)jc);
@@ -339,89 +376,130 @@ __global__ __launch_bounds__(kAdjThreads) void k_field_zb05_backward(FieldBackAr
             const float4 p = a.sorted[i];
             px = p.x - ox; py = p.y - oy; pz = p.z - oz;
         }
-        const float ax = 2.0f * px * inv_r2, ay = 2.0f * py * inv_r2, az = 2.0f * pz * inv_r2;
-        const float c0 = 1.0f - __builtin_fmaf(pz, pz, __builtin_fmaf(py, py, px * px)) * inv_r2;
+        vb[lane] = 2.0f * px * inv_r2;
+        vb[64 + lane] = 2.0f * py * inv_r2;
+        vb[128 + lane] = 2.0f * pz * inv_r2;
+        vb[192 + lane] = 1.0f - __builtin_fmaf(pz, pz, __builtin_fmaf(py, py, px * px)) * inv_r2;
         const float bx0 = wave_min(live ? px : 3.0e38f), bx1 = wave_max(live ? px : -3.0e38f);
         const float by0 = wave_min(live ? py : 3.0e38f), by1 = wave_max(live ? py : -3.0e38f);
         const float bz0 = wave_min(live ? pz : 3.0e38f), bz1 = wave_max(live ? pz : -3.0e38f);
+        wave_lds_sync();
 
-        GradSums g = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        uint32_t fill = 0;
-        // this lane's vertex of the next round is loaded one round ahead
-        float4 ncf = make_float4(0.f, 0.f, 0.f, 0.f), nvp = ncf;
-        {
-            const uint32_t off = run_offset(0u, lane);
-            if (lane < total) {
-                const uint32_t slot = off + lane;
-                ncf = a.coef[slot];
-                nvp = a.vpos[slot];
+        auto chunk = [&](auto ntc) {
+            constexpr int NT = decltype(ntc)::value;
+            float bt[NT], bk[NT];
+            v4f gu[NT], gx[NT];
+            const float* bsrc = vb + 64 * (lane >> 4) + (lane & 15u);
+            // (p', 1) of particle 16T + l%16, component l/16, from the LDS copy
+            // of 2p'/R^2 (times R^2 / 2) and a 1 for the fourth
+#pragma unroll
+            for (int T = 0; T < NT; ++T) {
+                bt[T] = bsrc[16 * T];
+                bk[T] = (lane >> 4) == 3u ? 1.0f : bt[T] * (0.5f * a.radius_sq);
+                gu[T] = v4f{0.f, 0.f, 0.f, 0.f};
+                gx[T] = v4f{0.f, 0.f, 0.f, 0.f};
             }
-        }
-        for (uint32_t base = 0; base < total; base += 64u) {
-            const uint32_t gi = base + lane;
-            const float4 cf = ncf, vp = nvp;
-            if (base + 64u < total) {
-                const uint32_t off = run_offset(base + 64u, gi + 64u);
-                if (gi + 64u < total) {
-                    const uint32_t slot = off + gi + 64u;
+            uint32_t head = 0, fill = 0;
+            float4 ncf = make_float4(0.f, 0.f, 0.f, 0.f), nvp = ncf;
+            {
+                const uint32_t off = run_offset(0u, lane);
+                if (lane < total) {
+                    const uint32_t slot = off + lane;
                     ncf = a.coef[slot];
                     nvp = a.vpos[slot];
                 }
             }
-            bool keep = false;
-            float4 sx, su;
-            if (gi < total) {
-                if (cf.x != 0.0f || cf.y != 0.0f || cf.z != 0.0f || cf.w != 0.0f) {
-                    const float xx = vp.x - ox;
-                    const float xy = vp.y - oy;
-                    const float xz = vp.z - oz;
+            for (uint32_t base = 0; base < total; base += 64u) {
+                const uint32_t gi = base + lane;
+                const float4 cf = ncf, vp = nvp;
+                if (base + 64u < total) {
+                    const uint32_t off = run_offset(base + 64u, gi + 64u);
+                    if (gi + 64u < total) {
+                        const uint32_t slot = off + gi + 64u;
+                        ncf = a.coef[slot];
+                        nvp = a.vpos[slot];
+                    }
+                }
+                bool keep = false;
+                float xx = 0.f, xy = 0.f, xz = 0.f;
+                if (gi < total && (cf.x != 0.0f || cf.y != 0.0f || cf.z != 0.0f || cf.w != 0.0f)) {
+                    xx = vp.x - ox;
+                    xy = vp.y - oy;
+                    xz = vp.z - oz;
                     const float dx = xx - __builtin_amdgcn_fmed3f(xx, bx0, bx1);
                     const float dy = xy - __builtin_amdgcn_fmed3f(xy, by0, by1);
                     const float dz = xz - __builtin_amdgcn_fmed3f(xz, bz0, bz1);
                     keep = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < cull_r2;
-                    sx = make_float4(xx, xy, xz,
-                                     __builtin_fmaf(xz, xz, __builtin_fmaf(xy, xy, xx * xx)) * inv_r2);
-                    // c - u . x' (the pair adds u . p')
-                    su = make_float4(cf.x, cf.y, cf.z,
-                                     cf.w - __builtin_fmaf(cf.z, xz, __builtin_fmaf(cf.y, xy, cf.x * xx)));
+                }
+                const uint64_t m = __ballot(keep);
+                if (keep) {
+                    const uint32_t slot = (head + fill + mask_rank(m)) & (kAdjQueue - 1);
+                    q[slot] = xx;
+                    q[kAdjStride + slot] = xy;
+                    q[2 * kAdjStride + slot] = xz;
+                    q[kAdjU * kAdjStride + slot] = cf.x;
+                    q[(kAdjU + 1) * kAdjStride + slot] = cf.y;
+                    q[(kAdjU + 2) * kAdjStride + slot] = cf.z;
+                    q[(kAdjU + 3) * kAdjStride + slot] =
+                        cf.w - __builtin_fmaf(cf.z, xz, __builtin_fmaf(cf.y, xy, cf.x * xx));
+                    q[kAdjQ * kAdjStride + slot] =
+                        -(__builtin_fmaf(xz, xz, __builtin_fmaf(xy, xy, xx * xx)) * inv_r2);
+                }
+                fill += (uint32_t)__popcll(m);
+                if (fill >= 64u) {
+                    wave_lds_sync();
+                    adj_tiles<NT>(q + head, 64u, bt, bk, gu, gx);
+                    wave_lds_sync();
+                    head ^= 64u;
+                    fill -= 64u;
                 }
             }
-            const uint64_t m = __ballot(keep);
-            if (keep) {
-                const uint32_t o = fill + mask_rank(m);
-                qx[o] = sx;
-                qu[o] = su;
-            }
-            fill += (uint32_t)__popcll(m);
-            if (fill >= 64u) {
-                wave_lds_sync();
-                gather_pairs(qx, qu, 64u, px, py, pz, ax, ay, az, c0, g);
-                wave_lds_sync();
-                const uint32_t rem = fill - 64u;
-                if (lane < rem) {
-                    qx[lane] = qx[64u + lane];
-                    qu[lane] = qu[64u + lane];
+            if (fill) {
+                // pad to 16 with entries that add 0: t < 0, u = 0
+                const uint32_t n = (fill + 15u) & ~15u;
+                if (lane >= fill && lane < n) {
+#pragma unroll
+                    for (int rr = 0; rr < kAdjQ; ++rr)
+                        if (rr != kAdjOne) q[rr * kAdjStride + head + lane] = 0.f;
+                    q[kAdjQ * kAdjStride + head + lane] = -1.0e30f;
                 }
                 wave_lds_sync();
-                fill = rem;
+                adj_tiles<NT>(q + head, n, bt, bk, gu, gx);
+                wave_lds_sync();
             }
-        }
-        if (fill) {
-            const uint32_t n = (fill + 7u) & ~7u;
-            if (lane >= fill && lane < n) {
-                qx[lane] = make_float4(0.f, 0.f, 0.f, 1.0e30f);
-                qu[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+            // the four partial sums of each particle; lane l keeps particle l
+            const uint32_t my = lane >> 4;
+            v4f su = gu[0], sx = gx[0];
+#pragma unroll
+            for (int T = 0; T < NT; ++T) {
+                v4f u4 = gu[T], x4 = gx[T];
+#pragma unroll
+                for (int cc = 0; cc < 4; ++cc) {
+                    float y = u4[cc];
+                    y += adj_swz_xor<16>(y);
+                    u4[cc] = y + __shfl_xor(y, 32, 64);
+                    float z = x4[cc];
+                    z += adj_swz_xor<16>(z);
+                    x4[cc] = z + __shfl_xor(z, 32, 64);
+                }
+                if (my == (uint32_t)T) {
+                    su = u4;
+                    sx = x4;
+                }
             }
-            wave_lds_sync();
-            gather_pairs(qx, qu, n, px, py, pz, ax, ay, az, c0, g);
-            wave_lds_sync();
-        }
-        if (live) {
-            const float s6 = 6.0f * inv_r2;
-            const uint64_t o = 3ull * a.perm[i];
-            a.g_pos[o + 0] = -__builtin_fmaf(s6, __builtin_fmaf(-g.sk, px, g.g2x), g.g1x);
-            a.g_pos[o + 1] = -__builtin_fmaf(s6, __builtin_fmaf(-g.sk, py, g.g2y), g.g1y);
-            a.g_pos[o + 2] = -__builtin_fmaf(s6, __builtin_fmaf(-g.sk, pz, g.g2z), g.g1z);
+            if (live) {
+                const float s6 = 6.0f * inv_r2;
+                const uint64_t o = 3ull * a.perm[i];
+                a.g_pos[o + 0] = -__builtin_fmaf(s6, __builtin_fmaf(-sx[3], px, sx[0]), su[0]);
+                a.g_pos[o + 1] = -__builtin_fmaf(s6, __builtin_fmaf(-sx[3], py, sx[1]), su[1]);
+                a.g_pos[o + 2] = -__builtin_fmaf(s6, __builtin_fmaf(-sx[3], pz, sx[2]), su[2]);
+            }
+        };
+        switch ((cnt + 15u) >> 4) {
+            case 1: chunk(std::integral_constant<int, 1>{}); break;
+            case 2: chunk(std::integral_constant<int, 2>{}); break;
+            case 3: chunk(std::integral_constant<int, 3>{}); break;
+            default: chunk(std::integral_constant<int, 4>{}); break;
         }
     }
 }
@@ -506,8 +584,8 @@ fsr_status field_zb05_backward(fsr_context* ctx, const float* g_phi, float* g_po
     a.radius_sq = st.radius_sq;
     a.g_pos = g_pos;
     const int blocks = persistent_blocks(ctx, kAdjThreads, 4, TUNE_FIELD);
-    hipLaunchKernelGGL(k_field_zb05_backward, dim3(blocks), dim3(kAdjThreads), 0, s, a);
-    FSR_LAUNCH_CHECK("k_field_zb05_backward");
+    hipLaunchKernelGGL(k_field_zb05_backward_mc, dim3(blocks), dim3(kAdjThreads), 0, s, a);
+    FSR_LAUNCH_CHECK("k_field_zb05_backward_mc");
     return FSR_OK;
 }
 
