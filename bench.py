@@ -447,7 +447,7 @@ def batched_vs_per_frame(method: str, frames: int, lanes: int, reps: int, autotu
     }
 
 
-def frame_sequence(fr, frames: int = 4):
+def frame_sequence(fr, frames: int = 6):
     """Frame-sequence driver (the reference's ReconstructorGPU::reconstruct flow:
     read .xyz, reconstruct, write .obj, frame after frame) on `frames` copies of
     the headline frame written as .xyz files: end-to-end frames/s of the
@@ -721,6 +721,9 @@ def main():
     ap.add_argument("--no-counters", action="store_true",
                     help="skip the rocprofv3 counter runs (traffic, L2 hit rate)")
     ap.add_argument("--counter-probe", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--only", choices=["zb05", "akinci12", "yu13", "adjoint", "batch"],
+                    help="run one workload alone (for kernel traces) and print its result "
+                         "instead of the benchmark line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus %d is not supported "
@@ -736,6 +739,26 @@ def main():
     from fluidsurfacereconstructiongpu_amd import datasets as ds
 
     K, W = args.steps, args.warmup
+    if args.only:
+        if args.only in ("zb05", "akinci12"):
+            fr = make_frame(args.only, args.particles, args.grid)
+            if args.only == "zb05":
+                rec, total_ms, stages, st = time_frames("zb05", fr, K, W)
+                res = {"value": K / (total_ms * 1e-3), "unit": "frames/s",
+                       **stage_summary(stages), "counts": counts_of(st)}
+            else:
+                res = akinci_line(fr, K, W)[0]
+        elif args.only == "yu13":
+            fr = make_frame("yu13", args.particles, 384)
+            res = yu13_line(fr, ds.yu13_params(fr, True), K, W)[0]
+        elif args.only == "adjoint":
+            res = adjoint_cost(reps=K)
+        else:
+            res = {m: batched_vs_per_frame(m, args.batch_frames, args.lanes, args.batch_reps,
+                                           not args.no_autotune) for m in ("zb05", "yu13")}
+        print(json.dumps({"only": args.only, "steps": K, "warmup": W, "result": res}),
+              flush=True)
+        return
     t0 = time.perf_counter()
     fr = make_frame("zb05", args.particles, args.grid)
     log(f"workload generated in {time.perf_counter() - t0:.1f} s: N={fr.n} sp={fr.sp.res} "

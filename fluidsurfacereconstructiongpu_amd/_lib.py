@@ -232,6 +232,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
                                   POINTER(SimParam), c_void_p, c_uint32, POINTER(FrameStats),
                                   c_char_p]
     lib.fsr_vis_write.restype = st
+    lib.fsr_set_surface_mode.argtypes = [c_void_p, c_int]
+    lib.fsr_set_surface_mode.restype = st
     lib.fsr_set_adjoint.argtypes = [c_void_p, c_int]
     lib.fsr_set_adjoint.restype = st
     lib.fsr_mc_vertex_backward.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]

@@ -287,9 +287,10 @@ fsr_status frame_impl(const char* who, fsr_context* ctx, fsr_method method,
     sa.counters = counters;
     sa.st = stats;
     sa.all_exceptions = band_dilation_exact(scg, sim->smoothing_radius) ? 0 : 1;
+    sa.radius_count = (!yu13 && ctx->surface_mode == FSR_SURFACE_RADIUS_COUNT) ? 1 : 0;
     uint32_t* involve = nullptr;
     if (yu13) FSR_TRY(arena_typed(ctx, A_INVOLVE, C, &involve));
-    if (method == FSR_METHOD_AKINCI12) {
+    if (method == FSR_METHOD_AKINCI12 || sa.radius_count) {
         FSR_TRY(surface_colorfield(ctx, sa, sim->particle_mass, s));
     } else {
         FSR_TRY(surface_particles(ctx, sa, involve, s));
@@ -530,6 +531,15 @@ fsr_status fsr_stage_times(fsr_context* ctx, float* ms, uint32_t max_frames, uin
     }
     *num_frames = n;
     ctx->prof_frames = 0;
+    return FSR_OK;
+}
+
+fsr_status fsr_set_surface_mode(fsr_context* ctx, fsr_surface_mode mode) {
+    if (!ctx || (mode != FSR_SURFACE_REFERENCE && mode != FSR_SURFACE_RADIUS_COUNT)) {
+        set_error("fsr_set_surface_mode: need a context and a known mode");
+        return FSR_ERR_INVALID_ARG;
+    }
+    ctx->surface_mode = mode;
     return FSR_OK;
 }
 

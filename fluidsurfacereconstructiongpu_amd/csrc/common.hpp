@@ -287,6 +287,7 @@ struct fsr_context {
     int tune[fsr::TUNE_NUM] = {0, 0, 0, 0};  // 0: kernel default
     bool capturing = false;                  // inside a graph capture: no arena growth
     bool save_adjoint = false;               // frames keep what the adjoint needs
+    int surface_mode = 0;                    // fsr_surface_mode of ZB05 / Akinci12 frames
     fsr::AdjointState adj;
     fsr::Arena arena;
     // stage profiling: ring of (FSR_NUM_STAGES + 1) events per frame
@@ -400,6 +401,9 @@ struct SurfaceArgs {
     Counters* counters;
     fsr_frame_stats* st;
     int all_exceptions;     // 1: every surface particle takes the exact path
+    // 1: surface particles are those with fewer than 25 other particles at
+    // squared distance (FP32) below h^2, instead of the method's own test
+    int radius_count;
 };
 fsr_status surface_occupancy(fsr_context* ctx, const SurfaceArgs& a, uint8_t* surf_cell,
                              uint32_t* involve, hipStream_t s);

@@ -16,13 +16,16 @@
 //         reference's Visualizer, same number format; index lists are written
 //         "i j k " on one line, followed by a line break only when the list is
 //         not empty (the occupied-cell list: when the hash grid is not empty).
+#include <algorithm>
 #include <cerrno>
+#include <charconv>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "common.hpp"
@@ -31,47 +34,118 @@ using namespace fsr;
 
 namespace {
 
-bool parse_floats(const std::string& line, float* out, int n) {
-    const char* p = line.c_str();
-    for (int i = 0; i < n; ++i) {
-        char* end = nullptr;
-        errno = 0;
-        const float v = std::strtof(p, &end);
-        if (end == p) return false;
-        out[i] = v;
-        p = end;
+// ---- .xyz body: the whole file is read into memory and its particle lines
+// are parsed by several host threads, each over a range of whole lines (a
+// count pass, then a parse pass into the rows its count assigns).  Numbers go
+// through std::from_chars, which rounds correctly like `stream >> float`.
+
+bool is_space(char c) { return c == ' ' || c == '\t' || c == '\r'; }
+
+// A float at p (leading blanks and one '+' skipped); false when there is none.
+bool next_float(const char*& p, const char* end, float* out) {
+    while (p < end && is_space(*p)) ++p;
+    if (p < end && *p == '+') ++p;
+    const std::from_chars_result r = std::from_chars(p, end, *out);
+    if (r.ec == std::errc::invalid_argument || r.ptr == p) return false;
+    p = r.ptr;  // out-of-range values parse to +-inf / 0 like strtof's, kept as read
+    return true;
+}
+
+bool blank_range(const char* a, const char* b) {
+    for (; a < b; ++a)
+        if (!is_space(*a)) return false;
+    return true;
+}
+
+fsr_status slurp(const char* path, const char* who, std::vector<char>* buf) {
+    std::FILE* f = std::fopen(path, "rb");
+    if (!f) {
+        set_error(std::string(who) + ": cannot open " + path);
+        return FSR_ERR_INVALID_ARG;
     }
-    return true;
+    buf->clear();
+    long size = -1;
+    if (std::fseek(f, 0, SEEK_END) == 0) size = std::ftell(f);
+    if (size < 0 || std::fseek(f, 0, SEEK_SET) != 0) {
+        std::fclose(f);
+        set_error(std::string(who) + ": cannot size " + path);
+        return FSR_ERR_INVALID_ARG;
+    }
+    buf->resize((size_t)size);
+    const size_t got = size ? std::fread(buf->data(), 1, (size_t)size, f) : 0;
+    std::fclose(f);
+    if (got != (size_t)size) {
+        set_error(std::string(who) + ": short read of " + path);
+        return FSR_ERR_INVALID_ARG;
+    }
+    return FSR_OK;
 }
 
-bool blank(const std::string& line) {
-    for (char c : line)
-        if (c != ' ' && c != '\t' && c != '\r' && c != '\n') return false;
-    return true;
-}
-
-fsr_status read_header(std::ifstream& f, const char* path, fsr_xyz_header* h) {
-    std::string line;
-    float v[3];
+// The five header lines; *body = offset of the first particle line.
+fsr_status header_from(const std::vector<char>& buf, const char* path, fsr_xyz_header* h,
+                       size_t* body) {
     const char* what[5] = {"AABB min", "AABB max", "kernel radius", "particle radius",
                            "particle mass"};
     const int count[5] = {3, 3, 1, 1, 1};
     float* dst[5] = {&h->aabb_min.x, &h->aabb_max.x, &h->kernel_radius, &h->particle_radius,
                      &h->particle_mass};
+    size_t at = 0;
     for (int k = 0; k < 5; ++k) {
-        if (!std::getline(f, line) || !parse_floats(line, v, count[k])) {
+        const char* b = buf.data() + at;
+        const char* e = static_cast<const char*>(std::memchr(b, '\n', buf.size() - at));
+        const char* le = e ? e : buf.data() + buf.size();
+        bool ok = true;
+        float v[3];
+        const char* p = b;
+        for (int j = 0; j < count[k] && ok; ++j) ok = next_float(p, le, &v[j]);
+        if (!ok || at >= buf.size()) {
             set_error(std::string(path) + ": cannot parse the " + what[k] + " header line");
             return FSR_ERR_INVALID_ARG;
         }
-        if (count[k] == 3) {
-            dst[k][0] = v[0];
-            dst[k][1] = v[1];
-            dst[k][2] = v[2];
-        } else {
-            *dst[k] = v[0];
-        }
+        for (int j = 0; j < count[k]; ++j) dst[k][j] = v[j];
+        at = e ? (size_t)(e - buf.data()) + 1 : buf.size();
     }
+    *body = at;
     return FSR_OK;
+}
+
+// Line-aligned ranges of the body, one per thread.
+std::vector<size_t> line_ranges(const std::vector<char>& buf, size_t body) {
+    const size_t len = buf.size() - body;
+    unsigned t = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (len < (1u << 20)) t = 1;
+    std::vector<size_t> cut{body};
+    for (unsigned k = 1; k < t; ++k) {
+        size_t at = std::max(cut.back(), body + len * k / t);
+        const void* nl = at < buf.size() ? std::memchr(buf.data() + at, '\n', buf.size() - at)
+                                         : nullptr;
+        at = nl ? (size_t)(static_cast<const char*>(nl) - buf.data()) + 1 : buf.size();
+        cut.push_back(at);
+    }
+    cut.push_back(buf.size());
+    return cut;
+}
+
+template <class F>
+void parallel_ranges(size_t n, F f) {
+    std::vector<std::thread> th;
+    for (size_t k = 1; k < n; ++k) th.emplace_back(f, k);
+    f(0);
+    for (auto& x : th) x.join();
+}
+
+// Lines and non-blank lines of [a, b).
+void count_lines(const char* a, const char* b, uint64_t* lines, uint64_t* particles) {
+    uint64_t nl = 0, np = 0;
+    while (a < b) {
+        const char* e = static_cast<const char*>(std::memchr(a, '\n', (size_t)(b - a)));
+        const char* le = e ? e : b;
+        ++nl;
+        if (!blank_range(a, le)) ++np;
+        a = e ? e + 1 : b;
+    }
+    *lines = nl;
+    *particles = np;
 }
 
 // Default iostream formatting (%g with 6 significant digits), as the
@@ -87,16 +161,17 @@ fsr_status fsr_xyz_info(const char* path, fsr_xyz_header* header, uint32_t* num_
         set_error("fsr_xyz_info: null argument");
         return FSR_ERR_INVALID_ARG;
     }
-    std::ifstream f(path);
-    if (!f) {
-        set_error(std::string("fsr_xyz_info: cannot open ") + path);
-        return FSR_ERR_INVALID_ARG;
-    }
-    FSR_TRY(read_header(f, path, header));
-    std::string line;
+    std::vector<char> buf;
+    FSR_TRY(slurp(path, "fsr_xyz_info", &buf));
+    size_t body = 0;
+    FSR_TRY(header_from(buf, path, header, &body));
+    const std::vector<size_t> cut = line_ranges(buf, body);
+    std::vector<uint64_t> lines(cut.size() - 1), parts(cut.size() - 1);
+    parallel_ranges(cut.size() - 1, [&](size_t k) {
+        count_lines(buf.data() + cut[k], buf.data() + cut[k + 1], &lines[k], &parts[k]);
+    });
     uint64_t n = 0;
-    while (std::getline(f, line))
-        if (!blank(line)) ++n;
+    for (uint64_t p : parts) n += p;
     if (n > 0xFFFFFFFFull) {
         set_error("fsr_xyz_info: more than 2^32 - 1 particles");
         return FSR_ERR_INVALID_ARG;
@@ -111,34 +186,63 @@ fsr_status fsr_xyz_read(const char* path, fsr_xyz_header* header, float* pos, fl
         set_error("fsr_xyz_read: null argument");
         return FSR_ERR_INVALID_ARG;
     }
-    std::ifstream f(path);
-    if (!f) {
-        set_error(std::string("fsr_xyz_read: cannot open ") + path);
-        return FSR_ERR_INVALID_ARG;
+    std::vector<char> buf;
+    FSR_TRY(slurp(path, "fsr_xyz_read", &buf));
+    size_t body = 0;
+    FSR_TRY(header_from(buf, path, header, &body));
+    const std::vector<size_t> cut = line_ranges(buf, body);
+    const size_t T = cut.size() - 1;
+    std::vector<uint64_t> lines(T), parts(T);
+    parallel_ranges(T, [&](size_t k) {
+        count_lines(buf.data() + cut[k], buf.data() + cut[k + 1], &lines[k], &parts[k]);
+    });
+    uint64_t n = 0;
+    std::vector<uint64_t> row0(T), line0(T);
+    uint64_t ln = 6;  // the first particle line is line 6 of the file
+    for (size_t k = 0; k < T; ++k) {
+        row0[k] = n;
+        line0[k] = ln;
+        n += parts[k];
+        ln += lines[k];
     }
-    FSR_TRY(read_header(f, path, header));
-    std::string line;
-    uint32_t n = 0;
-    uint64_t lineno = 5;
-    while (std::getline(f, line)) {
-        ++lineno;
-        if (blank(line)) continue;
-        if (n == capacity) {
-            set_error("fsr_xyz_read: more particles than the capacity");
-            return FSR_ERR_CAPACITY;
+    if (n > capacity) {
+        set_error("fsr_xyz_read: more particles than the capacity");
+        return FSR_ERR_CAPACITY;
+    }
+    // the first line (in file order) that does not parse, per range
+    std::vector<uint64_t> bad(T, 0);
+    parallel_ranges(T, [&](size_t k) {
+        const char* a = buf.data() + cut[k];
+        const char* b = buf.data() + cut[k + 1];
+        uint64_t r = row0[k], l = line0[k];
+        while (a < b) {
+            const char* e = static_cast<const char*>(std::memchr(a, '\n', (size_t)(b - a)));
+            const char* le = e ? e : b;
+            if (!blank_range(a, le)) {
+                float v[4];
+                const char* p = a;
+                bool ok = true;
+                for (int j = 0; j < 4 && ok; ++j) ok = next_float(p, le, &v[j]);
+                if (!ok) {
+                    bad[k] = l;
+                    return;
+                }
+                pos[3 * r + 0] = v[0];
+                pos[3 * r + 1] = v[1];
+                pos[3 * r + 2] = v[2];
+                dens[r] = v[3];
+                ++r;
+            }
+            ++l;
+            a = e ? e + 1 : b;
         }
-        float v[4];
-        if (!parse_floats(line, v, 4)) {
-            set_error(std::string(path) + ": cannot parse particle line " + std::to_string(lineno));
+    });
+    for (size_t k = 0; k < T; ++k)
+        if (bad[k]) {
+            set_error(std::string(path) + ": cannot parse particle line " + std::to_string(bad[k]));
             return FSR_ERR_INVALID_ARG;
         }
-        pos[3ull * n + 0] = v[0];
-        pos[3ull * n + 1] = v[1];
-        pos[3ull * n + 2] = v[2];
-        dens[n] = v[3];
-        ++n;
-    }
-    *num_particles = n;
+    *num_particles = (uint32_t)n;
     return FSR_OK;
 }
 

@@ -11,9 +11,9 @@
 // surface vertex ratios, cumulative stage times).
 //
 // Frames are pipelined (the reference runs them strictly one after another):
-// while frame k is reconstructed, a host thread parses frame k+1's .xyz into a
-// pinned buffer, and frame k+1 is uploaded on a copy stream while frame k's
-// mesh is written; --serial runs them strictly one after another instead
+// while frame k is reconstructed and its files are written, a host thread
+// parses frame k+1's .xyz into a pinned buffer, which is then uploaded on a
+// copy stream; --serial runs them strictly one after another instead
 // (read, upload, reconstruct, write), as the reference does.  Per-frame times
 // cover the reconstruction only, as the reference's timer does; the end-to-end
 // rate, from the first read to the last write, is printed at the end.
@@ -272,12 +272,6 @@ int main(int argc, char** argv) {
         CHECK_FSR(fsr_stage_times(ctx, stage, 1, &nf));
         times.push_back(ms);
         std::printf("Reconstructing Frame%ld took %g milliseconds.\n", frame, ms);
-        // the next frame's upload runs while this frame's files are written
-        if (next.valid()) {
-            next.get();
-            check_loaded(host[slot ^ 1]);
-            upload(slot ^ 1);
-        }
         if (save_vis) {
             const std::string path = stem + ".vis";
             std::printf("Writing to %s...\n", path.c_str());
@@ -312,6 +306,13 @@ int main(int argc, char** argv) {
                 CHECK_FSR(fsr_obj_write_soup(path.c_str(), tp.data(), tn.data(), T3));
             }
             std::printf("Writing to %s\n", path.c_str());
+        }
+        // the next frame was parsed while this one ran and its files were
+        // written; its upload overlaps the start of the next iteration
+        if (next.valid()) {
+            next.get();
+            check_loaded(host[slot ^ 1]);
+            upload(slot ^ 1);
         }
     }
     const double wall_ms =

@@ -12,7 +12,9 @@
 //   k_colorfield        extractionOfSurfaceParticlesUsingColorField
 //                       (ReconstructionCUDA.cu:909-971): surface iff fewer than 25
 //                       particles with |r|^2 > 0 in the clamped 3^3 cell block, or
-//                       |sum m/rho_j gradW_spiky(r) r_hat| > 3.
+//                       |sum m/rho_j gradW_spiky(r) r_hat| > 3.  With the radius-count
+//                       surface mode (either method): surface iff fewer than 25
+//                       particles at 0 < |r|^2 < h^2 (FP32 squared distances).
 // Every surface particle then seeds the band: its scalar cell is set in a bit
 // mask of "surface scalar cells" that band.hip dilates, or, when the dilation
 // is not exact for it, it is appended to an exception list that band.hip
@@ -249,7 +251,8 @@ __global__ __launch_bounds__(256) void k_colorfield_tail(SurfaceArgs a, float ma
             const int x0 = clampi(cx - 1, 0, a.sp.rx - 1), x1 = clampi(cx + 1, 0, a.sp.rx - 1);
             const int y0 = clampi(cy - 1, 0, a.sp.ry - 1), y1 = clampi(cy + 1, 0, a.sp.ry - 1);
             const int z0 = clampi(cz - 1, 0, a.sp.rz - 1), z1 = clampi(cz + 1, 0, a.sp.rz - 1);
-            uint32_t count = 0;
+            uint32_t count = 0, count_r = 0;
+            const float h2 = h * h;
             float gx = 0.f, gy = 0.f, gz = 0.f;
             for (int zz = z0; zz <= z1; ++zz)
                 for (int yy = y0; yy <= y1; ++yy) {
@@ -269,6 +272,7 @@ __global__ __launch_bounds__(256) void k_colorfield_tail(SurfaceArgs a, float ma
                         const float d2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
                         if (d2 > 0.0f) {
                             ++count;
+                            count_r += d2 < h2 ? 1u : 0u;
                             const float dist = __builtin_amdgcn_sqrtf(d2);  // hardware square root
                             if (dist < h) {
                                 const float x = 1.0f - dist * inv_h;
@@ -281,7 +285,7 @@ __global__ __launch_bounds__(256) void k_colorfield_tail(SurfaceArgs a, float ma
                     }
                 }
             const float g2 = __builtin_fmaf(gz, gz, __builtin_fmaf(gy, gy, gx * gx));
-            surf = count < 25u || __builtin_sqrtf(g2) > 3.0f;
+            surf = a.radius_count ? count_r < 25u : (count < 25u || __builtin_sqrtf(g2) > 3.0f);
             a.flags[i] = surf ? 1u : 0u;
         }
         const bool exc = a.sbits ? seed_particle(a, i, p.x, p.y, p.z, surf) : false;
@@ -336,6 +340,7 @@ __global__ __launch_bounds__(256) void k_colorfield(SurfaceArgs a, float mass, C
     const float inv_h = 1.0f / h;
     const float gcoef = -45.0f / (3.14159265358979f * (h * h) * (h * h));
     const float cull2 = (h * h) * 1.0001f;
+    const float h2 = h * h;
     uint32_t n_surf = 0, n_exc = 0;
     // work of this wave: candidates staged and kept (wave-uniform), and this
     // lane's pairs within h
@@ -397,7 +402,7 @@ __global__ __launch_bounds__(256) void k_colorfield(SurfaceArgs a, float mass, C
         const float bx0 = wave_minf(live ? p.x : 3.0e38f), bx1 = wave_maxf(live ? p.x : -3.0e38f);
         const float by0 = wave_minf(live ? p.y : 3.0e38f), by1 = wave_maxf(live ? p.y : -3.0e38f);
         const float bz0 = wave_minf(live ? p.z : 3.0e38f), bz1 = wave_maxf(live ? p.z : -3.0e38f);
-        uint32_t count = 0, kept = 0, fill = 0;
+        uint32_t count = 0, count_r = 0, kept = 0, fill = 0;
         float gx = 0.f, gy = 0.f, gz = 0.f;
         auto run = [&](uint32_t n) {
             for (uint32_t t = 0; t < n; ++t) {
@@ -406,6 +411,7 @@ __global__ __launch_bounds__(256) void k_colorfield(SurfaceArgs a, float mass, C
                 const float d2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
                 if (d2 > 0.0f) {
                     ++count;
+                    count_r += d2 < h2 ? 1u : 0u;
                     const float dist = __builtin_amdgcn_sqrtf(d2);  // hardware square root
                     if (dist < h) {
                         ++n_inh;
@@ -469,7 +475,8 @@ __global__ __launch_bounds__(256) void k_colorfield(SurfaceArgs a, float mass, C
         bool surf = false;
         if (live) {
             const float g2 = __builtin_fmaf(gz, gz, __builtin_fmaf(gy, gy, gx * gx));
-            surf = count < 25u || __builtin_sqrtf(g2) > 3.0f;
+            // dropped candidates lie beyond h: they never count in count_r
+            surf = a.radius_count ? count_r < 25u : (count < 25u || __builtin_sqrtf(g2) > 3.0f);
             a.flags[i] = surf ? 1u : 0u;
         }
         bool exc = false;

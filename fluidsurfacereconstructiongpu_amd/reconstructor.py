@@ -53,6 +53,12 @@ class Context:
         except Exception:
             pass
 
+    def set_surface_mode(self, radius_count: bool) -> None:
+        """Surface particles of ZB05 / Akinci12 frames: the method's own test
+        (False, the reference's) or fewer than 25 particles closer than the
+        smoothing radius (True)."""
+        _lib.check(self.lib.fsr_set_surface_mode(self.handle, 1 if radius_count else 0))
+
     def set_profiling(self, ring_frames: int) -> None:
         _lib.check(self.lib.fsr_set_profiling(self.handle, int(ring_frames)))
 

@@ -499,6 +499,25 @@ typedef enum {
     FSR_NUM_STAGES = 6
 } fsr_stage;
 
+/* Surface particles of the ZB05 and Akinci12 frames.
+ *   FSR_SURFACE_REFERENCE (default): each method's own test, as the reference
+ *     has it: the 27-cell occupancy test for ZB05
+ *     (extractionOfSurfaceParticles, ReconstructionCUDA.cu:70-120), the
+ *     27-cell count and colour-field gradient for Akinci12
+ *     (ReconstructionCUDA.cu:909-971).
+ *   FSR_SURFACE_RADIUS_COUNT: a particle is a surface particle iff fewer than
+ *     25 other particles lie closer than the smoothing radius (squared
+ *     distances in FP32, fmaf(dz, dz, fmaf(dy, dy, dx * dx)) < h * h).  No
+ *     reference counterpart: it is the "fewer than 25 neighbours within the
+ *     radius" band of the dam-break configuration.
+ * Yu13 frames always use the reference's test.  Applies to the following
+ * frames on ctx. */
+typedef enum {
+    FSR_SURFACE_REFERENCE = 0,
+    FSR_SURFACE_RADIUS_COUNT = 1
+} fsr_surface_mode;
+fsr_status fsr_set_surface_mode(fsr_context* ctx, fsr_surface_mode mode);
+
 /* ring_frames > 0: every following frame records HIP events at its stage
  * boundaries on its stream, into a ring of ring_frames slots; 0 disables. */
 fsr_status fsr_set_profiling(fsr_context* ctx, uint32_t ring_frames);

@@ -68,6 +68,8 @@ def load():
     lib.orc_frame.argtypes = [ctypes.c_int, f32p, f32p, ctypes.c_uint32, OGrid, OGrid,
                               ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                               ctypes.c_float, ctypes.c_void_p, u64p]
+    lib.orc_surface_radius_count.argtypes = [f32p, ctypes.c_uint32, u32p, OGrid, ctypes.c_float,
+                                             u32p, u32p]
     lib.orc_mc_tri_packed.argtypes = [ctypes.c_uint32]
     lib.orc_mc_tri_packed.restype = ctypes.c_uint64
     lib.orc_smooth.argtypes = [f32p, ctypes.c_uint32, u32p, OGrid, ctypes.c_float, ctypes.c_float,
@@ -214,6 +216,17 @@ def surface_occupancy(occ, ranges, sp, keys, involve=False):
     return surf, flags, inv
 
 
+def surface_radius_count(pos, ranges, sp, h):
+    """(flags, counts) of the radius-count surface test."""
+    lib = load()
+    n = pos.shape[0]
+    flags = np.empty(n, np.uint32)
+    cnt = np.empty(n, np.uint32)
+    lib.orc_surface_radius_count(np.ascontiguousarray(pos, np.float32).ravel(), n,
+                                 np.ascontiguousarray(ranges).ravel(), ogrid(sp), h, flags, cnt)
+    return flags, cnt
+
+
 def surface_colorfield(pos, dens, ranges, sp, h, mass):
     lib = load()
     n = pos.shape[0]
@@ -275,15 +288,17 @@ def mc(index, band_u8, phi, sc, iso, cpu=False, with_mesh=True, with_len=False):
     return nverts, pos, nrm
 
 
-def frame(method, fr, want_phi=False):
-    """Whole frame on copies of the frame's arrays; returns (counts, phi|None)."""
+def frame(method, fr, want_phi=False, radius_count=False):
+    """Whole frame on copies of the frame's arrays; returns (counts, phi|None).
+    radius_count: surface particles by the radius-count test."""
     lib = load()
     pos = np.ascontiguousarray(fr.pos, np.float32).copy()
     dens = np.ascontiguousarray(fr.dens, np.float32).copy()
     phi = np.empty(fr.sc.count, np.float32) if want_phi else None
     counts = np.zeros(6, np.uint64)
     h = np.float32(fr.h)
-    lib.orc_frame({"zb05": 0, "akinci12": 1}[method], pos.ravel(), dens, pos.shape[0],
+    lib.orc_frame({"zb05": 0, "akinci12": 1}[method] | (2 if radius_count else 0), pos.ravel(),
+                  dens, pos.shape[0],
                   ogrid(fr.sp), ogrid(fr.sc), float(h), float(h * h),
                   float(np.float32(fr.particle_radius)), float(np.float32(fr.mass)),
                   float(np.float32(fr.iso)), phi.ctypes.data if want_phi else None, counts)

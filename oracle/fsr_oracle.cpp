@@ -189,6 +189,41 @@ int orc_surface_colorfield(const float* pos, const float* dens, uint32_t n,
     return 0;
 }
 
+// Radius-count surface test (no reference counterpart: the "fewer than 25
+// neighbours within the radius" band of the dam-break configuration): i is a
+// surface particle iff fewer than 25 particles j of its clamped 3^3 cell block
+// have 0 < d2 < h * h, with d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) in FP32 of
+// d = p_i - p_j (the kernel's order).  The hash cell is at least h, so the
+// block holds every such j.
+int orc_surface_radius_count(const float* pos, uint32_t n, const uint32_t* ranges, orc_grid sp,
+                             float h, uint32_t* flags, uint32_t* count) {
+    const int rx = sp.rx, ry = sp.ry, rz = sp.rz;
+    const float h2 = h * h;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+        const int cx = trunc_q(cell_q(px, sp.minx, sp.cell), rx);
+        const int cy = trunc_q(cell_q(py, sp.miny, sp.cell), ry);
+        const int cz = trunc_q(cell_q(pz, sp.minz, sp.cell), rz);
+        uint32_t cnt = 0;
+        for (int zz = clampi(cz - 1, 0, rz - 1); zz <= clampi(cz + 1, 0, rz - 1); ++zz)
+            for (int yy = clampi(cy - 1, 0, ry - 1); yy <= clampi(cy + 1, 0, ry - 1); ++yy)
+                for (int xx = clampi(cx - 1, 0, rx - 1); xx <= clampi(cx + 1, 0, rx - 1); ++xx) {
+                    const uint64_t c = ((uint64_t)zz * ry + yy) * rx + xx;
+                    if (ranges[2 * c] == 0xFFFFFFFFu) continue;
+                    for (uint32_t j = ranges[2 * c]; j < ranges[2 * c + 1]; ++j) {
+                        const float fx = px - pos[3 * j], fy = py - pos[3 * j + 1],
+                                    fz = pz - pos[3 * j + 2];
+                        const float d2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+                        if (d2 > 0.0f && d2 < h2) ++cnt;
+                    }
+                }
+        flags[i] = cnt < 25u ? 1u : 0u;
+        if (count) count[i] = cnt;
+    }
+    return 0;
+}
+
 // estimationOfSurfaceVertices (ReconstructionCUDA.cu:554-595): box [i-3, i+3)
 // x [j-3, j+3] x [k-3, k+3] clamped to the grid, FP32 |v - p| compared in double
 // with 1.5 h.
@@ -640,7 +675,8 @@ int orc_field_aniso(const uint32_t* band_index, uint32_t B, const double* smooth
 }
 
 // One frame of the ZB05 (method 0) or Akinci12 (method 1) reconstructor
-// (GPU/ReconstructorGPUOursZB05.cpp:22-78, GPU/ReconstructorGPUAkinci12.cpp:34-84).
+// (GPU/ReconstructorGPUOursZB05.cpp:22-78, GPU/ReconstructorGPUAkinci12.cpp:34-84);
+// method | 2: surface particles by the radius-count test instead.
 int orc_frame(int method, float* pos, float* dens, uint32_t n, orc_grid sp, orc_grid sc, float h,
               float radius_sq, float particle_radius, float mass, float iso, float* phi_out,
               uint64_t* counts) {
@@ -652,7 +688,10 @@ int orc_frame(int method, float* pos, float* dens, uint32_t n, orc_grid sp, orc_
                   ranges.data(), occ.data());
     std::memcpy(pos, pos_s.data(), sizeof(float) * 3ull * n);
     std::memcpy(dens, dens_s.data(), sizeof(float) * n);
-    if (method == 1) {
+    const bool akinci = (method & 1) != 0;
+    if (method & 2) {
+        orc_surface_radius_count(pos, n, ranges.data(), sp, h, flags.data(), nullptr);
+    } else if (akinci) {
         orc_surface_colorfield(pos, dens, n, ranges.data(), sp, h, mass, flags.data(), nullptr,
                                nullptr);
     } else {
@@ -673,10 +712,10 @@ int orc_frame(int method, float* pos, float* dens, uint32_t n, orc_grid sp, orc_
     orc_field_zb05(index.data(), B, pos, ranges.data(), sp, sc, radius_sq, particle_radius,
                    phi.data(), nullptr);
     std::vector<uint32_t> nverts(B);
-    const uint64_t total = orc_mc(index.data(), B, band.data(), phi.data(), sc, iso, method == 1,
+    const uint64_t total = orc_mc(index.data(), B, band.data(), phi.data(), sc, iso, akinci,
                                   nverts.data(), nullptr, nullptr, 0, nullptr);
     std::vector<double> tp(3 * total), tn(3 * total);
-    orc_mc(index.data(), B, band.data(), phi.data(), sc, iso, method == 1, nverts.data(),
+    orc_mc(index.data(), B, band.data(), phi.data(), sc, iso, akinci, nverts.data(),
            tp.data(), tn.data(), total, nullptr);
     uint64_t q = 0;
     for (uint32_t b = 0; b < B; ++b) q += nverts[b] > 0;

@@ -85,8 +85,13 @@ int orc_field_aniso(const uint32_t* band_index, uint32_t B, const double* smooth
                     const uint32_t* involve_off, const double* G, orc_grid sp, orc_grid sc,
                     float mass, const uint8_t* ambiguous, double* phi, uint8_t* flag_out);
 
-/* Whole frame (method 0 = ZB05 occupancy band, 1 = Akinci12 colour-field band),
- * as the reconstructors run it.  Positions and densities are sorted in place.
+/* Radius-count surface test: flags[i] = 1 iff fewer than 25 particles of the
+ * clamped 3^3 block lie at 0 < d2 < h*h (FP32 d2).  count (may be NULL). */
+int orc_surface_radius_count(const float* pos, uint32_t n, const uint32_t* ranges, orc_grid sp,
+                             float h, uint32_t* flags, uint32_t* count);
+
+/* Whole frame (method 0 = ZB05 occupancy band, 1 = Akinci12 colour-field band;
+ * | 2 = radius-count surface particles instead), as the reconstructors run it.  Positions and densities are sorted in place.
  * phi_out (float[V], may be NULL).  counts[6] = {surface particles, B, Q, 3T,
  * out of grid, 0}.  Returns 0 on success. */
 int orc_frame(int method, float* pos, float* dens, uint32_t n, orc_grid sp, orc_grid sc,

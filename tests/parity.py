@@ -43,11 +43,14 @@ def run_gpu(fr, method="zb05", rec=None):
 class Stage:
     """Oracle results of every stage of one frame (ZB05 or Akinci12)."""
 
-    def __init__(self, fr, method="zb05"):
+    def __init__(self, fr, method="zb05", radius_count=False):
         self.fr = fr
         h = np.float32(fr.h)
         self.hs = oracle.hash_sort(fr.pos, fr.dens, fr.sp)
-        if method == "akinci12":
+        if radius_count:
+            self.flags, self.count = oracle.surface_radius_count(
+                self.hs["pos"], self.hs["ranges"], fr.sp, h)
+        elif method == "akinci12":
             self.flags, self.gnorm, self.count = oracle.surface_colorfield(
                 self.hs["pos"], self.hs["dens"], self.hs["ranges"], fr.sp, h,
                 np.float32(fr.mass))

@@ -130,6 +130,35 @@ def test_colorfield_brute_force():
         assert flags[i] == (1 if (cnt[i] < 25 or gn[i] > 3.0) else 0)
 
 
+def test_surface_radius_count_brute_force():
+    """The radius-count surface test against an all-pairs count: jittered
+    lattice (spacing s) with h = 2.3 s, so that 0 < d < h covers the first
+    shells of neighbours; an interior particle of the unjittered lattice has
+    6 + 12 + 8 + 6 + 24 = 56 (|d|^2 in {1, 2, 3, 4, 5} s^2) of them (the
+    jitter moves a few of the 5 s^2 shell past h), a corner one far fewer.  The pair decision is the FP32 one, d2 =
+    fmaf(dz, dz, fmaf(dy, dy, dx * dx)) < h*h, emulated in float64 with one
+    rounding per fused operation."""
+    fr = ds.small_block(nx=9, ny=8, nz=7, spacing=0.01)
+    h = np.float32(0.023)
+    sp = ds.spatial_grid(fr.pos.min(axis=0), fr.pos.max(axis=0), float(h), 1.0)
+    r = oracle.hash_sort(fr.pos, fr.dens, sp)
+    flags, cnt = oracle.surface_radius_count(r["pos"], r["ranges"], sp, h)
+    p = r["pos"]
+    f32, f64 = np.float32, np.float64
+    h2 = f32(h * h)
+    for i in range(len(p)):
+        d = (p[i][None, :] - p).astype(f32)
+        t = f32(d[:, 0] * d[:, 0])
+        t = (d[:, 1].astype(f64) * d[:, 1] + t).astype(f32)
+        d2 = (d[:, 2].astype(f64) * d[:, 2] + t).astype(f32)
+        n = int(((d2 > 0) & (d2 < h2)).sum())
+        assert cnt[i] == n, i
+        assert flags[i] == (1 if n < 25 else 0)
+    # the lattice has both kinds: interior particles are not surface particles
+    assert 0 < flags.sum() < len(p)
+    assert 40 <= cnt.max() <= 56
+
+
 # ------------------------------------------------------------------ band -----
 
 def np_band_box(p, sc, h):
