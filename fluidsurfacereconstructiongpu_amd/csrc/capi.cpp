@@ -293,7 +293,10 @@ fsr_status frame_impl(const char* who, fsr_context* ctx, fsr_method method,
     if (method == FSR_METHOD_AKINCI12 || sa.radius_count) {
         FSR_TRY(surface_colorfield(ctx, sa, sim->particle_mass, s));
     } else {
-        FSR_TRY(surface_particles(ctx, sa, involve, s));
+        // the occupancy test per cell, then flags and band seeds per particle
+        uint8_t* surf_cell;
+        FSR_TRY(arena_typed(ctx, A_SURF_CELL, C, &surf_cell));
+        FSR_TRY(surface_occupancy(ctx, sa, surf_cell, involve, s));
     }
 
     // 3. band

@@ -409,10 +409,6 @@ fsr_status surface_occupancy(fsr_context* ctx, const SurfaceArgs& a, uint8_t* su
                              uint32_t* involve, hipStream_t s);
 fsr_status surface_colorfield(fsr_context* ctx, const SurfaceArgs& a,
                               float mass, hipStream_t s);
-// surface_occupancy for a whole frame in one pass over the particles (no
-// per-cell surface array)
-fsr_status surface_particles(fsr_context* ctx, const SurfaceArgs& a, uint32_t* involve,
-                             hipStream_t s);
 
 // band.hip ------------------------------------------------------------------
 struct BandArgs {

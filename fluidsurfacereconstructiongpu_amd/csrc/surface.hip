@@ -152,82 +152,6 @@ __global__ __launch_bounds__(256) void k_flags_from_cells(SurfaceArgs a,
     if (a.st) block_count_add2(&a.st->num_surface_particles, n_surf, &a.st->band_exceptions, n_exc);
 }
 
-// The ZB05 / Yu13 surface particles in one pass over the particles: a particle
-// is a surface particle iff its hash cell is a surface cell (not on the last
-// layer, an empty cell in its clamped 3^3 block: extractionOfSurfaceParticles,
-// ReconstructionCUDA.cu:70-120); the lanes of a wave mostly share one or two
-// cells, so the 27 occupancy reads of a lane coalesce with its neighbours'.
-// With `involve`, the first particle of each surface cell writes the particle
-// counts of the cells in its clamped 5^3 block
-// (extractionOfSurfaceAndInvolveParticles, ReconstructionCUDA.cu:466-551).
-// Then band seeding and the frame's counts, as k_flags_from_cells.
-__global__ __launch_bounds__(256) void k_surface_particles(SurfaceArgs a,
-                                                           uint32_t* __restrict__ involve) {
-    const GridDev sp = a.sp;
-    const uint32_t C = sp.rx * sp.ry * sp.rz;
-    const uint32_t span = blockDim.x * kPerThread;
-    uint32_t n_surf = 0, n_exc = 0;
-    for (uint32_t base = blockIdx.x * span; base < a.n; base += gridDim.x * span) {
-        uint32_t key[kPerThread];
-        float4 p[kPerThread];
-#pragma unroll
-        for (uint32_t k = 0; k < kPerThread; ++k) {
-            const uint32_t i = base + k * blockDim.x + threadIdx.x;
-            key[k] = C;
-            p[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (i < a.n) {
-                key[k] = a.keys[i];
-                p[k] = a.sorted[i];
-            }
-        }
-        bool surf[kPerThread];
-#pragma unroll
-        for (uint32_t k = 0; k < kPerThread; ++k) {
-            const uint32_t i = base + k * blockDim.x + threadIdx.x;
-            const uint32_t c = key[k];
-            bool s = false;
-            if (c < C) {
-                const uint32_t x = c % sp.rx, y = (c / sp.rx) % sp.ry, z = c / (sp.rx * sp.ry);
-                if (x < sp.rx - 1 && y < sp.ry - 1 && z < sp.rz - 1) {
-                    const int x0 = clampi((int)x - 1, 0, sp.rx - 1), x1 = (int)x + 1;
-                    const int y0 = clampi((int)y - 1, 0, sp.ry - 1), y1 = (int)y + 1;
-                    const int z0 = clampi((int)z - 1, 0, sp.rz - 1), z1 = (int)z + 1;
-                    for (int zz = z0; zz <= z1 && !s; ++zz)
-                        for (int yy = y0; yy <= y1 && !s; ++yy)
-                            for (int xx = x0; xx <= x1; ++xx)
-                                if (a.occupancy[((uint32_t)zz * sp.ry + yy) * sp.rx + xx] < 1.0f) {
-                                    s = true;
-                                    break;
-                                }
-                }
-                if (involve && s && a.ranges[c].start == i) {
-                    const int x0 = clampi((int)x - 2, 0, sp.rx - 1), x1 = clampi((int)x + 2, 0, sp.rx - 1);
-                    const int y0 = clampi((int)y - 2, 0, sp.ry - 1), y1 = clampi((int)y + 2, 0, sp.ry - 1);
-                    const int z0 = clampi((int)z - 2, 0, sp.rz - 1), z1 = clampi((int)z + 2, 0, sp.rz - 1);
-                    for (int zz = z0; zz <= z1; ++zz)
-                        for (int yy = y0; yy <= y1; ++yy)
-                            for (int xx = x0; xx <= x1; ++xx) {
-                                const uint32_t cc = ((uint32_t)zz * sp.ry + yy) * sp.rx + xx;
-                                const fsr_index_range r = a.ranges[cc];
-                                // same value from every writer
-                                if (r.start != 0xFFFFFFFFu) involve[cc] = r.end - r.start;
-                            }
-                }
-            }
-            surf[k] = s;
-            if (i < a.n) a.flags[i] = s ? 1u : 0u;
-        }
-#pragma unroll
-        for (uint32_t k = 0; k < kPerThread; ++k) {
-            const uint32_t i = base + k * blockDim.x + threadIdx.x;
-            const bool exc = a.sbits ? seed_particle(a, i, p[k].x, p[k].y, p[k].z, surf[k]) : false;
-            n_surf += surf[k] ? 1u : 0u;
-            n_exc += exc ? 1u : 0u;
-        }
-    }
-    if (a.st) block_count_add2(&a.st->num_surface_particles, n_surf, &a.st->band_exceptions, n_exc);
-}
-
 // Particles outside the hash grid (sorted last, in no cell): one thread each,
 // over the clamped block around their own clamped cell.  Grid-stride over the
 // device-side count; the surface / exception counts are added once per block.
@@ -550,18 +474,6 @@ fsr_status surface_occupancy(fsr_context* ctx, const SurfaceArgs& a, uint8_t* su
         hipLaunchKernelGGL(k_flags_from_cells, dim3(count_blocks(ctx, a.n)), dim3(256), 0, s, a,
                            (const uint8_t*)surf_cell);
         FSR_LAUNCH_CHECK("k_flags_from_cells");
-    }
-    return FSR_OK;
-}
-
-fsr_status surface_particles(fsr_context* ctx, const SurfaceArgs& a, uint32_t* involve,
-                             hipStream_t s) {
-    const uint32_t C = a.sp.rx * a.sp.ry * a.sp.rz;
-    if (involve) FSR_TRY(fill_u32(involve, 0u, C, s));
-    if (a.n) {
-        hipLaunchKernelGGL(k_surface_particles, dim3(count_blocks(ctx, a.n)), dim3(256), 0, s, a,
-                           involve);
-        FSR_LAUNCH_CHECK("k_surface_particles");
     }
     return FSR_OK;
 }
