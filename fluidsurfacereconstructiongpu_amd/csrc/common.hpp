@@ -132,26 +132,37 @@ __device__ __forceinline__ uint32_t clip_word(uint32_t word, uint32_t w, uint32_
 }
 
 // XCD-aware dynamic work fetch for persistent kernels.  The work list [0, total)
-// is cut into 8 contiguous segments; workgroups are dispatched round-robin over
-// the 8 XCDs, so blockIdx % 8 names the XCD, and a wave takes items from its
-// XCD's segment first (neighbouring items share particle data, which then stays
-// in that XCD's L2), then from the other segments.  `step` (0 at start) is the
-// wave's position in that order.  The 8 counters next8[32 k] sit on separate
-// 128-byte lines (sharing one line across XCDs serialises the fetches) and start
-// at 0.  Returns total when every item is taken.  Wave-uniform.
-__device__ __forceinline__ uint32_t next_item_xcd(uint32_t* next8, uint32_t total,
+// is cut into 64 contiguous sub-segments, 8 per XCD in list order (XCD x owns
+// the x-th eighth of the list); workgroups are dispatched round-robin over the
+// 8 XCDs, so blockIdx % 8 names the XCD.  A wave takes items from its XCD's
+// sub-segments first (neighbouring items share particle data, which then stays
+// in that XCD's L2), starting at the sub-segment its workgroup index picks, then
+// from the other XCDs' eighths.  `step` (0 at start) is the wave's position in
+// that order.  Same-address atomics serialise (about 30 ns each here), so the
+// list has 64 counters, next64[32 k] on separate 128-byte lines, all starting at
+// 0, and a sub-segment seen exhausted by a plain (possibly stale, never early)
+// read is passed without an atomic.  Returns total when every item is taken.
+// Wave-uniform.
+constexpr uint32_t kFetchSegments = 64;
+
+__device__ __forceinline__ uint32_t next_item_xcd(uint32_t* next64, uint32_t total,
                                                   uint32_t& step) {
     const uint32_t xcd = blockIdx.x & 7u;
-    const uint32_t seg = (total + 7u) >> 3;
-    while (step < 8u) {
-        const uint32_t k = (xcd + step) & 7u;
+    const uint32_t sub0 = (blockIdx.x >> 3) & 7u;
+    const uint32_t seg = (total + kFetchSegments - 1u) / kFetchSegments;
+    while (step < kFetchSegments) {
+        const uint32_t k = ((xcd + (step >> 3)) & 7u) * 8u + ((sub0 + step) & 7u);
         const uint32_t lo = k * seg;
         const uint32_t hi = min(lo + seg, total);
         if (lo < hi) {
-            uint32_t idx = 0;
-            if (__lane_id() == 0) idx = atomicAdd(&next8[32u * k], 1u);
-            idx = __builtin_amdgcn_readfirstlane(idx);
-            if (lo + idx < hi) return lo + idx;
+            uint32_t* ctr = &next64[32u * k];
+            const uint32_t seen = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (seen < hi - lo) {
+                uint32_t idx = 0;
+                if (__lane_id() == 0) idx = atomicAdd(ctr, 1u);
+                idx = __builtin_amdgcn_readfirstlane(idx);
+                if (lo + idx < hi) return lo + idx;
+            }
         }
         ++step;
     }
@@ -243,9 +254,11 @@ struct Counters {
     uint32_t mc_next;
     uint32_t aux[12];
     uint32_t pad[12];
-    uint32_t xcd_next[8 * 32];  // per-XCD fetch counters of the field kernels, 128 B apart
-    uint32_t xcd_field[8 * 32]; // the same for the ZB05 field kernel, used once per frame,
-                                // so the frame's initial clear is its reset
+    // fetch counters of the persistent kernels (next_item_xcd), 128 B apart
+    uint32_t xcd_next[kFetchSegments * 32];
+    // the same for the ZB05 field kernel, used once per frame, so the frame's
+    // initial clear is its reset
+    uint32_t xcd_field[kFetchSegments * 32];
 };
 
 }  // namespace fsr
