@@ -424,11 +424,16 @@ __device__ __forceinline__ void nb_tiles(const float* __restrict__ rows, uint32_
         for (int T = 0; T < NT; ++T) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const float s2 = fmaxf(w[T][r], 0.0f);
+                // s2 may lie a rounding error (|s2| < 1e-6) below 0 for a
+                // candidate at the particle itself: it counts (s2 <= le_s2), and
+                // 1 + |s2|^1.5 clamps to 1 exactly as s2 = 0 would.  (No max
+                // with 0: on a matrix result it costs a canonicalisation too;
+                // the absolute value is a free source modifier of the square
+                // root, and the clamp one of the multiply-add.)
+                const float s2 = w[T][r];
                 if (kCov) cnt[T] += s2 <= le_s2 ? 1u : 0u;
-                // 1 - s2 sqrt(s2) <= 1: the clamp to [0, 1] is max(., 0)
                 w[T][r] = __builtin_amdgcn_fmed3f(
-                    __builtin_fmaf(-s2, __builtin_amdgcn_sqrtf(s2), 1.0f), 0.0f, 1.0f);
+                    __builtin_fmaf(-s2, __builtin_amdgcn_sqrtf(fabsf(s2)), 1.0f), 0.0f, 1.0f);
             }
         }
         const float4 a2 = *reinterpret_cast<const float4*>(a2p + tb);
