@@ -32,10 +32,12 @@
 // r^T M r with M = G^T G stored per particle (6 values): with v the vertex and
 // p the particle relative to the cell corner, |G(p - v)|^2 =
 // v^T M v - 2 v.(M p) + p^T M p, where M p and p^T M p are formed once per staged
-// candidate; the per-pair cost is 9 fused multiply-adds.  Particles of
+// candidate; eight of its nine vertex terms are two K = 4 matrix products per
+// 16 x 16 tile, the ninth one multiply-add (see aniso_tiles).  Particles of
 // non-involved cells carry M = I and the unscaled kernel, as the reference has
 // it; the field sums them through per-block moments (noninv_poly).
 #include <cmath>
+#include <type_traits>
 #include <limits>
 
 #include "common.hpp"
@@ -831,59 +833,74 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
-struct VertexTerms {
-    float xx, yy, zz, xy2, xz2, yz2, mx2, my2, mz2;
-};
+// LDS queue of one field wave: a ring of kAfQueue staged candidates, stored
+// as rows of kAfStride floats (structure of arrays).  Rows 0-3 and 4-7 are the
+// A operands of the two matrix products, (M00, M11, M22, M01) and (M02, M12,
+// bx, by) with b = M p; row 8 holds 1 - p^T M p, row 9 bz, row 10 the
+// coefficient c (p relative to the chunk's cell corner).  At a chunk's start
+// the ring is empty, and the upper half of rows 0-8 holds the chunk's vertex
+// operands until they are in registers.
+constexpr int kAfStride = 144;
+constexpr int kAfQueue = 128;
+constexpr int kAfOneMinusC = 8, kAfBz = 9, kAfCoef = 10, kAfRows = 11;
 
-// n staged candidates (a multiple of 8, at most 64) against this lane's
-// vertex, added to `sum` as one partial sum.  A vertex sums thousands of
-// candidates, most of them particles of non-involved cells whose weights are
-// all close to sigma m / rho (about 2 ulp of a sum near 1): added one by one
-// into the running FP32 sum, their rounding has the same sign every time and
-// drifts the field by ~1e-4 over a frame of the configured size, while a batch
-// partial sums them at its own, much smaller, magnitude.
-__device__ __forceinline__ float aniso_pairs(const float4* __restrict__ q0s,
-                                             const float4* __restrict__ q1s,
-                                             const float4* __restrict__ q2s, uint32_t n,
-                                             const VertexTerms& v, float sum) {
-    float part = 0.0f;
-    for (uint32_t t0 = 0; t0 < n; t0 += 8u) {
+// Sums c max(1 - |G (p - x)|^2, 0)^3 of n staged candidates (a multiple of 16)
+// for the NT 16-vertex tiles of a chunk.  With x relative to the same corner,
+//   |G (p - x)|^2 = p^T M p + [M00 M11 M22 M01].[xx yy zz 2xy]
+//                 + [M02 M12 bx by].[2xz 2yz -2x -2y] - 2 bz z,
+// so per 16 candidates and vertex tile two v_mfma_f32_16x16x4_f32 give
+// D = 1 - p^T M p - (the first eight terms) (C = the candidates' 1 - p^T M p,
+// B = the vertex terms negated), and e = D + bz (2z) takes one multiply-add
+// whose clamp bit gives max(., 0) (e exceeds 1 only by rounding, for a vertex
+// at the particle).  Lane l holds the pairs of candidates 4(l/16)+r (r = 0..3)
+// with vertex l%16 of each tile and adds c e^3 to part[T]; the caller adds
+// the partial sums of the four lane groups of a vertex.
+template <int NT>
+__device__ __forceinline__ void aniso_tiles(const float* __restrict__ rows, uint32_t n,
+                                            const float (&b1)[NT], const float (&b2)[NT],
+                                            const float (&z2)[NT], float (&part)[NT]) {
+    const uint32_t lane = __lane_id();
+    const float* a1p = rows + (lane >> 4) * kAfStride + (lane & 15u);
+    const float* a2p = rows + (4u + (lane >> 4)) * kAfStride + (lane & 15u);
+    const float* cp = rows + kAfOneMinusC * kAfStride + 4u * (lane >> 4);
+    const float* zp = rows + kAfBz * kAfStride + 4u * (lane >> 4);
+    const float* kp = rows + kAfCoef * kAfStride + 4u * (lane >> 4);
+    for (uint32_t tb = 0; tb < n; tb += 16u) {
+        const float a1 = a1p[tb];
+        const float a2 = a2p[tb];
+        const float4 c4 = *reinterpret_cast<const float4*>(cp + tb);
+        const float4 z4 = *reinterpret_cast<const float4*>(zp + tb);
+        const float4 k4 = *reinterpret_cast<const float4*>(kp + tb);
+        const v4f c = v4f{c4.x, c4.y, c4.z, c4.w};
+        const float bzr[4] = {z4.x, z4.y, z4.z, z4.w};
+        const float kr[4] = {k4.x, k4.y, k4.z, k4.w};
+        v4f d[NT];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const float4 q0 = q0s[t0 + t];
-            const float4 q1 = q1s[t0 + t];
-            const float4 q2 = q2s[t0 + t];
-            // |G(p - v)|^2 = v^T M v - 2 v.(M p) + p^T M p
-            float d2 = __builtin_fmaf(q0.x, v.xx, q2.y);
-            d2 = __builtin_fmaf(q0.y, v.yy, d2);
-            d2 = __builtin_fmaf(q0.z, v.zz, d2);
-            d2 = __builtin_fmaf(q0.w, v.xy2, d2);
-            d2 = __builtin_fmaf(q1.x, v.xz2, d2);
-            d2 = __builtin_fmaf(q1.y, v.yz2, d2);
-            d2 = __builtin_fmaf(q1.z, v.mx2, d2);
-            d2 = __builtin_fmaf(q1.w, v.my2, d2);
-            d2 = __builtin_fmaf(q2.x, v.mz2, d2);
-            // max(1 - d2, 0) as a clamp to [0, 1] (the subtraction's clamp
-            // bit): 1 - d2 exceeds 1 only where rounding made d2 < 0
-            const float x = __builtin_amdgcn_fmed3f(1.0f - d2, 0.0f, 1.0f);
-            part = __builtin_fmaf(q2.z, x * x * x, part);
+        for (int T = 0; T < NT; ++T) d[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1[T], c, 0, 0, 0);
+#pragma unroll
+        for (int T = 0; T < NT; ++T) d[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, b2[T], d[T], 0, 0, 0);
+#pragma unroll
+        for (int T = 0; T < NT; ++T) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float e = __builtin_amdgcn_fmed3f(__builtin_fmaf(bzr[r], z2[T], d[T][r]),
+                                                        0.0f, 1.0f);
+                part[T] = __builtin_fmaf(kr[r], e * e * e, part[T]);
+            }
         }
     }
-    return sum + part;
 }
 
 // kWaves: the resident waves per SIMD the register allocation is bounded for
-// (5: 96 VGPRs, a few spilled; 4: 128 VGPRs, none spilled).
+// (5: 96 VGPRs; 4: 128 VGPRs).
 template <int kWaves>
 __global__ __launch_bounds__(kAnisoThreads, kWaves) void k_field_aniso(AnisoFieldArgs a) {
-    __shared__ float4 queue[kAnisoThreads / 64][3][128];
+    __shared__ __align__(16) float queue[kAnisoThreads / 64][kAfRows * kAfStride];
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t nch = *a.num_chunks;
     const int R = a.bc.radius;
-    float4* q0s = queue[wave][0];
-    float4* q1s = queue[wave][1];
-    float4* q2s = queue[wave][2];
+    float* q = queue[wave];
     uint32_t step = 0;
     uint32_t n_staged = 0, n_kept = 0;
     for (;;) {
@@ -911,10 +928,17 @@ __global__ __launch_bounds__(kAnisoThreads, kWaves) void k_field_aniso(AnisoFiel
             vy = vertex_coord(iy, a.sc.miny, a.sc.cell) - oy;
             vz = vertex_coord(iz, a.sc.minz, a.sc.cell) - oz;
         }
-        VertexTerms vt;
-        vt.xx = vx * vx; vt.yy = vy * vy; vt.zz = vz * vz;
-        vt.xy2 = 2.0f * vx * vy; vt.xz2 = 2.0f * vx * vz; vt.yz2 = 2.0f * vy * vz;
-        vt.mx2 = -2.0f * vx; vt.my2 = -2.0f * vy; vt.mz2 = -2.0f * vz;
+        // the vertex operands (negated terms, and 2z) into the ring's upper half
+        float* vt = q + kAfQueue / 2;
+        vt[0 * kAfStride + lane] = -(vx * vx);
+        vt[1 * kAfStride + lane] = -(vy * vy);
+        vt[2 * kAfStride + lane] = -(vz * vz);
+        vt[3 * kAfStride + lane] = -(2.0f * vx * vy);
+        vt[4 * kAfStride + lane] = -(2.0f * vx * vz);
+        vt[5 * kAfStride + lane] = -(2.0f * vy * vz);
+        vt[6 * kAfStride + lane] = 2.0f * vx;
+        vt[7 * kAfStride + lane] = 2.0f * vy;
+        vt[8 * kAfStride + lane] = 2.0f * vz;
         const float bx0 = wave_min(live ? vx : 3.0e38f), bx1 = wave_max(live ? vx : -3.0e38f);
         const float by0 = wave_min(live ? vy : 3.0e38f), by1 = wave_max(live ? vy : -3.0e38f);
         const float bz0 = wave_min(live ? vz : 3.0e38f), bz1 = wave_max(live ? vz : -3.0e38f);
@@ -933,104 +957,153 @@ __global__ __launch_bounds__(kAnisoThreads, kWaves) void k_field_aniso(AnisoFiel
         }
         const uint32_t total = br.total;
         n_staged += total;
+        wave_lds_sync();
+        const int nt = (int)((cnt + 15u) >> 4);
         float sum = 0.f;
-        uint32_t fill = 0;
-        // Staging in two steps.  Each lane's candidate position (with its
-        // squared culling radius in w) is loaded one round ahead; after the
-        // box test, the two record words are loaded for the survivors only
-        // (a dropped candidate reads record 0, one shared line) and enter the
-        // queue one round later, so their load overlaps the next round's
-        // test.  Queue order stays candidate order.
-        float4 np4 = make_float4(0.f, 0.f, 0.f, 0.f);
-        uint32_t nj;
-        {
-            nj = br.offset(0u, lane) + lane;
-            if (lane < total) np4 = a.smooth[nj];
-        }
-        bool keep_p = false;
-        float px_p = 0.f, py_p = 0.f, pz_p = 0.f;
-        float4 r0_p = make_float4(0.f, 0.f, 0.f, 0.f), r1_p = r0_p;
-        for (uint32_t base = 0; base < total + 64u; base += 64u) {
-            const uint32_t g = base + lane;
-            bool keep = false;
-            float px = 0.f, py = 0.f, pz = 0.f;
-            float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0;
-            if (base < total) {
-                const float4 p4 = np4;
-                const uint32_t j = nj;
-                if (base + 64u < total) {
-                    nj = br.offset(base + 64u, g + 64u) + g + 64u;
-                    if (g + 64u < total) np4 = a.smooth[nj];
+        // The rest of the chunk, compiled once per tile count NT, so that the
+        // vertex operands and sums keep fixed registers across the staging.
+        auto run = [&](auto ntc) {
+            constexpr int NT = decltype(ntc)::value;
+            // lane l: component l/16 of the operands of vertex 16 T + l%16
+            float b1[NT], b2[NT], z2[NT], tot[NT];
+            {
+                const float* src = vt + (lane >> 4) * kAfStride + (lane & 15u);
+#pragma unroll
+                for (int T = 0; T < NT; ++T) {
+                    b1[T] = src[16 * T];
+                    b2[T] = src[4 * kAfStride + 16 * T];
+                    z2[T] = vt[8 * kAfStride + 16 * T + (lane & 15u)];
+                    tot[T] = 0.f;
                 }
-                if (g < total) {
-                    px = p4.x - ox;
-                    py = p4.y - oy;
-                    pz = p4.z - oz;
-                    const float dx = px - __builtin_amdgcn_fmed3f(px, bx0, bx1);
-                    const float dy = py - __builtin_amdgcn_fmed3f(py, by0, by1);
-                    const float dz = pz - __builtin_amdgcn_fmed3f(pz, bz0, bz1);
-                    keep = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < p4.w;
-                }
-                const uint32_t jr = keep ? j : 0u;
-                r0 = a.rec[2ull * jr + 0];
-                r1 = a.rec[2ull * jr + 1];
             }
-            if (base == 0) {
+            wave_lds_sync();
+            // Staging in two steps.  Each lane's candidate position (with its
+            // squared culling radius in w) is loaded one round ahead; after
+            // the box test, the two record words are loaded for the survivors
+            // only (a dropped candidate reads record 0, one shared line) and
+            // enter the queue one round later, so their load overlaps the next
+            // round's test.  Queue order stays candidate order.  Entries
+            // [head, head + fill) of the ring wait for evaluation; head is 0
+            // or 64.
+            uint32_t head = 0, fill = 0;
+            float4 np4 = make_float4(0.f, 0.f, 0.f, 0.f);
+            uint32_t nj;
+            {
+                nj = br.offset(0u, lane) + lane;
+                if (lane < total) np4 = a.smooth[nj];
+            }
+            bool keep_p = false;
+            float px_p = 0.f, py_p = 0.f, pz_p = 0.f;
+            float4 r0_p = make_float4(0.f, 0.f, 0.f, 0.f), r1_p = r0_p;
+            for (uint32_t base = 0; base < total + 64u; base += 64u) {
+                const uint32_t g = base + lane;
+                bool keep = false;
+                float px = 0.f, py = 0.f, pz = 0.f;
+                float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0;
+                if (base < total) {
+                    const float4 p4 = np4;
+                    const uint32_t j = nj;
+                    if (base + 64u < total) {
+                        nj = br.offset(base + 64u, g + 64u) + g + 64u;
+                        if (g + 64u < total) np4 = a.smooth[nj];
+                    }
+                    if (g < total) {
+                        px = p4.x - ox;
+                        py = p4.y - oy;
+                        pz = p4.z - oz;
+                        const float dx = px - __builtin_amdgcn_fmed3f(px, bx0, bx1);
+                        const float dy = py - __builtin_amdgcn_fmed3f(py, by0, by1);
+                        const float dz = pz - __builtin_amdgcn_fmed3f(pz, bz0, bz1);
+                        keep = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < p4.w;
+                    }
+                    const uint32_t jr = keep ? j : 0u;
+                    r0 = a.rec[2ull * jr + 0];
+                    r1 = a.rec[2ull * jr + 1];
+                }
+                if (base == 0) {
+                    keep_p = keep;
+                    px_p = px; py_p = py; pz_p = pz;
+                    r0_p = r0; r1_p = r1;
+                    continue;
+                }
+                // the previous round's survivors enter the queue
+                const uint64_t m = __ballot(keep_p);
+                if (keep_p) {
+                    const float M00 = r0_p.x, M11 = r0_p.y, M22 = r0_p.z, M01 = r0_p.w;
+                    const float M02 = r1_p.x, M12 = r1_p.y;
+                    const float bx = M00 * px_p + M01 * py_p + M02 * pz_p;
+                    const float by = M01 * px_p + M11 * py_p + M12 * pz_p;
+                    const float bz = M02 * px_p + M12 * py_p + M22 * pz_p;
+                    const float cc = px_p * bx + py_p * by + pz_p * bz;
+                    const uint32_t o = (head + fill + mask_rank(m)) & (kAfQueue - 1);
+                    q[0 * kAfStride + o] = M00;
+                    q[1 * kAfStride + o] = M11;
+                    q[2 * kAfStride + o] = M22;
+                    q[3 * kAfStride + o] = M01;
+                    q[4 * kAfStride + o] = M02;
+                    q[5 * kAfStride + o] = M12;
+                    q[6 * kAfStride + o] = bx;
+                    q[7 * kAfStride + o] = by;
+                    q[kAfOneMinusC * kAfStride + o] = 1.0f - cc;
+                    q[kAfBz * kAfStride + o] = bz;
+                    q[kAfCoef * kAfStride + o] = r1_p.z;
+                }
                 keep_p = keep;
                 px_p = px; py_p = py; pz_p = pz;
                 r0_p = r0; r1_p = r1;
-                continue;
-            }
-            // the previous round's survivors enter the queue
-            float4 s0, s1, s2;
-            if (keep_p) {
-                const float M00 = r0_p.x, M11 = r0_p.y, M22 = r0_p.z, M01 = r0_p.w;
-                const float M02 = r1_p.x, M12 = r1_p.y;
-                const float bx = M00 * px_p + M01 * py_p + M02 * pz_p;
-                const float by = M01 * px_p + M11 * py_p + M12 * pz_p;
-                const float bz = M02 * px_p + M12 * py_p + M22 * pz_p;
-                const float cc = px_p * bx + py_p * by + pz_p * bz;
-                s0 = make_float4(M00, M11, M22, M01);
-                s1 = make_float4(M02, M12, bx, by);
-                s2 = make_float4(bz, cc, r1_p.z, 0.f);
-            }
-            const uint64_t m = __ballot(keep_p);
-            if (keep_p) {
-                const uint32_t o = fill + mask_rank(m);
-                q0s[o] = s0;
-                q1s[o] = s1;
-                q2s[o] = s2;
-            }
-            keep_p = keep;
-            px_p = px; py_p = py; pz_p = pz;
-            r0_p = r0; r1_p = r1;
-            fill += (uint32_t)__popcll(m);
-            n_kept += (uint32_t)__popcll(m);
-            if (fill >= 64u) {
-                wave_lds_sync();
-                sum = aniso_pairs(q0s, q1s, q2s, 64u, vt, sum);
-                wave_lds_sync();
-                const uint32_t rem = fill - 64u;
-                if (lane < rem) {
-                    q0s[lane] = q0s[64u + lane];
-                    q1s[lane] = q1s[64u + lane];
-                    q2s[lane] = q2s[64u + lane];
+                fill += (uint32_t)__popcll(m);
+                n_kept += (uint32_t)__popcll(m);
+                if (fill >= 64u) {
+                    // one batch of 64: its own partial sums, added to the
+                    // running totals (a vertex sums many similar weights; a
+                    // batch sums them at its own, smaller, magnitude)
+                    float part[NT];
+#pragma unroll
+                    for (int T = 0; T < NT; ++T) part[T] = 0.f;
+                    wave_lds_sync();
+                    aniso_tiles<NT>(q + head, 64u, b1, b2, z2, part);
+                    wave_lds_sync();
+#pragma unroll
+                    for (int T = 0; T < NT; ++T) tot[T] += part[T];
+                    head ^= 64u;
+                    fill -= 64u;
                 }
+            }
+            if (fill) {
+                // pad to a multiple of 16 with entries of weight 0 (1 - p^T M p
+                // = -1, every other row 0); head + 64 <= kAfQueue, so the pad
+                // does not wrap
+                const uint32_t n = (fill + 15u) & ~15u;
+                if (lane >= fill && lane < n) {
+#pragma unroll
+                    for (int rr = 0; rr < kAfRows; ++rr)
+                        q[rr * kAfStride + head + lane] = rr == kAfOneMinusC ? -1.0f : 0.0f;
+                }
+                float part[NT];
+#pragma unroll
+                for (int T = 0; T < NT; ++T) part[T] = 0.f;
                 wave_lds_sync();
-                fill = rem;
+                aniso_tiles<NT>(q + head, n, b1, b2, z2, part);
+                wave_lds_sync();
+#pragma unroll
+                for (int T = 0; T < NT; ++T) tot[T] += part[T];
             }
-        }
-        if (fill) {
-            const uint32_t n = (fill + 7u) & ~7u;
-            if (lane >= fill && lane < n) {
-                // padding: |G r|^2 = 2 (outside the support) and zero weight
-                q0s[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
-                q1s[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
-                q2s[lane] = make_float4(0.f, 2.0f, 0.f, 0.f);
+            // add the four lane groups' sums of each vertex; lane l keeps those
+            // of vertex 16 (l/16) + l%16 = l
+            const uint32_t my = lane >> 4;
+#pragma unroll
+            for (int T = 0; T < NT; ++T) {
+                float x = tot[T];
+                x += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(x), 0x1F | (16 << 10)));
+                x += __shfl_xor(x, 32, 64);
+                if (my == (uint32_t)T) sum = x;
             }
-            wave_lds_sync();
-            sum = aniso_pairs(q0s, q1s, q2s, n, vt, sum);
-            wave_lds_sync();
+        };
+        switch (nt) {
+            case 1: run(std::integral_constant<int, 1>{}); break;
+            case 2: run(std::integral_constant<int, 2>{}); break;
+            case 3: run(std::integral_constant<int, 3>{}); break;
+            default: run(std::integral_constant<int, 4>{}); break;
         }
         if (live) {
             if (a.block_mom) {
