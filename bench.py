@@ -224,7 +224,10 @@ def work_sums(rec, name):
 
 def flop_roofline(kernel, flops, ms, **extra):
     achieved = flops / (ms * 1e-3) / 1e12
-    out = {"bound": "mfma" if kernel == "k_field_zb05" else "valu", "kernel": kernel,
+    # the pair tiles of these kernels run on the matrix cores (with VALU work
+    # between the products); the colour field is VALU code
+    mfma = kernel in ("k_field_zb05", "k_field_aniso", "k_smooth + k_covariance")
+    out = {"bound": "mfma" if mfma else "valu", "kernel": kernel,
            "achieved": achieved, "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
            "frac": achieved / (FP32_PEAK / 1e12), "flops_per_launch": float(flops),
            "kernel_ms": ms}

@@ -294,6 +294,7 @@ struct CovArgs {
     float4* sigma;          // [I] eigenvalues, descending
     float4* rec;            // [2N] field records
     uint32_t* work;         // work records (may be null)
+    const float4* cell_box; // per cell lo[C], hi[C] (k_cell_box)
 };
 
 // G and the field record of involved particle i (its involved index gi) from
@@ -618,6 +619,58 @@ __device__ __forceinline__ void nb_setup(bool live, float x, float y, float z, f
     rel[2] = pz;
 }
 
+// Per hash cell, the box of its particles (sorted positions) for the cell test
+// of the smoothing and covariance kernels (CellCull): widened by 1e-6 of the
+// largest coordinate (the test runs in absolute coordinates, the particle
+// test relative to the chunk's centre), w = r^2 kRadiusCull (1.002) + 1e-12
+// for an occupied cell, -1 for an empty one.  One wave per cell.
+__global__ __launch_bounds__(64) void k_cell_box(const fsr_index_range* __restrict__ ranges,
+                                                 const float4* __restrict__ sorted, float w,
+                                                 float4* __restrict__ lo,
+                                                 float4* __restrict__ hi) {
+    const uint32_t c = blockIdx.x;
+    const uint32_t lane = threadIdx.x;
+    const fsr_index_range r = ranges[c];
+    float4 l = make_float4(3.0e38f, 3.0e38f, 3.0e38f, -1.0f);
+    float4 u = make_float4(-3.0e38f, -3.0e38f, -3.0e38f, 0.0f);
+    if (r.start != 0xFFFFFFFFu) {
+        for (uint32_t j = r.start + lane; j < r.end; j += 64u) {
+            const float4 p = sorted[j];
+            l.x = fminf(l.x, p.x); l.y = fminf(l.y, p.y); l.z = fminf(l.z, p.z);
+            u.x = fmaxf(u.x, p.x); u.y = fmaxf(u.y, p.y); u.z = fmaxf(u.z, p.z);
+        }
+    }
+    l.x = wave_minf(l.x); l.y = wave_minf(l.y); l.z = wave_minf(l.z);
+    u.x = wave_maxf(u.x); u.y = wave_maxf(u.y); u.z = wave_maxf(u.z);
+    if (lane == 0) {
+        if (r.start != 0xFFFFFFFFu && r.end > r.start) {
+            l.w = w;
+            const float e = 1.0e-6f * fmaxf(fmaxf(fmaxf(fabsf(l.x), fabsf(u.x)),
+                                                  fmaxf(fabsf(l.y), fabsf(u.y))),
+                                            fmaxf(fabsf(l.z), fabsf(u.z)));
+            l.x -= e; l.y -= e; l.z -= e;
+            u.x += e; u.y += e; u.z += e;
+        }
+        lo[c] = l;
+        hi[c] = u;
+    }
+}
+
+// The cell test of a chunk: the cells of its block whose particle box lies
+// within the radius of the chunk's box [o + bb0, o + bb1].
+__device__ __forceinline__ CellCull chunk_cull(const float4* box, uint32_t C, const float* o,
+                                               const float* bb0, const float* bb1) {
+    CellCull cc;
+    cc.lo = box;
+    cc.hi = box + C;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        cc.b0[d] = bb0[d] + o[d];
+        cc.b1[d] = bb1[d] + o[d];
+    }
+    return cc;
+}
+
 template <bool kCov>
 struct NbLds {
     float queue[NbRows<kCov>::kRows * kNbStride];
@@ -626,8 +679,8 @@ struct NbLds {
 
 __global__ __launch_bounds__(kAnisoThreads, 5) void k_smooth_mc(
     const float4* __restrict__ sorted, const fsr_index_range* __restrict__ ranges, GridDev sp,
-    float radius, float lambda, ChunkList cl, float4* __restrict__ mean_out,
-    float4* __restrict__ smooth_out, uint32_t* __restrict__ work) {
+    float radius, float lambda, ChunkList cl, const float4* __restrict__ cell_box,
+    float4* __restrict__ mean_out, float4* __restrict__ smooth_out, uint32_t* __restrict__ work) {
     __shared__ __align__(16) NbLds<false> lds[kAnisoThreads / 64];
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x >> 6;
@@ -648,13 +701,15 @@ __global__ __launch_bounds__(kAnisoThreads, 5) void k_smooth_mc(
         const uint32_t i = own.start + 64u * k + lane;
         const bool live = i < own.end;
         const int cx = (int)(c % sp.rx), cy = (int)((c / sp.rx) % sp.ry), cz = (int)(c / (sp.rx * sp.ry));
-        BlockRows br = block_rows<2>(ranges, sp, cx, cy, cz);
-        const uint32_t total = br.total;
-        n_staged += total;
         float4 self = make_float4(0.f, 0.f, 0.f, 0.f);
         if (live) self = sorted[i];
         float o[3], bb0[3], bb1[3], rel[3];
         nb_setup(live, self.x, self.y, self.z, inv_r2, vt, o, bb0, bb1, rel);
+        // the block's rows, without the cells that cannot reach the chunk
+        const CellCull cc = chunk_cull(cell_box, sp.rx * sp.ry * sp.rz, o, bb0, bb1);
+        BlockRows br = block_rows<2>(ranges, sp, cx, cy, cz, &cc);
+        const uint32_t total = br.total;
+        n_staged += total;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -711,13 +766,15 @@ __global__ __launch_bounds__(kAnisoThreads, 4) void k_covariance_mc(CovArgs a, C
         const bool live = i < own.end;
         const int cx = (int)(c % a.sp.rx), cy = (int)((c / a.sp.rx) % a.sp.ry),
                   cz = (int)(c / (a.sp.rx * a.sp.ry));
-        BlockRows br = block_rows<2>(a.ranges, a.sp, cx, cy, cz);
-        const uint32_t total = br.total;
-        n_staged += total;
         float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
         if (live) m = a.mean[i];
         float o[3], bb0[3], bb1[3], rel[3];
         nb_setup(live, m.x, m.y, m.z, inv_r2, vt, o, bb0, bb1, rel);
+        // the block's rows, without the cells that cannot reach the chunk
+        const CellCull cc = chunk_cull(a.cell_box, a.sp.rx * a.sp.ry * a.sp.rz, o, bb0, bb1);
+        BlockRows br = block_rows<2>(a.ranges, a.sp, cx, cy, cz, &cc);
+        const uint32_t total = br.total;
+        n_staged += total;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1306,8 +1363,28 @@ static fsr_status pass_scratch(fsr_context* ctx, uint32_t C, uint32_t n, PassScr
     return FSR_OK;
 }
 
+// The cell boxes of the smoothing and covariance: a.cell_box, or computed here.
+static fsr_status cell_boxes(fsr_context* ctx, const AnisoArgs& a, hipStream_t s,
+                             const float4** out) {
+    if (a.cell_box) {
+        *out = a.cell_box;
+        return FSR_OK;
+    }
+    const uint32_t C = a.sp.rx * a.sp.ry * a.sp.rz;
+    float4* box;
+    FSR_TRY(arena_typed(ctx, A_CELL_BOX_RAW, 2ull * C, &box));
+    const float w = a.aniso_radius * a.aniso_radius * kRadiusCull * 1.002f + 1.0e-12f;
+    hipLaunchKernelGGL(k_cell_box, dim3(C), dim3(64), 0, s, a.ranges, a.sorted, w, box,
+                       box + C);
+    FSR_LAUNCH_CHECK("k_cell_box");
+    *out = box;
+    return FSR_OK;
+}
+
 fsr_status aniso_smooth(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
     const uint32_t C = a.sp.rx * a.sp.ry * a.sp.rz;
+    const float4* box;
+    FSR_TRY(cell_boxes(ctx, a, s, &box));
     const int blocks = persistent_blocks(ctx, kAnisoThreads, 5, TUNE_ANISO);
     PassScratch ps;
     FSR_TRY(pass_scratch(ctx, C, a.n, &ps));
@@ -1324,7 +1401,8 @@ fsr_status aniso_smooth(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
     uint32_t* work;
     FSR_TRY(work_section(ctx, WORK_SMOOTH, &work));
     hipLaunchKernelGGL(k_smooth_mc, dim3(blocks), dim3(kAnisoThreads), 0, s, a.sorted,
-                       a.ranges, a.sp, a.aniso_radius, a.lambda, cl, a.mean, a.smooth, work);
+                       a.ranges, a.sp, a.aniso_radius, a.lambda, cl, box, a.mean, a.smooth,
+                       work);
     FSR_LAUNCH_CHECK("k_smooth_mc");
     return FSR_OK;
 }
@@ -1376,6 +1454,7 @@ fsr_status aniso_cov(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
     ca.sigma = a.sigma;
     ca.rec = a.rec;
     FSR_TRY(work_section(ctx, WORK_COV, &ca.work));
+    FSR_TRY(cell_boxes(ctx, a, s, &ca.cell_box));
     FSR_TRY(reset_xcd(a.counters, s));
     hipLaunchKernelGGL(k_covariance_mc, dim3(blocks_cov), dim3(kAnisoThreads), 0, s, ca, cl2);
     FSR_LAUNCH_CHECK("k_covariance_mc");
@@ -1383,9 +1462,12 @@ fsr_status aniso_cov(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
 }
 
 fsr_status aniso_run(fsr_context* ctx, const AnisoArgs& a, hipStream_t s) {
-    FSR_TRY(aniso_smooth(ctx, a, s));
-    FSR_TRY(aniso_involved(ctx, a, s));
-    return aniso_cov(ctx, a, s);
+    // the cell boxes once for both neighbour passes
+    AnisoArgs b = a;
+    FSR_TRY(cell_boxes(ctx, a, s, &b.cell_box));
+    FSR_TRY(aniso_smooth(ctx, b, s));
+    FSR_TRY(aniso_involved(ctx, b, s));
+    return aniso_cov(ctx, b, s);
 }
 
 // Whether every particle of a clamped 5^3 block lies within distance 1 of

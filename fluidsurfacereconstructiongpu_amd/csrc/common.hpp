@@ -213,7 +213,7 @@ enum ArenaSlot {
     A_INVOLVED_IDX, A_MEAN, A_SMOOTH, A_GMAT, A_SIGMA, A_AUX0, A_AUX1, A_REC,
     A_ADJ_VERT, A_ADJ_COEF, A_ADJ_AUX, A_ADJ_TOUCH, A_EDGE_MASK, A_EDGE_OFF, A_SURF_CHUNKS, A_SLAB, A_BAND_PARTIALS,
     A_FIELD_WORK, A_COLOR_WORK, A_MOM_CELL, A_MOM_T1, A_MOM_T2,
-    A_MOM_BLOCK, A_CELL_BOX, A_ANISO_WORK, A_NUM_SLOTS
+    A_MOM_BLOCK, A_CELL_BOX, A_ANISO_WORK, A_CELL_BOX_RAW, A_NUM_SLOTS
 };
 
 // Band vertices grouped by hash cell (band.hip).
@@ -531,6 +531,10 @@ struct AnisoArgs {
     Counters* counters;
     fsr_frame_stats* st;
     int noninv_moments;           // 1: the field sums non-involved particles by moments
+    // per hash cell, the box of its sorted positions for the cell test of the
+    // smoothing and covariance (float4 lo[C], hi[C]); computed by the first
+    // of those kernels when null
+    const float4* cell_box = nullptr;
 };
 fsr_status aniso_run(fsr_context* ctx, const AnisoArgs& a, hipStream_t s);
 // steps of aniso_run: smoothing (K10), involved list + non-involved field
