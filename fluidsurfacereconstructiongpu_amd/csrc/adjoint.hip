@@ -9,9 +9,10 @@
 // sits at c_a + t (c_b - c_a), t = (iso - phi_a) / (phi_b - phi_a), so
 //   dt/dphi_a = (iso - phi_b) / D^2,  dt/dphi_b = -(iso - phi_a) / D^2,  D = phi_b - phi_a
 // and an edge with |D| < 1e-7 (t fixed at 0.5) carries no gradient.  Normals
-// are treated as constants.  Each band vertex gathers the terms of the soup
-// vertices on its edges from the (up to) 8 cubes it is a corner of, in a fixed
-// order: no atomics, the result is deterministic.
+// are treated as constants.  Each emitted cube sums, per corner, the terms of
+// its soup vertices on the edges at that corner (8 lanes per cube); each band
+// vertex then adds the sums of the (up to) 8 cubes it is a corner of, in a
+// fixed order: no floating-point atomics, the result is deterministic.
 //
 // Field term (field_zb05_backward).  With w_i = t_i^3, t_i = 1 - |x - p_i|^2/R^2,
 // W = sum w_i, xbar = sum w_i p_i / W and phi = |x - xbar| - r_p:
@@ -71,6 +72,7 @@ struct McBackArgs {
     const float* phi;
     const float* g_tri;
     float* g_phi;
+    float* corner;              // [8 (capacity / 3)] corner sums per emitted cube
     const fsr_frame_stats* st;
     GridDev sc;
     float iso;
@@ -78,21 +80,29 @@ struct McBackArgs {
     int cpu_flavor;
 };
 
-// Marks the 8 corners of every emitted cube, so that the gather below skips,
+// Pass 1, one 8-lane group per emitted cube (lane c = corner c): the terms of
+// the cube's soup vertices (table order) on the edges at corner c, summed into
+// corner[8 t0 + c], t0 = the cube's first triangle (vert_off / 3, unique per
+// cube); and the touched bits of the cube's corners, so that pass 2 skips,
 // after one bit test, the band vertices no soup vertex depends on.
-__global__ __launch_bounds__(kAdjThreads) void k_mc_touch(McBackArgs a) {
+__global__ __launch_bounds__(kAdjThreads) void k_mc_corner_terms(McBackArgs a) {
     const uint32_t Q = a.st->num_valid_cubes;
     const uint32_t rx = a.sc.rx, ry = a.sc.ry;
     const uint64_t rxy = (uint64_t)rx * ry;
     const uint32_t wx = row_words(rx);
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < Q; q += gridDim.x * blockDim.x) {
+    const uint32_t c = threadIdx.x & 7u;
+    const int g0 = (int)(__lane_id() & ~7u);
+    const uint32_t groups = gridDim.x * (blockDim.x / 8u);
+    for (uint32_t q = blockIdx.x * (blockDim.x / 8u) + threadIdx.x / 8u; q < Q; q += groups) {
         const uint32_t cb = a.valid_idx[q];
-        if ((uint64_t)a.vert_off[cb] + a.nverts[cb] > a.capacity) continue;
+        const uint32_t off = a.vert_off[cb], nv = a.nverts[cb];
+        if ((uint64_t)off + nv > a.capacity) continue;  // the forward did not emit it
         const uint32_t vid = a.band_index[cb];
         const uint32_t x = vid % rx, y = (vid / rx) % ry, z = (uint32_t)(vid / rxy);
-        // corners x and x + 1 of the 4 (y, z) rows; both bits may share a word
-        for (uint32_t d = 0; d < 4; ++d) {
-            const uint64_t row = (uint64_t)(z + (d >> 1)) * ry + (y + (d & 1u));
+        const uint32_t dx = corner_dx(c), dy = corner_dy(c), dz = corner_dz(c);
+        if (dx == 0u) {
+            // corners x and x + 1 of this lane's (y, z) row; both bits may share a word
+            const uint64_t row = (uint64_t)(z + dz) * ry + (y + dy);
             const uint64_t w0 = row * wx + (x >> 5);
             const uint32_t bit = x & 31u;
             if (bit < 31u) {
@@ -102,9 +112,53 @@ __global__ __launch_bounds__(kAdjThreads) void k_mc_touch(McBackArgs a) {
                 atomicOr(&a.touched[w0 + 1], 1u);
             }
         }
+        const uint64_t u = ((uint64_t)z * ry + y) * rx + x;
+        const float val = a.phi[u + dx + (uint64_t)dy * rx + (uint64_t)dz * rxy];
+        uint32_t f = (val <= a.iso) ? (1u << c) : 0u;
+        f |= (uint32_t)__builtin_amdgcn_ds_swizzle((int)f, 0x1F | (1 << 10));
+        f |= (uint32_t)__builtin_amdgcn_ds_swizzle((int)f, 0x1F | (2 << 10));
+        f |= (uint32_t)__builtin_amdgcn_ds_swizzle((int)f, 0x1F | (4 << 10));
+        const uint64_t tri = cAdjTri[f];
+        float acc = 0.0f;
+        for (uint32_t k = 0; k < nv; ++k) {
+            const uint32_t e = (uint32_t)(tri >> (4 * k)) & 0xFu;
+            const uint32_t s0 = (e < 8) ? (e & 3u) + (e & 4u) : e - 8u;
+            const uint32_t s1 = (e < 8) ? (((e & 3u) + 1u) & 3u) + (e & 4u) : e - 4u;
+            // the edge's corner values from their lanes (the whole group runs
+            // the same k: nv is the group's)
+            const float v0 = __shfl(val, g0 + (int)s0, 64);
+            const float v1 = __shfl(val, g0 + (int)s1, 64);
+            if (s0 != c && s1 != c) continue;
+            const float delta = v1 - v0;
+            const bool flat = a.cpu_flavor
+                                  ? ((double)delta > -1.0e-7 && (double)delta < 1.0e-7)
+                                  : (delta > -1.0e-7f && delta < 1.0e-7f);
+            if (flat) continue;
+            // d(position)/dt: corner s1 minus corner s0 as the forward evaluates them
+            const uint32_t X0 = x + corner_dx(s0), Y0 = y + corner_dy(s0), Z0 = z + corner_dz(s0);
+            const uint32_t X1 = x + corner_dx(s1), Y1 = y + corner_dy(s1), Z1 = z + corner_dz(s1);
+            float ex, ey, ez;
+            if (a.cpu_flavor) {
+                ex = (a.sc.minx + (float)X1 * a.sc.cell) - (a.sc.minx + (float)X0 * a.sc.cell);
+                ey = (a.sc.miny + (float)Y1 * a.sc.cell) - (a.sc.miny + (float)Y0 * a.sc.cell);
+                ez = (a.sc.minz + (float)Z1 * a.sc.cell) - (a.sc.minz + (float)Z0 * a.sc.cell);
+            } else {
+                ex = vertex_coord(X1, a.sc.minx, a.sc.cell) - vertex_coord(X0, a.sc.minx, a.sc.cell);
+                ey = vertex_coord(Y1, a.sc.miny, a.sc.cell) - vertex_coord(Y0, a.sc.miny, a.sc.cell);
+                ez = vertex_coord(Z1, a.sc.minz, a.sc.cell) - vertex_coord(Z0, a.sc.minz, a.sc.cell);
+            }
+            const uint64_t o = 3ull * (off + k);
+            const float gt = __builtin_fmaf(a.g_tri[o + 2], ez,
+                                            __builtin_fmaf(a.g_tri[o + 1], ey, a.g_tri[o] * ex));
+            const float inv_d2 = 1.0f / (delta * delta);
+            acc += (s0 == c) ? gt * (a.iso - v1) * inv_d2 : -gt * (a.iso - v0) * inv_d2;
+        }
+        a.corner[8ull * (off / 3u) + c] = acc;
     }
 }
 
+// Pass 2, one thread per band vertex: the corner sums of the (up to) 8 emitted
+// cubes it is a corner of, in corner order.
 __global__ __launch_bounds__(kAdjThreads) void k_mc_vertex_backward(McBackArgs a) {
     const uint32_t B = a.st->num_band_vertices;
     const uint32_t rx = a.sc.rx, ry = a.sc.ry, rz = a.sc.rz;
@@ -115,6 +169,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_mc_vertex_backward(McBackArgs a
         const uint32_t x = vid % rx, y = (vid / rx) % ry, z = (uint32_t)(vid / rxy);
         if (!((a.touched[((uint64_t)z * ry + y) * wx + (x >> 5)] >> (x & 31u)) & 1u)) continue;
         float acc = 0.0f;
+#pragma unroll
         for (uint32_t cv = 0; cv < 8; ++cv) {
             // the cube whose corner cv is this vertex
             const uint32_t ox = corner_dx(cv), oy = corner_dy(cv), oz = corner_dz(cv);
@@ -129,45 +184,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_mc_vertex_backward(McBackArgs a
             if (nv == 0) continue;
             const uint32_t off = a.vert_off[cb];
             if ((uint64_t)off + nv > a.capacity) continue;  // the forward did not emit it
-            const uint64_t u = ((uint64_t)uz * ry + uy) * rx + ux;
-            float val[8];
-            uint32_t f = 0;
-#pragma unroll
-            for (uint32_t c = 0; c < 8; ++c) {
-                val[c] = a.phi[u + corner_dx(c) + corner_dy(c) * rx + corner_dz(c) * rxy];
-                f |= (val[c] <= a.iso) ? (1u << c) : 0u;
-            }
-            const uint64_t tri = cAdjTri[f];
-            for (uint32_t k = 0; k < nv; ++k) {
-                const uint32_t e = (uint32_t)(tri >> (4 * k)) & 0xFu;
-                const uint32_t s0 = (e < 8) ? (e & 3u) + (e & 4u) : e - 8u;
-                const uint32_t s1 = (e < 8) ? (((e & 3u) + 1u) & 3u) + (e & 4u) : e - 4u;
-                if (s0 != cv && s1 != cv) continue;
-                const float v0 = val[s0], v1 = val[s1];
-                const float delta = v1 - v0;
-                const bool flat = a.cpu_flavor
-                                      ? ((double)delta > -1.0e-7 && (double)delta < 1.0e-7)
-                                      : (delta > -1.0e-7f && delta < 1.0e-7f);
-                if (flat) continue;
-                // d(position)/dt: corner s1 minus corner s0 as the forward evaluates them
-                const uint32_t X0 = ux + corner_dx(s0), Y0 = uy + corner_dy(s0), Z0 = uz + corner_dz(s0);
-                const uint32_t X1 = ux + corner_dx(s1), Y1 = uy + corner_dy(s1), Z1 = uz + corner_dz(s1);
-                float ex, ey, ez;
-                if (a.cpu_flavor) {
-                    ex = (a.sc.minx + (float)X1 * a.sc.cell) - (a.sc.minx + (float)X0 * a.sc.cell);
-                    ey = (a.sc.miny + (float)Y1 * a.sc.cell) - (a.sc.miny + (float)Y0 * a.sc.cell);
-                    ez = (a.sc.minz + (float)Z1 * a.sc.cell) - (a.sc.minz + (float)Z0 * a.sc.cell);
-                } else {
-                    ex = vertex_coord(X1, a.sc.minx, a.sc.cell) - vertex_coord(X0, a.sc.minx, a.sc.cell);
-                    ey = vertex_coord(Y1, a.sc.miny, a.sc.cell) - vertex_coord(Y0, a.sc.miny, a.sc.cell);
-                    ez = vertex_coord(Z1, a.sc.minz, a.sc.cell) - vertex_coord(Z0, a.sc.minz, a.sc.cell);
-                }
-                const uint64_t o = 3ull * (off + k);
-                const float gt = __builtin_fmaf(a.g_tri[o + 2], ez,
-                                                __builtin_fmaf(a.g_tri[o + 1], ey, a.g_tri[o] * ex));
-                const float inv_d2 = 1.0f / (delta * delta);
-                acc += (s0 == cv) ? gt * (a.iso - v1) * inv_d2 : -gt * (a.iso - v0) * inv_d2;
-            }
+            acc += a.corner[8ull * (off / 3u) + cv];
         }
         if (acc != 0.0f) a.g_phi[vid] += acc;
     }
@@ -528,9 +545,10 @@ fsr_status mc_vertex_backward(fsr_context* ctx, const float* g_tri_pos, float* g
     a.iso = st.iso;
     a.capacity = st.tri_capacity;
     a.cpu_flavor = st.cpu_flavor;
+    FSR_TRY(arena_typed(ctx, A_ADJ_CORNER, 8ull * (st.tri_capacity / 3u) + 8u, &a.corner));
     const int blocks = persistent_blocks(ctx, kAdjThreads, 8, TUNE_MC);
-    hipLaunchKernelGGL(k_mc_touch, dim3(blocks), dim3(kAdjThreads), 0, s, a);
-    FSR_LAUNCH_CHECK("k_mc_touch");
+    hipLaunchKernelGGL(k_mc_corner_terms, dim3(blocks), dim3(kAdjThreads), 0, s, a);
+    FSR_LAUNCH_CHECK("k_mc_corner_terms");
     hipLaunchKernelGGL(k_mc_vertex_backward, dim3(blocks), dim3(kAdjThreads), 0, s, a);
     FSR_LAUNCH_CHECK("k_mc_vertex_backward");
     return FSR_OK;
