@@ -428,6 +428,17 @@ def batched_vs_per_frame(method: str, frames: int, lanes: int, reps: int, autotu
         per_frame_pass(sync=True)
     torch.cuda.synchronize(dev)
     sync_ms = (time.perf_counter() - t_sync) * 1e3 / reps
+    # the frames' neighbour-kernel time (field stage, and for Yu13 the prep
+    # stage: smoothing and covariance), from one more per-frame pass with stage
+    # events: what a batch has left to overlap is the rest of the per-frame time
+    # (and the idle tails of those persistent kernels on small frames)
+    n_frames = len(frs)
+    batch.ctx.set_profiling(n_frames)
+    per_frame_pass()
+    torch.cuda.synchronize(dev)
+    stages = batch.ctx.stage_times(n_frames)
+    batch.ctx.set_profiling(0)
+    heavy_ms = float(stages[:, 4].sum() + (stages[:, 3].sum() if method == "yu13" else 0.0))
     batch.run(stream=stream)                # eager + capture
     torch.cuda.synchronize(dev)
     batch.run(stream=stream)                # first replay
@@ -452,6 +463,8 @@ def batched_vs_per_frame(method: str, frames: int, lanes: int, reps: int, autotu
         "per_frame_async_frames_per_s": n / (per_ms * 1e-3),
         "speedup_vs_per_frame_sync": sync_ms / bat_ms,
         "speedup_vs_per_frame_async": per_ms / bat_ms,
+        "neighbour_kernel_ms_sum": heavy_ms,
+        "neighbour_kernel_share_of_per_frame_sync": heavy_ms / sync_ms,
         "batched_frame_ms_p50": float(np.percentile(bat_frame_ms, 50)),
         "batched_frame_ms_p99": float(np.percentile(bat_frame_ms, 99)),
         "per_frame_ms_p50": float(np.percentile(per_frame_ms, 50)),
