@@ -152,6 +152,70 @@ void count_lines(const char* a, const char* b, uint64_t* lines, uint64_t* partic
 // reference's `file << float` writes.
 inline int put_float(char* buf, float x) { return std::snprintf(buf, 32, "%g", (double)x); }
 
+// The same text without the printf machinery: std::to_chars in the general
+// format with precision 6 is specified as printf's "%.6g" (correctly rounded
+// from the float's exact value, which (double)x keeps).
+inline char* put_g(char* p, float x) {
+    return std::to_chars(p, p + 32, x, std::chars_format::general, 6).ptr;
+}
+inline char* put_u(char* p, unsigned long long v) { return std::to_chars(p, p + 24, v).ptr; }
+
+// Lines 0 .. n-1, line i written by line(i, p) (returning the end), at most
+// max_len bytes each: formatted on up to 16 threads in contiguous ranges and
+// written in order.  Returns false on a write error.
+template <typename L>
+bool write_lines(std::FILE* f, uint64_t n, size_t max_len, L line) {
+    if (!n) return true;
+    unsigned t = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < 4096) t = 1;
+    std::vector<std::vector<char>> out(t);
+    parallel_ranges(t, [&](size_t k) {
+        const uint64_t lo = n * k / t, hi = n * (k + 1) / t;
+        std::vector<char>& b = out[k];
+        b.resize((size_t)(hi - lo) * max_len);
+        char* p = b.data();
+        for (uint64_t i = lo; i < hi; ++i) p = line(i, p);
+        b.resize((size_t)(p - b.data()));
+    });
+    for (const auto& b : out)
+        if (!b.empty() && std::fwrite(b.data(), 1, b.size(), f) != b.size()) return false;
+    return true;
+}
+
+// "v x y z" / "vn x y z" lines of n float3s.
+bool write_vec_lines(std::FILE* f, const char* tag, const float* v, uint64_t n) {
+    const size_t tl = std::strlen(tag);
+    return write_lines(f, n, tl + 3 * 16 + 1, [&](uint64_t i, char* p) {
+        std::memcpy(p, tag, tl);
+        p += tl;
+        for (int c = 0; c < 3; ++c) {
+            *p++ = ' ';
+            p = put_g(p, v[3 * i + c]);
+        }
+        *p++ = '\n';
+        return p;
+    });
+}
+
+// "f a//a b//b c//c " lines (1-based vertex numbers).
+template <typename V>
+bool write_face_lines(std::FILE* f, uint64_t num_tris, V vertex) {
+    return write_lines(f, num_tris, 2 + 3 * (2 * 20 + 3) + 1, [&](uint64_t t, char* p) {
+        *p++ = 'f';
+        for (int c = 0; c < 3; ++c) {
+            const unsigned long long v = vertex(3 * t + c);
+            *p++ = ' ';
+            p = put_u(p, v);
+            *p++ = '/';
+            *p++ = '/';
+            p = put_u(p, v);
+        }
+        *p++ = ' ';
+        *p++ = '\n';
+        return p;
+    });
+}
+
 }  // namespace
 
 extern "C" {
@@ -257,26 +321,12 @@ fsr_status fsr_obj_write_soup(const char* path, const float* pos, const float* n
         set_error(std::string("fsr_obj_write_soup: cannot write ") + path);
         return FSR_ERR_INVALID_ARG;
     }
-    char a[32], b[32], c[32];
-    for (uint32_t i = 0; i < num_vertices; ++i) {
-        put_float(a, pos[3ull * i]);
-        put_float(b, pos[3ull * i + 1]);
-        put_float(c, pos[3ull * i + 2]);
-        std::fprintf(f, "v %s %s %s\n", a, b, c);
-    }
-    if (nrm)
-        for (uint32_t i = 0; i < num_vertices; ++i) {
-            put_float(a, nrm[3ull * i]);
-            put_float(b, nrm[3ull * i + 1]);
-            put_float(c, nrm[3ull * i + 2]);
-            std::fprintf(f, "vn %s %s %s\n", a, b, c);
-        }
-    for (uint64_t i = 1; i <= num_vertices; i += 3)
-        std::fprintf(f, "f %llu//%llu %llu//%llu %llu//%llu \n", (unsigned long long)i,
-                     (unsigned long long)i, (unsigned long long)(i + 1),
-                     (unsigned long long)(i + 1), (unsigned long long)(i + 2),
-                     (unsigned long long)(i + 2));
-    const bool ok = std::fclose(f) == 0;
+    bool ok = write_vec_lines(f, "v", pos, num_vertices);
+    if (ok && nrm) ok = write_vec_lines(f, "vn", nrm, num_vertices);
+    if (ok)
+        ok = write_face_lines(f, num_vertices / 3,
+                              [](uint64_t k) { return (unsigned long long)k + 1ull; });
+    ok = (std::fclose(f) == 0) && ok;
     if (!ok) {
         set_error(std::string("fsr_obj_write_soup: write error on ") + path);
         return FSR_ERR_INVALID_ARG;
@@ -301,26 +351,12 @@ fsr_status fsr_obj_write_indexed(const char* path, const float* pos, const float
         set_error(std::string("fsr_obj_write_indexed: cannot write ") + path);
         return FSR_ERR_INVALID_ARG;
     }
-    char a[32], b[32], c[32];
-    for (uint32_t i = 0; i < num_vertices; ++i) {
-        put_float(a, pos[3ull * i]);
-        put_float(b, pos[3ull * i + 1]);
-        put_float(c, pos[3ull * i + 2]);
-        std::fprintf(f, "v %s %s %s\n", a, b, c);
-    }
-    if (nrm)
-        for (uint32_t i = 0; i < num_vertices; ++i) {
-            put_float(a, nrm[3ull * i]);
-            put_float(b, nrm[3ull * i + 1]);
-            put_float(c, nrm[3ull * i + 2]);
-            std::fprintf(f, "vn %s %s %s\n", a, b, c);
-        }
-    for (uint32_t t = 0; t + 2 < num_indices; t += 3) {
-        const unsigned long long i0 = indices[t] + 1ull, i1 = indices[t + 1] + 1ull,
-                                 i2 = indices[t + 2] + 1ull;
-        std::fprintf(f, "f %llu//%llu %llu//%llu %llu//%llu \n", i0, i0, i1, i1, i2, i2);
-    }
-    const bool ok = std::fclose(f) == 0;
+    bool ok = write_vec_lines(f, "v", pos, num_vertices);
+    if (ok && nrm) ok = write_vec_lines(f, "vn", nrm, num_vertices);
+    if (ok)
+        ok = write_face_lines(f, num_indices / 3,
+                              [&](uint64_t k) { return indices[k] + 1ull; });
+    ok = (std::fclose(f) == 0) && ok;
     if (!ok) {
         set_error(std::string("fsr_obj_write_indexed: write error on ") + path);
         return FSR_ERR_INVALID_ARG;

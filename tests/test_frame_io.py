@@ -103,6 +103,41 @@ def test_obj_indexed_layout(tmp_path):
         frame_io.write_obj_indexed(p, pos, None, np.array([0, 1, 4], np.uint32))
 
 
+def _g(x) -> str:
+    # C's printf("%g") of the float's exact value, as `file << float` writes
+    return "%g" % float(x)
+
+
+def test_obj_text_matches_printf_g(tmp_path):
+    """The .obj writers format on several threads with std::to_chars; their
+    text must equal printf's "%g" (6 significant digits) line for line, over
+    magnitudes from denormals to FLT_MAX, signed zeros and infinities, values
+    that round up a digit (999999.5), and more lines than one thread takes."""
+    rng = np.random.default_rng(5)
+    n = 3 * 4000
+    with np.errstate(over="ignore"):
+        pos = (rng.normal(size=(n, 3)) * 10.0 ** rng.integers(-45, 39, size=(n, 3)))
+    pos = pos.astype(np.float32)
+    special = np.array([0.0, -0.0, np.inf, -np.inf, 1e-45, -1e-45, 3.4028235e38, 1.17549435e-38,
+                        0.1, 123456.5, 999999.5, 1e6, 1e-5, 1e-4, 0.000123456789], np.float32)
+    pos.ravel()[:special.size] = special
+    nrm = rng.random((n, 3)).astype(np.float32)
+    p = str(tmp_path / "s.obj")
+    frame_io.write_obj_soup(p, pos, nrm)
+    exp = ["v %s %s %s\n" % tuple(_g(x) for x in v) for v in pos]
+    exp += ["vn %s %s %s\n" % tuple(_g(x) for x in v) for v in nrm]
+    exp += ["f %d//%d %d//%d %d//%d \n" % (i, i, i + 1, i + 1, i + 2, i + 2)
+            for i in range(1, n + 1, 3)]
+    assert open(p).read() == "".join(exp)
+    idx = rng.integers(0, n, size=3 * 3000).astype(np.uint32)
+    frame_io.write_obj_indexed(p, pos, None, idx)
+    exp = ["v %s %s %s\n" % tuple(_g(x) for x in v) for v in pos]
+    for t in range(0, idx.size, 3):
+        a, b, c = (int(idx[t + k]) + 1 for k in range(3))
+        exp.append("f %d//%d %d//%d %d//%d \n" % (a, a, b, b, c, c))
+    assert open(p).read() == "".join(exp)
+
+
 def test_reconstructor_constants():
     head = dict(aabb_min=(0, 0, 0), aabb_max=(1, 1, 1), kernel_radius=0.036,
                 particle_radius=0.01, particle_mass=0.004)
