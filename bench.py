@@ -746,7 +746,8 @@ def main():
     ap.add_argument("--no-counters", action="store_true",
                     help="skip the rocprofv3 counter runs (traffic, L2 hit rate)")
     ap.add_argument("--counter-probe", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--only", choices=["zb05", "akinci12", "yu13", "adjoint", "batch"],
+    ap.add_argument("--only", choices=["zb05", "akinci12", "yu13", "adjoint", "batch",
+                                       "sequence"],
                     help="run one workload alone (for kernel traces) and print its result "
                          "instead of the benchmark line")
     args = ap.parse_args()
@@ -778,6 +779,8 @@ def main():
             res = yu13_line(fr, ds.yu13_params(fr, True), K, W)[0]
         elif args.only == "adjoint":
             res = adjoint_cost(reps=K)
+        elif args.only == "sequence":
+            res = frame_sequence(make_frame("zb05", args.particles, args.grid))
         else:
             res = {m: batched_vs_per_frame(m, args.batch_frames, args.lanes, args.batch_reps,
                                            not args.no_autotune) for m in ("zb05", "yu13")}

@@ -196,11 +196,11 @@ __global__ __launch_bounds__(256) void k_colorfield_tail(SurfaceArgs a, float ma
                         const float d2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
                         if (d2 > 0.0f) {
                             ++count;
-                            count_r += d2 < h2 ? 1u : 0u;
-                            const float dist = __builtin_amdgcn_sqrtf(d2);  // hardware square root
-                            if (dist < h) {
-                                const float x = 1.0f - dist * inv_h;
-                                const float c = mass / q.w * gcoef * x * x * __builtin_amdgcn_rsqf(d2);
+                            if (d2 < h2) {
+                                ++count_r;
+                                const float ri = __builtin_amdgcn_rsqf(d2);
+                                const float x = 1.0f - d2 * ri * inv_h;
+                                const float c = mass / q.w * gcoef * x * x * ri;
                                 gx = __builtin_fmaf(c, rx, gx);
                                 gy = __builtin_fmaf(c, ry, gy);
                                 gz = __builtin_fmaf(c, rz, gz);
@@ -251,9 +251,11 @@ struct ColorChunks {
 // h * (1 + 5e-5) from the bounding box of the chunk's particles can only add to
 // the neighbour count (d^2 > 0, dist >= h): they are counted and dropped, the
 // others are compacted in order and evaluated with the per-thread kernel's
-// arithmetic, so counts, gradients and flags are identical to it.  Both use
-// the hardware square root for the distance: it enters the gradient through
-// (1 - d/h)^2, which vanishes where the d < h test could flip.
+// arithmetic, so counts, gradients and flags are identical to it.  Both take
+// the reference's test |r| < h as |r|^2 < h^2 and the distance as
+// |r|^2 rsqrt(|r|^2), one reciprocal square root per pair within h and no
+// square root: the distance enters the gradient through (1 - |r|/h)^2, which
+// vanishes where the two forms of the test could differ.
 __global__ __launch_bounds__(256) void k_colorfield(SurfaceArgs a, float mass, ColorChunks cl) {
     __shared__ float4 queue[4][128];
     const uint32_t lane = __lane_id();
@@ -335,12 +337,11 @@ __global__ __launch_bounds__(256) void k_colorfield(SurfaceArgs a, float mass, C
                 const float d2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
                 if (d2 > 0.0f) {
                     ++count;
-                    count_r += d2 < h2 ? 1u : 0u;
-                    const float dist = __builtin_amdgcn_sqrtf(d2);  // hardware square root
-                    if (dist < h) {
-                        ++n_inh;
-                        const float x = 1.0f - dist * inv_h;
-                        const float cc = qq.w * x * x * __builtin_amdgcn_rsqf(d2);
+                    if (d2 < h2) {
+                        ++count_r;
+                        const float ri = __builtin_amdgcn_rsqf(d2);
+                        const float x = 1.0f - d2 * ri * inv_h;
+                        const float cc = qq.w * x * x * ri;
                         gx = __builtin_fmaf(cc, rx, gx);
                         gy = __builtin_fmaf(cc, ry, gy);
                         gz = __builtin_fmaf(cc, rz, gz);
@@ -394,6 +395,7 @@ __global__ __launch_bounds__(256) void k_colorfield(SurfaceArgs a, float mass, C
         }
         // every dropped candidate has d^2 > 0: it counts as a neighbour
         count += total - kept;
+        n_inh += count_r;
         n_staged += total;
         n_kept += kept;
         bool surf = false;
