@@ -213,7 +213,7 @@ enum ArenaSlot {
     A_INVOLVED_IDX, A_MEAN, A_SMOOTH, A_GMAT, A_SIGMA, A_AUX0, A_AUX1, A_REC,
     A_ADJ_VERT, A_ADJ_COEF, A_ADJ_AUX, A_ADJ_TOUCH, A_EDGE_MASK, A_EDGE_OFF, A_SURF_CHUNKS, A_SLAB, A_BAND_PARTIALS,
     A_FIELD_WORK, A_COLOR_WORK, A_MOM_CELL, A_MOM_T1, A_MOM_T2,
-    A_MOM_BLOCK, A_CELL_BOX, A_ANISO_WORK, A_CELL_BOX_RAW, A_ADJ_CORNER, A_NUM_SLOTS
+    A_MOM_BLOCK, A_CELL_BOX, A_ANISO_WORK, A_CELL_BOX_RAW, A_ADJ_CORNER, A_INVOLVE_TMP, A_NUM_SLOTS
 };
 
 // Band vertices grouped by hash cell (band.hip).

@@ -1,12 +1,13 @@
 // Surface-particle detection and band seeding.
 //
 //   k_cell_surface      extractionOfSurfaceParticles (ReconstructionCUDA.cu:70-120)
-//                       and, with `involve`, extractionOfSurfaceAndInvolveParticles
+//                       and the surface half of extractionOfSurfaceAndInvolveParticles
 //                       (ReconstructionCUDA.cu:466-551): an occupied cell that is
 //                       not on the last layer of the grid and has an empty cell in
-//                       its clamped 3^3 block is a surface cell; with `involve`,
-//                       every cell of the clamped 5^3 block of a surface cell gets
-//                       its particle count.
+//                       its clamped 3^3 block is a surface cell.
+//   k_involve_window    the involve half of the latter: every cell of the clamped
+//                       5^3 block of a surface cell gets its particle count (the
+//                       block test as three separable 5-wide passes).
 //   k_flags_from_cells  marks the particles of surface cells (the reference's
 //                       per-cell loop over the cell's range, written per particle).
 //   k_colorfield        extractionOfSurfaceParticlesUsingColorField
@@ -42,9 +43,7 @@ __device__ __forceinline__ int trunc_cell(float q, int res) {
 }
 
 __global__ __launch_bounds__(256) void k_cell_surface(const float* __restrict__ occupancy,
-                                                      const fsr_index_range* __restrict__ ranges,
-                                                      GridDev sp, uint8_t* __restrict__ surf_cell,
-                                                      uint32_t* __restrict__ involve) {
+                                                      GridDev sp, uint8_t* __restrict__ surf_cell) {
     const uint32_t C = sp.rx * sp.ry * sp.rz;
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
@@ -63,17 +62,33 @@ __global__ __launch_bounds__(256) void k_cell_surface(const float* __restrict__ 
                     }
     }
     surf_cell[c] = surf;
-    if (involve && surf) {
-        const int x0 = clampi((int)x - 2, 0, sp.rx - 1), x1 = clampi((int)x + 2, 0, sp.rx - 1);
-        const int y0 = clampi((int)y - 2, 0, sp.ry - 1), y1 = clampi((int)y + 2, 0, sp.ry - 1);
-        const int z0 = clampi((int)z - 2, 0, sp.rz - 1), z1 = clampi((int)z + 2, 0, sp.rz - 1);
-        for (int zz = z0; zz <= z1; ++zz)
-            for (int yy = y0; yy <= y1; ++yy)
-                for (int xx = x0; xx <= x1; ++xx) {
-                    const uint32_t cc = ((uint32_t)zz * sp.ry + yy) * sp.rx + xx;
-                    const fsr_index_range r = ranges[cc];
-                    if (r.start != 0xFFFFFFFFu) involve[cc] = r.end - r.start;  // same value from every writer
-                }
+}
+
+// Involve counts (K4, the Yu13 half): an occupied cell within the clamped 5^3
+// block of some surface cell gets its particle count, every other cell 0.  The
+// block test is separable: three passes of a 5-wide OR along x, y and z over
+// the surface-cell bytes (clamped to the grid, as the block is); the z pass
+// writes the counts.  Every cell is written, so no clearing pass is needed.
+template <int kAxis>
+__global__ __launch_bounds__(256) void k_involve_window(const uint8_t* __restrict__ in,
+                                                       uint8_t* __restrict__ out, GridDev sp,
+                                                       const fsr_index_range* __restrict__ ranges,
+                                                       uint32_t* __restrict__ involve) {
+    const uint32_t C = sp.rx * sp.ry * sp.rz;
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const uint32_t x = c % sp.rx, y = (c / sp.rx) % sp.ry, z = c / (sp.rx * sp.ry);
+    const uint32_t pos = kAxis == 0 ? x : (kAxis == 1 ? y : z);
+    const uint32_t res = kAxis == 0 ? sp.rx : (kAxis == 1 ? sp.ry : sp.rz);
+    const uint32_t stride = kAxis == 0 ? 1u : (kAxis == 1 ? sp.rx : sp.rx * sp.ry);
+    const uint32_t lo = pos >= 2u ? pos - 2u : 0u, hi = min(pos + 2u, res - 1u);
+    uint8_t any = 0;
+    for (uint32_t p = lo; p <= hi; ++p) any |= in[c + (p - pos) * stride];
+    if (kAxis < 2) {
+        out[c] = any;
+    } else {
+        const fsr_index_range r = ranges[c];
+        involve[c] = (any && r.start != 0xFFFFFFFFu) ? r.end - r.start : 0u;
     }
 }
 
@@ -466,12 +481,22 @@ fsr_status surface_seed_from_flags(fsr_context* ctx, const SurfaceArgs& a, hipSt
 
 fsr_status surface_occupancy(fsr_context* ctx, const SurfaceArgs& a, uint8_t* surf_cell,
                              uint32_t* involve, hipStream_t s) {
-    (void)ctx;
     const uint32_t C = a.sp.rx * a.sp.ry * a.sp.rz;
-    if (involve) FSR_TRY(fill_u32(involve, 0u, C, s));
     hipLaunchKernelGGL(k_cell_surface, dim3((C + 255) / 256), dim3(256), 0, s, a.occupancy,
-                       a.ranges, a.sp, surf_cell, involve);
+                       a.sp, surf_cell);
     FSR_LAUNCH_CHECK("k_cell_surface");
+    if (involve) {
+        uint8_t* tmp;
+        FSR_TRY(arena_typed(ctx, A_INVOLVE_TMP, 2ull * C, &tmp));
+        const dim3 g((C + 255) / 256);
+        hipLaunchKernelGGL(k_involve_window<0>, g, dim3(256), 0, s, (const uint8_t*)surf_cell,
+                           tmp, a.sp, a.ranges, involve);
+        hipLaunchKernelGGL(k_involve_window<1>, g, dim3(256), 0, s, (const uint8_t*)tmp,
+                           tmp + C, a.sp, a.ranges, involve);
+        hipLaunchKernelGGL(k_involve_window<2>, g, dim3(256), 0, s, (const uint8_t*)(tmp + C),
+                           tmp, a.sp, a.ranges, involve);
+        FSR_LAUNCH_CHECK("k_involve_window");
+    }
     if (a.n) {
         hipLaunchKernelGGL(k_flags_from_cells, dim3(count_blocks(ctx, a.n)), dim3(256), 0, s, a,
                            (const uint8_t*)surf_cell);
