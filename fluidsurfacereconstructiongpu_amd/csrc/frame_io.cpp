@@ -47,7 +47,13 @@ bool next_float(const char*& p, const char* end, float* out) {
     if (p < end && *p == '+') ++p;
     const std::from_chars_result r = std::from_chars(p, end, *out);
     if (r.ec == std::errc::invalid_argument || r.ptr == p) return false;
-    p = r.ptr;  // out-of-range values parse to +-inf / 0 like strtof's, kept as read
+    if (r.ec == std::errc::result_out_of_range) {
+        // from_chars leaves the value unset here: take strtof's (+-HUGE_VALF,
+        // or the nearest denormal or +-0 for an underflow)
+        const std::string token(p, r.ptr);
+        *out = std::strtof(token.c_str(), nullptr);
+    }
+    p = r.ptr;
     return true;
 }
 

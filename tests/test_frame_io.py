@@ -138,6 +138,87 @@ def test_obj_text_matches_printf_g(tmp_path):
     assert open(p).read() == "".join(exp)
 
 
+def _nearest_f32(text: str) -> np.float32:
+    """The float32 nearest to the decimal `text` (ties to even), by exact
+    rational comparison of the three candidates around a first guess."""
+    from fractions import Fraction
+
+    neg = text.lstrip().startswith("-")
+    x = abs(Fraction(text))
+    big = np.finfo(np.float32).max
+    with np.errstate(over="ignore"):
+        g = np.float32(float(x)) if abs(x) <= Fraction(2) ** 128 else np.float32(np.sign(x) * big)
+        cands = [np.nextafter(g, np.float32(-np.inf)), g, np.nextafter(g, np.float32(np.inf))]
+
+    def exact(c):
+        # infinity stands for the first value past FLT_MAX's binade, 2^128
+        return Fraction(float(c)) if np.isfinite(c) else Fraction(int(np.sign(c)) * 2 ** 128)
+
+    def key(c):
+        lsb = int(np.frombuffer(np.float32(c).tobytes(), np.uint32)[0]) & 1
+        return (abs(exact(c) - x), lsb)
+
+    best = np.float32(min(cands, key=key))
+    return -best if neg else best  # the sign survives a zero result
+
+
+def test_xyz_numbers_round_correctly(tmp_path):
+    """The reader's numbers are the correctly rounded float32 of their decimal
+    text (as `stream >> float` gives): 9-digit round trips of random floats
+    over many binades, and strings near rounding boundaries: exact midpoints
+    of two floats (ties to even), decimals a hair above and below a midpoint,
+    more than 19 significant digits, large exponents, subnormals, values that
+    underflow to zero or overflow to infinity (from_chars leaves those unset;
+    the reader takes strtof's value), signed zeros, leading zeros and a bare
+    exponent marker."""
+    from fractions import Fraction
+
+    rng = np.random.default_rng(11)
+    vals = (rng.normal(size=4000) * 10.0 ** rng.integers(-30, 30, size=4000)).astype(np.float32)
+    texts = ["%.9g" % float(v) for v in vals]
+    expect = list(vals)
+    for v in vals[:300]:
+        v = np.float32(abs(v)) if v != 0 else np.float32(1.5)
+        nxt = np.nextafter(v, np.float32(np.inf))
+        mid = (Fraction(float(v)) + Fraction(float(nxt))) / 2
+        exact = format(mid.numerator / mid.denominator, ".40g")  # a long decimal near it
+        for t in (exact, "%.17g" % float(mid), "%.12g" % float(mid)):
+            texts.append(t)
+            expect.append(_nearest_f32(t))
+        # exact midpoint written out in full: a tie
+        num, den = mid.numerator, mid.denominator
+        k = den.bit_length() - 1  # den = 2^k
+        digits = str(num * 5 ** k)
+        t = digits[:-k] + "." + digits[-k:] if k < len(digits) else "0." + digits.zfill(k)
+        texts.append(t)
+        expect.append(_nearest_f32(t))
+    for t in ["0.000000000000000000000000000000000001234567", "1e-40", "-1e-45", "3.4028235e38",
+              "3.4028236e38", "1.17549435e-38", "1.1754942e-38", "-0.0", "0", "000123.4500",
+              "12345678901234567890123", "0.1234567890123456789012", "9007199254740993",
+              "1e23", "1e22", "4.9406564584124654e-324", "2.5e-1", "-7.00000000000000000001"]:
+        texts.append(t)
+        expect.append(_nearest_f32(t))
+    n = len(texts)
+    while n % 4:
+        texts.append("1")
+        expect.append(np.float32(1))
+        n += 1
+    rows = [" ".join(texts[i:i + 4]) for i in range(0, n, 4)]
+    p = tmp_path / "t.xyz"
+    p.write_text("0 0 0\n1 1 1\n0.1\n0.01\n0.001\n" + "\n".join(rows) + "\n1.5e 2 3 4\n")
+    try:
+        frame_io.read_xyz(str(p))
+        raise AssertionError("a line with a bare exponent marker must not parse")
+    except _lib.FsrError:
+        pass
+    p.write_text("0 0 0\n1 1 1\n0.1\n0.01\n0.001\n" + "\n".join(rows) + "\n")
+    _, pos, dens = frame_io.read_xyz(str(p))
+    got = np.column_stack([pos, dens]).ravel()
+    want = np.array(expect, np.float32)
+    bad = np.nonzero(got.view(np.uint32) != want.view(np.uint32))[0]
+    assert bad.size == 0, [(texts[i], got[i], want[i]) for i in bad[:5]]
+
+
 def test_reconstructor_constants():
     head = dict(aabb_min=(0, 0, 0), aabb_max=(1, 1, 1), kernel_radius=0.036,
                 particle_radius=0.01, particle_mass=0.004)
