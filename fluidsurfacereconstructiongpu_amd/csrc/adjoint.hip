@@ -168,23 +168,41 @@ __global__ __launch_bounds__(kAdjThreads) void k_mc_vertex_backward(McBackArgs a
         const uint32_t vid = a.band_index[b];
         const uint32_t x = vid % rx, y = (vid / rx) % ry, z = (uint32_t)(vid / rxy);
         if (!((a.touched[((uint64_t)z * ry + y) * wx + (x >> 5)] >> (x & 31u)) & 1u)) continue;
+        // the cubes whose corner cv is this vertex, level by level over the 8
+        // (each level's loads issued together): mask word, band rank, vertex
+        // count, offset, corner sum
+        uint64_t word[8];
+        uint32_t bw[8], sh[8];
+        bool ok[8];
+#pragma unroll
+        for (uint32_t cv = 0; cv < 8; ++cv) {
+            const uint32_t ox = corner_dx(cv), oy = corner_dy(cv), oz = corner_dz(cv);
+            const uint32_t ux = x - ox, uy = y - oy, uz = z - oz;
+            ok[cv] = x >= ox && y >= oy && z >= oz && ux + 1 < rx && uy + 1 < ry && uz + 1 < rz;
+            word[cv] = ((uint64_t)uz * ry + uy) * wx + (ux >> 5);
+            sh[cv] = ux & 31u;
+            bw[cv] = ok[cv] ? a.bits[word[cv]] : 0u;
+        }
+        uint32_t cb[8];
+#pragma unroll
+        for (uint32_t cv = 0; cv < 8; ++cv) {
+            ok[cv] = ok[cv] && ((bw[cv] >> sh[cv]) & 1u);
+            cb[cv] = ok[cv] ? a.word_off[word[cv]] + __popc(bw[cv] & ((1u << sh[cv]) - 1u)) : 0u;
+        }
+        uint32_t nv[8];
+#pragma unroll
+        for (uint32_t cv = 0; cv < 8; ++cv) nv[cv] = ok[cv] ? a.nverts[cb[cv]] : 0u;
+        uint32_t off[8];
+#pragma unroll
+        for (uint32_t cv = 0; cv < 8; ++cv) {
+            off[cv] = nv[cv] ? a.vert_off[cb[cv]] : 0u;
+        }
         float acc = 0.0f;
 #pragma unroll
         for (uint32_t cv = 0; cv < 8; ++cv) {
-            // the cube whose corner cv is this vertex
-            const uint32_t ox = corner_dx(cv), oy = corner_dy(cv), oz = corner_dz(cv);
-            if (x < ox || y < oy || z < oz) continue;
-            const uint32_t ux = x - ox, uy = y - oy, uz = z - oz;
-            if (ux + 1 >= rx || uy + 1 >= ry || uz + 1 >= rz) continue;
-            const uint64_t word = ((uint64_t)uz * ry + uy) * wx + (ux >> 5);
-            const uint32_t bw = a.bits[word];
-            if (!((bw >> (ux & 31u)) & 1u)) continue;
-            const uint32_t cb = a.word_off[word] + __popc(bw & ((1u << (ux & 31u)) - 1u));
-            const uint32_t nv = a.nverts[cb];
-            if (nv == 0) continue;
-            const uint32_t off = a.vert_off[cb];
-            if ((uint64_t)off + nv > a.capacity) continue;  // the forward did not emit it
-            acc += a.corner[8ull * (off / 3u) + cv];
+            // a cube the forward did not emit (beyond the soup capacity) adds nothing
+            if (nv[cv] && (uint64_t)off[cv] + nv[cv] <= a.capacity)
+                acc += a.corner[8ull * (off[cv] / 3u) + cv];
         }
         if (acc != 0.0f) a.g_phi[vid] += acc;
     }

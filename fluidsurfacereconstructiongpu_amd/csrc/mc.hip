@@ -42,11 +42,6 @@ __device__ __forceinline__ uint32_t case_vertices(uint32_t flag) {
     return 16u - (uint32_t)__popcll(f);
 }
 
-__device__ __forceinline__ bool band_bit(const uint32_t* __restrict__ bits, uint32_t wx,
-                                         uint32_t ry, uint32_t x, uint32_t y, uint32_t z) {
-    return (bits[((uint64_t)z * ry + y) * wx + (x >> 5)] >> (x & 31u)) & 1u;
-}
-
 __device__ __forceinline__ uint32_t cube_flag(const float* __restrict__ phi, uint64_t v,
                                               uint32_t rx, uint64_t rxy, float iso) {
     uint32_t f = 0;
@@ -70,14 +65,17 @@ __device__ __forceinline__ uint32_t classify_cube(const McArgs& a, uint32_t vid)
     uint32_t nv = 0;
     if (x < rx - 1 && y < ry - 1 && z < rz - 1) {
         // the corners' band bits first: the field is read only for cubes
-        // whose 8 corners are band vertices
-        bool all = true;
+        // whose 8 corners are band vertices.  The 8 words (x and x + 1 of the
+        // 4 rows, both inside the grid) are loaded together, not one after
+        // another behind a short-circuit test.
+        uint32_t all = 1u;
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const uint32_t dx = (c >= 4) ? 1u : 0u;
-            const uint32_t dy = (c == 2 || c == 3 || c == 6 || c == 7) ? 1u : 0u;
-            const uint32_t dz = (c == 1 || c == 2 || c == 5 || c == 6) ? 1u : 0u;
-            all = all && band_bit(a.bits, wx, ry, x + dx, y + dy, z + dz);
+        for (int r = 0; r < 4; ++r) {
+            const uint32_t dy = (r == 1 || r == 2) ? 1u : 0u, dz = (r >= 2) ? 1u : 0u;
+            const uint64_t row = ((uint64_t)(z + dz) * ry + (y + dy)) * wx;
+            const uint32_t w0 = a.bits[row + (x >> 5)];
+            const uint32_t w1 = a.bits[row + ((x + 1u) >> 5)];
+            all &= (w0 >> (x & 31u)) & (w1 >> ((x + 1u) & 31u)) & 1u;
         }
         if (all) nv = case_vertices(cube_flag(a.phi, vid, rx, rxy, a.iso));
     }
