@@ -110,14 +110,13 @@ struct CellCull {
     float b0[3], b1[3];
 };
 
-__device__ __forceinline__ bool cell_reaches(const CellCull& cc, uint32_t c) {
-    const float4 lo = cc.lo[c];
-    if (!(lo.w > 0.0f)) return false;
-    const float4 hi = cc.hi[c];
+// The test on a cell's box (lo, hi) as loaded.
+__device__ __forceinline__ bool cell_reaches(const CellCull& cc, const float4& lo,
+                                             const float4& hi) {
     const float dx = fmaxf(fmaxf(lo.x - cc.b1[0], cc.b0[0] - hi.x), 0.0f);
     const float dy = fmaxf(fmaxf(lo.y - cc.b1[1], cc.b0[1] - hi.y), 0.0f);
     const float dz = fmaxf(fmaxf(lo.z - cc.b1[2], cc.b0[2] - hi.z), 0.0f);
-    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < lo.w;
+    return lo.w > 0.0f && __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < lo.w;
 }
 
 template <int R>
@@ -134,12 +133,38 @@ __device__ __forceinline__ BlockRows block_rows(const fsr_index_range* __restric
     if ((int)lane < nrows) {
         const int yy = y0 + (int)lane % ny, zz = z0 + (int)lane / ny;
         const uint32_t rowc = ((uint32_t)zz * sp.ry + yy) * sp.rx;
+        // the row's 2R + 1 cells as clamped offsets (a repeated edge cell
+        // leaves the run unchanged), their loads issued together
+        constexpr int kW = 2 * R + 1;
+        uint32_t cell[kW];
+        fsr_index_range rg[kW];
+        bool in[kW];
+#pragma unroll
+        for (int d = 0; d < kW; ++d) {
+            cell[d] = rowc + (uint32_t)clampi(cx - R + d, x0, x1);
+            rg[d] = ranges[cell[d]];
+        }
+        if (cull) {
+            // every cell has a box entry (empty cells w = -1)
+            float4 lo[kW], hi[kW];
+#pragma unroll
+            for (int d = 0; d < kW; ++d) {
+                lo[d] = cull->lo[cell[d]];
+                hi[d] = cull->hi[cell[d]];
+            }
+#pragma unroll
+            for (int d = 0; d < kW; ++d)
+                in[d] = rg[d].start != 0xFFFFFFFFu && cell_reaches(*cull, lo[d], hi[d]);
+        } else {
+#pragma unroll
+            for (int d = 0; d < kW; ++d) in[d] = rg[d].start != 0xFFFFFFFFu;
+        }
         uint32_t s = 0xFFFFFFFFu, e = 0;
-        for (int xx = x0; xx <= x1; ++xx) {
-            const fsr_index_range rg = ranges[rowc + xx];
-            if (rg.start != 0xFFFFFFFFu && (!cull || cell_reaches(*cull, rowc + xx))) {
-                if (s == 0xFFFFFFFFu) s = rg.start;
-                e = rg.end;
+#pragma unroll
+        for (int d = 0; d < kW; ++d) {
+            if (in[d]) {
+                if (s == 0xFFFFFFFFu) s = rg[d].start;
+                e = rg[d].end;
             }
         }
         if (s != 0xFFFFFFFFu) { rs = s; len = e - s; }

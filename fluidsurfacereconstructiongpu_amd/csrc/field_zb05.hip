@@ -315,12 +315,17 @@ __global__ __launch_bounds__(kFieldThreads, 6) void k_field_zb05(FieldArgs a, Ba
         if ((int)lane < nrows) {
             const int yy = y0 + (int)lane % ny, zz = z0 + (int)lane / ny;
             const uint32_t rowc = ((uint32_t)zz * a.sp.ry + yy) * a.sp.rx;
+            // the row's 3 cells as clamped offsets (a repeated edge cell leaves
+            // the run unchanged), loaded together
+            fsr_index_range rg[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) rg[d] = a.ranges[rowc + clampi(cx - 1 + d, x0, x1)];
             uint32_t s = 0xFFFFFFFFu, e = 0;
-            for (int xx = x0; xx <= x1; ++xx) {
-                const fsr_index_range rg = a.ranges[rowc + xx];
-                if (rg.start != 0xFFFFFFFFu) {
-                    if (s == 0xFFFFFFFFu) s = rg.start;
-                    e = rg.end;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                if (rg[d].start != 0xFFFFFFFFu) {
+                    if (s == 0xFFFFFFFFu) s = rg[d].start;
+                    e = rg[d].end;
                 }
             }
             if (s != 0xFFFFFFFFu) { rs = s; len = e - s; }

@@ -50,16 +50,22 @@ __global__ __launch_bounds__(256) void k_cell_surface(const float* __restrict__ 
     const uint32_t x = c % sp.rx, y = (c / sp.rx) % sp.ry, z = c / (sp.rx * sp.ry);
     uint8_t surf = 0;
     if (x < sp.rx - 1 && y < sp.ry - 1 && z < sp.rz - 1 && occupancy[c] >= 1.0f) {
-        const int x0 = clampi((int)x - 1, 0, sp.rx - 1), x1 = clampi((int)x + 1, 0, sp.rx - 1);
-        const int y0 = clampi((int)y - 1, 0, sp.ry - 1), y1 = clampi((int)y + 1, 0, sp.ry - 1);
-        const int z0 = clampi((int)z - 1, 0, sp.rz - 1), z1 = clampi((int)z + 1, 0, sp.rz - 1);
-        for (int zz = z0; zz <= z1 && !surf; ++zz)
-            for (int yy = y0; yy <= y1 && !surf; ++yy)
-                for (int xx = x0; xx <= x1; ++xx)
-                    if (occupancy[((uint32_t)zz * sp.ry + yy) * sp.rx + xx] < 1.0f) {
-                        surf = 1;
-                        break;
-                    }
+        // the clamped 3^3 block as 27 clamped offsets (a clamped offset repeats
+        // a cell of the block, which leaves the test unchanged), all read with
+        // no early exit, so that their loads issue together
+#pragma unroll
+        for (int dz = -1; dz <= 1; ++dz) {
+            const uint32_t zz = (uint32_t)clampi((int)z + dz, 0, sp.rz - 1);
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy) {
+                const uint32_t yy = (uint32_t)clampi((int)y + dy, 0, sp.ry - 1);
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const uint32_t xx = (uint32_t)clampi((int)x + dx, 0, sp.rx - 1);
+                    surf |= occupancy[(zz * sp.ry + yy) * sp.rx + xx] < 1.0f ? 1 : 0;
+                }
+            }
+        }
     }
     surf_cell[c] = surf;
 }
