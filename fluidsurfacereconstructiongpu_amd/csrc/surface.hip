@@ -312,12 +312,17 @@ __global__ __launch_bounds__(256) void k_colorfield(SurfaceArgs a, float mass, C
         if ((int)lane < nrows) {
             const int yy = y0 + (int)lane % ny, zz = z0 + (int)lane / ny;
             const uint32_t rowc = ((uint32_t)zz * a.sp.ry + yy) * a.sp.rx;
+            // the row's 3 cells as clamped offsets (a repeated edge cell leaves
+            // the run unchanged), loaded together
+            fsr_index_range rg[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) rg[d] = a.ranges[rowc + clampi(cx - 1 + d, x0, x1)];
             uint32_t s0 = 0xFFFFFFFFu, e0 = 0;
-            for (int xx = x0; xx <= x1; ++xx) {
-                const fsr_index_range rg = a.ranges[rowc + xx];
-                if (rg.start != 0xFFFFFFFFu) {
-                    if (s0 == 0xFFFFFFFFu) s0 = rg.start;
-                    e0 = rg.end;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                if (rg[d].start != 0xFFFFFFFFu) {
+                    if (s0 == 0xFFFFFFFFu) s0 = rg[d].start;
+                    e0 = rg[d].end;
                 }
             }
             if (s0 != 0xFFFFFFFFu) { rs = s0; len = e0 - s0; }
