@@ -121,9 +121,11 @@ __device__ __forceinline__ uint32_t dilate_x_word(const uint8_t* __restrict__ in
 // with a halo of 3 rows and 3 layers on each side into LDS (from the byte
 // mask), ORs the 7 rows (y), then the 7 layers (z), and writes the tile.
 // Rows and layers outside the grid contribute nothing, as in the separate
-// passes.
+// passes.  1024 threads per tile: a thread loads the tile's 3,872 input
+// words in 4 rounds rather than 16, each round waiting for its loads.
 constexpr int kTW = 8, kTY = 16, kTZ = 16, kH = 3;
-__global__ __launch_bounds__(256) void k_dilate_box(const uint8_t* __restrict__ in,
+constexpr int kDilateThreads = 1024;
+__global__ __launch_bounds__(kDilateThreads) void k_dilate_box(const uint8_t* __restrict__ in,
                                                     uint32_t* __restrict__ out, uint32_t wx,
                                                     GridDev sc) {
     __shared__ uint32_t xs[kTZ + 2 * kH][kTY + 2 * kH][kTW];
@@ -384,7 +386,15 @@ __global__ __launch_bounds__(256) void k_band_emit(const uint32_t* __restrict__ 
     if (scanned) {
         before = threadIdx.x == 0 ? partials[blockIdx.x] : 0u;
     } else {
-        for (uint32_t i = threadIdx.x; i < blockIdx.x; i += blockDim.x) before += partials[i];
+        // at most 4096 blocks: up to 16 counts per thread, loaded together
+        uint32_t pv[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const uint32_t i = threadIdx.x + 256u * k;
+            pv[k] = i < blockIdx.x ? partials[i] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) before += pv[k];
     }
     before = wave_sum(before);
     // exclusive scan of the words' counts inside the block
@@ -667,7 +677,7 @@ fsr_status band_build(fsr_context* ctx, const BandArgs& a, hipStream_t s) {
     uint32_t* sb = const_cast<uint32_t*>(a.sbits);
     const dim3 tiles((wx + kTW - 1) / kTW, (a.sc.ry + kTY - 1) / kTY, (a.sc.rz + kTZ - 1) / kTZ);
     if (tiles.y <= 65535u && tiles.z <= 65535u) {
-        hipLaunchKernelGGL(k_dilate_box, tiles, dim3(256), 0, s, (const uint8_t*)sb, a.bits, wx,
+        hipLaunchKernelGGL(k_dilate_box, tiles, dim3(kDilateThreads), 0, s, (const uint8_t*)sb, a.bits, wx,
                            a.sc);
         FSR_LAUNCH_CHECK("k_dilate_box");
     } else {

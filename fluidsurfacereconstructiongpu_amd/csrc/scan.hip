@@ -117,8 +117,19 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(
     const uint64_t seg = segment_len(n, gridDim.x);
     const uint64_t lo = (uint64_t)blockIdx.x * seg;
     const uint64_t hi = lo + seg < n ? lo + seg : n;
+    // the partials of the blocks before this one (at most kMaxScanBlocks),
+    // loaded together
     uint32_t s = 0;
-    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kScanThreads) s += partials[i];
+    {
+        uint32_t pv[kMaxScanBlocks / kScanThreads];
+#pragma unroll
+        for (uint32_t k = 0; k < kMaxScanBlocks / kScanThreads; ++k) {
+            const uint32_t i = threadIdx.x + kScanThreads * k;
+            pv[k] = i < blockIdx.x ? partials[i] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kMaxScanBlocks / kScanThreads; ++k) s += pv[k];
+    }
     uint32_t excl;
     uint32_t running = block_exclusive(s, lds, &excl);
     if (!out) {
@@ -218,10 +229,22 @@ __global__ __launch_bounds__(kScanThreads) void k_mc_scan_down(
     const uint64_t seg = segment_len(n, gridDim.x);
     const uint64_t lo = (uint64_t)blockIdx.x * seg;
     const uint64_t hi = lo + seg < n ? lo + seg : n;
+    // the two partials of the blocks before this one (at most kMaxScanBlocks),
+    // loaded together
     uint32_t s0 = 0, s1 = 0;
-    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kScanThreads) {
-        s0 += partials[i];
-        s1 += partials[gridDim.x + i];
+    {
+        uint32_t p0[kMaxScanBlocks / kScanThreads], p1[kMaxScanBlocks / kScanThreads];
+#pragma unroll
+        for (uint32_t k = 0; k < kMaxScanBlocks / kScanThreads; ++k) {
+            const uint32_t i = threadIdx.x + kScanThreads * k;
+            p0[k] = i < blockIdx.x ? partials[i] : 0u;
+            p1[k] = i < blockIdx.x ? partials[gridDim.x + i] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kMaxScanBlocks / kScanThreads; ++k) {
+            s0 += p0[k];
+            s1 += p1[k];
+        }
     }
     uint32_t excl;
     uint32_t run_v = block_exclusive(s0, lds, &excl);
@@ -326,10 +349,22 @@ __global__ __launch_bounds__(kScanThreads) void k_cells_scan_down(
     const uint64_t seg = cell_segment(n, gridDim.x);
     const uint64_t lo = (uint64_t)blockIdx.x * seg;
     const uint64_t hi = lo + seg < n ? lo + seg : n;
+    // the two partials of the blocks before this one (at most kMaxCellBlocks),
+    // loaded together
     uint32_t s0 = 0, s1 = 0;
-    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kScanThreads) {
-        s0 += partials[i];
-        s1 += partials[gridDim.x + i];
+    {
+        uint32_t p0[kMaxCellBlocks / kScanThreads], p1[kMaxCellBlocks / kScanThreads];
+#pragma unroll
+        for (uint32_t k = 0; k < kMaxCellBlocks / kScanThreads; ++k) {
+            const uint32_t i = threadIdx.x + kScanThreads * k;
+            p0[k] = i < blockIdx.x ? partials[i] : 0u;
+            p1[k] = i < blockIdx.x ? partials[gridDim.x + i] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kMaxCellBlocks / kScanThreads; ++k) {
+            s0 += p0[k];
+            s1 += p1[k];
+        }
     }
     uint32_t excl;
     uint32_t run_v = block_exclusive(s0, lds, &excl);
