@@ -569,11 +569,37 @@ __global__ __launch_bounds__(256) void k_slab_rows(const uint32_t* __restrict__ 
         for (uint32_t yb = 0; nw && yb < ny; yb += 32u) {
             uint32_t rows = layer_rows(m, sc, z, ylo, yb, ny);
             while (rows) {
-                const uint64_t row = (uint64_t)z * sc.ry + ylo + yb + __builtin_ctz(rows);
-                rows &= rows - 1u;
-                for (uint32_t j = 0; j < nw; ++j) {
-                    const uint32_t w = w0 + j;
-                    o += __popc(clip_word(bits[row * wx + w], w, xlo, xhi));
+                // up to 4 occupied rows per round; with at most 2 words per
+                // row (the common case) their 8 words are loaded together
+                uint64_t rw[4];
+                bool ok[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    ok[k] = rows != 0u;
+                    const uint32_t dy = ok[k] ? (uint32_t)__builtin_ctz(rows) : 0u;
+                    rw[k] = ((uint64_t)z * sc.ry + ylo + yb + dy) * wx;
+                    rows &= rows - 1u;
+                }
+                if (nw <= 2u) {
+                    uint32_t v0[4], v1[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        v0[k] = ok[k] ? bits[rw[k] + w0] : 0u;
+                        v1[k] = ok[k] && nw == 2u ? bits[rw[k] + w0 + 1u] : 0u;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        o += __popc(clip_word(v0[k], w0, xlo, xhi));
+                        if (nw == 2u) o += __popc(clip_word(v1[k], w0 + 1u, xlo, xhi));
+                    }
+                } else {
+                    for (int k = 0; k < 4; ++k) {
+                        if (!ok[k]) break;
+                        for (uint32_t j = 0; j < nw; ++j) {
+                            const uint32_t w = w0 + j;
+                            o += __popc(clip_word(bits[rw[k] + w], w, xlo, xhi));
+                        }
+                    }
                 }
             }
         }
@@ -623,21 +649,48 @@ __global__ __launch_bounds__(256) void k_slab_fill(const uint32_t* __restrict__ 
         const uint32_t ny = yhi > ylo ? yhi - ylo : 0u;
         const uint32_t w0 = xlo >> 5;
         const uint32_t nw = xhi > xlo ? ((xhi + 31u) >> 5) - w0 : 0u;
+        // the slab's ids in row order, then word order
+        auto put = [&](uint64_t row, uint32_t w, uint32_t word) {
+            word = clip_word(word, w, xlo, xhi);
+            const uint32_t base = (uint32_t)(row * sc.rx) + w * 32u;
+            while (word) {
+                const uint32_t id = base + __builtin_ctz(word);
+                if (staged) stage[o - lo] = id;
+                else if (o < cap) cell_band[o] = id;
+                ++o;
+                word &= word - 1u;
+            }
+        };
         for (uint32_t yb = 0; nw && yb < ny; yb += 32u) {
             uint32_t rows = layer_rows(m, sc, z, ylo, yb, ny);
             while (rows) {
-                const uint64_t row = (uint64_t)z * sc.ry + ylo + yb + __builtin_ctz(rows);
-                rows &= rows - 1u;
-                for (uint32_t j = 0; j < nw; ++j) {
-                    const uint32_t w = w0 + j;
-                    uint32_t word = clip_word(bits[row * wx + w], w, xlo, xhi);
-                    const uint32_t base = (uint32_t)(row * sc.rx) + w * 32u;
-                    while (word) {
-                        const uint32_t id = base + __builtin_ctz(word);
-                        if (staged) stage[o - lo] = id;
-                        else if (o < cap) cell_band[o] = id;
-                        ++o;
-                        word &= word - 1u;
+                // up to 4 occupied rows per round; with at most 2 words per
+                // row (the common case) their 8 words are loaded together
+                uint64_t rr[4];
+                bool ok[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    ok[k] = rows != 0u;
+                    const uint32_t dy = ok[k] ? (uint32_t)__builtin_ctz(rows) : 0u;
+                    rr[k] = (uint64_t)z * sc.ry + ylo + yb + dy;
+                    rows &= rows - 1u;
+                }
+                if (nw <= 2u) {
+                    uint32_t v0[4], v1[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        v0[k] = ok[k] ? bits[rr[k] * wx + w0] : 0u;
+                        v1[k] = ok[k] && nw == 2u ? bits[rr[k] * wx + w0 + 1u] : 0u;
+                    }
+                    for (int k = 0; k < 4; ++k) {
+                        if (!ok[k]) break;
+                        put(rr[k], w0, v0[k]);
+                        if (nw == 2u) put(rr[k], w0 + 1u, v1[k]);
+                    }
+                } else {
+                    for (int k = 0; k < 4; ++k) {
+                        if (!ok[k]) break;
+                        for (uint32_t j = 0; j < nw; ++j) put(rr[k], w0 + j, bits[rr[k] * wx + w0 + j]);
                     }
                 }
             }

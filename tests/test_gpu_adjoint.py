@@ -203,7 +203,7 @@ def test_c5_ball256k_against_fd(torch_cuda):
     loss = sphere_loss(phi, soup, fr)
     loss.backward()
     g_gpu = x.grad.cpu().numpy().astype(np.float64)
-    print("  gpu done, loss %.6e" % float(loss), flush=True)
+    print("  gpu done, loss %.6e" % float(loss.detach()), flush=True)
     A = oracle.AdjointWorkload("zb05", fr)
     print("  oracle workload B=%d K=%d (gpu B=%d K=%d) loss %.6e" % (
         A.B, A.K, sf.stats["num_band_vertices"], sf.stats["num_mesh_vertices"], A.loss()),
