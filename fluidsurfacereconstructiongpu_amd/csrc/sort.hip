@@ -32,22 +32,26 @@ namespace {
 constexpr int kSortThreads = 256;
 constexpr int kSortRounds = 8;
 constexpr int kSortTile = kSortThreads * kSortRounds;
+// The histogram kernels take a tile with 1024 threads in 2 rounds, so that
+// four times as many waves have their loads in flight.
+constexpr int kHistThreads = 1024;
+constexpr int kHistRounds = kSortTile / kHistThreads;
 
 // Cell id per particle, the packed (x, y, z, density) copy, and the first
 // pass's digit histogram of the block's 2048-key tile.
-__global__ __launch_bounds__(kSortThreads) void k_hash(const float* __restrict__ pos,
+__global__ __launch_bounds__(kHistThreads) void k_hash(const float* __restrict__ pos,
                                                        const float* __restrict__ dens, uint32_t n,
                                                        GridDev sp, float4* __restrict__ packed,
                                                        uint32_t* __restrict__ keys,
                                                        uint32_t* __restrict__ hist, uint32_t tiles) {
     __shared__ uint32_t cnt[256];
-    cnt[threadIdx.x] = 0;
+    if (threadIdx.x < 256) cnt[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t C = sp.rx * sp.ry * sp.rz;
     const uint32_t base = blockIdx.x * kSortTile;
 #pragma unroll
-    for (int r = 0; r < kSortRounds; ++r) {
-        const uint32_t i = base + r * kSortThreads + threadIdx.x;
+    for (int r = 0; r < kHistRounds; ++r) {
+        const uint32_t i = base + r * kHistThreads + threadIdx.x;
         if (i >= n) continue;
         const float px = pos[3ull * i + 0];
         const float py = pos[3ull * i + 1];
@@ -69,24 +73,24 @@ __global__ __launch_bounds__(kSortThreads) void k_hash(const float* __restrict__
         atomicAdd(&cnt[key & 255u], 1u);
     }
     __syncthreads();
-    hist[threadIdx.x * tiles + blockIdx.x] = cnt[threadIdx.x];
+    if (threadIdx.x < 256) hist[threadIdx.x * tiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ keys,
+__global__ __launch_bounds__(kHistThreads) void k_radix_hist(const uint32_t* __restrict__ keys,
                                                              uint32_t n, uint32_t shift,
                                                              uint32_t* __restrict__ hist,
                                                              uint32_t tiles) {
     __shared__ uint32_t cnt[256];
-    cnt[threadIdx.x] = 0;
+    if (threadIdx.x < 256) cnt[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * kSortTile;
 #pragma unroll
-    for (int r = 0; r < kSortRounds; ++r) {
-        const uint32_t i = base + r * kSortThreads + threadIdx.x;
+    for (int r = 0; r < kHistRounds; ++r) {
+        const uint32_t i = base + r * kHistThreads + threadIdx.x;
         if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
     }
     __syncthreads();
-    hist[threadIdx.x * tiles + blockIdx.x] = cnt[threadIdx.x];
+    if (threadIdx.x < 256) hist[threadIdx.x * tiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
 // Per digit (one block each): exclusive prefix of the digit's per-tile counts
@@ -307,7 +311,7 @@ fsr_status spatial_hash(fsr_context* ctx, float* pos, float* dens, uint32_t n,
         return FSR_OK;
     }
     const uint32_t blocks = (n + 255) / 256;
-    hipLaunchKernelGGL(k_hash, dim3(tiles), dim3(kSortThreads), 0, s, pos, dens, n, sp, packed, k0,
+    hipLaunchKernelGGL(k_hash, dim3(tiles), dim3(kHistThreads), 0, s, pos, dens, n, sp, packed, k0,
                        hist, tiles);
     FSR_LAUNCH_CHECK("k_hash");
 
@@ -321,7 +325,7 @@ fsr_status spatial_hash(fsr_context* ctx, float* pos, float* dens, uint32_t n,
         const uint32_t shift = 8u * p;
         // the first pass's histogram comes from k_hash
         if (p > 0) {
-            hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kSortThreads), 0, s, kin, n, shift,
+            hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kHistThreads), 0, s, kin, n, shift,
                                hist, tiles);
             FSR_LAUNCH_CHECK("k_radix_hist");
         }
