@@ -19,10 +19,10 @@
 // owns a 2048-key tile.  Pass = per-tile digit histogram in LDS (for the first
 // pass, made by k_hash) -> per digit, an exclusive scan over the tiles and the
 // digit total (one block per digit) -> stable scatter (which adds the totals of
-// the smaller digits): each wave ranks its 512 contiguous
-// keys with eight ballots per round and wave-private running digit counts, the
-// block combines the four waves' counts once, reorders the tile by digit in LDS
-// and writes each digit's run with consecutive threads.
+// the smaller digits): each of 8 waves ranks its 256 contiguous keys with
+// eight ballots per round and wave-private running digit counts, the block
+// combines the waves' counts once, reorders the tile by digit in LDS and
+// writes each digit's run with consecutive threads.
 #include "common.hpp"
 
 namespace fsr {
@@ -127,56 +127,62 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Stable scatter of one 2048-key tile.  Wave w owns the contiguous keys
-// [512 w, 512 w + 512) of the tile, in 8 rounds of 64:
+// Stable scatter of one 2048-key tile by kScatterWaves waves.  Wave w owns the
+// contiguous keys [256 w, 256 w + 256) of the tile, in 4 rounds of 64:
 //   1. per round, the lanes holding the same digit are found with 8 ballots;
 //      a key's rank in the wave's sub-tile is the wave's running count of its
 //      digit plus its rank among those lanes (wave-private LDS counters);
-//   2. one block step turns the four waves' digit counts into the start of
-//      every (wave, digit) run in the tile's digit order;
+//   2. one block step (threads 0..255, one digit each) turns the waves' digit
+//      counts into the start of every (wave, digit) run in the tile's digit
+//      order;
 //   3. keys and values are placed in LDS in that order and written out with
 //      consecutive threads on consecutive positions of each digit's run.
+constexpr int kScatterWaves = 8;
+constexpr int kScatterThreads = 64 * kScatterWaves;
+constexpr int kScatterRounds = kSortTile / kScatterThreads;
 template <bool kFirst>
-__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
+__global__ __launch_bounds__(kScatterThreads) void k_radix_scatter(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n,
     uint32_t shift, const uint32_t* __restrict__ offsets, uint32_t tiles,
     const uint32_t* __restrict__ totals) {
-    __shared__ uint32_t wcnt[4][256];   // counts, then run starts, of (wave, digit)
-    __shared__ uint32_t dstart[256];    // first tile slot of each digit
-    __shared__ uint32_t gbase[256];     // output position of the digit's first key of the tile
+    __shared__ uint32_t wcnt[kScatterWaves][256];  // counts, then run starts, of (wave, digit)
+    __shared__ uint32_t dstart[256];               // first tile slot of each digit
+    __shared__ uint32_t gbase[256];  // output position of the digit's first key of the tile
     __shared__ uint32_t wsum[4];
     __shared__ uint32_t skey[kSortTile];
     __shared__ uint32_t sval[kSortTile];
     const uint32_t t = threadIdx.x;
     const uint32_t wave = t >> 6, lane = t & 63u;
-#pragma unroll
-    for (int w = 0; w < 4; ++w) wcnt[w][t] = 0;
+    const bool digit_thread = t < 256u;  // waves 0..3: thread t stands for digit t
+    for (uint32_t k = t; k < kScatterWaves * 256u; k += kScatterThreads) (&wcnt[0][0])[k] = 0;
     {
         // output position of digit t's first key of this tile: the keys of
         // all smaller digits, then digit t's keys of the tiles before this one
-        const uint32_t tot = totals[t];
+        const uint32_t tot = digit_thread ? totals[t] : 0u;
         const uint32_t incl = wave_inclusive_sum(tot);
-        if ((t & 63u) == 63u) wsum[t >> 6] = incl;
+        if (digit_thread && lane == 63u) wsum[wave] = incl;
         __syncthreads();
-        uint32_t before = 0;
+        if (digit_thread) {
+            uint32_t before = 0;
 #pragma unroll
-        for (int w = 0; w < 4; ++w) before += (w < (int)(t >> 6)) ? wsum[w] : 0u;
-        gbase[t] = before + incl - tot + offsets[t * tiles + blockIdx.x];
+            for (int w = 0; w < 4; ++w) before += (w < (int)wave) ? wsum[w] : 0u;
+            gbase[t] = before + incl - tot + offsets[t * tiles + blockIdx.x];
+        }
     }
     __syncthreads();
     const uint32_t tile0 = blockIdx.x * kSortTile;
-    const uint32_t base = tile0 + wave * (kSortTile / 4);
-    uint32_t key[kSortRounds], val[kSortRounds], loc[kSortRounds];
+    const uint32_t base = tile0 + wave * (kSortTile / kScatterWaves);
+    uint32_t key[kScatterRounds], val[kScatterRounds], loc[kScatterRounds];
 #pragma unroll
-    for (int j = 0; j < kSortRounds; ++j) {
+    for (int j = 0; j < kScatterRounds; ++j) {
         const uint32_t i = base + j * 64u + lane;
         const bool valid = i < n;
         key[j] = valid ? keys_in[i] : 0u;
         val[j] = kFirst ? i : (valid ? vals_in[i] : 0u);
     }
 #pragma unroll
-    for (int j = 0; j < kSortRounds; ++j) {
+    for (int j = 0; j < kScatterRounds; ++j) {
         const bool valid = base + j * 64u + lane < n;
         const uint32_t d = (key[j] >> shift) & 255u;
         uint64_t peers = __ballot(valid);
@@ -195,26 +201,34 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     }
     __syncthreads();
     {
-        // thread t: digit t.  Tile total of the digit, its start in the tile
-        // (exclusive scan over digits), and the start of each wave's run.
-        const uint32_t c0 = wcnt[0][t], c1 = wcnt[1][t], c2 = wcnt[2][t], c3 = wcnt[3][t];
-        const uint32_t tot = c0 + c1 + c2 + c3;
-        const uint32_t incl = wave_inclusive_sum(tot);
-        if (lane == 63) wsum[wave] = incl;
-        __syncthreads();
-        uint32_t before = 0;
+        // thread t < 256: digit t.  Tile total of the digit, its start in the
+        // tile (exclusive scan over digits), and the start of each wave's run.
+        uint32_t c[kScatterWaves];
+        uint32_t tot = 0;
 #pragma unroll
-        for (int w = 0; w < 4; ++w) before += (w < (int)wave) ? wsum[w] : 0u;
-        const uint32_t st = before + incl - tot;
-        dstart[t] = st;
-        wcnt[0][t] = st;
-        wcnt[1][t] = st + c0;
-        wcnt[2][t] = st + c0 + c1;
-        wcnt[3][t] = st + c0 + c1 + c2;
+        for (int w = 0; w < kScatterWaves; ++w) {
+            c[w] = digit_thread ? wcnt[w][t] : 0u;
+            tot += c[w];
+        }
+        const uint32_t incl = wave_inclusive_sum(tot);
+        if (digit_thread && lane == 63u) wsum[wave] = incl;
+        __syncthreads();
+        if (digit_thread) {
+            uint32_t before = 0;
+#pragma unroll
+            for (int w = 0; w < 4; ++w) before += (w < (int)wave) ? wsum[w] : 0u;
+            uint32_t st = before + incl - tot;
+            dstart[t] = st;
+#pragma unroll
+            for (int w = 0; w < kScatterWaves; ++w) {
+                wcnt[w][t] = st;
+                st += c[w];
+            }
+        }
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < kSortRounds; ++j) {
+    for (int j = 0; j < kScatterRounds; ++j) {
         if (base + j * 64u + lane < n) {
             const uint32_t slot = wcnt[wave][(key[j] >> shift) & 255u] + loc[j];
             skey[slot] = key[j];
@@ -223,7 +237,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     }
     __syncthreads();
     const uint32_t count = min((uint32_t)kSortTile, n - tile0);
-    for (uint32_t q = t; q < count; q += kSortThreads) {
+    for (uint32_t q = t; q < count; q += kScatterThreads) {
         const uint32_t k = skey[q];
         const uint32_t d = (k >> shift) & 255u;
         const uint32_t p = gbase[d] + (q - dstart[d]);
@@ -332,10 +346,10 @@ fsr_status spatial_hash(fsr_context* ctx, float* pos, float* dens, uint32_t n,
         hipLaunchKernelGGL(k_digit_scan, dim3(256), dim3(kSortThreads), 0, s, hist, tiles, totals);
         FSR_LAUNCH_CHECK("k_digit_scan");
         if (p == 0)
-            hipLaunchKernelGGL(k_radix_scatter<true>, dim3(tiles), dim3(kSortThreads), 0, s, kin,
+            hipLaunchKernelGGL(k_radix_scatter<true>, dim3(tiles), dim3(kScatterThreads), 0, s, kin,
                                vin, kout, vout, n, shift, hist, tiles, totals);
         else
-            hipLaunchKernelGGL(k_radix_scatter<false>, dim3(tiles), dim3(kSortThreads), 0, s, kin,
+            hipLaunchKernelGGL(k_radix_scatter<false>, dim3(tiles), dim3(kScatterThreads), 0, s, kin,
                                vin, kout, vout, n, shift, hist, tiles, totals);
         FSR_LAUNCH_CHECK("k_radix_scatter");
         uint32_t* t;
