@@ -144,16 +144,10 @@ __global__ __launch_bounds__(256) void k_flags_from_cells(SurfaceArgs a,
     uint32_t n_surf = 0, n_exc = 0;
     for (uint32_t base = blockIdx.x * span; base < a.n; base += gridDim.x * span) {
         uint32_t key[kPerThread];
-        float4 p[kPerThread];
 #pragma unroll
         for (uint32_t k = 0; k < kPerThread; ++k) {
             const uint32_t i = base + k * blockDim.x + threadIdx.x;
-            key[k] = C;
-            p[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (i < a.n) {
-                key[k] = a.keys[i];
-                p[k] = a.sorted[i];
-            }
+            key[k] = i < a.n ? a.keys[i] : C;
         }
         bool surf[kPerThread];
 #pragma unroll
@@ -161,6 +155,13 @@ __global__ __launch_bounds__(256) void k_flags_from_cells(SurfaceArgs a,
             const uint32_t i = base + k * blockDim.x + threadIdx.x;
             surf[k] = key[k] < C && surf_cell[key[k]] != 0;
             if (i < a.n) a.flags[i] = surf[k] ? 1u : 0u;
+        }
+        // positions only of the surface particles (the band seeds)
+        float4 p[kPerThread];
+#pragma unroll
+        for (uint32_t k = 0; k < kPerThread; ++k) {
+            const uint32_t i = base + k * blockDim.x + threadIdx.x;
+            p[k] = (a.sbits && surf[k]) ? a.sorted[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
 #pragma unroll
         for (uint32_t k = 0; k < kPerThread; ++k) {
